@@ -1,0 +1,567 @@
+// gp_ctx.hip -- context, C ABI and the launch groups of the reference driver.
+#include "gp_internal.h"
+#include "../../include/gaspacho2.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+static thread_local std::string g_err;
+
+void gp_set_error(const std::string &msg) { g_err = msg; }
+
+extern "C" const char *gp_last_error(void) { return g_err.c_str(); }
+extern "C" int gp_version(void) { return 1 * 10000 + 0 * 100 + 0; }
+
+// ------------------------------------------------------------------ timing
+void gp_timer_begin(gp_ctx *c, gp_timer &t) {
+    if (!c->timing) return;
+    if (!t.a) { (void)hipEventCreate(&t.a); (void)hipEventCreate(&t.b); }
+    (void)hipEventRecord(t.a, c->stream);
+    t.open = true;
+}
+void gp_timer_end(gp_ctx *c, gp_timer &t) {
+    if (!c->timing || !t.open) return;
+    (void)hipEventRecord(t.b, c->stream);
+    (void)hipEventSynchronize(t.b);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, t.a, t.b);
+    t.total_ms += ms;
+    t.count += 1;
+    t.open = false;
+}
+
+// ------------------------------------------------------------------ memory
+template <class T> static int dmalloc(T **p, size_t n) {
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+    if (e != hipSuccess) {
+        gp_set_error(std::string("hipMalloc of ") + std::to_string(n * sizeof(T)) +
+                     " bytes: " + hipGetErrorString(e));
+        return (int)e;
+    }
+    e = hipMemset(*p, 0, n * sizeof(T));
+    if (e != hipSuccess) { gp_set_error("hipMemset failed"); return (int)e; }
+    return 0;
+}
+
+static void free_all(gp_ctx *c) {
+    void *bufs[] = {c->Yp, c->Yi, c->Yx, c->d_nh, c->Z, c->ls, c->Omega, c->delta, c->X, c->T,
+                    c->rho, c->phi, c->zeta, c->K, c->Ksym, c->Kchol, c->Kinv, c->A, c->titsias,
+                    c->DeltaInv, c->DeltaChol, c->W, c->Oinvt, c->Nu, c->Psi, c->Sinv, c->Beta,
+                    c->tBeta, c->bvec, c->tBphi, c->lkhds, c->gq, c->gs, c->grad, c->grad0, c->ss,
+                    c->SS, c->XX, c->lbq, c->gradDeltaTmp, c->gradLogRhoTmp, c->C1, c->C2, c->U,
+                    c->ZB, c->R, c->C1a, c->AO, c->G2, c->B2, c->S2, c->Ecell, c->red, c->scal,
+                    c->lkhd, c->info, c->Zh, c->Zl, c->gram_part, c->pwork};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    gp_timer *ts[] = {&c->t_gram, &c->t_chol, &c->t_gc, &c->t_kuf, &c->t_elbo, &c->t_grad};
+    for (gp_timer *t : ts) {
+        if (t->a) (void)hipEventDestroy(t->a);
+        if (t->b) (void)hipEventDestroy(t->b);
+    }
+    if (c->blas) rocblas_destroy_handle(c->blas);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+// ------------------------------------------------------------------ create
+extern "C" int gp_create(gp_ctx **out, int N, int J, int M, int Q, int p, const int *nh,
+                         const int *Yp, const int *Yi, const double *Yx, long long L,
+                         const double *Zcov, const double *ls, const double *X0,
+                         const double *T0, int ard_only, int precision) {
+    if (!out) { gp_set_error("out is NULL"); return 1; }
+    *out = nullptr;
+    if (N <= 0 || J <= 0 || M <= 0 || Q <= 0 || p < 2 || !nh) { gp_set_error("bad dimensions"); return 1; }
+    if (Q > 64) { gp_set_error("Q > 64 unsupported"); return 1; }
+    if (nh[0] != 0 || nh[p] - nh[p - 1] != M) { gp_set_error("nh must end with the M-column Knm block"); return 1; }
+    for (int l = 0; l < p; l++)
+        if (nh[l + 1] <= nh[l]) { gp_set_error("nh must be increasing"); return 1; }
+    if (precision != GP_PREC_FP64 && precision != GP_PREC_MIXED) { gp_set_error("bad precision"); return 1; }
+    if (L < 0 || (L > 0 && (!Yi || !Yx)) || !Yp || !ls || !X0 || !T0 || !Zcov) {
+        gp_set_error("NULL input array"); return 1;
+    }
+    if (Yp[0] != 0 || Yp[J] != L) { gp_set_error("Yp must start at 0 and end at L"); return 1; }
+    for (int j = 0; j < J; j++) {
+        if (Yp[j + 1] < Yp[j]) { gp_set_error("Yp must be non-decreasing"); return 1; }
+        for (long long l = Yp[j]; l < Yp[j + 1]; l++) {
+            if (Yi[l] < 0 || Yi[l] >= N) { gp_set_error("row index out of range"); return 1; }
+            if (l > Yp[j] && Yi[l] <= Yi[l - 1]) { gp_set_error("row indices must ascend within a gene"); return 1; }
+        }
+    }
+    gp_ctx *c = new gp_ctx();
+    c->N = N; c->J = J; c->M = M; c->Q = Q; c->p = p; c->L = L;
+    c->nh.assign(nh, nh + p + 1);
+    c->P = nh[p];
+    c->g0 = nh[p - 1];
+    c->Npad = ((N + 255) / 256) * 256;
+    c->ard_only = ard_only ? 1 : 0;
+    c->precision = precision;
+    const size_t Np = c->Npad, P = c->P;
+    c->tot_max = (size_t)N + p + ((size_t)N + M + 1) * Q;
+    int r = 0;
+#define ALLOC(ptr, n) do { if ((r = dmalloc(&(ptr), (n))) != 0) { free_all(c); delete c; return r; } } while (0)
+    hipError_t e = hipGetDevice(&c->dev);
+    if (e != hipSuccess) { gp_set_error(std::string("hipGetDevice: ") + hipGetErrorString(e)); delete c; return (int)e; }
+    e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { gp_set_error("hipStreamCreate failed"); delete c; return (int)e; }
+    if (rocblas_create_handle(&c->blas) != rocblas_status_success) { gp_set_error("rocblas_create_handle failed"); free_all(c); delete c; return 950; }
+    rocblas_set_stream(c->blas, c->stream);
+    ALLOC(c->Yp, (size_t)J + 1);
+    ALLOC(c->Yi, (size_t)L + 1);
+    ALLOC(c->Yx, (size_t)L + 1);
+    ALLOC(c->d_nh, (size_t)p + 1);
+    ALLOC(c->Z, Np * P);
+    ALLOC(c->ls, (size_t)N);
+    ALLOC(c->Omega, 2 * (size_t)N);
+    ALLOC(c->delta, 2 * (size_t)p);
+    ALLOC(c->X, 2 * (size_t)N * Q);
+    ALLOC(c->T, 2 * (size_t)M * Q);
+    ALLOC(c->rho, 2 * (size_t)Q);
+    ALLOC(c->phi, (size_t)J);
+    ALLOC(c->zeta, P);
+    ALLOC(c->K, (size_t)M * M);
+    ALLOC(c->Ksym, (size_t)M * M);
+    ALLOC(c->Kchol, (size_t)M * M);
+    ALLOC(c->Kinv, (size_t)M * M);
+    ALLOC(c->A, Np * M);
+    ALLOC(c->titsias, (size_t)N);
+    ALLOC(c->DeltaInv, P * P);
+    ALLOC(c->DeltaChol, P * P);
+    ALLOC(c->W, Np * J);
+    ALLOC(c->Oinvt, (size_t)N);
+    ALLOC(c->Nu, Np * J);
+    ALLOC(c->Psi, (size_t)J * P * P);
+    ALLOC(c->Sinv, (size_t)J * P * P);
+    ALLOC(c->Beta, P * J);
+    ALLOC(c->tBeta, P * J);
+    ALLOC(c->bvec, P * J);
+    ALLOC(c->tBphi, P * J);
+    ALLOC(c->pwork, P * J);
+    ALLOC(c->lkhds, (size_t)J);
+    ALLOC(c->gq, (size_t)J);
+    ALLOC(c->gs, (size_t)J);
+    ALLOC(c->grad, c->tot_max);
+    ALLOC(c->grad0, c->tot_max);
+    ALLOC(c->ss, c->tot_max);
+    ALLOC(c->SS, c->tot_max * GP_HIST);
+    ALLOC(c->XX, c->tot_max * GP_HIST);
+    ALLOC(c->lbq, c->tot_max);
+    ALLOC(c->gradDeltaTmp, P * J);
+    ALLOC(c->gradLogRhoTmp, (size_t)M * Q);
+    ALLOC(c->C1, Np * M);
+    ALLOC(c->C2, (size_t)M * M);
+    ALLOC(c->U, Np * P);
+    ALLOC(c->ZB, Np * J);
+    ALLOC(c->R, Np * J);
+    ALLOC(c->C1a, Np * M);
+    ALLOC(c->AO, Np * M);
+    ALLOC(c->G2, (size_t)M * M);
+    ALLOC(c->B2, (size_t)M * M);
+    ALLOC(c->S2, (size_t)M * M);
+    ALLOC(c->Ecell, (size_t)N);
+    ALLOC(c->red, 4096);
+    ALLOC(c->scal, 64);
+    ALLOC(c->lkhd, GP_LKHD_SLOTS);
+    ALLOC(c->info, 16);
+#undef ALLOC
+#define UP(dst, src, n) do { e = hipMemcpy((dst), (src), (n), hipMemcpyHostToDevice); \
+        if (e != hipSuccess) { gp_set_error(std::string("upload: ") + hipGetErrorString(e)); free_all(c); delete c; return (int)e; } } while (0)
+    UP(c->Yp, Yp, sizeof(int) * ((size_t)J + 1));
+    if (L > 0) { UP(c->Yi, Yi, sizeof(int) * (size_t)L); UP(c->Yx, Yx, sizeof(double) * (size_t)L); }
+    UP(c->d_nh, nh, sizeof(int) * ((size_t)p + 1));
+    e = hipMemcpy2D(c->Z, Np * sizeof(double), Zcov, (size_t)N * sizeof(double),
+                    (size_t)N * sizeof(double), c->g0, hipMemcpyHostToDevice);
+    if (e != hipSuccess) { gp_set_error("Zcov upload failed"); free_all(c); delete c; return (int)e; }
+    UP(c->ls, ls, sizeof(double) * (size_t)N);
+    UP(c->X, X0, sizeof(double) * (size_t)N * Q);
+    UP(c->T, T0, sizeof(double) * (size_t)M * Q);
+    // initParams init.cu:27-48
+    {
+        std::vector<double> d(2 * (size_t)p, 0.0), om(2 * (size_t)N, 0.0), ph(J, 1.0), rh(2 * (size_t)Q, 0.0);
+        d[0] = 1.5;
+        for (int i = 1; i < p - 1; i++) d[i] = 0.01;
+        d[p - 1] = 0.3;
+        for (int i = 0; i < N; i++) om[i] = 1.0;
+        rh[0] = 5.0;
+        for (int k = 1; k < Q; k++) rh[k] = 10.0;
+        UP(c->delta, d.data(), sizeof(double) * d.size());
+        UP(c->Omega, om.data(), sizeof(double) * om.size());
+        UP(c->phi, ph.data(), sizeof(double) * ph.size());
+        UP(c->rho, rh.data(), sizeof(double) * rh.size());
+    }
+#undef UP
+    *out = c;
+    return 0;
+}
+
+extern "C" int gp_destroy(gp_ctx *c) {
+    if (!c) return 0;
+    (void)hipStreamSynchronize(c->stream);
+    free_all(c);
+    delete c;
+    return 0;
+}
+
+extern "C" int gp_set_precision(gp_ctx *c, int precision) {
+    if (!c) { gp_set_error("NULL context"); return 1; }
+    if (precision != GP_PREC_FP64 && precision != GP_PREC_MIXED) { gp_set_error("bad precision"); return 1; }
+    c->precision = precision;
+    return 0;
+}
+
+// ------------------------------------------------------------------ buffers
+namespace {
+enum Kind { DIRECT, ROWS_N, TRANS_A, W_OINVT, PSI2, DELTA_CHOL };
+struct BufDesc {
+    double *dev = nullptr;
+    size_t len = 0;      // doubles in the reference layout
+    Kind kind = DIRECT;
+    int cols = 0;        // ROWS_N: number of columns
+    bool writable = true;
+};
+BufDesc describe(gp_ctx *c, const char *nm) {
+    const size_t N = c->N, J = c->J, M = c->M, Q = c->Q, P = c->P, p = c->p;
+    BufDesc d;
+    auto is = [&](const char *s) { return std::strcmp(nm, s) == 0; };
+    if (is("Omega")) { d.dev = c->Omega; d.len = 2 * N; }
+    else if (is("delta")) { d.dev = c->delta; d.len = 2 * p; }
+    else if (is("X")) { d.dev = c->X; d.len = 2 * N * Q; }
+    else if (is("T")) { d.dev = c->T; d.len = 2 * M * Q; }
+    else if (is("rho")) { d.dev = c->rho; d.len = 2 * Q; }
+    else if (is("phi")) { d.dev = c->phi; d.len = J; }
+    else if (is("zeta")) { d.dev = c->zeta; d.len = P; }
+    else if (is("ls")) { d.dev = c->ls; d.len = N; }
+    else if (is("K")) { d.dev = c->K; d.len = M * M; }
+    else if (is("Kinv")) { d.dev = c->Kchol; d.len = M * M; }
+    else if (is("KinvExplicit")) { d.dev = c->Kinv; d.len = M * M; }
+    else if (is("KinvKmn")) { d.dev = c->A; d.len = N * M; d.kind = TRANS_A; d.writable = false; }
+    else if (is("titsias")) { d.dev = c->titsias; d.len = N; }
+    else if (is("DeltaInv")) { d.dev = c->DeltaInv; d.len = P * P; }
+    else if (is("DeltaChol")) { d.dev = c->DeltaChol; d.len = P * P; d.kind = DELTA_CHOL; d.writable = false; }
+    else if (is("Z")) { d.dev = c->Z; d.len = N * P; d.kind = ROWS_N; d.cols = (int)P; }
+    else if (is("W")) { d.dev = c->W; d.len = N * (J + 1); d.kind = W_OINVT; }
+    else if (is("Nu")) { d.dev = c->Nu; d.len = N * J; d.kind = ROWS_N; d.cols = (int)J; }
+    else if (is("Psi")) { d.dev = c->Psi; d.len = 2 * J * P * P; d.kind = PSI2; }
+    else if (is("Beta")) { d.dev = c->Beta; d.len = P * J; }
+    else if (is("tBeta")) { d.dev = c->tBeta; d.len = P * J; }
+    else if (is("lkhds")) { d.dev = c->lkhds; d.len = J; }
+    else if (is("grad")) { d.dev = c->grad; d.len = c->tot_max; }
+    else if (is("grad0")) { d.dev = c->grad0; d.len = c->tot_max; }
+    else if (is("ss")) { d.dev = c->ss; d.len = c->tot_max; }
+    else if (is("SS")) { d.dev = c->SS; d.len = c->tot_max * GP_HIST; }
+    else if (is("XX")) { d.dev = c->XX; d.len = c->tot_max * GP_HIST; }
+    else if (is("gradDeltaTmp")) { d.dev = c->gradDeltaTmp; d.len = P * J; }
+    else if (is("C1")) { d.dev = c->C1; d.len = N * M; d.kind = ROWS_N; d.cols = (int)M; }
+    else if (is("C2")) { d.dev = c->C2; d.len = M * M; }
+    else if (is("U")) { d.dev = c->U; d.len = N * P; d.kind = ROWS_N; d.cols = (int)P; d.writable = false; }
+    else if (is("lkhd")) { d.dev = c->lkhd; d.len = GP_LKHD_SLOTS; }
+    return d;
+}
+}  // namespace
+
+extern "C" size_t gp_buffer_len(gp_ctx *c, const char *name) {
+    if (!c || !name) return 0;
+    return describe(c, name).len;
+}
+
+extern "C" int gp_get_buffer(gp_ctx *c, const char *name, double *host, size_t n) {
+    if (!c || !name || !host) { gp_set_error("NULL argument"); return 1; }
+    BufDesc d = describe(c, name);
+    if (!d.dev) { gp_set_error(std::string("unknown buffer ") + name); return 1; }
+    if (n != d.len) { gp_set_error(std::string("length mismatch for ") + name); return 1; }
+    GP_HIP(hipStreamSynchronize(c->stream));
+    const size_t N = c->N, Np = c->Npad, M = c->M, P = c->P, J = c->J;
+    switch (d.kind) {
+    case DIRECT:
+        GP_HIP(hipMemcpy(host, d.dev, n * sizeof(double), hipMemcpyDeviceToHost));
+        break;
+    case ROWS_N:
+        GP_HIP(hipMemcpy2D(host, N * sizeof(double), d.dev, Np * sizeof(double), N * sizeof(double),
+                           d.cols, hipMemcpyDeviceToHost));
+        break;
+    case W_OINVT:
+        GP_HIP(hipMemcpy2D(host, N * sizeof(double), c->W, Np * sizeof(double), N * sizeof(double),
+                           J, hipMemcpyDeviceToHost));
+        GP_HIP(hipMemcpy(host + N * J, c->Oinvt, N * sizeof(double), hipMemcpyDeviceToHost));
+        break;
+    case TRANS_A: {
+        std::vector<double> tmp(N * M);
+        GP_HIP(hipMemcpy2D(tmp.data(), N * sizeof(double), c->A, Np * sizeof(double),
+                           N * sizeof(double), M, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < N; i++)
+            for (size_t m = 0; m < M; m++) host[m + i * M] = tmp[i + m * N];
+        break;
+    }
+    case PSI2:
+        GP_HIP(hipMemcpy(host, c->Psi, J * P * P * sizeof(double), hipMemcpyDeviceToHost));
+        GP_HIP(hipMemcpy(host + J * P * P, c->Sinv, J * P * P * sizeof(double), hipMemcpyDeviceToHost));
+        break;
+    case DELTA_CHOL: {
+        std::vector<double> up(P * P);
+        GP_HIP(hipMemcpy(host, c->DeltaChol, P * P * sizeof(double), hipMemcpyDeviceToHost));
+        GP_HIP(hipMemcpy(up.data(), c->DeltaInv, P * P * sizeof(double), hipMemcpyDeviceToHost));
+        for (size_t b = 0; b < P; b++)
+            for (size_t a = 0; a < b; a++) host[a + b * P] = up[a + b * P];
+        break;
+    }
+    }
+    return 0;
+}
+
+extern "C" int gp_set_buffer(gp_ctx *c, const char *name, const double *host, size_t n) {
+    if (!c || !name || !host) { gp_set_error("NULL argument"); return 1; }
+    BufDesc d = describe(c, name);
+    if (!d.dev) { gp_set_error(std::string("unknown buffer ") + name); return 1; }
+    if (!d.writable) { gp_set_error(std::string("buffer is read-only: ") + name); return 1; }
+    if (n != d.len) { gp_set_error(std::string("length mismatch for ") + name); return 1; }
+    GP_HIP(hipStreamSynchronize(c->stream));
+    const size_t N = c->N, Np = c->Npad, P = c->P, J = c->J;
+    switch (d.kind) {
+    case DIRECT:
+        GP_HIP(hipMemcpy(d.dev, host, n * sizeof(double), hipMemcpyHostToDevice));
+        break;
+    case ROWS_N:
+        GP_HIP(hipMemcpy2D(d.dev, Np * sizeof(double), host, N * sizeof(double), N * sizeof(double),
+                           d.cols, hipMemcpyHostToDevice));
+        break;
+    case W_OINVT:
+        GP_HIP(hipMemcpy2D(c->W, Np * sizeof(double), host, N * sizeof(double), N * sizeof(double),
+                           J, hipMemcpyHostToDevice));
+        GP_HIP(hipMemcpy(c->Oinvt, host + N * J, N * sizeof(double), hipMemcpyHostToDevice));
+        break;
+    case PSI2:
+        GP_HIP(hipMemcpy(c->Psi, host, J * P * P * sizeof(double), hipMemcpyHostToDevice));
+        GP_HIP(hipMemcpy(c->Sinv, host + J * P * P, J * P * P * sizeof(double), hipMemcpyHostToDevice));
+        break;
+    default:
+        gp_set_error("buffer not settable");
+        return 1;
+    }
+    return 0;
+}
+
+// ------------------------------------------------------------------ groups
+static int read_info(gp_ctx *c, int *bad) {
+    int h = 0;
+    GP_HIP(hipMemcpyAsync(&h, c->info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    GP_HIP(hipStreamSynchronize(c->stream));
+    *bad = h;
+    return 0;
+}
+static int clear_info(gp_ctx *c) {
+    GP_HIP(hipMemsetAsync(c->info, 0, sizeof(int), c->stream));
+    return 0;
+}
+
+// Kernel matrices: getKernelMats, cholKmm, explicit Kmm^{-1}, getKinvKmn.
+static int kernel_mats(gp_ctx *c) {
+    gp_timer_begin(c, c->t_kuf);
+    GP_TRY(gpk_kuf(c));
+    GP_TRY(gpk_chol_batched(c, c->Kchol, c->M, 1, 0, c->info + 1));
+    GP_TRY(gpk_chol_inverse_batched(c, c->Kchol, c->M, 1, 0, c->Kinv));
+    GP_TRY(gpk_titsias(c));
+    gp_timer_end(c, c->t_kuf);
+    return 0;
+}
+
+static int gram_chol(gp_ctx *c) {
+    gp_timer_begin(c, c->t_gram);
+    GP_TRY(gpk_gram(c, c->W, c->Psi, c->J));
+    gp_timer_end(c, c->t_gram);
+    gp_timer_begin(c, c->t_chol);
+    GP_TRY(gpk_chol_batched(c, c->Psi, c->P, c->J, (size_t)c->P * c->P, c->info));
+    gp_timer_end(c, c->t_chol);
+    return 0;
+}
+
+extern "C" int gp_init_fit(gp_ctx *c) {
+    if (!c) { gp_set_error("NULL context"); return 1; }
+    GP_TRY(clear_info(c));
+    GP_TRY(kernel_mats(c));
+    GP_TRY(gpk_expand_delta(c));
+    // initPsi + initPsi2: unweighted Gram in the first slot, factored
+    GP_TRY(gpk_gram(c, nullptr, c->Psi, 1));
+    GP_TRY(gpk_chol_batched(c, c->Psi, c->P, 1, 0, c->info));
+    // initBeta
+    GP_TRY(gpk_init_y0(c));
+    GP_TRY(gpk_cholsl_batched(c, c->Psi, c->P, 0, c->bvec, c->Beta, c->J));
+    int bad = 0;
+    GP_TRY(read_info(c, &bad));
+    return bad ? -bad : 0;
+}
+
+extern "C" int gp_irls_step(gp_ctx *c, int jtr, double *obj) {
+    if (!c) { gp_set_error("NULL context"); return 1; }
+    if (jtr < 0 || jtr >= GP_LKHD_SLOTS) { gp_set_error("jtr out of range"); return 1; }
+    GP_TRY(clear_info(c));
+    GP_TRY(gpk_w_nu(c));
+    GP_TRY(gram_chol(c));
+    GP_TRY(gpk_r_and_sums(c));
+    GP_TRY(gpk_cholsl_batched(c, c->Psi, c->P, (size_t)c->P * c->P, c->bvec, c->tBphi, c->J));
+    GP_TRY(gpk_beta_phi(c, jtr));
+    GP_TRY(gpk_lkhd_pois(c));
+    GP_TRY(gpk_collect(c, 0, jtr, obj));
+    int bad = 0;
+    GP_TRY(read_info(c, &bad));
+    return bad ? -bad : 0;
+}
+
+// getLkhd: b_j, beta-tilde_j (scratch), the factor diagonals and the sums.
+static int lkhd_genes(gp_ctx *c) {
+    GP_TRY(gpk_r_and_sums(c));
+    GP_TRY(gpk_cholsl_batched(c, c->Psi, c->P, (size_t)c->P * c->P, c->bvec, c->pwork, c->J));
+    GP_TRY(gpk_lkhd_genes_with(c, c->pwork));
+    return 0;
+}
+
+extern "C" int gp_ql_start(gp_ctx *c, double *obj) {
+    if (!c) { gp_set_error("NULL context"); return 1; }
+    GP_TRY(clear_info(c));
+    GP_TRY(gpk_oinvt(c));
+    GP_TRY(gpk_chol_batched(c, c->DeltaChol, c->P, 1, 0, c->info + 2));
+    GP_TRY(lkhd_genes(c));
+    GP_TRY(gpk_collect(c, c->N, 0, obj));
+    return 0;
+}
+
+extern "C" int gp_elbo(gp_ctx *c, int slot, double *obj) {
+    if (!c) { gp_set_error("NULL context"); return 1; }
+    if (slot < 0 || slot >= GP_LKHD_SLOTS) { gp_set_error("slot out of range"); return 1; }
+    gp_timer_begin(c, c->t_elbo);
+    GP_TRY(clear_info(c));
+    GP_TRY(gpk_oinvt(c));
+    GP_TRY(kernel_mats(c));
+    GP_TRY(gpk_expand_delta(c));
+    GP_TRY(gram_chol(c));
+    GP_TRY(gpk_chol_batched(c, c->DeltaChol, c->P, 1, 0, c->info + 2));
+    GP_TRY(lkhd_genes(c));
+    GP_TRY(gpk_collect(c, c->N, slot, obj));
+    gp_timer_end(c, c->t_elbo);
+    int bad = 0;
+    GP_TRY(read_info(c, &bad));
+    return bad ? -bad : 0;
+}
+
+extern "C" int gp_grad(gp_ctx *c) {
+    if (!c) { gp_set_error("NULL context"); return 1; }
+    gp_timer_begin(c, c->t_grad);
+    GP_TRY(gpk_expand_delta(c));
+    GP_TRY(gpk_oinvt(c));
+    // getTildeBeta
+    GP_TRY(gpk_r_and_sums(c));
+    GP_TRY(gpk_cholsl_batched(c, c->Psi, c->P, (size_t)c->P * c->P, c->bvec, c->tBeta, c->J));
+    // solvePsi (explicit inverses, needed by the delta gradient, U and C2)
+    GP_TRY(gpk_chol_inverse_batched(c, c->Psi, c->P, c->J, (size_t)c->P * c->P, c->Sinv));
+    GP_TRY(gpk_grad_delta(c));
+    gp_timer_begin(c, c->t_gc);
+    GP_TRY(gpk_gradcontract(c));
+    gp_timer_end(c, c->t_gc);
+    GP_TRY(gpk_tn(c));
+    GP_TRY(gpk_cell_grad(c));
+    GP_TRY(gpk_c2(c));
+    GP_TRY(gpk_grad_tau_rho(c));
+    gp_timer_end(c, c->t_grad);
+    GP_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int gp_lbfgs_step(gp_ctx *c, int jtr, int rhoflag) {
+    if (!c) { gp_set_error("NULL context"); return 1; }
+    if (jtr < 0) { gp_set_error("jtr < 0"); return 1; }
+    GP_TRY(gpk_lbfgs(c, jtr, rhoflag ? 1 : 0));
+    return 0;
+}
+
+extern "C" int gp_downdate(gp_ctx *c, int jtr, double sr, int rhoflag) {
+    if (!c) { gp_set_error("NULL context"); return 1; }
+    GP_TRY(gpk_downdate(c, jtr, sr, rhoflag ? 1 : 0));
+    return 0;
+}
+
+extern "C" int gp_end_outer(gp_ctx *c) {
+    if (!c) { gp_set_error("NULL context"); return 1; }
+    GP_TRY(gpk_expand_delta(c));
+    GP_TRY(gpk_beta_from_tilde(c));
+    GP_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ggplvm.cu:1146-1330
+extern "C" int gp_fit(gp_ctx *c, int itrmax, int jtr_cap, int verbose, gp_outer_cb on_outer,
+                      void *user, double *obj) {
+    if (!c) { gp_set_error("NULL context"); return 1; }
+    std::vector<double> lk(GP_LKHD_SLOTS, 0.0);
+    int r = gp_init_fit(c);
+    if (r > 0) return r;
+    int rhoflag = 0;
+    double last = 0.0;
+    for (int itr = 0; itr < itrmax; itr++) {
+        int JTRMAX = (itr == 0 ? 200 : 100);
+        if (jtr_cap > 0 && JTRMAX > jtr_cap) JTRMAX = jtr_cap;
+        for (int jtr = 0; jtr < JTRMAX; jtr++) {
+            r = gp_irls_step(c, jtr, &lk[jtr]);
+            if (r > 0) return r;
+            if (verbose) printf("lkhd[%d,%d]=%lf\n", itr, jtr, lk[jtr]);
+            if (jtr > 0) {
+                if (std::isnan(lk[jtr])) { gp_set_error("GLM objective is NaN"); if (obj) *obj = NAN; return 2; }
+                if (jtr == JTRMAX - 1 || std::fabs(lk[jtr - 1] - lk[jtr]) / std::fabs(lk[jtr]) < 1e-7) break;
+            }
+        }
+        r = gp_ql_start(c, &lk[0]);
+        if (r > 0) return r;
+        last = lk[0];
+        if (verbose) printf("lkhd[%d,0]=%lf\n", itr, lk[0]);
+        if (itr > 20) rhoflag = 1;
+        if (itr == 30) JTRMAX = 10000;
+        if (jtr_cap > 0 && JTRMAX > jtr_cap) JTRMAX = jtr_cap;
+        for (int jtr = 0; jtr < JTRMAX && jtr + 1 < GP_LKHD_SLOTS; jtr++) {
+            if ((r = gp_grad(c)) > 0) return r;
+            if ((r = gp_lbfgs_step(c, jtr, rhoflag)) > 0) return r;
+            double sr = 1.0;
+            for (int ktr = 1; ktr < 8; ktr++) {
+                r = gp_elbo(c, jtr + 1, &lk[jtr + 1]);
+                if (r > 0) return r;
+                if (verbose) printf("lkhd[%d,%d]=%lf\n", itr, jtr + 1, lk[jtr + 1]);
+                if (lk[jtr + 1] > lk[jtr] || ktr == 7) break;
+                else if (std::isnan(lk[jtr + 1]) || std::isinf(lk[jtr + 1])) sr *= 0.01;
+                else if (lk[jtr + 1] < lk[jtr]) sr *= 0.1;
+                if ((r = gp_downdate(c, jtr, sr, rhoflag)) > 0) return r;
+            }
+            last = lk[jtr + 1];
+            if (jtr > 0 && std::fabs(lk[jtr + 1] - lk[jtr]) < std::fabs(lk[jtr + 1]) * 1e-8) break;
+        }
+        if (on_outer) on_outer(c, itr, last, user);
+        if ((r = gp_end_outer(c)) > 0) return r;
+    }
+    if (obj) *obj = last;
+    return 0;
+}
+
+// ------------------------------------------------------------------ timing API
+extern "C" int gp_timing_enable(gp_ctx *c, int on) {
+    if (!c) { gp_set_error("NULL context"); return 1; }
+    c->timing = on ? 1 : 0;
+    return 0;
+}
+extern "C" int gp_timing_reset(gp_ctx *c) {
+    if (!c) { gp_set_error("NULL context"); return 1; }
+    gp_timer *ts[] = {&c->t_gram, &c->t_chol, &c->t_gc, &c->t_kuf, &c->t_elbo, &c->t_grad};
+    for (gp_timer *t : ts) { t->total_ms = 0.0; t->count = 0; }
+    return 0;
+}
+extern "C" int gp_timing_get(gp_ctx *c, const char *name, double *avg_ms, int *count) {
+    if (!c || !name) { gp_set_error("NULL argument"); return 1; }
+    gp_timer *t = nullptr;
+    if (!std::strcmp(name, "gram")) t = &c->t_gram;
+    else if (!std::strcmp(name, "chol")) t = &c->t_chol;
+    else if (!std::strcmp(name, "gradcontract")) t = &c->t_gc;
+    else if (!std::strcmp(name, "kuf")) t = &c->t_kuf;
+    else if (!std::strcmp(name, "elbo")) t = &c->t_elbo;
+    else if (!std::strcmp(name, "grad")) t = &c->t_grad;
+    if (!t) { gp_set_error(std::string("unknown timer ") + name); return 1; }
+    if (avg_ms) *avg_ms = t->count ? t->total_ms / t->count : 0.0;
+    if (count) *count = t->count;
+    return 0;
+}

@@ -1,0 +1,161 @@
+// gp_internal.h -- device context and launch wrappers shared by the library's
+// translation units.  Host code only sees plain device pointers; every kernel
+// lives in exactly one .hip file and is reached through the wrappers below.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#define GP_HIST 10            // L-BFGS history length (ggplvm.cu:872-880)
+#define GP_LKHD_SLOTS 50000   // objective history length (ggplvm.cu:1025)
+
+struct gp_timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    double total_ms = 0.0;
+    int count = 0;
+    bool open = false;
+};
+
+struct gp_ctx {
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+
+    int N = 0, J = 0, M = 0, Q = 0, P = 0, p = 0, g0 = 0, Npad = 0;
+    long long L = 0;
+    int ard_only = 0, precision = 0;
+    std::vector<int> nh;
+
+    // counts, CSC by gene
+    int *Yp = nullptr, *Yi = nullptr;
+    double *Yx = nullptr;
+    int *d_nh = nullptr;
+
+    // model matrix Npad x P (column-major, ld = Npad); GP block at column g0
+    double *Z = nullptr;
+    double *ls = nullptr;
+    // parameters with backup halves (reference layout)
+    double *Omega = nullptr, *delta = nullptr, *X = nullptr, *T = nullptr, *rho = nullptr;
+    double *phi = nullptr, *zeta = nullptr;
+
+    double *K = nullptr;       // M x M: upper + unit diag, lower 0 (reference K)
+    double *Ksym = nullptr;    // M x M full symmetric Kmm
+    double *Kchol = nullptr;   // M x M: lower+diag = chol(Kmm), upper = Kmm
+    double *Kinv = nullptr;    // M x M explicit inverse of Kmm
+    double *A = nullptr;       // Npad x M: KinvKmn transposed
+    double *titsias = nullptr; // N
+    double *DeltaInv = nullptr;  // P x P reference layout (upper)
+    double *DeltaChol = nullptr; // P x P factor of DeltaInv (lower+diag)
+
+    double *W = nullptr;       // Npad x J
+    double *Oinvt = nullptr;   // N
+    double *Nu = nullptr;      // Npad x J
+    double *Psi = nullptr;     // J x P x P: upper = Psi, lower+diag = chol
+    double *Sinv = nullptr;    // J x P x P explicit inverse (symmetric)
+    double *Beta = nullptr, *tBeta = nullptr, *bvec = nullptr, *tBphi = nullptr;  // P x J
+    double *pwork = nullptr;   // P x J scratch (beta-tilde of the likelihood, beta work)
+    double *lkhds = nullptr, *gq = nullptr, *gs = nullptr;   // J
+    double *grad = nullptr, *grad0 = nullptr, *ss = nullptr, *SS = nullptr, *XX = nullptr;
+    double *lbq = nullptr;     // L-BFGS work vector (tot)
+    double *gradDeltaTmp = nullptr;  // P x J
+    double *gradLogRhoTmp = nullptr; // Q x M
+    double *C1 = nullptr;      // Npad x M
+    double *C2 = nullptr;      // M x M
+    double *U = nullptr;       // Npad x P  sum_j o_ij Psi_j^{-1} z_i
+    double *ZB = nullptr;      // Npad x J  Z*beta (IRLS) / Z*tBeta (gradient)
+    double *R = nullptr;       // Npad x J  W nu / Omega ; reused for TN/phi
+    double *C1a = nullptr;     // Npad x M
+    double *AO = nullptr;      // Npad x M  diag(Oinvt) A
+    double *G2 = nullptr, *B2 = nullptr, *S2 = nullptr;  // M x M pieces of C2
+    double *Ecell = nullptr;   // N
+    double *red = nullptr;     // reduction scratch
+    double *scal = nullptr;    // device scalars (L-BFGS alphas etc.)
+    double *lkhd = nullptr;    // GP_LKHD_SLOTS
+    int *info = nullptr;       // non-PD counters
+    // mixed precision staging (fp16 hi/lo of the model matrix, tiled)
+    void *Zh = nullptr, *Zl = nullptr;
+    double *gram_part = nullptr;  // FP64 split-N partial Grams (mixed mode)
+    int gram_splits = 1;
+
+    size_t tot_max = 0;        // N+p+(N+M+1)Q
+    int timing = 0;
+    gp_timer t_gram, t_chol, t_gc, t_kuf, t_elbo, t_grad;
+    std::string err;
+};
+
+// ---------------- error handling ----------------
+void gp_set_error(const std::string &msg);
+#define GP_HIP(call)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (call);                                                   \
+        if (e_ != hipSuccess) {                                                   \
+            gp_set_error(std::string(#call) + ": " + hipGetErrorString(e_));      \
+            return (int)e_;                                                       \
+        }                                                                         \
+    } while (0)
+#define GP_BLAS(call)                                                             \
+    do {                                                                          \
+        rocblas_status s_ = (call);                                               \
+        if (s_ != rocblas_status_success) {                                       \
+            gp_set_error(std::string(#call) + ": rocblas status " +               \
+                         std::to_string((int)s_));                                \
+            return 900 + (int)s_;                                                 \
+        }                                                                         \
+    } while (0)
+#define GP_TRY(call)                                                              \
+    do {                                                                          \
+        int r_ = (call);                                                          \
+        if (r_ > 0) return r_;                                                    \
+    } while (0)
+
+// ---------------- launch wrappers (gp_kernels.hip) ----------------
+int gpk_kuf(gp_ctx *c);                       // Knm into Z, K, Ksym, Kchol
+int gpk_expand_delta(gp_ctx *c);              // DeltaInv + DeltaChol copy
+int gpk_oinvt(gp_ctx *c);
+int gpk_titsias(gp_ctx *c);                   // A = Knm Kinv (rocBLAS) + titsias
+int gpk_w_nu(gp_ctx *c);                      // IRLS W / Nu from ZB = Z*Beta
+int gpk_r_and_sums(gp_ctx *c);                // R = W nu / Omega, gq, gs
+int gpk_init_y0(gp_ctx *c);                   // R = log((y+0.05)/ls) dense
+int gpk_beta_phi(gp_ctx *c, int jtr);
+int gpk_lkhd_pois(gp_ctx *c);
+int gpk_lkhd_genes_with(gp_ctx *c, const double *tb);  // tb = Psi_j^{-1} b_j
+int gpk_collect(gp_ctx *c, int Nterm, int slot, double *obj);
+int gpk_grad_delta(gp_ctx *c);
+int gpk_tn(gp_ctx *c);
+int gpk_cell_grad(gp_ctx *c);
+int gpk_c2(gp_ctx *c);
+int gpk_grad_tau_rho(gp_ctx *c);
+int gpk_lbfgs(gp_ctx *c, int itr, int rhoflag);
+int gpk_downdate(gp_ctx *c, int itr, double r, int rhoflag);
+int gpk_beta_from_tilde(gp_ctx *c);
+int gpk_fill(gp_ctx *c, double *p, size_t n, double v);
+
+// ---------------- gp_chol.hip ----------------
+// Batched in-place Cholesky of `batch` n x n column-major matrices spaced
+// `stride` doubles apart; reads the lower triangle, writes L to lower+diag,
+// zero-fills a matrix that meets a non-positive pivot and counts it in *info.
+int gpk_chol_batched(gp_ctx *c, double *A, int n, int batch, size_t stride, int *info);
+// x_j = A_j^{-1} b_j for factored A_j (stride 0 = one shared factor).
+int gpk_cholsl_batched(gp_ctx *c, const double *L, int n, size_t lstride,
+                       const double *b, double *x, int batch);
+// S_j = A_j^{-1} (full symmetric) from the factors.
+int gpk_chol_inverse_batched(gp_ctx *c, const double *L, int n, int batch,
+                             size_t stride, double *S);
+
+// ---------------- gp_gram.hip ----------------
+// Psi_j = DeltaInv + Z^T diag(W_j / Omega) Z, full symmetric, all genes.
+// W == nullptr: unweighted single Gram (initPsi).
+int gpk_gram(gp_ctx *c, const double *W, double *Psi, int batch);
+
+// ---------------- gp_gradcontract.hip ----------------
+// U = sum_j diag(W_j/Omega) Z Sinv_j  (Npad x P)
+int gpk_gradcontract(gp_ctx *c);
+
+// ---------------- timing ----------------
+void gp_timer_begin(gp_ctx *c, gp_timer &t);
+void gp_timer_end(gp_ctx *c, gp_timer &t);

@@ -1,0 +1,950 @@
+// gp_kernels.hip -- element-wise, reduction and epilogue kernels of the fit:
+// the Kuf/Kmm build, IRLS pseudo-data, likelihood assembly, the per-cell and
+// per-gene gradient epilogues, C2, the T/rho gradients and the L-BFGS update.
+// Every reduction is two-stage with a fixed partition, so results are
+// bit-reproducible from run to run.
+#include "gp_internal.h"
+
+#include <cmath>
+
+namespace {
+
+constexpr int NT = 256;          // threads per block for 1-D kernels
+constexpr int RED_BLOCKS = 256;  // stage-1 blocks of the vector reductions
+
+__device__ inline double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+// Sum over a 256-thread block; result valid on thread 0.
+__device__ inline double block_sum256(double v, double *sh) {
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+    return r;
+}
+
+inline unsigned nblk(size_t n, int nt = NT) { return (unsigned)((n + nt - 1) / nt); }
+
+// ---------------------------------------------------------------- Kuf / Kmm
+// kernel_funcs.cu:61-97 (periodic dim 0 + SE-ARD) and 221-255 (SE-ARD only).
+// 256 cells x 8 inducing points per block; T rows staged in LDS.
+__global__ void __launch_bounds__(NT) k_kuf(int N, int Npad, int M, int Q,
+        const double *__restrict__ X, const double *__restrict__ T,
+        const double *__restrict__ rho, int ard, double *__restrict__ Knm) {
+    __shared__ double Ts[8][64];
+    __shared__ double rh[64];
+    const int i = blockIdx.x * NT + threadIdx.x;
+    const int m0 = blockIdx.y * 8;
+    for (int t = threadIdx.x; t < 8 * Q; t += NT) {
+        int mm = t % 8, q = t / 8;
+        Ts[mm][q] = (m0 + mm < M) ? T[(m0 + mm) + (size_t)q * M] : 0.0;
+    }
+    for (int q = threadIdx.x; q < Q; q += NT) rh[q] = rho[q];
+    __syncthreads();
+    if (i >= Npad) return;
+    for (int mm = 0; mm < 8; mm++) {
+        const int m = m0 + mm;
+        if (m >= M) break;
+        double v = 0.0;
+        if (i < N) {
+            double sum = 0.0, t;
+            int k0 = 0;
+            if (!ard) {
+                t = sin((X[i] - Ts[mm][0]) / 2.0);
+                sum += t * t / rh[0];
+                k0 = 1;
+            }
+            for (int k = k0; k < Q; k++) {
+                t = X[i + (size_t)k * N] - Ts[mm][k];
+                sum += t * t / 2.0 / rh[k];
+            }
+            v = exp(-sum);
+        }
+        Knm[i + (size_t)m * Npad] = v;
+    }
+}
+
+__device__ inline double kmm_entry(int a, int b, int M, int Q, const double *T,
+                                   const double *rho, int ard) {
+    // value for a < b, evaluated exactly as the reference's ii < j branch
+    double sum = 0.0, t;
+    int k0 = 0;
+    if (!ard) {
+        t = sin((T[a] - T[b]) / 2.0);
+        sum += t * t / rho[0];
+        k0 = 1;
+    }
+    for (int k = k0; k < Q; k++) {
+        t = T[a + (size_t)k * M] - T[b + (size_t)k * M];
+        sum += t * t / 2.0 / rho[k];
+    }
+    return exp(-sum);
+}
+
+__global__ void k_kmm(int M, int Q, const double *T, const double *rho, int ard,
+                      double *K, double *Ksym, double *Kchol) {
+    const size_t idx = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (idx >= (size_t)M * M) return;
+    const int a = (int)(idx % M), b = (int)(idx / M);
+    double v, vs;
+    if (a == b) { v = 1.0; vs = 1.0; }
+    else if (a < b) { v = kmm_entry(a, b, M, Q, T, rho, ard); vs = v; }
+    else { v = 0.0; vs = kmm_entry(b, a, M, Q, T, rho, ard); }
+    K[idx] = v;
+    Ksym[idx] = vs;
+    Kchol[idx] = vs;
+}
+
+// ---------------------------------------------------------------- DeltaInv
+// util_cuda.cu:249-269 expandDelta_k; DeltaChol gets the full symmetric copy.
+__global__ void k_expand_delta(int P, int p, int M, int g0, const int *nh,
+                               const double *delta, const double *K,
+                               double *DeltaInv, double *DeltaChol) {
+    const size_t idx = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (idx >= (size_t)P * P) return;
+    const int r = (int)(idx % P), c = (int)(idx / P);
+    const int lo = r < c ? r : c, hi = r < c ? c : r;
+    double sym = 0.0;
+    if (lo == hi && lo < g0) {
+        int l = 0;
+        while (l < p - 1 && !(nh[l] <= lo && lo < nh[l + 1])) l++;
+        sym = 1.0 / delta[l];
+    } else if (lo >= g0) {
+        sym = K[(lo - g0) + (size_t)(hi - g0) * M] / delta[p - 1];
+    }
+    DeltaInv[idx] = (r <= c) ? sym : 0.0;
+    DeltaChol[idx] = sym;
+}
+
+// ---------------------------------------------------------------- Oinvt
+// ggplvm.cu:284-294: Oinvt_i = sum_j W_ij / Omega_i (same j order).
+__global__ void k_oinvt(int N, int Npad, int J, const double *W, const double *Omega,
+                        double *Oinvt) {
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= N) return;
+    const double om = Omega[i];
+    double o = 0.0;
+    for (int j = 0; j < J; j++) o += W[i + (size_t)j * Npad] / om;
+    Oinvt[i] = o;
+}
+
+// ---------------------------------------------------------------- titsias
+// kernel_funcs.cu:160: t_i = (1 - k_i^T Kmm^{-1} k_i) / 2, with A = Knm Kmm^{-1}.
+__global__ void k_titsias(int N, int Npad, int M, const double *Knm, const double *A,
+                          double *tits) {
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= N) return;
+    double s = 0.0;
+    for (int m = 0; m < M; m++) s += Knm[i + (size_t)m * Npad] * A[i + (size_t)m * Npad];
+    tits[i] = (1.0 - s) / 2.0;
+}
+
+// ---------------------------------------------------------------- IRLS W / Nu
+// ggplvm.cu:468-514 with zeta == 0; eta = z_i^T beta_j precomputed in ZB.
+__global__ void k_w_nu(int N, int Npad, const double *ZB, const double *ls,
+                       const double *tits, double dtwt_delta, double wt,
+                       double *W, double *Nu) {
+    const int i = blockIdx.x * NT + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= Npad) return;
+    const size_t a = i + (size_t)j * Npad;
+    if (i >= N) { W[a] = 0.0; Nu[a] = 0.0; return; }
+    const double eta = ZB[a];
+    W[a] = exp(eta) * ls[i] * exp(dtwt_delta * tits[i] * wt);
+    Nu[a] = eta - 1.0;
+}
+
+__global__ void k_nu_sparse(int Npad, const int *Yp, const int *Yi, const double *Yx,
+                            const double *ZB, const double *ls, const double *tits,
+                            double dtwt_delta, double wt, double *Nu) {
+    const int j = blockIdx.x;
+    for (int l = Yp[j] + threadIdx.x; l < Yp[j + 1]; l += NT) {
+        const int i = Yi[l];
+        const size_t a = i + (size_t)j * Npad;
+        const double eta = ZB[a];
+        const double logW = eta + log(ls[i]) + dtwt_delta * tits[i] * wt;
+        Nu[a] = eta + exp(log(Yx[l]) - logW) - 1.0;
+    }
+}
+
+// R = (W/Omega) nu, gq_j = sum nu^2 W / Omega, gs_j = sum log(Omega / W).
+__global__ void __launch_bounds__(NT) k_r_sums(int N, int Npad, const double *W,
+        const double *Nu, const double *Omega, double *R, double *gq, double *gs) {
+    __shared__ double sh[4];
+    const int j = blockIdx.x;
+    const double *Wj = W + (size_t)j * Npad, *nj = Nu + (size_t)j * Npad;
+    double *Rj = R + (size_t)j * Npad;
+    double q = 0.0, s = 0.0;
+    for (int i = threadIdx.x; i < Npad; i += NT) {
+        if (i < N) {
+            const double w = Wj[i], nu = nj[i], om = Omega[i];
+            Rj[i] = w / om * nu;
+            q += nu * nu * w / om;
+            s += log(om / w);
+        } else {
+            Rj[i] = 0.0;
+        }
+    }
+    q = block_sum256(q, sh);
+    s = block_sum256(s, sh);
+    if (threadIdx.x == 0) { gq[j] = q; gs[j] = s; }
+}
+
+// init.cu:193-221 right-hand side: log((y + 0.05) / ls) at non-zeros,
+// log(0.05 / ls) elsewhere.
+__global__ void k_y0_dense(int N, int Npad, const double *ls, double *R) {
+    const int i = blockIdx.x * NT + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= Npad) return;
+    R[i + (size_t)j * Npad] = (i < N) ? log(0.05 / ls[i]) : 0.0;
+}
+__global__ void k_y0_sparse(int Npad, const int *Yp, const int *Yi, const double *Yx,
+                            const double *ls, double *R) {
+    const int j = blockIdx.x;
+    for (int l = Yp[j] + threadIdx.x; l < Yp[j + 1]; l += NT) {
+        const int i = Yi[l];
+        R[i + (size_t)j * Npad] = log((Yx[l] + 0.05) / ls[i]);
+    }
+}
+
+// ---------------------------------------------------------------- getBetaPhi
+// ggplvm.cu:418-464: damped beta update and phi from L^T beta.
+__global__ void __launch_bounds__(NT) k_beta_phi(int N, int P, int jtr,
+        const double *Psi, const double *tmpsol, const double *zeta,
+        const double *gq, double *Beta, double *phi, double *work) {
+    __shared__ double sh[4];
+    const int j = blockIdx.x;
+    double r = (double)(jtr + 1) / 100.0;
+    if (jtr > 100) r = 1.0;
+    double *Bj = Beta + (size_t)j * P;
+    double *tb = work + (size_t)j * P;
+    const double *sol = tmpsol + (size_t)j * P;
+    for (int k = threadIdx.x; k < P; k += NT) {
+        double b = Bj[k] * (1 - r) + sol[k] * r;
+        tb[k] = b;
+        Bj[k] = b + zeta[k];
+    }
+    __syncthreads();
+    const double *Lj = Psi + (size_t)j * P * P;
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < P; k += NT) {
+        double t = 0.0;
+        for (int i = k; i < P; i++) t += Lj[i + (size_t)k * P] * tb[i];
+        acc += t * t;
+    }
+    acc = block_sum256(acc, sh);
+    if (threadIdx.x == 0) {
+        double ph = (gq[j] - acc) / (double)N;
+        if (isnan(ph)) ph = 1.0;
+        if (ph < 0.0) ph = 1.0;
+        if (ph < 0.01) ph = 0.01;
+        if (ph > 1000.0) ph = 1000.0;
+        phi[j] = ph;
+    }
+}
+
+// ---------------------------------------------------------------- getLkhdPois
+// ggplvm.cu:703-724: sum_nz (log W - delta t wt) y - sum_i W.
+__global__ void __launch_bounds__(NT) k_lkhd_pois(int N, int Npad, const int *Yp,
+        const int *Yi, const double *Yx, const double *W, const double *tits,
+        double dt, double wt, double *lkhds) {
+    __shared__ double sh[4];
+    const int j = blockIdx.x;
+    const double *Wj = W + (size_t)j * Npad;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < N; i += NT) s -= Wj[i];
+    for (int l = Yp[j] + threadIdx.x; l < Yp[j + 1]; l += NT) {
+        const int i = Yi[l];
+        s += (log(Wj[i]) - dt * tits[i] * wt) * Yx[l];
+    }
+    s = block_sum256(s, sh);
+    if (threadIdx.x == 0) lkhds[j] = s;
+}
+
+// ---------------------------------------------------------------- getLkhdj
+// ggplvm.cu:744-770 from the factor diagonals, b_j, tBeta_j and the sums.
+__global__ void __launch_bounds__(NT) k_lkhd_genes(int N, int P, const double *Psi,
+        const double *DeltaChol, const double *bvec, const double *tBeta,
+        const double *gq, const double *gs, const double *phi, double *lkhds) {
+    __shared__ double sh[4];
+    const int j = blockIdx.x;
+    const double *Lj = Psi + (size_t)j * P * P;
+    double ld = 0.0, bt = 0.0;
+    for (int k = threadIdx.x; k < P; k += NT) {
+        ld += log(DeltaChol[k + (size_t)k * P]) - log(Lj[k + (size_t)k * P]);
+        bt += bvec[k + (size_t)j * P] * tBeta[k + (size_t)j * P];
+    }
+    ld = block_sum256(ld, sh);
+    bt = block_sum256(bt, sh);
+    if (threadIdx.x == 0) {
+        const double ph = phi[j];
+        double lk = -((double)N) / 2.0 * log(ph);
+        lk -= gs[j] / 2.0;
+        lk += ld;
+        lk -= gq[j] / 2.0 / ph;
+        lk += bt / 2.0 / ph;
+        lkhds[j] = lk;
+    }
+}
+
+// ---------------------------------------------------------------- collectLkhd
+// ggplvm.cu:788-815; stage 1 over cells, stage 2 adds the gene sum.
+__global__ void __launch_bounds__(NT) k_collect_cells(int N, const double *Oinvt,
+        const double *tits, const double *Omega, double wtdt, double igb, double igap1,
+        double *part) {
+    __shared__ double sh[4];
+    const int per = (N + RED_BLOCKS - 1) / RED_BLOCKS;
+    const int lo = blockIdx.x * per, hi = min(N, lo + per);
+    double s = 0.0;
+    for (int i = lo + threadIdx.x; i < hi; i += NT)
+        s += wtdt * Oinvt[i] * tits[i] + (igb / Omega[i] + igap1 * log(Omega[i]));
+    s = block_sum256(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(NT) k_collect_final(int J, const double *lkhds,
+        const double *part, int npart, double *lkhd, int slot) {
+    __shared__ double sh[4];
+    double g = 0.0, c = 0.0;
+    for (int j = threadIdx.x; j < J; j += NT) g += lkhds[j];
+    for (int b = threadIdx.x; b < npart; b += NT) c += part[b];
+    g = block_sum256(g, sh);
+    c = block_sum256(c, sh);
+    if (threadIdx.x == 0) lkhd[slot] = g - c;
+}
+
+// ---------------------------------------------------------------- getGradLogDelta
+// ggplvm.cu:210-253 per (gene, column) with the explicit inverse S_j.
+__global__ void __launch_bounds__(NT) k_grad_delta_gene(int P, int p, int M, int g0,
+        const int *nh, const double *delta, const double *Ksym, const double *Sinv,
+        const double *tBeta, const double *phi, double *gdt) {
+    __shared__ double sh[4];
+    __shared__ double tb[2048];
+    const int j = blockIdx.x;
+    const double *S = Sinv + (size_t)j * P * P;
+    const double *tbj = tBeta + (size_t)j * P;
+    for (int k = threadIdx.x; k < P; k += NT) tb[k] = tbj[k];
+    __syncthreads();
+    const double ph = phi[j];
+    // quadratic form tb_GP^T K tb_GP
+    double qf = 0.0;
+    for (int a = threadIdx.x; a < M; a += NT) {
+        double t = 0.0;
+        for (int b = 0; b < M; b++) t += Ksym[a + (size_t)b * M] * tb[g0 + b];
+        qf += tb[g0 + a] * t;
+    }
+    qf = block_sum256(qf, sh);
+    __shared__ double qfs;
+    if (threadIdx.x == 0) qfs = qf;
+    __syncthreads();
+    for (int k = threadIdx.x; k < P; k += NT) {
+        int l = 0;
+        while (l < p - 1 && !(nh[l] <= k && k < nh[l + 1])) l++;
+        const double dk = delta[l];
+        double psi;
+        if (k < g0) {
+            psi = S[k + (size_t)k * P];
+        } else {
+            const int kk = k - g0;
+            psi = 0.0;
+            for (int b = 0; b < M; b++) psi += S[(g0 + b) + (size_t)k * P] * Ksym[b + (size_t)kk * M];
+        }
+        double sum = 0.0;
+        sum -= (dk - psi) / dk / dk / 2.0;
+        if (k < g0) sum += tb[k] * tb[k] / dk / dk / ph / 2.0;
+        else if (k == g0) sum += qfs / dk / dk / ph / 2.0;
+        gdt[k + (size_t)j * P] = sum * dk;
+    }
+}
+
+// collectGradLogDelta ggplvm.cu:256-281
+__global__ void __launch_bounds__(NT) k_grad_delta_collect(int N, int J, int P, int p,
+        const int *nh, const double *gdt, const double *Oinvt, const double *tits,
+        const double *delta, double wt, double *gd) {
+    __shared__ double sh[4];
+    const int l = blockIdx.x;
+    const int c0 = nh[l], c1 = nh[l + 1], w = c1 - c0;
+    double s = 0.0;
+    for (size_t t = threadIdx.x; t < (size_t)J * w; t += NT) {
+        const int j = (int)(t / w), k = c0 + (int)(t % w);
+        s += gdt[k + (size_t)j * P];
+    }
+    if (l == p - 1)
+        for (int i = threadIdx.x; i < N; i += NT) s -= wt * Oinvt[i] * tits[i] * delta[p - 1];
+    s = block_sum256(s, sh);
+    if (threadIdx.x == 0) gd[l] = s;
+}
+
+// ---------------------------------------------------------------- tn / E
+// ggplvm.cu:335-345 per-cell pieces: tn_ij = o nu - o z^T tBeta_j,
+// E_i = sum_j tn^2 / W / phi / 2, TN_ij = tn_ij / phi_j.
+__global__ void k_tn(int N, int Npad, int J, const double *W, const double *Nu,
+                     const double *ZB, const double *Omega, const double *phi,
+                     double *TN, double *E) {
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= Npad) return;
+    if (i >= N) {
+        for (int j = 0; j < J; j++) TN[i + (size_t)j * Npad] = 0.0;
+        return;
+    }
+    const double om = Omega[i];
+    double e = 0.0;
+    for (int j = 0; j < J; j++) {
+        const size_t a = i + (size_t)j * Npad;
+        const double w = W[a], o = w / om;
+        const double tn = o * Nu[a] - o * ZB[a];
+        e += tn * tn / w / phi[j] / 2.0;
+        TN[a] = tn / phi[j];
+    }
+    E[i] = e;
+}
+
+// ---------------------------------------------------------------- per-cell gradient
+// ggplvm.cu:346-375: gOmega, C1 and gX from U = sum_j o_ij Psi_j^{-1} z_i.
+__global__ void k_cell_grad(int N, int Npad, int J, int M, int Q, int P, int g0, int ard,
+        const double *Z, const double *U, const double *C1a, const double *A,
+        const double *Omega, const double *Oinvt, const double *tits, const double *E,
+        const double *X, const double *T, const double *rho, double dt, double wt,
+        double igb, double igap1, double *gO, double *gX, double *C1) {
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= Npad) return;
+    if (i >= N) {
+        for (int k = 0; k < M; k++) C1[i + (size_t)k * Npad] = 0.0;
+        return;
+    }
+    const double om = Omega[i], oii = Oinvt[i];
+    double D = 0.0;
+    for (int k = 0; k < P; k++) D += Z[i + (size_t)k * Npad] * U[i + (size_t)k * Npad];
+    double sum = -((double)J) / om / 2.0 + D / om / 2.0 + E[i];
+    sum += wt * dt * oii / om * tits[i];
+    gO[i] = sum * om + igb / om - igap1;
+    const double *Knm = Z + (size_t)g0 * Npad;
+    double ci1 = 0.0, gx0 = 0.0;
+    for (int k = 0; k < M; k++) {
+        const size_t a = i + (size_t)k * Npad;
+        double c = -U[i + (size_t)(g0 + k) * Npad] + C1a[a];
+        c += wt * dt * oii * A[a];
+        c *= Knm[a];
+        C1[a] = c;
+        ci1 += c;
+        if (!ard) {
+            const double t = (X[i] - T[k]) / 2.0;
+            gx0 -= c * sin(t) * cos(t) / rho[0];
+        }
+    }
+    int q0 = 0;
+    if (!ard) { gX[i] = gx0; q0 = 1; }
+    for (int q = q0; q < Q; q++) {
+        double s = 0.0;
+        for (int k = 0; k < M; k++) s += C1[i + (size_t)k * Npad] * T[k + (size_t)q * M];
+        gX[i + (size_t)q * N] = (s - ci1 * X[i + (size_t)q * N]) / rho[q];
+    }
+}
+
+// ---------------------------------------------------------------- C2
+__global__ void k_scale_rows(int N, int Npad, int M, const double *A, const double *o,
+                             double *AO) {
+    const size_t idx = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (idx >= (size_t)Npad * M) return;
+    const int i = (int)(idx % Npad);
+    AO[idx] = (i < N) ? A[idx] * o[i] : 0.0;
+}
+__global__ void k_scale_cols(int P, int J, const double *B, const double *phi, double *Bo) {
+    const size_t idx = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (idx >= (size_t)P * J) return;
+    Bo[idx] = B[idx] / phi[idx / P];
+}
+// Sum over genes of the GP block of Psi_j^{-1}.
+__global__ void k_sum_sinv(int J, int P, int M, int g0, const double *Sinv, double *S2) {
+    const size_t idx = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (idx >= (size_t)M * M) return;
+    const int a = (int)(idx % M), b = (int)(idx / M);
+    double s = 0.0;
+    for (int j = 0; j < J; j++) s += Sinv[(size_t)j * P * P + (g0 + a) + (size_t)(g0 + b) * P];
+    S2[idx] = s;
+}
+// ggplvm.cu:57-100 getC2
+__global__ void k_c2(int M, int J, double wt, double dt, const double *G2, const double *B2,
+                     const double *S2, const double *Kinv, const double *K, double *C2) {
+    const size_t idx = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (idx >= (size_t)M * M) return;
+    const int j = (int)(idx % M), k = (int)(idx / M);
+    if (j > k) return;
+    const size_t a = j + (size_t)k * M;
+    double c = -wt * dt * G2[a] / 2.0;
+    c -= B2[a] / dt / 2.0;
+    c -= S2[a] / dt / 2.0;
+    c += ((double)J) * Kinv[a] / 2.0;
+    c *= K[a];
+    C2[a] = c;
+    if (j < k) C2[k + (size_t)j * M] = c;
+}
+
+// ---------------------------------------------------------------- getGradTau / rho
+// ggplvm.cu:103-145 and 165-205: one block per inducing point m.
+__global__ void __launch_bounds__(NT) k_grad_tau_rho(int N, int Npad, int M, int Q, int ard,
+        const double *C1, const double *C2, const double *X, const double *T,
+        const double *rho, double *gT, double *grt) {
+    __shared__ double sh[4];
+    const int m = blockIdx.x;
+    const double *c1 = C1 + (size_t)m * Npad;
+    for (int q = 0; q < Q; q++) {
+        const double tm = T[m + (size_t)q * M];
+        const double rq = rho[q];
+        double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, r1 = 0.0, r2 = 0.0;
+        const bool per = (q == 0 && !ard);
+        for (int i = threadIdx.x; i < N; i += NT) {
+            const double c = c1[i], x = X[i + (size_t)q * N];
+            if (per) {
+                const double t = (x - tm) / 2.0;
+                s1 += c * sin(t) * cos(t);
+                const double st = sin((x - tm) / 2.0);
+                r1 += c * st * st / rq / rq;
+            } else {
+                s1 += c * x;
+                s2 += c;
+                const double d = x - tm;
+                r1 += c * d * d / rq / rq / 2.0;
+            }
+        }
+        for (int i = threadIdx.x; i < M; i += NT) {
+            const double c = C2[m + (size_t)i * M], cc = C2[i + (size_t)m * M];
+            const double ti = T[i + (size_t)q * M];
+            if (per) {
+                const double t = (ti - tm) / 2.0;
+                s3 += c * sin(t) * cos(t);
+                const double st = sin((ti - tm) / 2.0);
+                r2 += cc * st * st / rq / rq;
+            } else {
+                s3 += c * ti;
+                s4 += c;
+                const double d = ti - tm;
+                r2 += cc * d * d / rq / rq / 2.0;
+            }
+        }
+        s1 = block_sum256(s1, sh);
+        s2 = block_sum256(s2, sh);
+        s3 = block_sum256(s3, sh);
+        s4 = block_sum256(s4, sh);
+        r1 = block_sum256(r1, sh);
+        r2 = block_sum256(r2, sh);
+        if (threadIdx.x == 0) {
+            double g;
+            if (per) g = s1 / rq + 2.0 * s3 / rq;
+            else g = (s1 - s2 * tm) / rq + 2.0 * (s3 - s4 * tm) / rq;
+            gT[m + (size_t)q * M] = g;
+            grt[q + (size_t)m * Q] = (r1 + r2) * rq;
+        }
+    }
+}
+__global__ void k_collect_rho(int M, int Q, const double *grt, double *gr) {
+    const int q = threadIdx.x;
+    if (q >= Q) return;
+    double s = 0.0;
+    for (int k = 0; k < M; k++) s += grt[q + (size_t)k * Q];
+    gr[q] = s;
+}
+
+// ---------------------------------------------------------------- L-BFGS
+// util_cuda.cu:187-232 and ggplvm.cu:865-934 / 818-859.
+__global__ void __launch_bounds__(NT) k_dot2(size_t n, const double *a, const double *b,
+        const double *c, const double *d, double *part) {
+    __shared__ double sh[4];
+    const size_t per = (n + RED_BLOCKS - 1) / RED_BLOCKS;
+    const size_t lo = blockIdx.x * per, hi = lo + per < n ? lo + per : n;
+    double s = 0.0, t = 0.0;
+    for (size_t k = lo + threadIdx.x; k < hi; k += NT) {
+        s += a[k] * b[k];
+        t += c[k] * d[k];
+    }
+    s = block_sum256(s, sh);
+    t = block_sum256(t, sh);
+    if (threadIdx.x == 0) { part[2 * blockIdx.x] = s; part[2 * blockIdx.x + 1] = t; }
+}
+__global__ void __launch_bounds__(NT) k_dot2_final(const double *part, double *out) {
+    __shared__ double sh[4];
+    double s = 0.0, t = 0.0;
+    for (int b = threadIdx.x; b < RED_BLOCKS; b += NT) { s += part[2 * b]; t += part[2 * b + 1]; }
+    s = block_sum256(s, sh);
+    t = block_sum256(t, sh);
+    if (threadIdx.x == 0) { out[0] = s; out[1] = t; }
+}
+// mode 0: q -= (num/den) * y, store alpha in *alpha
+// mode 1: z  = (num/den) * q
+// mode 2: z += s * (alpha - num/den)
+__global__ void k_lb_axpy(size_t n, int mode, const double *nd, double *alpha,
+                          const double *x, double *y) {
+    const size_t k = (size_t)blockIdx.x * NT + threadIdx.x;
+    const double r = nd[0] / nd[1];
+    if (k == 0 && mode == 0) *alpha = r;
+    if (k >= n) return;
+    if (mode == 0) y[k] -= r * x[k];
+    else if (mode == 1) y[k] = r * x[k];
+    else y[k] += x[k] * (*alpha - r);
+}
+__global__ void k_lb_first(size_t n, const double *g, double *ss, double *g0) {
+    const size_t k = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (k >= n) return;
+    double v = -g[k] / 100000.0;
+    if (v > 1.0) v = 1.0;
+    if (v < -1.0) v = -1.0;
+    ss[k] = v;
+    g0[k] = g[k];
+}
+__global__ void k_lb_diff(size_t n, const double *g, double *g0, double *xx) {
+    const size_t k = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (k >= n) return;
+    xx[k] = g[k] - g0[k];
+    g0[k] = g[k];
+}
+__global__ void k_lb_clip(size_t n, const double *z, double *ss) {
+    const size_t k = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (k >= n) return;
+    double v = z[k];
+    if (v > 1.0) v = 1.0;
+    if (v < -1.0) v = -1.0;
+    ss[k] = -v;
+}
+// Parameter update (mode 0 = LBFGS with backups, mode 1 = Downdate from backups).
+__global__ void k_lb_update(int N, int p, int M, int Q, int mode, double r, int rhoflag,
+                            const double *ss, double *Omega, double *delta, double *X,
+                            double *T, double *rho, double *SSslot, size_t tot) {
+    const size_t k = (size_t)blockIdx.x * NT + threadIdx.x;
+    const size_t NQ = (size_t)N * Q, MQ = (size_t)M * Q;
+    if (k < tot) SSslot[k] = r * ss[k];
+    if (k < (size_t)N) {
+        double v = (mode == 0) ? exp(log(Omega[k]) - ss[k]) : exp(log(Omega[k + N]) - r * ss[k]);
+        if (mode == 0) Omega[k + N] = Omega[k];
+        if (v > 1000.) v = 1000.;
+        if (v < .001) v = .001;
+        Omega[k] = v;
+    }
+    if (k < (size_t)p) {
+        if (mode == 0) { delta[k + p] = delta[k]; delta[k] = exp(log(delta[k]) - ss[N + k]); }
+        else delta[k] = exp(log(delta[k + p]) - r * ss[N + k]);
+    }
+    if (k < NQ) {
+        if (mode == 0) { X[k + NQ] = X[k]; X[k] = X[k] - ss[N + p + k]; }
+        else X[k] = X[k + NQ] - r * ss[N + p + k];
+    }
+    if (k < MQ) {
+        if (mode == 0) { T[k + MQ] = T[k]; T[k] = T[k] - ss[N + p + NQ + k]; }
+        else T[k] = T[k + MQ] - r * ss[N + p + NQ + k];
+    }
+}
+// rho_0 update and mscale_d (util_cuda.cu:546-555) of rho_1..Q-1.
+__global__ void k_lb_rho(int N, int p, int M, int Q, int mode, double r, int rhoflag,
+                         const double *ss, double *rho) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const size_t off = (size_t)N + p + (size_t)N * Q + (size_t)M * Q;
+    if (mode == 0) {
+        rho[Q] = rho[0];
+        rho[0] = exp(log(rho[0]) - ss[off] * (double)rhoflag);
+    } else {
+        rho[0] = exp(log(rho[Q]) - r * ss[off] * (double)rhoflag);
+    }
+    double mh = 0.0;
+    for (int i = 1; i < Q; i++) mh += rho[i];
+    mh /= (double)(Q - 1);
+    for (int i = 1; i < Q; i++) rho[i] *= 10.0 / mh;
+}
+
+__global__ void k_add_zeta(size_t n, int P, const double *tB, const double *zeta, double *B) {
+    const size_t k = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (k >= n) return;
+    B[k] = tB[k] + zeta[k % P];
+}
+
+__global__ void k_fill(size_t n, double *p, double v) {
+    const size_t k = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (k < n) p[k] = v;
+}
+
+}  // namespace
+
+// ====================================================================== wrappers
+
+#define LAUNCH_CHECK() GP_HIP(hipGetLastError())
+
+int gpk_fill(gp_ctx *c, double *p, size_t n, double v) {
+    if (!n) return 0;
+    k_fill<<<nblk(n), NT, 0, c->stream>>>(n, p, v);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int gpk_kuf(gp_ctx *c) {
+    dim3 g(nblk(c->Npad), (c->M + 7) / 8);
+    k_kuf<<<g, NT, 0, c->stream>>>(c->N, c->Npad, c->M, c->Q, c->X, c->T, c->rho,
+                                   c->ard_only, c->Z + (size_t)c->g0 * c->Npad);
+    LAUNCH_CHECK();
+    k_kmm<<<nblk((size_t)c->M * c->M), NT, 0, c->stream>>>(c->M, c->Q, c->T, c->rho,
+                                                           c->ard_only, c->K, c->Ksym, c->Kchol);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int gpk_expand_delta(gp_ctx *c) {
+    k_expand_delta<<<nblk((size_t)c->P * c->P), NT, 0, c->stream>>>(
+        c->P, c->p, c->M, c->g0, c->d_nh, c->delta, c->K, c->DeltaInv, c->DeltaChol);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int gpk_oinvt(gp_ctx *c) {
+    k_oinvt<<<nblk(c->N), NT, 0, c->stream>>>(c->N, c->Npad, c->J, c->W, c->Omega, c->Oinvt);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int gpk_titsias(gp_ctx *c) {
+    const double one = 1.0, zero = 0.0;
+    const double *Knm = c->Z + (size_t)c->g0 * c->Npad;
+    GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, c->Npad,
+                          c->M, c->M, &one, Knm, c->Npad, c->Kinv, c->M, &zero, c->A, c->Npad));
+    k_titsias<<<nblk(c->N), NT, 0, c->stream>>>(c->N, c->Npad, c->M, Knm, c->A, c->titsias);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int gpk_w_nu(gp_ctx *c) {
+    // ZB = Z * Beta (eta for every cell and gene)
+    const double one = 1.0, zero = 0.0;
+    GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, c->Npad,
+                          c->J, c->P, &one, c->Z, c->Npad, c->Beta, c->P, &zero, c->ZB, c->Npad));
+    double dt;
+    GP_HIP(hipMemcpyAsync(&dt, c->delta + c->p - 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GP_HIP(hipStreamSynchronize(c->stream));
+    dim3 g(nblk(c->Npad), c->J);
+    k_w_nu<<<g, NT, 0, c->stream>>>(c->N, c->Npad, c->ZB, c->ls, c->titsias, dt, 0.1, c->W, c->Nu);
+    LAUNCH_CHECK();
+    k_nu_sparse<<<c->J, NT, 0, c->stream>>>(c->Npad, c->Yp, c->Yi, c->Yx, c->ZB, c->ls,
+                                            c->titsias, dt, 0.1, c->Nu);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int gpk_r_and_sums(gp_ctx *c) {
+    k_r_sums<<<c->J, NT, 0, c->stream>>>(c->N, c->Npad, c->W, c->Nu, c->Omega, c->R, c->gq, c->gs);
+    LAUNCH_CHECK();
+    // b_j = Z^T R_j  (util_cuda.cu:462-487 getZtOinvjnj for every gene)
+    const double one = 1.0, zero = 0.0;
+    GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->P,
+                          c->J, c->Npad, &one, c->Z, c->Npad, c->R, c->Npad, &zero, c->bvec, c->P));
+    return 0;
+}
+
+int gpk_init_y0(gp_ctx *c) {
+    dim3 g(nblk(c->Npad), c->J);
+    k_y0_dense<<<g, NT, 0, c->stream>>>(c->N, c->Npad, c->ls, c->R);
+    LAUNCH_CHECK();
+    k_y0_sparse<<<c->J, NT, 0, c->stream>>>(c->Npad, c->Yp, c->Yi, c->Yx, c->ls, c->R);
+    LAUNCH_CHECK();
+    const double one = 1.0, zero = 0.0;
+    GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->P,
+                          c->J, c->Npad, &one, c->Z, c->Npad, c->R, c->Npad, &zero, c->bvec, c->P));
+    return 0;
+}
+
+int gpk_beta_phi(gp_ctx *c, int jtr) {
+    // tBphi holds Psi_j^{-1} b_j on entry (solved by the caller)
+    k_beta_phi<<<c->J, NT, 0, c->stream>>>(c->N, c->P, jtr, c->Psi, c->tBphi, c->zeta, c->gq,
+                                           c->Beta, c->phi, c->pwork);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int gpk_lkhd_pois(gp_ctx *c) {
+    double dt;
+    GP_HIP(hipMemcpyAsync(&dt, c->delta + c->p - 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GP_HIP(hipStreamSynchronize(c->stream));
+    k_lkhd_pois<<<c->J, NT, 0, c->stream>>>(c->N, c->Npad, c->Yp, c->Yi, c->Yx, c->W, c->titsias,
+                                            dt, 0.1, c->lkhds);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int gpk_lkhd_genes_with(gp_ctx *c, const double *tb) {
+    k_lkhd_genes<<<c->J, NT, 0, c->stream>>>(c->N, c->P, c->Psi, c->DeltaChol, c->bvec, tb,
+                                             c->gq, c->gs, c->phi, c->lkhds);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int gpk_collect(gp_ctx *c, int Nterm, int slot, double *obj) {
+    double dt;
+    GP_HIP(hipMemcpyAsync(&dt, c->delta + c->p - 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GP_HIP(hipStreamSynchronize(c->stream));
+    int npart = 0;
+    if (Nterm > 0) {
+        k_collect_cells<<<RED_BLOCKS, NT, 0, c->stream>>>(Nterm, c->Oinvt, c->titsias, c->Omega,
+                                                          0.1 * dt, 100.0, 100.0, c->red);
+        LAUNCH_CHECK();
+        npart = RED_BLOCKS;
+    }
+    k_collect_final<<<1, NT, 0, c->stream>>>(c->J, c->lkhds, c->red, npart, c->lkhd, slot);
+    LAUNCH_CHECK();
+    if (obj) {
+        GP_HIP(hipMemcpyAsync(obj, c->lkhd + slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        GP_HIP(hipStreamSynchronize(c->stream));
+    }
+    return 0;
+}
+
+int gpk_grad_delta(gp_ctx *c) {
+    if (c->P > 2048) { gp_set_error("P > 2048 unsupported"); return 1; }
+    k_grad_delta_gene<<<c->J, NT, 0, c->stream>>>(c->P, c->p, c->M, c->g0, c->d_nh, c->delta,
+                                                  c->Ksym, c->Sinv, c->tBeta, c->phi,
+                                                  c->gradDeltaTmp);
+    LAUNCH_CHECK();
+    k_grad_delta_collect<<<c->p, NT, 0, c->stream>>>(c->N, c->J, c->P, c->p, c->d_nh,
+                                                     c->gradDeltaTmp, c->Oinvt, c->titsias,
+                                                     c->delta, 0.1, c->grad + c->N);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int gpk_tn(gp_ctx *c) {
+    // ZB = Z * tBeta
+    const double one = 1.0, zero = 0.0;
+    GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, c->Npad,
+                          c->J, c->P, &one, c->Z, c->Npad, c->tBeta, c->P, &zero, c->ZB, c->Npad));
+    k_tn<<<nblk(c->Npad), NT, 0, c->stream>>>(c->N, c->Npad, c->J, c->W, c->Nu, c->ZB, c->Omega,
+                                              c->phi, c->R, c->Ecell);
+    LAUNCH_CHECK();
+    // C1a = (TN / phi) * tBeta_GP^T
+    GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, c->Npad,
+                          c->M, c->J, &one, c->R, c->Npad, c->tBeta + c->g0, c->P, &zero, c->C1a,
+                          c->Npad));
+    return 0;
+}
+
+int gpk_cell_grad(gp_ctx *c) {
+    double dt;
+    GP_HIP(hipMemcpyAsync(&dt, c->delta + c->p - 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GP_HIP(hipStreamSynchronize(c->stream));
+    k_cell_grad<<<nblk(c->Npad), NT, 0, c->stream>>>(
+        c->N, c->Npad, c->J, c->M, c->Q, c->P, c->g0, c->ard_only, c->Z, c->U, c->C1a, c->A,
+        c->Omega, c->Oinvt, c->titsias, c->Ecell, c->X, c->T, c->rho, dt, 0.1, 100.0, 100.0,
+        c->grad, c->grad + c->N + c->p, c->C1);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int gpk_c2(gp_ctx *c) {
+    double dt;
+    GP_HIP(hipMemcpyAsync(&dt, c->delta + c->p - 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GP_HIP(hipStreamSynchronize(c->stream));
+    const double one = 1.0, zero = 0.0;
+    k_scale_rows<<<nblk((size_t)c->Npad * c->M), NT, 0, c->stream>>>(c->N, c->Npad, c->M, c->A,
+                                                                      c->Oinvt, c->AO);
+    LAUNCH_CHECK();
+    GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->M, c->M,
+                          c->Npad, &one, c->A, c->Npad, c->AO, c->Npad, &zero, c->G2, c->M));
+    k_scale_cols<<<nblk((size_t)c->P * c->J), NT, 0, c->stream>>>(c->P, c->J, c->tBeta, c->phi,
+                                                                   c->tBphi);
+    LAUNCH_CHECK();
+    GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, c->M, c->M,
+                          c->J, &one, c->tBphi + c->g0, c->P, c->tBeta + c->g0, c->P, &zero, c->B2,
+                          c->M));
+    k_sum_sinv<<<nblk((size_t)c->M * c->M), NT, 0, c->stream>>>(c->J, c->P, c->M, c->g0, c->Sinv,
+                                                                 c->S2);
+    LAUNCH_CHECK();
+    k_c2<<<nblk((size_t)c->M * c->M), NT, 0, c->stream>>>(c->M, c->J, 0.1, dt, c->G2, c->B2, c->S2,
+                                                          c->Kinv, c->K, c->C2);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int gpk_grad_tau_rho(gp_ctx *c) {
+    double *gT = c->grad + c->N + c->p + (size_t)c->N * c->Q;
+    double *gr = gT + (size_t)c->M * c->Q;
+    k_grad_tau_rho<<<c->M, NT, 0, c->stream>>>(c->N, c->Npad, c->M, c->Q, c->ard_only, c->C1,
+                                                c->C2, c->X, c->T, c->rho, gT, c->gradLogRhoTmp);
+    LAUNCH_CHECK();
+    k_collect_rho<<<1, 64, 0, c->stream>>>(c->M, c->Q, c->gradLogRhoTmp, gr);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+static int lb_dots(gp_ctx *c, size_t n, const double *a, const double *b, const double *x,
+                   const double *y, double *out) {
+    k_dot2<<<RED_BLOCKS, NT, 0, c->stream>>>(n, a, b, x, y, c->red);
+    LAUNCH_CHECK();
+    k_dot2_final<<<1, NT, 0, c->stream>>>(c->red, out);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int gpk_lbfgs(gp_ctx *c, int itr, int rhoflag) {
+    const size_t tot = (size_t)c->N + c->p + (size_t)c->N * c->Q + (size_t)c->M * c->Q + rhoflag;
+    const int pt = (itr - 1) % GP_HIST, ptp1 = itr % GP_HIST;
+    const unsigned nb = nblk(tot);
+    if (itr == 0) {
+        k_lb_first<<<nb, NT, 0, c->stream>>>(tot, c->grad, c->ss, c->grad0);
+        LAUNCH_CHECK();
+    } else {
+        k_lb_diff<<<nb, NT, 0, c->stream>>>(tot, c->grad, c->grad0, c->XX + (size_t)pt * tot);
+        LAUNCH_CHECK();
+        double *q = c->lbq, *z = c->ss;
+        GP_HIP(hipMemcpyAsync(q, c->grad, tot * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        const int lo = (itr > GP_HIST ? pt - GP_HIST + 1 : 0);
+        double *nd = c->scal, *alpha = c->scal + 4;
+        for (int i = pt; i >= lo; i--) {
+            const int ii = (i + GP_HIST) % GP_HIST;
+            const double *S = c->SS + (size_t)ii * tot, *Y = c->XX + (size_t)ii * tot;
+            GP_TRY(lb_dots(c, tot, S, q, S, Y, nd));
+            k_lb_axpy<<<nb, NT, 0, c->stream>>>(tot, 0, nd, alpha + ii, Y, q);
+            LAUNCH_CHECK();
+        }
+        {
+            const double *S = c->SS + (size_t)pt * tot, *Y = c->XX + (size_t)pt * tot;
+            GP_TRY(lb_dots(c, tot, S, Y, Y, Y, nd));
+            k_lb_axpy<<<nb, NT, 0, c->stream>>>(tot, 1, nd, nullptr, q, z);
+            LAUNCH_CHECK();
+        }
+        for (int i = lo; i <= pt; i++) {
+            const int ii = (i + GP_HIST) % GP_HIST;
+            const double *S = c->SS + (size_t)ii * tot, *Y = c->XX + (size_t)ii * tot;
+            GP_TRY(lb_dots(c, tot, Y, z, S, Y, nd));
+            k_lb_axpy<<<nb, NT, 0, c->stream>>>(tot, 2, nd, alpha + ii, S, z);
+            LAUNCH_CHECK();
+        }
+        k_lb_clip<<<nb, NT, 0, c->stream>>>(tot, z, c->ss);
+        LAUNCH_CHECK();
+    }
+    const size_t nmax = std::max(tot, (size_t)c->N * c->Q);
+    k_lb_update<<<nblk(nmax), NT, 0, c->stream>>>(c->N, c->p, c->M, c->Q, 0, 1.0, rhoflag, c->ss,
+                                                   c->Omega, c->delta, c->X, c->T, c->rho,
+                                                   c->SS + (size_t)ptp1 * tot, tot);
+    LAUNCH_CHECK();
+    k_lb_rho<<<1, 64, 0, c->stream>>>(c->N, c->p, c->M, c->Q, 0, 1.0, rhoflag, c->ss, c->rho);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int gpk_downdate(gp_ctx *c, int itr, double r, int rhoflag) {
+    const size_t tot = (size_t)c->N + c->p + (size_t)c->N * c->Q + (size_t)c->M * c->Q + rhoflag;
+    const int ptp1 = itr % GP_HIST;
+    const size_t nmax = std::max(tot, (size_t)c->N * c->Q);
+    k_lb_update<<<nblk(nmax), NT, 0, c->stream>>>(c->N, c->p, c->M, c->Q, 1, r, rhoflag, c->ss,
+                                                   c->Omega, c->delta, c->X, c->T, c->rho,
+                                                   c->SS + (size_t)ptp1 * tot, tot);
+    LAUNCH_CHECK();
+    k_lb_rho<<<1, 64, 0, c->stream>>>(c->N, c->p, c->M, c->Q, 1, r, rhoflag, c->ss, c->rho);
+    LAUNCH_CHECK();
+    return 0;
+}
+
+int gpk_beta_from_tilde(gp_ctx *c) {
+    const size_t n = (size_t)c->P * c->J;
+    k_add_zeta<<<nblk(n), NT, 0, c->stream>>>(n, c->P, c->tBeta, c->zeta, c->Beta);
+    LAUNCH_CHECK();
+    return 0;
+}

@@ -1,0 +1,207 @@
+"""ctypes binding of libgaspacho2.so (the C ABI declared in include/gaspacho2.h).
+
+The binding mirrors the reference driver's launch groups (src/ggplvm.cu main):
+``init_fit`` (ggplvm.cu:1111-1137), ``irls_step`` (1156-1181), ``ql_start``
+(1205-1211), ``elbo`` (1259-1278), ``grad`` (1215-1240), ``lbfgs_step`` (1251),
+``downdate`` (1289), ``end_outer`` (1319-1324) and ``fit`` (1146-1330).
+There is no CPU fallback: if the library is not built, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgaspacho2.so")
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+_lib = None
+
+
+class GPError(RuntimeError):
+    pass
+
+
+def load() -> C.CDLL:
+    """Load libgaspacho2.so; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise GPError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+    lib = C.CDLL(LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    vp = C.c_void_p
+    sig = {
+        "gp_version": (C.c_int, []),
+        "gp_last_error": (C.c_char_p, []),
+        "gp_create": (C.c_int, [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ip,
+                                ip, ip, dp, C.c_longlong, dp, dp, dp, dp, C.c_int, C.c_int]),
+        "gp_destroy": (C.c_int, [vp]),
+        "gp_set_precision": (C.c_int, [vp, C.c_int]),
+        "gp_buffer_len": (C.c_size_t, [vp, C.c_char_p]),
+        "gp_get_buffer": (C.c_int, [vp, C.c_char_p, dp, C.c_size_t]),
+        "gp_set_buffer": (C.c_int, [vp, C.c_char_p, dp, C.c_size_t]),
+        "gp_init_fit": (C.c_int, [vp]),
+        "gp_irls_step": (C.c_int, [vp, C.c_int, dp]),
+        "gp_ql_start": (C.c_int, [vp, dp]),
+        "gp_elbo": (C.c_int, [vp, C.c_int, dp]),
+        "gp_grad": (C.c_int, [vp]),
+        "gp_lbfgs_step": (C.c_int, [vp, C.c_int, C.c_int]),
+        "gp_downdate": (C.c_int, [vp, C.c_int, C.c_double, C.c_int]),
+        "gp_end_outer": (C.c_int, [vp]),
+        "gp_fit": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp, dp]),
+        "gp_timing_enable": (C.c_int, [vp, C.c_int]),
+        "gp_timing_reset": (C.c_int, [vp]),
+        "gp_timing_get": (C.c_int, [vp, C.c_char_p, dp, ip]),
+        "gp_init_ta_with_per": (C.c_int, [C.c_int, C.c_int, dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def init_ta_with_per(M: int, Q: int) -> np.ndarray:
+    """Initial inducing points (util.c:73-87); M x Q column-major as a flat array."""
+    lib = load()
+    T = np.zeros(M * Q, dtype=np.float64)
+    if lib.gp_init_ta_with_per(M, Q, _dp(T)) != 0:
+        raise GPError(lib.gp_last_error().decode())
+    return T
+
+
+class GPContext:
+    """One fit on one GPU (a gp_ctx)."""
+
+    def __init__(self, N: int, J: int, M: int, Q: int, nh, Yp, Yi, Yx, Zcov, ls, X0, T0,
+                 ard_only: bool = False, precision: int = PREC_FP64):
+        self._lib = load()
+        self.N, self.J, self.M, self.Q = int(N), int(J), int(M), int(Q)
+        self.nh = np.ascontiguousarray(nh, dtype=np.int32)
+        self.p = len(self.nh) - 1
+        self.P = int(self.nh[-1])
+        self._keep = [np.ascontiguousarray(Yp, dtype=np.int32),
+                      np.ascontiguousarray(Yi, dtype=np.int32),
+                      np.ascontiguousarray(Yx, dtype=np.float64),
+                      np.asfortranarray(np.asarray(Zcov, dtype=np.float64).reshape(self.N, -1, order="F")).ravel(order="F"),
+                      np.ascontiguousarray(ls, dtype=np.float64),
+                      np.ascontiguousarray(X0, dtype=np.float64).ravel(order="F") if np.ndim(X0) == 1 else np.asfortranarray(X0, dtype=np.float64).ravel(order="F"),
+                      np.ascontiguousarray(T0, dtype=np.float64).ravel(order="F") if np.ndim(T0) == 1 else np.asfortranarray(T0, dtype=np.float64).ravel(order="F")]
+        Ypa, Yia, Yxa, Zc, lsa, Xa, Ta = self._keep
+        h = C.c_void_p()
+        st = self._lib.gp_create(C.byref(h), self.N, self.J, self.M, self.Q, self.p, _ip(self.nh),
+                                 _ip(Ypa), _ip(Yia), _dp(Yxa), len(Yxa), _dp(Zc), _dp(lsa),
+                                 _dp(Xa), _dp(Ta), int(bool(ard_only)), int(precision))
+        self._check(st)
+        self._h = h
+        self.last_nonpd = 0
+
+    # -- plumbing
+    def _check(self, st: int) -> int:
+        if st > 0:
+            raise GPError(f"status {st}: {self._lib.gp_last_error().decode()}")
+        self.last_nonpd = -st if st < 0 else 0
+        return st
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.gp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- buffers
+    def buffer_len(self, name: str) -> int:
+        return int(self._lib.gp_buffer_len(self._h, name.encode()))
+
+    def get(self, name: str) -> np.ndarray:
+        n = self.buffer_len(name)
+        if n == 0:
+            raise GPError(f"unknown buffer {name}")
+        out = np.empty(n, dtype=np.float64)
+        self._check(self._lib.gp_get_buffer(self._h, name.encode(), _dp(out), n))
+        return out
+
+    def set(self, name: str, arr) -> None:
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        self._check(self._lib.gp_set_buffer(self._h, name.encode(), _dp(a), a.size))
+
+    def set_precision(self, precision: int) -> None:
+        self._check(self._lib.gp_set_precision(self._h, int(precision)))
+
+    # -- launch groups
+    def init_fit(self) -> None:
+        self._check(self._lib.gp_init_fit(self._h))
+
+    def irls_step(self, jtr: int) -> float:
+        o = C.c_double()
+        self._check(self._lib.gp_irls_step(self._h, int(jtr), C.byref(o)))
+        return o.value
+
+    def ql_start(self) -> float:
+        o = C.c_double()
+        self._check(self._lib.gp_ql_start(self._h, C.byref(o)))
+        return o.value
+
+    def elbo(self, slot: int = 0) -> float:
+        o = C.c_double()
+        self._check(self._lib.gp_elbo(self._h, int(slot), C.byref(o)))
+        return o.value
+
+    def grad(self) -> None:
+        self._check(self._lib.gp_grad(self._h))
+
+    def lbfgs_step(self, jtr: int, rhoflag: int) -> None:
+        self._check(self._lib.gp_lbfgs_step(self._h, int(jtr), int(rhoflag)))
+
+    def downdate(self, jtr: int, sr: float, rhoflag: int) -> None:
+        self._check(self._lib.gp_downdate(self._h, int(jtr), float(sr), int(rhoflag)))
+
+    def end_outer(self) -> None:
+        self._check(self._lib.gp_end_outer(self._h))
+
+    def fit(self, itrmax: int, jtr_cap: int = 0, verbose: bool = False) -> float:
+        o = C.c_double()
+        self._check(self._lib.gp_fit(self._h, int(itrmax), int(jtr_cap), int(bool(verbose)),
+                                     None, None, C.byref(o)))
+        return o.value
+
+    # -- timing
+    def timing(self, on: bool = True) -> None:
+        self._check(self._lib.gp_timing_enable(self._h, int(bool(on))))
+
+    def timing_reset(self) -> None:
+        self._check(self._lib.gp_timing_reset(self._h))
+
+    def timing_get(self, name: str):
+        ms = C.c_double()
+        cnt = C.c_int()
+        self._check(self._lib.gp_timing_get(self._h, name.encode(), C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value

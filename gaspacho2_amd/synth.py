@@ -1,0 +1,174 @@
+"""Synthetic single-cell count data in the reference's file formats.
+
+Generative model (the workloads' recipe): latent X ~ N(0, I_Q); generating
+inducing points ~ N(0, I_Q); SE-ARD lengthscales l_q ~ U(0.5, 2) (rho = l^2);
+per-gene f_g = K_xz a_g with a_g ~ N(0, 0.5^2); intercepts b_g ~ N(-1, 1);
+size factors s_n ~ LogNormal(0, 0.5^2); counts y ~ Poisson(s_n exp(f_g + b_g)).
+Counts are stored CSC by gene (SparseMat.cu:37-52) with double values.
+
+The PCA initial latent follows init.cu:304-357 (getNu0 / getNuTNu / getX):
+nu0 = log((y + 0.5) / s) - Z beta, X = nu0 V / sqrt(lambda) * sqrt(N) with
+the top-Q eigenpairs of nu0^T nu0, largest first.  Here beta is the
+intercept-only fit (per-gene mean of log((y + 0.5) / s)).
+"""
+from __future__ import annotations
+
+import struct
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+
+@dataclass
+class Dataset:
+    N: int
+    J: int
+    Q: int
+    M: int
+    Yp: np.ndarray           # int32 [J+1]
+    Yi: np.ndarray           # int32 [L]
+    Yx: np.ndarray           # float64 [L]
+    ls: np.ndarray           # float64 [N]
+    Zcov: np.ndarray         # float64 N x ncol (Fortran), intercept first
+    nh: np.ndarray           # int32 [p+1]
+    X0: np.ndarray           # float64 N x Q (Fortran)
+    c: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    v: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    body: bytes = b""
+    X_true: Optional[np.ndarray] = None
+
+    @property
+    def L(self) -> int:
+        return int(self.Yp[-1])
+
+    @property
+    def P(self) -> int:
+        return int(self.nh[-1])
+
+    def zero_fraction(self) -> float:
+        return 1.0 - self.L / float(self.N * self.J)
+
+
+def _se_ard(X: np.ndarray, Z: np.ndarray, rho: np.ndarray) -> np.ndarray:
+    d = np.zeros((X.shape[0], Z.shape[0]))
+    for q in range(X.shape[1]):
+        d += (X[:, q:q + 1] - Z[None, :, q]) ** 2 / (2.0 * rho[q])
+    return np.exp(-d)
+
+
+def make_dataset(N: int, G: int, Q: int, M: int, seed: int = 0, n_numeric: int = 0,
+                 n_levels: int = 0, gen_M: Optional[int] = None, pca: bool = True,
+                 block: int = 256) -> Dataset:
+    """Counts from the generative model above; optional covariates
+    (n_numeric N(0,1) columns and one categorical variable with n_levels)."""
+    rng = np.random.default_rng(seed)
+    Mg = gen_M or M
+    X = rng.standard_normal((N, Q))
+    Zg = rng.standard_normal((Mg, Q))
+    rho = rng.uniform(0.5, 2.0, Q) ** 2
+    Kxz = _se_ard(X, Zg, rho)
+    a = rng.normal(0.0, 0.5, (Mg, G))
+    b = rng.normal(-1.0, 1.0, G)
+    s = np.exp(rng.normal(0.0, 0.5, N))
+    # covariates
+    cov_cols = [np.ones(N)]
+    c_list, v_list, body = [], [], b""
+    if n_numeric:
+        num = rng.standard_normal((N, n_numeric))
+        for k in range(n_numeric):
+            cov_cols.append(num[:, k])
+        c_list.append(1)
+        v_list.append(n_numeric)
+        body += np.asfortranarray(num).tobytes(order="F")
+    if n_levels:
+        cat = rng.integers(0, n_levels, N).astype(np.int32)
+        for lev in range(n_levels):
+            cov_cols.append((cat == lev).astype(np.float64))
+        c_list.append(n_levels)
+        v_list.append(1)
+        body += cat.tobytes()
+    Zcov = np.asfortranarray(np.stack(cov_cols, axis=1))
+    beta_cov = rng.normal(0.0, 0.2, (Zcov.shape[1], G)) if Zcov.shape[1] > 1 else None
+    Yp = np.zeros(G + 1, dtype=np.int64)
+    Yi_parts, Yx_parts = [], []
+    nu0_gram = None
+    for g0 in range(0, G, block):
+        g1 = min(G, g0 + block)
+        eta = Kxz @ a[:, g0:g1] + b[g0:g1]
+        if beta_cov is not None:
+            eta += Zcov[:, 1:] @ beta_cov[1:, g0:g1]
+        lam = s[:, None] * np.exp(eta)
+        y = rng.poisson(lam)
+        for k in range(g1 - g0):
+            nz = np.flatnonzero(y[:, k])
+            Yi_parts.append(nz.astype(np.int32))
+            Yx_parts.append(y[nz, k].astype(np.float64))
+            Yp[g0 + k + 1] = Yp[g0 + k] + nz.size
+    Yi = np.concatenate(Yi_parts) if Yi_parts else np.zeros(0, np.int32)
+    Yx = np.concatenate(Yx_parts) if Yx_parts else np.zeros(0)
+    ncol = Zcov.shape[1]
+    nh = np.array([0, 1] + ([] if not n_numeric else [1 + n_numeric]) +
+                  ([] if not n_levels else [1 + n_numeric + n_levels]), dtype=np.int32)
+    nh = np.concatenate([nh, [ncol + M]]).astype(np.int32)
+    ds = Dataset(N=N, J=G, Q=Q, M=M, Yp=Yp.astype(np.int32), Yi=Yi, Yx=Yx, ls=s,
+                 Zcov=Zcov, nh=nh, X0=np.zeros((N, Q), order="F"),
+                 c=np.array(c_list, np.int32), v=np.array(v_list, np.int32), body=body,
+                 X_true=X)
+    ds.X0 = pca_init(ds) if pca else np.asfortranarray(X)
+    return ds
+
+
+def dense_counts(ds: Dataset, genes: Optional[slice] = None) -> np.ndarray:
+    g = range(ds.J) if genes is None else range(*genes.indices(ds.J))
+    out = np.zeros((ds.N, len(g)))
+    for k, j in enumerate(g):
+        lo, hi = ds.Yp[j], ds.Yp[j + 1]
+        out[ds.Yi[lo:hi], k] = ds.Yx[lo:hi]
+    return out
+
+
+def pca_init(ds: Dataset, max_cells: int = 20000, seed: int = 1) -> np.ndarray:
+    """init.cu:304-357 PCA initial latent (see module docstring).
+
+    nu0 = S + u 1^T - 1 m^T with S sparse (log((y+0.5)/0.5) at non-zeros),
+    u_i = log(0.5 / s_i) and m_j the per-gene mean; the eigenvectors come from
+    a random cell subsample, the projection uses every cell."""
+    import scipy.sparse as sp
+    N, J, Q = ds.N, ds.J, ds.Q
+    vals = np.log(ds.Yx + 0.5) - np.log(0.5)
+    S = sp.csc_matrix((vals, ds.Yi, ds.Yp), shape=(N, J)).tocsr()
+    u = np.log(0.5) - np.log(ds.ls)
+    m = (np.asarray(S.sum(axis=0)).ravel() + u.sum()) / N
+    rng = np.random.default_rng(seed)
+    rows = np.arange(N) if N <= max_cells else np.sort(rng.choice(N, max_cells, replace=False))
+    nu = S[rows].toarray() + u[rows, None] - m[None, :]
+    G = nu.T @ nu
+    lam, V = np.linalg.eigh(G)
+    lam, V = lam[::-1][:Q], V[:, ::-1][:, :Q]
+    proj = np.asarray(S @ V) + u[:, None] * V.sum(axis=0)[None, :] - (m @ V)[None, :]
+    X = proj / np.sqrt(lam) * np.sqrt(rows.size)
+    return np.asfortranarray(X)
+
+
+# ----------------------------------------------------------------------------
+# reference file formats
+def write_sparsemat(path: str, ds: Dataset) -> None:
+    with open(path, "wb") as f:
+        f.write(struct.pack("<iii", ds.N, ds.J, ds.L))
+        f.write(ds.Yi.astype("<i4").tobytes())
+        f.write(ds.Yp.astype("<i4").tobytes())
+        f.write(ds.Yx.astype("<f8").tobytes())
+
+
+def write_dataframe(path: str, ds: Dataset) -> None:
+    with open(path, "wb") as f:
+        f.write(struct.pack("<ii", ds.N, len(ds.c)))
+        f.write(ds.c.astype("<i4").tobytes())
+        f.write(ds.v.astype("<i4").tobytes())
+        f.write(ds.body)
+
+
+def write_doubles(path: str, x: np.ndarray) -> None:
+    with open(path, "wb") as f:
+        f.write(np.asarray(x, dtype="<f8").ravel(order="F").tobytes())

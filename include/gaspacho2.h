@@ -1,0 +1,141 @@
+/*
+ * gaspacho2.h -- C ABI of the MI355X (gfx950) sparse GP-LVM fit library.
+ *
+ * The reference (natsuhiko/GASPACHO2) has no plugin layer: its boundary is the
+ * set of <<<>>> launch groups issued by main() in src/ggplvm.cu.  Each entry
+ * point below replaces one such launch group and keeps its argument meaning;
+ * the citation on each line names the reference code it replaces.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Host arrays are in the reference's
+ *    layouts (column-major FP64, see gp_get_buffer).
+ *  - Every call returns int status: 0 ok; >0 a hipError_t / library error
+ *    code (message in gp_last_error()); <0 minus the number of per-gene
+ *    Cholesky factorizations that met a non-positive pivot.  Such a factor is
+ *    zero-filled, exactly as chol.cu:26-29 does, so the objective becomes
+ *    +-inf/NaN the way the reference's does.
+ *  - One context = one HIP device, one stream; calls on a context must come
+ *    from one host thread at a time.
+ */
+#ifndef GASPACHO2_H
+#define GASPACHO2_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gp_ctx gp_ctx;
+
+/* Precision of the weighted-Gram and gradient contractions. */
+enum {
+    GP_PREC_FP64 = 0,    /* FP64 throughout (the reference's arithmetic) */
+    GP_PREC_MIXED = 1    /* FP16 hi/lo split MFMA Gram, FP32 accumulate,
+                            FP64 cross-chunk sums, FP64 Cholesky/log-det */
+};
+
+/* Library version (major*10000 + minor*100 + patch). */
+int gp_version(void);
+
+/* Message for the last non-zero status returned on this host thread. */
+const char *gp_last_error(void);
+
+/* Replaces the allocation + upload block of main(): ggplvm.cu:964-968
+ * (newSparseMatOnDevice), :981-982 (newDataFrameOnDevice), :1019-1060
+ * (cudaMalloc of every buffer, ls / X upload, T upload after initTaWithPer)
+ * and initParams<<<1,1>>> ggplvm.cu:1101 / init.cu:27-48.
+ *   N cells, J genes, M inducing points, Q latent dims.
+ *   p, nh[p+1]: variables and their model-matrix column offsets as built by
+ *     loadDataFrame (DataFrame.cu:56-90): variable 0 = intercept, p-1 = Knm.
+ *   Yp[J+1], Yi[L], Yx[L]: counts, CSC by gene (SparseMat.cu:37-52); row
+ *     indices ascending within a gene.
+ *   Zcov: N x nh[p-1] covariate model-matrix columns, column-major, with the
+ *     intercept in column 0 and categorical levels as 0/1 indicator columns
+ *     (what getModelMatrix, DataFrame.cu:40-50, returns).
+ *   ls[N] size factors; X0[N*Q], T0[M*Q] column-major.
+ *   ard_only: 0 = periodic dim 0 + SE-ARD (kernel_funcs.cu:61-97, the
+ *     reference default); 1 = SE-ARD on all dims (kernel_funcs.cu:221-255).
+ *   precision: GP_PREC_FP64 or GP_PREC_MIXED. */
+int gp_create(gp_ctx **out, int N, int J, int M, int Q, int p, const int *nh,
+              const int *Yp, const int *Yi, const double *Yx, long long L,
+              const double *Zcov, const double *ls, const double *X0,
+              const double *T0, int ard_only, int precision);
+int gp_destroy(gp_ctx *ctx);
+
+/* Switch the contraction precision of an existing context. */
+int gp_set_precision(gp_ctx *ctx, int precision);
+
+/* Named state buffers in the reference layout (host <-> device copies).
+ * Names: Omega delta X T rho phi (value|backup halves where the reference
+ * keeps them), Z, K, Kinv, KinvKmn, titsias, DeltaInv, W (incl. Oinvt at
+ * +N*J), Nu, Psi (factor, incl. inverse at +J*P*P), Beta, tBeta, lkhds,
+ * grad, grad0, ss, SS, XX, C1, C2, lkhd.  Lengths in doubles match the
+ * reference's cudaMalloc sizes (ggplvm.cu:1024-1060).
+ * Replaces readParams (util_cuda.cu:520-544) / writeParams (557-580) copies. */
+size_t gp_buffer_len(gp_ctx *ctx, const char *name);
+int gp_get_buffer(gp_ctx *ctx, const char *name, double *host, size_t n);
+int gp_set_buffer(gp_ctx *ctx, const char *name, const double *host, size_t n);
+
+/* ggplvm.cu:1111-1137: getKernelMats, cholKmm, getKinvKmn, expandDelta_k,
+ * initPsi, initPsi2, initBeta. */
+int gp_init_fit(gp_ctx *ctx);
+
+/* ggplvm.cu:1156-1181: one IRLS/Laplace step of the GLM phase
+ * (getWForBeta, getPsi, cholPsi, getBetaPhi(jtr), getLkhdPois, collectLkhd
+ * with N=0).  *obj receives lkhd[jtr]. */
+int gp_irls_step(gp_ctx *ctx, int jtr, double *obj);
+
+/* ggplvm.cu:1205-1211: getOinvt, cholDeltaInv, getLkhd, collectLkhd(jtr=0):
+ * the objective at the start of the quasi-likelihood phase. */
+int gp_ql_start(gp_ctx *ctx, double *obj);
+
+/* ggplvm.cu:1259-1278: one ELBO evaluation at the current parameters
+ * (getOinvt, getKernelMats, expandDelta_k, cholKmm, getKinvKmn, getPsi,
+ * cholPsi, cholDeltaInv, getLkhd, collectLkhd); stored at lkhd[slot]. */
+int gp_elbo(gp_ctx *ctx, int slot, double *obj);
+
+/* ggplvm.cu:1215-1240: one gradient evaluation (expandDelta_k, getOinvt,
+ * getTildeBeta, getGradLogDelta, collectGradLogDelta, getGradLogOmega,
+ * solvePsi, getC2, getGradTau, getGradLogRho, collectGradLogRho) into the
+ * device grad buffer [gOmega N | gdelta p | gX NQ | gT MQ | grho Q]. */
+int gp_grad(gp_ctx *ctx);
+
+/* ggplvm.cu:1251 LBFGS<<<1,1>>> (ggplvm.cu:865-934, util_cuda.cu:187-232):
+ * L-BFGS direction from the history ring and the parameter update with
+ * backups. */
+int gp_lbfgs_step(gp_ctx *ctx, int jtr, int rhoflag);
+
+/* ggplvm.cu:1289 Downdate<<<1,1>>> (ggplvm.cu:818-859). */
+int gp_downdate(gp_ctx *ctx, int jtr, double sr, int rhoflag);
+
+/* ggplvm.cu:1319-1324: expandDelta_k, getBetaFromTildeBeta. */
+int gp_end_outer(gp_ctx *ctx);
+
+/* ggplvm.cu:1146-1330: the whole outer loop (phase A, phase B with line
+ * search, end-of-iteration), run by the library with the reference's
+ * schedule.  jtr_cap > 0 caps every inner loop (0 = reference schedule).
+ * on_outer, if non-NULL, is called after each outer iteration (the driver
+ * writes X9.gz / phi.gz / Params9.bin there).  *obj = last objective. */
+typedef void (*gp_outer_cb)(gp_ctx *ctx, int itr, double obj, void *user);
+int gp_fit(gp_ctx *ctx, int itrmax, int jtr_cap, int verbose,
+           gp_outer_cb on_outer, void *user, double *obj);
+
+/* util.c:73-87 initTaWithPer: initial inducing points T (M x Q,
+ * column-major).  Dim 0 equispaced on (0, 2*pi); dims 1..Q-1 N(0,1) from
+ * MT19937 seeded the way GSL seeds seed 0 (-> 4357), GSL polar Box-Muller.
+ * Host-only; needs no device. */
+int gp_init_ta_with_per(int M, int Q, double *T);
+
+/* Average device time (ms) of the last launch of a named kernel group and
+ * its launch count since gp_timing_reset: "gram", "chol", "gradcontract",
+ * "kuf", "elbo", "grad".  Replaces the cudaEvent printf timers of
+ * ggplvm.cu:1155-1276. */
+int gp_timing_enable(gp_ctx *ctx, int on);
+int gp_timing_reset(gp_ctx *ctx);
+int gp_timing_get(gp_ctx *ctx, const char *name, double *avg_ms, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

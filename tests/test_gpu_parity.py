@@ -1,0 +1,169 @@
+"""GPU (HIP, through the C ABI) vs the FP64 CPU oracle, launch group by launch
+group, on the same seeded inputs.  FP64 path: tolerances are a few hundred
+ulps relative to the largest entry (summation order differs: tree reductions
+and blocked factorizations on the GPU, serial loops in the oracle)."""
+import numpy as np
+import pytest
+
+from conftest import make_oracle, rel
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-9          # arrays, FP64 path
+TOL_OBJ = 1e-11     # objectives, FP64 path
+
+
+def _pair(ds, ard_only=False, precision=0):
+    from gaspacho2_amd import lib
+    T0 = lib.init_ta_with_per(ds.M, ds.Q)
+    o = make_oracle(ds, T0=T0, ard_only=ard_only)
+    g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0,
+                      T0, ard_only=ard_only, precision=precision)
+    return o, g
+
+
+def _cmp(o, g, name, tol=TOL, sl=None):
+    a, b = g.get(name), o.get(name)
+    if sl is not None:
+        a, b = a[sl], b[sl]
+    r = rel(a, b)
+    assert r < tol, f"{name}: rel err {r:.3e}"
+    return r
+
+
+def _psi_lower(x, J, P):
+    x = x[:J * P * P].reshape(J, P, P).transpose(0, 2, 1)
+    return np.tril(x)
+
+
+def test_init_fit(small_ds):
+    o, g = _pair(small_ds)
+    o.init_fit()
+    g.init_fit()
+    for name in ("Z", "K", "Kinv", "KinvKmn", "titsias", "DeltaInv", "Beta"):
+        _cmp(o, g, name)
+
+
+@pytest.mark.parametrize("ard", [False, True])
+def test_irls_and_quasi_likelihood_start(small_ds, ard):
+    o, g = _pair(small_ds, ard_only=ard)
+    o.init_fit()
+    g.init_fit()
+    J, P = small_ds.J, small_ds.P
+    for jtr in range(4):
+        lo, lg = o.irls_step(jtr), g.irls_step(jtr)
+        assert abs(lo - lg) <= TOL_OBJ * abs(lo), (jtr, lo, lg)
+    for name in ("W", "Nu", "Beta", "phi", "lkhds"):
+        _cmp(o, g, name, sl=slice(0, small_ds.N * small_ds.J) if name == "W" else None)
+    assert rel(_psi_lower(g.get("Psi"), J, P), _psi_lower(o.get("Psi"), J, P)) < TOL
+    lo, lg = o.ql_start(), g.ql_start()
+    assert abs(lo - lg) <= TOL_OBJ * abs(lo)
+    _cmp(o, g, "lkhds")
+
+
+def test_elbo_and_gradient(small_ds):
+    o, g = _pair(small_ds)
+    o.init_fit()
+    g.init_fit()
+    for jtr in range(3):
+        o.irls_step(jtr)
+        g.irls_step(jtr)
+    o.ql_start()
+    g.ql_start()
+    o.grad()
+    g.grad()
+    N, p, Q, M, J, P = small_ds.N, len(small_ds.nh) - 1, small_ds.Q, small_ds.M, small_ds.J, small_ds.P
+    _cmp(o, g, "tBeta")
+    _cmp(o, g, "gradDeltaTmp")
+    _cmp(o, g, "C1")
+    _cmp(o, g, "C2")
+    go, gg = o.get("grad"), g.get("grad")
+    blocks = {"gOmega": slice(0, N), "gdelta": slice(N, N + p), "gX": slice(N + p, N + p + N * Q),
+              "gT": slice(N + p + N * Q, N + p + N * Q + M * Q),
+              "grho": slice(N + p + N * Q + M * Q, N + p + N * Q + M * Q + Q)}
+    for k, s in blocks.items():
+        assert rel(gg[s], go[s]) < TOL, k
+    inv_g = g.get("Psi")[J * P * P:]
+    inv_o = o.get("Psi")[J * P * P:]
+    assert rel(inv_g, inv_o) < 1e-8
+    # ELBO at a moved point
+    X = o.get("X")
+    X[: N * Q] += 0.01 * np.sin(np.arange(N * Q))
+    o.set("X", X)
+    g.set("X", X)
+    lo, lg = o.elbo(1), g.elbo(1)
+    assert abs(lo - lg) <= TOL_OBJ * abs(lo)
+
+
+def test_lbfgs_line_search_sequence(small_ds):
+    o, g = _pair(small_ds)
+    o.init_fit()
+    g.init_fit()
+    for jtr in range(3):
+        o.irls_step(jtr)
+        g.irls_step(jtr)
+    lk_o = [o.ql_start()]
+    lk_g = [g.ql_start()]
+    for jtr in range(6):
+        o.grad()
+        g.grad()
+        o.lbfgs_step(jtr, 0)
+        g.lbfgs_step(jtr, 0)
+        sr = 1.0
+        for ktr in range(1, 8):
+            a, b = o.elbo(jtr + 1), g.elbo(jtr + 1)
+            assert abs(a - b) <= 1e-9 * abs(a), (jtr, ktr, a, b)
+            if a > lk_o[-1] or ktr == 7:
+                break
+            sr *= 0.1
+            o.downdate(jtr, sr, 0)
+            g.downdate(jtr, sr, 0)
+        lk_o.append(a)
+        lk_g.append(b)
+    for name in ("Omega", "delta", "X", "T", "rho"):
+        _cmp(o, g, name, tol=1e-8)
+
+
+def test_fit_short_schedule(tiny_ds):
+    o, g = _pair(tiny_ds)
+    lo = o.fit(2, 6)
+    lg = g.fit(2, 6)
+    assert abs(lo - lg) <= 1e-8 * abs(lo)
+    _cmp(o, g, "X", tol=1e-7)
+
+
+def test_config0_size_elbo_and_gradient():
+    """N=2,000, G=200, Q=5, M=32 (the small CPU-runnable workload)."""
+    from gaspacho2_amd import synth
+    ds = synth.make_dataset(2000, 200, 5, 32, seed=0)
+    o, g = _pair(ds)
+    o.init_fit()
+    g.init_fit()
+    for jtr in range(2):
+        o.irls_step(jtr)
+        g.irls_step(jtr)
+    a, b = o.ql_start(), g.ql_start()
+    assert abs(a - b) <= TOL_OBJ * abs(a)
+    o.grad()
+    g.grad()
+    assert rel(g.get("grad"), o.get("grad")) < TOL
+
+
+def test_non_pd_factor_is_zero_filled(tiny_ds):
+    o, g = _pair(tiny_ds)
+    o.init_fit()
+    g.init_fit()
+    o.irls_step(0)
+    g.irls_step(0)
+    N, J, P = tiny_ds.N, tiny_ds.J, tiny_ds.P
+    W = o.get("W")
+    W[N * 2: N * 3] *= -50.0          # gene 2 gets negative weights -> indefinite Gram
+    o.set("W", W)
+    g.set("W", W)
+    o.call("orc_get_psi")
+    bad_o = o.call("orc_chol_psi")
+    r = g.elbo(0)
+    assert bad_o == 1 and g.last_nonpd == 1
+    Pg = g.get("Psi")[:J * P * P].reshape(J, P * P)
+    assert not Pg[2].any()
+    assert not np.isfinite(r)
