@@ -81,9 +81,137 @@ __global__ void __launch_bounds__(UT) k_gradcontract_f64(int N, int Npad, int J,
         }
 }
 
+
+// ---------------------------------------------------------------------------
+// FP64 MFMA path (v_mfma_f64_16x16x4_f64), P16 = 16*T16 <= 272.
+// Workgroup = 4 waves = 64 cells; wave w owns cells i0+16w .. i0+16w+15 and
+// all T16 column tiles of U.  Its model-matrix fragments stay in registers for
+// the whole kernel:  zf[ks], lane l = Z[cell 16w + l%16][k = 4 ks + l/16].
+// For every gene the A fragment is zf[ks] * o_ij (o = W/Omega of the lane's
+// cell) and the B fragments are rows of Sinv_j, streamed through two LDS
+// buffers of 16 rows x P16 columns (read as columns, Sinv being symmetric):
+//   B frag, lane l: Sinv_j[k = 4 ks + l/16][n = 16 ct + l%16]
+//   D,      lane l, r: U[cell 16w + 4(l/16) + r][16 ct + l%16]
+typedef double gc_d4 __attribute__((ext_vector_type(4)));
+
+template <int T16>
+__global__ void __launch_bounds__(256) k_gc_mfma_f64(int N, int Npad, int J, int P,
+        const double *__restrict__ Z, const double *__restrict__ W,
+        const double *__restrict__ Omega, const double *__restrict__ Sinv,
+        double *__restrict__ U) {
+    constexpr int P16 = 16 * T16, KS = 4 * T16;
+    extern __shared__ double sl[];               // [2][16][P16]
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int i0 = blockIdx.x * 64;
+    const int ci = i0 + 16 * w + (lane % 16);    // cell of this lane's A rows
+    double zf[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+        const int k = 4 * ks + lane / 16;
+        zf[ks] = (k < P) ? Z[ci + (size_t)k * Npad] : 0.0;
+    }
+    gc_d4 acc[T16];
+#pragma unroll
+    for (int ct = 0; ct < T16; ct++) acc[ct] = gc_d4{0.0, 0.0, 0.0, 0.0};
+
+    // staging map: thread -> row r = tid/16 of the 16-row chunk, columns tid%16 + 16 t;
+    // a running pointer walks the chunks so the loads use immediate offsets
+    const int sr = tid / 16, sc = tid % 16;
+    const size_t PP = (size_t)P * P;
+    const size_t chunk_step = (size_t)16 * P;
+    double pf[T16];
+    auto fetch = [&](const double *rowp, int k) {
+        const bool krow = k < P;
+#pragma unroll
+        for (int t = 0; t < T16; t++)
+            pf[t] = (krow && sc + 16 * t < P) ? rowp[16 * t] : 0.0;
+    };
+    auto store = [&](int buf) {
+        double *d = sl + buf * 16 * P16 + sr * P16 + sc;
+#pragma unroll
+        for (int t = 0; t < T16; t++) d[16 * t] = pf[t];
+    };
+    const double *rp = Sinv + (size_t)sr * P + sc;   // row sr of chunk 0, gene 0
+    fetch(rp, sr);
+    store(0);
+    double o = (ci < N) ? W[ci] / Omega[ci] : 0.0;
+    double on = 0.0;
+    __syncthreads();
+    int cur = 0;
+    int kn = sr;                                      // row index of the prefetched chunk
+    for (int j = 0; j < J; j++) {
+#pragma unroll
+        for (int kc = 0; kc < T16; kc++) {
+            const bool next = (kc + 1 < T16) || (j + 1 < J);
+            if (next) {
+                if (kc + 1 < T16) { rp += chunk_step; kn += 16; }
+                else { rp = Sinv + (size_t)(j + 1) * PP + (size_t)sr * P + sc; kn = sr; }
+                asm volatile("" : "+v"(kn));
+                fetch(rp, kn);
+            }
+            if (kc + 1 == T16 && j + 1 < J)
+                on = (ci < N) ? W[ci + (size_t)(j + 1) * Npad] / Omega[ci] : 0.0;
+            const double *B = sl + cur * 16 * P16 + (lane / 16) * P16 + (lane % 16);
+#pragma unroll
+            for (int kk = 0; kk < 4; kk++) {
+                const double a = zf[4 * kc + kk] * o;
+#pragma unroll
+                for (int ct = 0; ct < T16; ct++)
+                    acc[ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, B[4 * kk * P16 + 16 * ct],
+                                                                   acc[ct], 0, 0, 0);
+            }
+            if (next) store(cur ^ 1);
+            __syncthreads();
+            cur ^= 1;
+        }
+        o = on;
+    }
+#pragma unroll
+    for (int ct = 0; ct < T16; ct++) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = i0 + 16 * w + 4 * (lane / 16) + r, col = 16 * ct + (lane % 16);
+            if (col < P) U[row + (size_t)col * Npad] = acc[ct][r];
+        }
+    }
+}
+
 }  // namespace
 
+template <int T16>
+static int launch_gc(gp_ctx *c) {
+    const size_t shm = 2 * 16 * (size_t)(16 * T16) * sizeof(double);
+    if (shm > 65536)
+        GP_HIP(hipFuncSetAttribute((const void *)k_gc_mfma_f64<T16>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    k_gc_mfma_f64<T16><<<c->Npad / 64, 256, shm, c->stream>>>(c->N, c->Npad, c->J, c->P, c->Z,
+                                                               c->W, c->Omega, c->Sinv, c->U);
+    GP_HIP(hipGetLastError());
+    return 0;
+}
+
 int gpk_gradcontract(gp_ctx *c) {
+    switch ((c->P + 15) / 16) {
+    case 1: return launch_gc<1>(c);
+    case 2: return launch_gc<2>(c);
+    case 3: return launch_gc<3>(c);
+    case 4: return launch_gc<4>(c);
+    case 5: return launch_gc<5>(c);
+    case 6: return launch_gc<6>(c);
+    case 7: return launch_gc<7>(c);
+    case 8: return launch_gc<8>(c);
+    case 9: return launch_gc<9>(c);
+    case 10: return launch_gc<10>(c);
+    case 11: return launch_gc<11>(c);
+    case 12: return launch_gc<12>(c);
+    case 13: return launch_gc<13>(c);
+    case 14: return launch_gc<14>(c);
+    case 15: return launch_gc<15>(c);
+    case 16: return launch_gc<16>(c);
+    case 17: return launch_gc<17>(c);
+    default: break;
+    }
     dim3 g(c->Npad / TI, (c->P + TN - 1) / TN);
     k_gradcontract_f64<<<g, UT, 0, c->stream>>>(c->N, c->Npad, c->J, c->P, c->Z, c->W, c->Omega,
                                                 c->Sinv, c->U);
