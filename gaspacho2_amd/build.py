@@ -22,6 +22,9 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HIP_SOURCES = ["gp_ctx.hip", "gp_kernels.hip", "gp_chol.hip", "gp_gram.hip",
                "gp_gradcontract.hip"]
+# per-source extra flags: the Gram kernel keeps its MFMA accumulators in
+# VGPRs (no per-chunk AGPR<->VGPR copies, two waves per SIMD fit)
+EXTRA_FLAGS = {"gp_gram.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -51,7 +54,8 @@ def build_library(jobs: int = 8) -> str:
     for src in HIP_SOURCES:
         o = os.path.join(OBJ, src.replace(".hip", ".o"))
         objs.append(o)
-        cmds.append([hipcc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
+        cmds.append([hipcc, *HIPCC_FLAGS, *EXTRA_FLAGS.get(src, []), "-c",
+                     os.path.join(CSRC, src), "-o", o])
     with ThreadPoolExecutor(max_workers=max(1, min(jobs, len(cmds)))) as ex:
         list(ex.map(_run, cmds))
     io_obj = os.path.join(OBJ, "gp_io.o")

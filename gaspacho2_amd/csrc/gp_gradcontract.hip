@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(UT) k_gradcontract_f64(int N, int Npad, int J,
 // cell) and the B fragments are rows of Sinv_j, streamed through two LDS
 // buffers of 16 rows x P16 columns (read as columns, Sinv being symmetric):
 //   B frag, lane l: Sinv_j[k = 4 ks + l/16][n = 16 ct + l%16]
-//   D,      lane l, r: U[cell 16w + 4(l/16) + r][16 ct + l%16]
+//   D,      lane l, r: U[cell 16w + 4r + l/16][16 ct + l%16]
 typedef double gc_d4 __attribute__((ext_vector_type(4)));
 
 template <int T16>
@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(256) k_gc_mfma_f64(int N, int Npad, int J, int
     for (int ct = 0; ct < T16; ct++) {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const int row = i0 + 16 * w + 4 * (lane / 16) + r, col = 16 * ct + (lane % 16);
+            const int row = i0 + 16 * w + 4 * r + lane / 16, col = 16 * ct + (lane % 16);
             if (col < P) U[row + (size_t)col * Npad] = acc[ct][r];
         }
     }

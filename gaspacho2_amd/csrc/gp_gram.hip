@@ -95,7 +95,7 @@ __global__ void __launch_bounds__(GT) k_gram_f64(int N, int Npad, int P, int nti
 // reads its A/B fragments straight from LDS:
 //   A frag, lane l: (w z)[cell = k0 + l/16][col = 16a + l%16]
 //   B frag, lane l:    z [cell = k0 + l/16][col = 16b + l%16]
-//   D,      lane l, r: Psi[16a + 4(l/16) + r][16b + l%16]
+//   D,      lane l, r: Psi[16a + 4r + l/16][16b + l%16]
 // The next chunk is fetched into registers while the current one is
 // multiplied, so global latency hides behind the MFMAs.
 typedef double gp_d4 __attribute__((ext_vector_type(4)));
@@ -112,8 +112,10 @@ __global__ void __launch_bounds__(256) k_gram_mfma_f64(int N, int Npad, int P, i
         const double *__restrict__ Omega, const double *__restrict__ DeltaInv,
         double *__restrict__ Psi) {
     extern __shared__ double lds[];
-    double *As = lds;                       // [KCM][P16]
-    double *Bs = lds + KCM * P16;           // [KCM][P16]
+    const int LS = P16 + 1;                 // LDS row stride: odd, so the 16 cells of a
+                                            // staging store fall in distinct banks
+    double *As = lds;                       // [KCM][LS]
+    double *Bs = lds + KCM * LS;            // [KCM][LS]
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int j = blockIdx.y;
@@ -135,8 +137,8 @@ __global__ void __launch_bounds__(256) k_gram_mfma_f64(int N, int Npad, int P, i
     const int cell = tid % KCM, col0 = tid / KCM;
     const double *zb = Z + cell + (size_t)col0 * Npad;
     const size_t zstride = (size_t)KCM * Npad;
-    double *as = As + cell * P16 + col0;
-    double *bs = Bs + cell * P16 + col0;
+    double *as = As + cell * LS + col0;
+    double *bs = Bs + cell * LS + col0;
     double pf[PF];
     double pw = 0.0;
     auto fetch = [&](int i0) {
@@ -162,7 +164,7 @@ __global__ void __launch_bounds__(256) k_gram_mfma_f64(int N, int Npad, int P, i
         int base[KCM / 4];
 #pragma unroll
         for (int ks = 0; ks < KCM / 4; ks++) {
-            base[ks] = (ks * 4 + lane / 16) * P16 + (lane % 16);
+            base[ks] = (ks * 4 + lane / 16) * LS + (lane % 16);
             asm volatile("" : "+v"(base[ks]));   // formed per chunk, not hoisted
         }
         int ta = a0, tb = b0;
@@ -186,7 +188,7 @@ __global__ void __launch_bounds__(256) k_gram_mfma_f64(int N, int Npad, int P, i
         if (s < nvalid) {
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int row = ta * 16 + 4 * (lane / 16) + r, col = tb * 16 + lane % 16;
+                const int row = ta * 16 + 4 * r + lane / 16, col = tb * 16 + lane % 16;
                 if (row < P && col < P && row <= col) {
                     const double v = acc[s][r] + DeltaInv[row + (size_t)col * P];
                     Pj[row + (size_t)col * P] = v;
@@ -204,7 +206,7 @@ template <int PF>
 static int launch_gram_mfma(gp_ctx *c, const double *W, double *Psi, int batch, int P16, int T16) {
     const int ntot = T16 * (T16 + 1) / 2;
     const int ng = (ntot + TPG - 1) / TPG;
-    const size_t shm = 2 * (size_t)KCM * P16 * sizeof(double);
+    const size_t shm = 2 * (size_t)KCM * (P16 + 1) * sizeof(double);
     if (shm > 65536)
         GP_HIP(hipFuncSetAttribute((const void *)k_gram_mfma_f64<PF>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
