@@ -167,3 +167,13 @@ int gpk_chol_inverse_batched(gp_ctx *c, const double *L, int n, int batch, size_
                                           (rocblas_stride)stride, batch));
     return 0;
 }
+
+int gpk_tri_inverse(gp_ctx *c, const double *L, int n, double *V) {
+    const size_t tot = (size_t)n * n;
+    k_identity<<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(V, n, 1, 0);
+    GP_HIP(hipGetLastError());
+    const double one = 1.0;
+    GP_BLAS(rocblas_dtrsm(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                          rocblas_diagonal_non_unit, n, n, &one, L, n, V, n));
+    return 0;
+}

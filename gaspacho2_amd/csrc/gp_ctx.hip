@@ -54,7 +54,8 @@ static void free_all(gp_ctx *c) {
                     c->tBeta, c->bvec, c->tBphi, c->lkhds, c->gq, c->gs, c->grad, c->grad0, c->ss,
                     c->SS, c->XX, c->lbq, c->gradDeltaTmp, c->gradLogRhoTmp, c->C1, c->C2, c->U,
                     c->ZB, c->R, c->C1a, c->AO, c->G2, c->B2, c->S2, c->Ecell, c->red, c->scal,
-                    c->lkhd, c->info, c->Zh, c->Zl, c->gram_part, c->pwork};
+                    c->lkhd, c->info, c->Zh, c->Zl, c->gram_part, c->pwork, c->w32, c->wsc,
+                    c->wun, c->zun, c->Linv, c->Zw, c->DeltaW, c->DeltaWChol, c->Uw, c->tBw};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     gp_timer *ts[] = {&c->t_gram, &c->t_chol, &c->t_gc, &c->t_kuf, &c->t_elbo, &c->t_grad};
@@ -78,7 +79,7 @@ extern "C" int gp_create(gp_ctx **out, int N, int J, int M, int Q, int p, const 
     if (nh[0] != 0 || nh[p] - nh[p - 1] != M) { gp_set_error("nh must end with the M-column Knm block"); return 1; }
     for (int l = 0; l < p; l++)
         if (nh[l + 1] <= nh[l]) { gp_set_error("nh must be increasing"); return 1; }
-    if (precision != GP_PREC_FP64 && precision != GP_PREC_MIXED) { gp_set_error("bad precision"); return 1; }
+    if (precision < GP_PREC_FP64 || precision > GP_PREC_FP64_WHITENED) { gp_set_error("bad precision"); return 1; }
     if (L < 0 || (L > 0 && (!Yi || !Yx)) || !Yp || !ls || !X0 || !T0 || !Zcov) {
         gp_set_error("NULL input array"); return 1;
     }
@@ -129,6 +130,12 @@ extern "C" int gp_create(gp_ctx **out, int N, int J, int M, int Q, int p, const 
     ALLOC(c->titsias, (size_t)N);
     ALLOC(c->DeltaInv, P * P);
     ALLOC(c->DeltaChol, P * P);
+    ALLOC(c->Linv, (size_t)M * M);
+    ALLOC(c->Zw, Np * P);
+    ALLOC(c->DeltaW, P * P);
+    ALLOC(c->DeltaWChol, P * P);
+    ALLOC(c->Uw, Np * P);
+    ALLOC(c->tBw, P * J);
     ALLOC(c->W, Np * J);
     ALLOC(c->Oinvt, (size_t)N);
     ALLOC(c->Nu, Np * J);
@@ -206,7 +213,7 @@ extern "C" int gp_destroy(gp_ctx *c) {
 
 extern "C" int gp_set_precision(gp_ctx *c, int precision) {
     if (!c) { gp_set_error("NULL context"); return 1; }
-    if (precision != GP_PREC_FP64 && precision != GP_PREC_MIXED) { gp_set_error("bad precision"); return 1; }
+    if (precision < GP_PREC_FP64 || precision > GP_PREC_FP64_WHITENED) { gp_set_error("bad precision"); return 1; }
     c->precision = precision;
     return 0;
 }
@@ -359,6 +366,8 @@ static int clear_info(gp_ctx *c) {
 static int kernel_mats(gp_ctx *c) {
     gp_timer_begin(c, c->t_kuf);
     GP_TRY(gpk_kuf(c));
+    c->zsplit_valid = 0;
+    c->zw_valid = 0;
     GP_TRY(gpk_chol_batched(c, c->Kchol, c->M, 1, 0, c->info + 1));
     GP_TRY(gpk_chol_inverse_batched(c, c->Kchol, c->M, 1, 0, c->Kinv));
     GP_TRY(gpk_titsias(c));
@@ -366,9 +375,27 @@ static int kernel_mats(gp_ctx *c) {
     return 0;
 }
 
-static int gram_chol(gp_ctx *c) {
+// Weighted Grams + batched Cholesky.  mixed: FP16 hi/lo MFMA Gram in the
+// whitened basis [covariates | Knm L^-T] with the diagonal prior DeltaW, whose
+// smallest eigenvalue is min(1/delta) -- well conditioned for the FP32-accumulated
+// Gram; otherwise the FP64 Gram in the reference basis.
+static int gram_chol(gp_ctx *c, int precision) {
     gp_timer_begin(c, c->t_gram);
-    GP_TRY(gpk_gram(c, c->W, c->Psi, c->J));
+    if (precision != GP_PREC_FP64) {
+        if (!c->zw_valid) {
+            GP_TRY(gpk_whiten(c));
+            c->zw_valid = 1;
+            c->zsplit_valid = 0;
+        }
+        if (precision == GP_PREC_MIXED)
+            GP_TRY(gpk_gram_mixed(c, c->W, c->Psi, c->J, c->Zw, c->DeltaW));
+        else
+            GP_TRY(gpk_gram_src(c, c->W, c->Psi, c->J, c->Zw, c->DeltaW));
+        c->psi_whitened = 1;
+    } else {
+        GP_TRY(gpk_gram(c, c->W, c->Psi, c->J));
+        c->psi_whitened = 0;
+    }
     gp_timer_end(c, c->t_gram);
     gp_timer_begin(c, c->t_chol);
     GP_TRY(gpk_chol_batched(c, c->Psi, c->P, c->J, (size_t)c->P * c->P, c->info));
@@ -397,8 +424,8 @@ extern "C" int gp_irls_step(gp_ctx *c, int jtr, double *obj) {
     if (jtr < 0 || jtr >= GP_LKHD_SLOTS) { gp_set_error("jtr out of range"); return 1; }
     GP_TRY(clear_info(c));
     GP_TRY(gpk_w_nu(c));
-    GP_TRY(gram_chol(c));
-    GP_TRY(gpk_r_and_sums(c));
+    GP_TRY(gram_chol(c, GP_PREC_FP64));   // the GLM phase runs in the reference basis, FP64
+    GP_TRY(gpk_r_and_sums(c, c->Z));
     GP_TRY(gpk_cholsl_batched(c, c->Psi, c->P, (size_t)c->P * c->P, c->bvec, c->tBphi, c->J));
     GP_TRY(gpk_beta_phi(c, jtr));
     GP_TRY(gpk_lkhd_pois(c));
@@ -410,9 +437,10 @@ extern "C" int gp_irls_step(gp_ctx *c, int jtr, double *obj) {
 
 // getLkhd: b_j, beta-tilde_j (scratch), the factor diagonals and the sums.
 static int lkhd_genes(gp_ctx *c) {
-    GP_TRY(gpk_r_and_sums(c));
+    const bool w = c->psi_whitened;
+    GP_TRY(gpk_r_and_sums(c, w ? c->Zw : c->Z));
     GP_TRY(gpk_cholsl_batched(c, c->Psi, c->P, (size_t)c->P * c->P, c->bvec, c->pwork, c->J));
-    GP_TRY(gpk_lkhd_genes_with(c, c->pwork));
+    GP_TRY(gpk_lkhd_genes_with(c, c->pwork, w ? c->DeltaWChol : c->DeltaChol));
     return 0;
 }
 
@@ -434,7 +462,7 @@ extern "C" int gp_elbo(gp_ctx *c, int slot, double *obj) {
     GP_TRY(gpk_oinvt(c));
     GP_TRY(kernel_mats(c));
     GP_TRY(gpk_expand_delta(c));
-    GP_TRY(gram_chol(c));
+    GP_TRY(gram_chol(c, c->precision));
     GP_TRY(gpk_chol_batched(c, c->DeltaChol, c->P, 1, 0, c->info + 2));
     GP_TRY(lkhd_genes(c));
     GP_TRY(gpk_collect(c, c->N, slot, obj));
@@ -449,18 +477,21 @@ extern "C" int gp_grad(gp_ctx *c) {
     gp_timer_begin(c, c->t_grad);
     GP_TRY(gpk_expand_delta(c));
     GP_TRY(gpk_oinvt(c));
-    // getTildeBeta
-    GP_TRY(gpk_r_and_sums(c));
-    GP_TRY(gpk_cholsl_batched(c, c->Psi, c->P, (size_t)c->P * c->P, c->bvec, c->tBeta, c->J));
+    const bool w = c->psi_whitened;
+    // getTildeBeta (in the basis of the current factor)
+    GP_TRY(gpk_r_and_sums(c, w ? c->Zw : c->Z));
+    GP_TRY(gpk_cholsl_batched(c, c->Psi, c->P, (size_t)c->P * c->P, c->bvec,
+                              w ? c->tBw : c->tBeta, c->J));
     // solvePsi (explicit inverses, needed by the delta gradient, U and C2)
     GP_TRY(gpk_chol_inverse_batched(c, c->Psi, c->P, c->J, (size_t)c->P * c->P, c->Sinv));
-    GP_TRY(gpk_grad_delta(c));
+    GP_TRY(gpk_grad_delta(c, w));
     gp_timer_begin(c, c->t_gc);
-    GP_TRY(gpk_gradcontract(c));
+    GP_TRY(gpk_gradcontract(c, w ? c->Zw : c->Z, w ? c->Uw : c->U));
     gp_timer_end(c, c->t_gc);
+    if (w) GP_TRY(gpk_unwhiten_grad(c));
     GP_TRY(gpk_tn(c));
     GP_TRY(gpk_cell_grad(c));
-    GP_TRY(gpk_c2(c));
+    GP_TRY(gpk_c2(c, w));
     GP_TRY(gpk_grad_tau_rho(c));
     gp_timer_end(c, c->t_grad);
     GP_HIP(hipStreamSynchronize(c->stream));

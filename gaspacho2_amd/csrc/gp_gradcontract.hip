@@ -180,41 +180,41 @@ __global__ void __launch_bounds__(256) k_gc_mfma_f64(int N, int Npad, int J, int
 }  // namespace
 
 template <int T16>
-static int launch_gc(gp_ctx *c) {
+static int launch_gc(gp_ctx *c, const double *Zs, double *Uo) {
     const size_t shm = 2 * 16 * (size_t)(16 * T16) * sizeof(double);
     if (shm > 65536)
         GP_HIP(hipFuncSetAttribute((const void *)k_gc_mfma_f64<T16>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    k_gc_mfma_f64<T16><<<c->Npad / 64, 256, shm, c->stream>>>(c->N, c->Npad, c->J, c->P, c->Z,
-                                                               c->W, c->Omega, c->Sinv, c->U);
+    k_gc_mfma_f64<T16><<<c->Npad / 64, 256, shm, c->stream>>>(c->N, c->Npad, c->J, c->P, Zs,
+                                                               c->W, c->Omega, c->Sinv, Uo);
     GP_HIP(hipGetLastError());
     return 0;
 }
 
-int gpk_gradcontract(gp_ctx *c) {
+int gpk_gradcontract(gp_ctx *c, const double *Zs, double *Uo) {
     switch ((c->P + 15) / 16) {
-    case 1: return launch_gc<1>(c);
-    case 2: return launch_gc<2>(c);
-    case 3: return launch_gc<3>(c);
-    case 4: return launch_gc<4>(c);
-    case 5: return launch_gc<5>(c);
-    case 6: return launch_gc<6>(c);
-    case 7: return launch_gc<7>(c);
-    case 8: return launch_gc<8>(c);
-    case 9: return launch_gc<9>(c);
-    case 10: return launch_gc<10>(c);
-    case 11: return launch_gc<11>(c);
-    case 12: return launch_gc<12>(c);
-    case 13: return launch_gc<13>(c);
-    case 14: return launch_gc<14>(c);
-    case 15: return launch_gc<15>(c);
-    case 16: return launch_gc<16>(c);
-    case 17: return launch_gc<17>(c);
+    case 1: return launch_gc<1>(c, Zs, Uo);
+    case 2: return launch_gc<2>(c, Zs, Uo);
+    case 3: return launch_gc<3>(c, Zs, Uo);
+    case 4: return launch_gc<4>(c, Zs, Uo);
+    case 5: return launch_gc<5>(c, Zs, Uo);
+    case 6: return launch_gc<6>(c, Zs, Uo);
+    case 7: return launch_gc<7>(c, Zs, Uo);
+    case 8: return launch_gc<8>(c, Zs, Uo);
+    case 9: return launch_gc<9>(c, Zs, Uo);
+    case 10: return launch_gc<10>(c, Zs, Uo);
+    case 11: return launch_gc<11>(c, Zs, Uo);
+    case 12: return launch_gc<12>(c, Zs, Uo);
+    case 13: return launch_gc<13>(c, Zs, Uo);
+    case 14: return launch_gc<14>(c, Zs, Uo);
+    case 15: return launch_gc<15>(c, Zs, Uo);
+    case 16: return launch_gc<16>(c, Zs, Uo);
+    case 17: return launch_gc<17>(c, Zs, Uo);
     default: break;
     }
     dim3 g(c->Npad / TI, (c->P + TN - 1) / TN);
-    k_gradcontract_f64<<<g, UT, 0, c->stream>>>(c->N, c->Npad, c->J, c->P, c->Z, c->W, c->Omega,
-                                                c->Sinv, c->U);
+    k_gradcontract_f64<<<g, UT, 0, c->stream>>>(c->N, c->Npad, c->J, c->P, Zs, c->W, c->Omega,
+                                                c->Sinv, Uo);
     GP_HIP(hipGetLastError());
     return 0;
 }

@@ -203,7 +203,8 @@ __global__ void __launch_bounds__(256) k_gram_mfma_f64(int N, int Npad, int P, i
 }  // namespace
 
 template <int PF>
-static int launch_gram_mfma(gp_ctx *c, const double *W, double *Psi, int batch, int P16, int T16) {
+static int launch_gram_mfma(gp_ctx *c, const double *W, double *Psi, int batch, int P16, int T16,
+                            const double *Zs, const double *Dl) {
     const int ntot = T16 * (T16 + 1) / 2;
     const int ng = (ntot + TPG - 1) / TPG;
     const size_t shm = 2 * (size_t)KCM * (P16 + 1) * sizeof(double);
@@ -211,25 +212,30 @@ static int launch_gram_mfma(gp_ctx *c, const double *W, double *Psi, int batch, 
         GP_HIP(hipFuncSetAttribute((const void *)k_gram_mfma_f64<PF>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     dim3 g(ng, batch);
-    k_gram_mfma_f64<PF><<<g, 256, shm, c->stream>>>(c->N, c->Npad, c->P, P16, T16, ntot, c->Z, W,
-                                                     c->Omega, c->DeltaInv, Psi);
+    k_gram_mfma_f64<PF><<<g, 256, shm, c->stream>>>(c->N, c->Npad, c->P, P16, T16, ntot, Zs, W,
+                                                     c->Omega, Dl, Psi);
     GP_HIP(hipGetLastError());
     return 0;
 }
 
 int gpk_gram(gp_ctx *c, const double *W, double *Psi, int batch) {
+    return gpk_gram_src(c, W, Psi, batch, c->Z, c->DeltaInv);
+}
+
+int gpk_gram_src(gp_ctx *c, const double *W, double *Psi, int batch, const double *Zs,
+                 const double *Dl) {
     const int T16 = (c->P + 15) / 16, P16 = 16 * T16;
     const int pf = T16;   // columns per thread: P16 / 16
     if (P16 <= P16MAX) {
-        if (pf <= 4) return launch_gram_mfma<4>(c, W, Psi, batch, P16, T16);
-        if (pf <= 9) return launch_gram_mfma<9>(c, W, Psi, batch, P16, T16);
-        if (pf <= 17) return launch_gram_mfma<17>(c, W, Psi, batch, P16, T16);
-        if (pf <= 25) return launch_gram_mfma<25>(c, W, Psi, batch, P16, T16);
-        return launch_gram_mfma<36>(c, W, Psi, batch, P16, T16);
+        if (pf <= 4) return launch_gram_mfma<4>(c, W, Psi, batch, P16, T16, Zs, Dl);
+        if (pf <= 9) return launch_gram_mfma<9>(c, W, Psi, batch, P16, T16, Zs, Dl);
+        if (pf <= 17) return launch_gram_mfma<17>(c, W, Psi, batch, P16, T16, Zs, Dl);
+        if (pf <= 25) return launch_gram_mfma<25>(c, W, Psi, batch, P16, T16, Zs, Dl);
+        return launch_gram_mfma<36>(c, W, Psi, batch, P16, T16, Zs, Dl);
     }
     const int nt = (c->P + TILE - 1) / TILE;
     dim3 g(nt * (nt + 1) / 2, batch);
-    k_gram_f64<<<g, GT, 0, c->stream>>>(c->N, c->Npad, c->P, nt, c->Z, W, c->Omega, c->DeltaInv, Psi);
+    k_gram_f64<<<g, GT, 0, c->stream>>>(c->N, c->Npad, c->P, nt, Zs, W, c->Omega, Dl, Psi);
     GP_HIP(hipGetLastError());
     return 0;
 }

@@ -1,0 +1,324 @@
+// gp_gram_mixed.hip -- mixed-precision weighted Gram
+//   Psi_j = DeltaInv + Z^T diag(w_j) Z,   w_j = W_j / Omega
+// on FP16 MFMA (v_mfma_f32_16x16x32_f16) with a hi/lo split of both operands,
+// FP32 accumulation inside a cell segment and FP64 sums across segments.
+//
+// Operands.  z~ = z 2^s_a = zh + zl (two FP16 values, pre-split once per Kuf
+// build, each column scaled so its maximum sits just under 2^14) and, per gene,
+// a = (w 2^-e_j) z~ split the same way in registers, where 2^e_j is a power of
+// two >= max_i w_ij (so |a| <= |z~| < 2^14 and nothing overflows FP16).
+// Each product uses three MFMAs: ah*zh + ah*zl + al*zh (al*zl ~ 2^-22 dropped).
+//
+// Work decomposition.  The P16 x P16 output (P16 = 16*ceil(P/16)) is cut into
+// 128x128 blocks; workgroup (block pair, gene, segment) computes one block for
+// one gene over one segment of cells.  Wave w owns the 64x64 quadrant
+// (w/2, w%2) = 4x4 tiles of 16x16; tiles below the diagonal or past P16 are
+// skipped.  Per 32-cell chunk:
+//   A (weighted) : 128 rows x 32 cells hi/lo, built cooperatively into LDS
+//                  (double-buffered), fragment lane l = a[row 16t + l%16][cells 8(l/16)..+7]
+//   B (plain z)  : read by each wave straight from global zh/zl (column-major,
+//                  cells contiguous -> one 16-byte load per lane per fragment)
+//   D            : lane l, r -> block[16t_a + 4(l/16) + r][16t_b + l%16]
+// Partials (FP32, per segment) go to a scratch buffer; k_gram_reduce adds the
+// segments in FP64, rescales by 2^e_j, adds DeltaInv and writes both triangles.
+#include "gp_internal.h"
+
+#include <hip/hip_fp16.h>
+
+namespace {
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int BLK = 128;     // output block edge
+constexpr int CH = 32;       // cells per chunk (one MFMA k-step)
+
+// zh/zl (FP16, Npad x P16 column-major) from the FP64 model matrix, each
+// column scaled by a power of two 2^s_a that puts its largest magnitude in
+// [2^13, 2^14): values down to 2^-16 of the column maximum keep a normal FP16
+// lo part, i.e. the full ~22-bit hi/lo precision.  zun[a] = 2^-s_a.
+__global__ void __launch_bounds__(256) k_split_z(int Npad, int P, const double *__restrict__ Z,
+        _Float16 *__restrict__ zh, _Float16 *__restrict__ zl, double *__restrict__ zun) {
+    __shared__ double sh[4];
+    __shared__ float scale;
+    const int a = blockIdx.x;
+    const double *za = Z + (size_t)a * Npad;
+    double mx = 0.0;
+    if (a < P)
+        for (int i = threadIdx.x; i < Npad; i += 256) mx = fmax(mx, fabs(za[i]));
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double m = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+        int e = 0;
+        if (m > 0.0) (void)frexp(m, &e);          // m < 2^e
+        const int sa = 14 - e;                     // m * 2^sa < 2^14
+        scale = ldexpf(1.0f, sa);
+        zun[a] = ldexp(1.0, -sa);
+    }
+    __syncthreads();
+    const float sc = scale;
+    _Float16 *h = zh + (size_t)a * Npad, *l = zl + (size_t)a * Npad;
+    for (int i = threadIdx.x; i < Npad; i += 256) {
+        const float v = (a < P) ? (float)(za[i] * (double)sc) : 0.0f;
+        const _Float16 hv = (_Float16)v;
+        h[i] = hv;
+        l[i] = (_Float16)(v - (float)hv);
+    }
+}
+
+// Per gene: w32 = W/Omega (FP32) and the power-of-two scale 2^-e_j.
+__global__ void __launch_bounds__(256) k_wprep(int N, int Npad, const double *__restrict__ W,
+        const double *__restrict__ Omega, float *__restrict__ w32, float *__restrict__ wsc,
+        double *__restrict__ wunscale) {
+    __shared__ float sh[4];
+    const int j = blockIdx.x;
+    const double *Wj = W + (size_t)j * Npad;
+    float *wj = w32 + (size_t)j * Npad;
+    float mx = 0.0f;
+    for (int i = threadIdx.x; i < Npad; i += 256) {
+        const float v = (i < N) ? (float)(Wj[i] / Omega[i]) : 0.0f;
+        wj[i] = v;
+        mx = fmaxf(mx, fabsf(v));
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_down(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3]));
+        int e = 0;
+        if (m > 0.0f) { (void)frexpf(m, &e); }      // m <= 2^e
+        wsc[j] = ldexpf(1.0f, -e);
+        wunscale[j] = ldexp(1.0, e);
+    }
+}
+
+__device__ inline void split8(const float *p, h8 &hi, h8 &lo) {
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const _Float16 h = (_Float16)p[e];
+        hi[e] = h;
+        lo[e] = (_Float16)(p[e] - (float)h);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_gram_mixed(int Npad, int P16, int T16, int nbk, int seg,
+        const _Float16 *__restrict__ zh, const _Float16 *__restrict__ zl,
+        const float *__restrict__ w32, const float *__restrict__ wsc,
+        float *__restrict__ part) {
+    // LDS: [2 buffers][hi/lo][128 rows][32 cells] halves = 32 KB
+    __shared__ __attribute__((aligned(16))) _Float16 As[2][2][BLK][CH];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // block pair (bi <= bj) from blockIdx.x
+    int bi = 0, t = blockIdx.x;
+    while (t >= nbk - bi) { t -= nbk - bi; bi++; }
+    const int bj = bi + t;
+    const int j = blockIdx.y, s = blockIdx.z;   // gene, cell segment
+    const int c0 = s * seg, c1 = min(Npad, c0 + seg);
+    const float sc = wsc[j];
+    const float *wj = w32 + (size_t)j * Npad;
+    // quadrant tiles (global tile indices)
+    const int qa = w >> 1, qb = w & 1;
+    const int ta0 = bi * 8 + qa * 4, tb0 = bj * 8 + qb * 4;
+    bool valid[4][4];
+    bool any = false;
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            valid[x][y] = (ta0 + x < T16) && (tb0 + y < T16) && (ta0 + x <= tb0 + y);
+            any |= valid[x][y];
+        }
+    f4 acc[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) acc[x][y] = f4{0.f, 0.f, 0.f, 0.f};
+
+    // staging map: 512 items (128 rows x 4 groups of 8 cells) -> 2 per thread
+    const int srow0 = tid >> 2, sgrp = tid & 3;         // items (srow0, sgrp) and (srow0+64, sgrp)
+    const int arow0 = bi * BLK + srow0, arow1 = arow0 + 64;
+    h8 pzh[2], pzl[2];
+    float pw[8];
+    auto fetchA = [&](int cc) {
+        const size_t off = (size_t)cc + 8 * sgrp;
+        pzh[0] = (arow0 < P16) ? *(const h8 *)(zh + (size_t)arow0 * Npad + off) : h8{};
+        pzl[0] = (arow0 < P16) ? *(const h8 *)(zl + (size_t)arow0 * Npad + off) : h8{};
+        pzh[1] = (arow1 < P16) ? *(const h8 *)(zh + (size_t)arow1 * Npad + off) : h8{};
+        pzl[1] = (arow1 < P16) ? *(const h8 *)(zl + (size_t)arow1 * Npad + off) : h8{};
+#pragma unroll
+        for (int e = 0; e < 8; e++) pw[e] = wj[off + e] * sc;
+    };
+    auto storeA = [&](int buf) {
+#pragma unroll
+        for (int it = 0; it < 2; it++) {
+            float p[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) p[e] = ((float)pzh[it][e] + (float)pzl[it][e]) * pw[e];
+            h8 hi, lo;
+            split8(p, hi, lo);
+            *(h8 *)&As[buf][0][srow0 + 64 * it][8 * sgrp] = hi;
+            *(h8 *)&As[buf][1][srow0 + 64 * it][8 * sgrp] = lo;
+        }
+    };
+    // B fragments: lane l loads z[col 16 tb + l%16][cells cc + 8(l/16) ..+7]
+    h8 bh[4], bl[4];
+    auto fetchB = [&](int cc) {
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            const int col = (tb0 + y) * 16 + (lane & 15);
+            const size_t off = (size_t)col * Npad + cc + 8 * (lane >> 4);
+            const bool ok = (tb0 + y) < T16;
+            bh[y] = ok ? *(const h8 *)(zh + off) : h8{};
+            bl[y] = ok ? *(const h8 *)(zl + off) : h8{};
+        }
+    };
+
+    int cur = 0;
+    fetchA(c0);
+    storeA(0);
+    if (any) fetchB(c0);
+    __syncthreads();
+    for (int cc = c0; cc < c1; cc += CH) {
+        const bool more = cc + CH < c1;
+        h8 nbh[4], nbl[4];
+        if (more) {
+            fetchA(cc + CH);
+            if (any) {
+#pragma unroll
+                for (int y = 0; y < 4; y++) {
+                    const int col = (tb0 + y) * 16 + (lane & 15);
+                    const size_t off = (size_t)col * Npad + cc + CH + 8 * (lane >> 4);
+                    const bool ok = (tb0 + y) < T16;
+                    nbh[y] = ok ? *(const h8 *)(zh + off) : h8{};
+                    nbl[y] = ok ? *(const h8 *)(zl + off) : h8{};
+                }
+            }
+        }
+        if (any) {
+            h8 ah[4], al[4];
+#pragma unroll
+            for (int x = 0; x < 4; x++) {
+                const int r = qa * 64 + 16 * x + (lane & 15);
+                ah[x] = *(const h8 *)&As[cur][0][r][8 * (lane >> 4)];
+                al[x] = *(const h8 *)&As[cur][1][r][8 * (lane >> 4)];
+            }
+#pragma unroll
+            for (int x = 0; x < 4; x++)
+#pragma unroll
+                for (int y = 0; y < 4; y++)
+                    if (valid[x][y]) {
+                        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[x], bh[y], acc[x][y], 0, 0, 0);
+                        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[x], bl[y], acc[x][y], 0, 0, 0);
+                        acc[x][y] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[x], bh[y], acc[x][y], 0, 0, 0);
+                    }
+            if (more) {
+#pragma unroll
+                for (int y = 0; y < 4; y++) { bh[y] = nbh[y]; bl[y] = nbl[y]; }
+            }
+        }
+        if (more) storeA(cur ^ 1);
+        __syncthreads();
+        cur ^= 1;
+    }
+    // partial block (FP32), row-major 128x128 inside [s][j][block pair]
+    const int nbp = nbk * (nbk + 1) / 2;
+    float *pb = part + (((size_t)s * gridDim.y + j) * nbp + blockIdx.x) * (size_t)(BLK * BLK);
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = qa * 64 + 16 * x + 4 * (lane >> 4) + r;
+                const int col = qb * 64 + 16 * y + (lane & 15);
+                pb[row * BLK + col] = valid[x][y] ? acc[x][y][r] : 0.0f;
+            }
+        }
+}
+
+// Psi_j[r,c] = DeltaInv[r,c] + 2^e_j 2^-s_r 2^-s_c sum_s part[s][j][blk][r',c'] for r <= c,
+// written to both triangles.
+__global__ void k_gram_reduce(int P, int nbk, int nseg, int J, const float *__restrict__ part,
+                              const double *__restrict__ wun, const double *__restrict__ zun,
+                              const double *__restrict__ DeltaInv, double *__restrict__ Psi) {
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (idx >= (size_t)P * P) return;
+    const int r = (int)(idx % P), c = (int)(idx / P);
+    if (r > c) return;
+    const int bi = r / BLK, bj = c / BLK;
+    const int bp = bi * nbk - bi * (bi - 1) / 2 + (bj - bi);
+    const int nbp = nbk * (nbk + 1) / 2;
+    const size_t inblk = (size_t)(r % BLK) * BLK + (c % BLK);
+    double s = 0.0;
+    for (int g = 0; g < nseg; g++)
+        s += (double)part[(((size_t)g * J + j) * nbp + bp) * (size_t)(BLK * BLK) + inblk];
+    const double v = s * (wun[j] * zun[r] * zun[c]) + DeltaInv[r + (size_t)c * P];
+    double *Pj = Psi + (size_t)j * P * P;
+    Pj[r + (size_t)c * P] = v;
+    Pj[c + (size_t)r * P] = v;
+}
+
+}  // namespace
+
+// Scratch sizes for the mixed Gram (allocated on first use).
+static int ensure_mixed(gp_ctx *c, int nseg) {
+    const int T16 = (c->P + 15) / 16, P16 = 16 * T16;
+    const int nbk = (P16 + BLK - 1) / BLK;
+    const size_t nbp = (size_t)nbk * (nbk + 1) / 2;
+    const size_t zbytes = (size_t)c->Npad * P16 * sizeof(_Float16);
+    const size_t pbytes = (size_t)nseg * c->J * nbp * BLK * BLK * sizeof(float);
+    if (!c->Zh) {
+        GP_HIP(hipMalloc(&c->Zh, zbytes));
+        GP_HIP(hipMalloc(&c->Zl, zbytes));
+        GP_HIP(hipMalloc((void **)&c->w32, (size_t)c->Npad * c->J * sizeof(float)));
+        GP_HIP(hipMalloc((void **)&c->wsc, (size_t)c->J * sizeof(float)));
+        GP_HIP(hipMalloc((void **)&c->wun, (size_t)c->J * sizeof(double)));
+        GP_HIP(hipMalloc((void **)&c->zun, (size_t)P16 * sizeof(double)));
+    }
+    if (c->gram_part_bytes < pbytes) {
+        if (c->gram_part) (void)hipFree(c->gram_part);
+        c->gram_part = nullptr;
+        GP_HIP(hipMalloc((void **)&c->gram_part, pbytes));
+        c->gram_part_bytes = pbytes;
+    }
+    return 0;
+}
+
+int gpk_split_z(gp_ctx *c, const double *Zsrc) {
+    const int T16 = (c->P + 15) / 16, P16 = 16 * T16;
+    GP_TRY(ensure_mixed(c, c->gram_splits));
+    k_split_z<<<P16, 256, 0, c->stream>>>(c->Npad, c->P, Zsrc, (_Float16 *)c->Zh,
+                                          (_Float16 *)c->Zl, c->zun);
+    GP_HIP(hipGetLastError());
+    c->zsplit_valid = 1;
+    return 0;
+}
+
+int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const double *Zsrc,
+                   const double *Delta) {
+    const int T16 = (c->P + 15) / 16, P16 = 16 * T16;
+    const int nbk = (P16 + BLK - 1) / BLK;
+    const int nbp = nbk * (nbk + 1) / 2;
+    int nseg = c->gram_splits > 0 ? c->gram_splits : 1;
+    GP_TRY(ensure_mixed(c, nseg));
+    if (!c->zsplit_valid) GP_TRY(gpk_split_z(c, Zsrc));
+    // segment length: multiple of the chunk, covering Npad
+    int seg = (c->Npad + nseg - 1) / nseg;
+    seg = (seg + CH - 1) / CH * CH;
+    nseg = (c->Npad + seg - 1) / seg;
+    k_wprep<<<batch, 256, 0, c->stream>>>(c->N, c->Npad, W, c->Omega, c->w32, c->wsc, c->wun);
+    GP_HIP(hipGetLastError());
+    dim3 g(nbp, batch, nseg);
+    k_gram_mixed<<<g, 256, 0, c->stream>>>(c->Npad, P16, T16, nbk, seg, (const _Float16 *)c->Zh,
+                                            (const _Float16 *)c->Zl, c->w32, c->wsc, c->gram_part);
+    GP_HIP(hipGetLastError());
+    dim3 gr((unsigned)(((size_t)c->P * c->P + 255) / 256), batch);
+    k_gram_reduce<<<gr, 256, 0, c->stream>>>(c->P, nbk, nseg, batch, c->gram_part, c->wun,
+                                              c->zun, Delta, Psi);
+    GP_HIP(hipGetLastError());
+    return 0;
+}

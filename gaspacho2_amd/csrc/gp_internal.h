@@ -51,6 +51,15 @@ struct gp_ctx {
     double *titsias = nullptr; // N
     double *DeltaInv = nullptr;  // P x P reference layout (upper)
     double *DeltaChol = nullptr; // P x P factor of DeltaInv (lower+diag)
+    // whitened basis (mixed path): GP columns Knm L^-T with Kmm = L L^T
+    double *Linv = nullptr;      // M x M  L^{-1}
+    double *Zw = nullptr;        // Npad x P  [covariates | Knm L^-T]
+    double *DeltaW = nullptr;    // P x P  diag(1/delta_l): the whitened prior
+    double *DeltaWChol = nullptr;// P x P  diag(sqrt(1/delta_l))
+    double *Uw = nullptr;        // Npad x P  contraction in the whitened basis
+    double *tBw = nullptr;       // P x J  beta-tilde in the whitened basis
+    int psi_whitened = 0;        // basis of the current Psi factor
+    int zw_valid = 0;            // Zw / Linv match the current Kmm factor
 
     double *W = nullptr;       // Npad x J
     double *Oinvt = nullptr;   // N
@@ -78,9 +87,15 @@ struct gp_ctx {
     double *lkhd = nullptr;    // GP_LKHD_SLOTS
     int *info = nullptr;       // non-PD counters
     // mixed precision staging (fp16 hi/lo of the model matrix, tiled)
-    void *Zh = nullptr, *Zl = nullptr;
-    double *gram_part = nullptr;  // FP64 split-N partial Grams (mixed mode)
-    int gram_splits = 1;
+    void *Zh = nullptr, *Zl = nullptr;  // FP16 hi/lo of Z (Npad x P16)
+    float *w32 = nullptr;      // Npad x J  W/Omega in FP32
+    float *wsc = nullptr;      // J  2^-e_j weight scale
+    double *wun = nullptr;     // J  2^e_j
+    double *zun = nullptr;     // P16  2^-s_a column unscale of the FP16 model matrix
+    float *gram_part = nullptr;   // per-segment FP32 partial Gram blocks
+    size_t gram_part_bytes = 0;
+    int gram_splits = 16;      // cell segments of the mixed Gram
+    int zsplit_valid = 0;
 
     size_t tot_max = 0;        // N+p+(N+M+1)Q
     int timing = 0;
@@ -119,16 +134,18 @@ int gpk_expand_delta(gp_ctx *c);              // DeltaInv + DeltaChol copy
 int gpk_oinvt(gp_ctx *c);
 int gpk_titsias(gp_ctx *c);                   // A = Knm Kinv (rocBLAS) + titsias
 int gpk_w_nu(gp_ctx *c);                      // IRLS W / Nu from ZB = Z*Beta
-int gpk_r_and_sums(gp_ctx *c);                // R = W nu / Omega, gq, gs
+int gpk_r_and_sums(gp_ctx *c, const double *Zsrc);  // R = W nu / Omega, gq, gs, b = Zsrc^T R
 int gpk_init_y0(gp_ctx *c);                   // R = log((y+0.05)/ls) dense
 int gpk_beta_phi(gp_ctx *c, int jtr);
 int gpk_lkhd_pois(gp_ctx *c);
-int gpk_lkhd_genes_with(gp_ctx *c, const double *tb);  // tb = Psi_j^{-1} b_j
+int gpk_lkhd_genes_with(gp_ctx *c, const double *tb, const double *Dchol);  // tb = Psi_j^{-1} b_j
 int gpk_collect(gp_ctx *c, int Nterm, int slot, double *obj);
-int gpk_grad_delta(gp_ctx *c);
+int gpk_grad_delta(gp_ctx *c, int whitened);
+int gpk_whiten(gp_ctx *c);                    // Linv, Zw from the Kmm factor
+int gpk_unwhiten_grad(gp_ctx *c);             // tBeta, U from tBw, Uw
 int gpk_tn(gp_ctx *c);
 int gpk_cell_grad(gp_ctx *c);
-int gpk_c2(gp_ctx *c);
+int gpk_c2(gp_ctx *c, int whitened);
 int gpk_grad_tau_rho(gp_ctx *c);
 int gpk_lbfgs(gp_ctx *c, int itr, int rhoflag);
 int gpk_downdate(gp_ctx *c, int itr, double r, int rhoflag);
@@ -146,15 +163,25 @@ int gpk_cholsl_batched(gp_ctx *c, const double *L, int n, size_t lstride,
 // S_j = A_j^{-1} (full symmetric) from the factors.
 int gpk_chol_inverse_batched(gp_ctx *c, const double *L, int n, int batch,
                              size_t stride, double *S);
+// V = L^{-1} (lower triangular) of one factored n x n matrix.
+int gpk_tri_inverse(gp_ctx *c, const double *L, int n, double *V);
 
 // ---------------- gp_gram.hip ----------------
 // Psi_j = DeltaInv + Z^T diag(W_j / Omega) Z, full symmetric, all genes.
 // W == nullptr: unweighted single Gram (initPsi).
 int gpk_gram(gp_ctx *c, const double *W, double *Psi, int batch);
+// the same from an explicit model matrix and prior (whitened basis)
+int gpk_gram_src(gp_ctx *c, const double *W, double *Psi, int batch, const double *Zsrc,
+                 const double *Delta);
+
+// ---------------- gp_gram_mixed.hip ----------------
+int gpk_split_z(gp_ctx *c, const double *Zsrc);
+int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const double *Zsrc,
+                   const double *Delta);
 
 // ---------------- gp_gradcontract.hip ----------------
-// U = sum_j diag(W_j/Omega) Z Sinv_j  (Npad x P)
-int gpk_gradcontract(gp_ctx *c);
+// U = sum_j diag(W_j/Omega) Zsrc Sinv_j  (Npad x P)
+int gpk_gradcontract(gp_ctx *c, const double *Zsrc, double *Uout);
 
 // ---------------- timing ----------------
 void gp_timer_begin(gp_ctx *c, gp_timer &t);

@@ -105,7 +105,8 @@ __global__ void k_kmm(int M, int Q, const double *T, const double *rho, int ard,
 // util_cuda.cu:249-269 expandDelta_k; DeltaChol gets the full symmetric copy.
 __global__ void k_expand_delta(int P, int p, int M, int g0, const int *nh,
                                const double *delta, const double *K,
-                               double *DeltaInv, double *DeltaChol) {
+                               double *DeltaInv, double *DeltaChol, double *DeltaW,
+                               double *DeltaWChol) {
     const size_t idx = (size_t)blockIdx.x * NT + threadIdx.x;
     if (idx >= (size_t)P * P) return;
     const int r = (int)(idx % P), c = (int)(idx / P);
@@ -120,6 +121,17 @@ __global__ void k_expand_delta(int P, int p, int M, int g0, const int *nh,
     }
     DeltaInv[idx] = (r <= c) ? sym : 0.0;
     DeltaChol[idx] = sym;
+    if (DeltaW) {
+        // whitened prior: diag(1/delta_l) (the GP block K/delta becomes I/delta)
+        double dw = 0.0;
+        if (r == c) {
+            int l = 0;
+            while (l < p - 1 && !(nh[l] <= r && r < nh[l + 1])) l++;
+            dw = 1.0 / delta[l];
+        }
+        DeltaW[idx] = dw;
+        DeltaWChol[idx] = (r == c) ? sqrt(dw) : 0.0;
+    }
 }
 
 // ---------------------------------------------------------------- Oinvt
@@ -321,9 +333,12 @@ __global__ void __launch_bounds__(NT) k_collect_final(int J, const double *lkhds
 
 // ---------------------------------------------------------------- getGradLogDelta
 // ggplvm.cu:210-253 per (gene, column) with the explicit inverse S_j.
+// whitened: Sinv and tBeta are in the basis [covariates | Knm L^-T]; then the GP
+// trace term tr(Psi^-1_GP K) = tr(Sinv_GP) and tb_GP^T K tb_GP = |tb_w,GP|^2, so
+// each GP column carries its own diagonal entry and the column sums agree.
 __global__ void __launch_bounds__(NT) k_grad_delta_gene(int P, int p, int M, int g0,
         const int *nh, const double *delta, const double *Ksym, const double *Sinv,
-        const double *tBeta, const double *phi, double *gdt) {
+        const double *tBeta, const double *phi, double *gdt, int whitened) {
     __shared__ double sh[4];
     __shared__ double tb[2048];
     const int j = blockIdx.x;
@@ -336,7 +351,9 @@ __global__ void __launch_bounds__(NT) k_grad_delta_gene(int P, int p, int M, int
     double qf = 0.0;
     for (int a = threadIdx.x; a < M; a += NT) {
         double t = 0.0;
-        for (int b = 0; b < M; b++) t += Ksym[a + (size_t)b * M] * tb[g0 + b];
+        if (whitened) t = tb[g0 + a];
+        else
+            for (int b = 0; b < M; b++) t += Ksym[a + (size_t)b * M] * tb[g0 + b];
         qf += tb[g0 + a] * t;
     }
     qf = block_sum256(qf, sh);
@@ -348,7 +365,7 @@ __global__ void __launch_bounds__(NT) k_grad_delta_gene(int P, int p, int M, int
         while (l < p - 1 && !(nh[l] <= k && k < nh[l + 1])) l++;
         const double dk = delta[l];
         double psi;
-        if (k < g0) {
+        if (k < g0 || whitened) {
             psi = S[k + (size_t)k * P];
         } else {
             const int kk = k - g0;
@@ -692,7 +709,8 @@ int gpk_kuf(gp_ctx *c) {
 
 int gpk_expand_delta(gp_ctx *c) {
     k_expand_delta<<<nblk((size_t)c->P * c->P), NT, 0, c->stream>>>(
-        c->P, c->p, c->M, c->g0, c->d_nh, c->delta, c->K, c->DeltaInv, c->DeltaChol);
+        c->P, c->p, c->M, c->g0, c->d_nh, c->delta, c->K, c->DeltaInv, c->DeltaChol, c->DeltaW,
+        c->DeltaWChol);
     LAUNCH_CHECK();
     return 0;
 }
@@ -730,13 +748,13 @@ int gpk_w_nu(gp_ctx *c) {
     return 0;
 }
 
-int gpk_r_and_sums(gp_ctx *c) {
+int gpk_r_and_sums(gp_ctx *c, const double *Zsrc) {
     k_r_sums<<<c->J, NT, 0, c->stream>>>(c->N, c->Npad, c->W, c->Nu, c->Omega, c->R, c->gq, c->gs);
     LAUNCH_CHECK();
     // b_j = Z^T R_j  (util_cuda.cu:462-487 getZtOinvjnj for every gene)
     const double one = 1.0, zero = 0.0;
     GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->P,
-                          c->J, c->Npad, &one, c->Z, c->Npad, c->R, c->Npad, &zero, c->bvec, c->P));
+                          c->J, c->Npad, &one, Zsrc, c->Npad, c->R, c->Npad, &zero, c->bvec, c->P));
     return 0;
 }
 
@@ -770,8 +788,8 @@ int gpk_lkhd_pois(gp_ctx *c) {
     return 0;
 }
 
-int gpk_lkhd_genes_with(gp_ctx *c, const double *tb) {
-    k_lkhd_genes<<<c->J, NT, 0, c->stream>>>(c->N, c->P, c->Psi, c->DeltaChol, c->bvec, tb,
+int gpk_lkhd_genes_with(gp_ctx *c, const double *tb, const double *Dchol) {
+    k_lkhd_genes<<<c->J, NT, 0, c->stream>>>(c->N, c->P, c->Psi, Dchol, c->bvec, tb,
                                              c->gq, c->gs, c->phi, c->lkhds);
     LAUNCH_CHECK();
     return 0;
@@ -797,11 +815,11 @@ int gpk_collect(gp_ctx *c, int Nterm, int slot, double *obj) {
     return 0;
 }
 
-int gpk_grad_delta(gp_ctx *c) {
+int gpk_grad_delta(gp_ctx *c, int whitened) {
     if (c->P > 2048) { gp_set_error("P > 2048 unsupported"); return 1; }
     k_grad_delta_gene<<<c->J, NT, 0, c->stream>>>(c->P, c->p, c->M, c->g0, c->d_nh, c->delta,
-                                                  c->Ksym, c->Sinv, c->tBeta, c->phi,
-                                                  c->gradDeltaTmp);
+                                                  c->Ksym, c->Sinv, whitened ? c->tBw : c->tBeta,
+                                                  c->phi, c->gradDeltaTmp, whitened);
     LAUNCH_CHECK();
     k_grad_delta_collect<<<c->p, NT, 0, c->stream>>>(c->N, c->J, c->P, c->p, c->d_nh,
                                                      c->gradDeltaTmp, c->Oinvt, c->titsias,
@@ -837,11 +855,21 @@ int gpk_cell_grad(gp_ctx *c) {
     return 0;
 }
 
-int gpk_c2(gp_ctx *c) {
+int gpk_c2(gp_ctx *c, int whitened) {
     double dt;
     GP_HIP(hipMemcpyAsync(&dt, c->delta + c->p - 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     GP_HIP(hipStreamSynchronize(c->stream));
     const double one = 1.0, zero = 0.0;
+    k_sum_sinv<<<nblk((size_t)c->M * c->M), NT, 0, c->stream>>>(c->J, c->P, c->M, c->g0, c->Sinv,
+                                                                 c->S2);
+    LAUNCH_CHECK();
+    if (whitened) {
+        // sum_j Psi_j^{-1}[GP,GP] = L^-T (sum_j Sw_j[GP,GP]) L^-1  (G2 as scratch)
+        GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, c->M, c->M,
+                              c->M, &one, c->S2, c->M, c->Linv, c->M, &zero, c->G2, c->M));
+        GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->M,
+                              c->M, c->M, &one, c->Linv, c->M, c->G2, c->M, &zero, c->S2, c->M));
+    }
     k_scale_rows<<<nblk((size_t)c->Npad * c->M), NT, 0, c->stream>>>(c->N, c->Npad, c->M, c->A,
                                                                       c->Oinvt, c->AO);
     LAUNCH_CHECK();
@@ -853,9 +881,6 @@ int gpk_c2(gp_ctx *c) {
     GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, c->M, c->M,
                           c->J, &one, c->tBphi + c->g0, c->P, c->tBeta + c->g0, c->P, &zero, c->B2,
                           c->M));
-    k_sum_sinv<<<nblk((size_t)c->M * c->M), NT, 0, c->stream>>>(c->J, c->P, c->M, c->g0, c->Sinv,
-                                                                 c->S2);
-    LAUNCH_CHECK();
     k_c2<<<nblk((size_t)c->M * c->M), NT, 0, c->stream>>>(c->M, c->J, 0.1, dt, c->G2, c->B2, c->S2,
                                                           c->Kinv, c->K, c->C2);
     LAUNCH_CHECK();
@@ -946,5 +971,36 @@ int gpk_beta_from_tilde(gp_ctx *c) {
     const size_t n = (size_t)c->P * c->J;
     k_add_zeta<<<nblk(n), NT, 0, c->stream>>>(n, c->P, c->tBeta, c->zeta, c->Beta);
     LAUNCH_CHECK();
+    return 0;
+}
+
+// Whitened basis for the mixed path: L^{-1} of the Kmm factor and the model
+// matrix with its GP block replaced by Knm L^-T (rows of norm <= 1).
+int gpk_whiten(gp_ctx *c) {
+    const double one = 1.0, zero = 0.0;
+    GP_TRY(gpk_tri_inverse(c, c->Kchol, c->M, c->Linv));
+    GP_HIP(hipMemcpyAsync(c->Zw, c->Z, (size_t)c->Npad * c->g0 * sizeof(double),
+                          hipMemcpyDeviceToDevice, c->stream));
+    const double *Knm = c->Z + (size_t)c->g0 * c->Npad;
+    GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, c->Npad,
+                          c->M, c->M, &one, Knm, c->Npad, c->Linv, c->M, &zero,
+                          c->Zw + (size_t)c->g0 * c->Npad, c->Npad));
+    return 0;
+}
+
+// Back to the reference basis: tBeta = T^-T tBw and U = Uw T^-1 with
+// T = blockdiag(I, L):  GP rows of tBeta = L^-T tBw_GP, GP columns of U = Uw_GP L^-1.
+int gpk_unwhiten_grad(gp_ctx *c) {
+    const double one = 1.0, zero = 0.0;
+    GP_HIP(hipMemcpy2DAsync(c->tBeta, c->P * sizeof(double), c->tBw, c->P * sizeof(double),
+                            c->g0 * sizeof(double), c->J, hipMemcpyDeviceToDevice, c->stream));
+    GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->M, c->J,
+                          c->M, &one, c->Linv, c->M, c->tBw + c->g0, c->P, &zero,
+                          c->tBeta + c->g0, c->P));
+    GP_HIP(hipMemcpyAsync(c->U, c->Uw, (size_t)c->Npad * c->g0 * sizeof(double),
+                          hipMemcpyDeviceToDevice, c->stream));
+    GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, c->Npad, c->M,
+                          c->M, &one, c->Uw + (size_t)c->g0 * c->Npad, c->Npad, c->Linv, c->M,
+                          &zero, c->U + (size_t)c->g0 * c->Npad, c->Npad));
     return 0;
 }

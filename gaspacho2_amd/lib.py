@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(_HERE, "libgaspacho2.so")
 
 PREC_FP64 = 0
 PREC_MIXED = 1
+PREC_FP64_WHITENED = 2
 
 _lib = None
 

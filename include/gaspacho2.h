@@ -31,8 +31,12 @@ typedef struct gp_ctx gp_ctx;
 /* Precision of the weighted-Gram and gradient contractions. */
 enum {
     GP_PREC_FP64 = 0,    /* FP64 throughout (the reference's arithmetic) */
-    GP_PREC_MIXED = 1    /* FP16 hi/lo split MFMA Gram, FP32 accumulate,
-                            FP64 cross-chunk sums, FP64 Cholesky/log-det */
+    GP_PREC_MIXED = 1,   /* ELBO/gradient in the whitened basis
+                            [covariates | Knm L^-T] (Kmm = L L^T): FP16 hi/lo
+                            split MFMA Gram, FP32 accumulate, FP64 segment
+                            sums, FP64 Cholesky/log-det */
+    GP_PREC_FP64_WHITENED = 2  /* the whitened basis with FP64 arithmetic
+                                  (validates the basis change on its own) */
 };
 
 /* Library version (major*10000 + minor*100 + patch). */

@@ -167,3 +167,85 @@ def test_non_pd_factor_is_zero_filled(tiny_ds):
     Pg = g.get("Psi")[:J * P * P].reshape(J, P * P)
     assert not Pg[2].any()
     assert not np.isfinite(r)
+
+
+def test_mixed_precision_gram_error_bounds():
+    """FP16 hi/lo MFMA Gram (FP32 accumulate inside a cell segment, FP64 across
+    segments) against the FP64 Gram of the same whitened basis at the same W
+    (IRLS run in FP64) at the N=2,000 / G=200 / M=32 workload: entries within
+    3e-6 of the largest entry of their gene (FP32 accumulation dominates; the
+    hi/lo operand split alone gives ~2e-8), ELBO within 1e-7 of the oracle."""
+    from gaspacho2_amd import lib, synth
+    ds = synth.make_dataset(2000, 200, 5, 32, seed=0, n_numeric=2)
+    o, g = _pair(ds, precision=lib.PREC_FP64)
+    o.init_fit()
+    g.init_fit()
+    for jtr in range(2):
+        o.irls_step(jtr)
+        g.irls_step(jtr)
+    o.ql_start()
+    g.ql_start()
+    J, P = ds.J, ds.P
+    iu = np.triu_indices(P, 1)
+    g.set_precision(lib.PREC_FP64_WHITENED)
+    g.elbo(1)
+    Pw = g.get("Psi")[:J * P * P].reshape(J, P, P)[:, iu[1], iu[0]]
+    g.set_precision(lib.PREC_MIXED)
+    f = g.elbo(1)
+    e = o.elbo(1)
+    Pm = g.get("Psi")[:J * P * P].reshape(J, P, P)[:, iu[1], iu[0]]
+    err = np.max(np.abs(Pm - Pw), axis=1) / np.max(np.abs(Pw), axis=1)
+    print(f"mixed Gram max rel err per gene: {err.max():.3e}; ELBO rel err {abs(e - f) / abs(e):.3e}")
+    assert err.max() < 3e-6
+    assert abs(e - f) <= 1e-7 * abs(e), (e, f, abs(e - f) / abs(e))
+
+
+def test_whitened_basis_fp64_matches_oracle():
+    """The whitened basis [covariates | Knm L^-T] with FP64 arithmetic gives the
+    reference ELBO and gradient (the basis change is exact algebra: the log|L|
+    terms cancel and Psi^-1, beta-tilde, U map back through L^-1)."""
+    from gaspacho2_amd import lib
+    from gaspacho2_amd import synth
+    ds = synth.make_dataset(600, 30, 4, 16, seed=5, n_numeric=2, n_levels=2)
+    o, g = _pair(ds)
+    o.init_fit()
+    g.init_fit()
+    for jtr in range(3):
+        o.irls_step(jtr)
+        g.irls_step(jtr)
+    o.ql_start()
+    g.ql_start()
+    g.set_precision(lib.PREC_FP64_WHITENED)
+    a, b = o.elbo(1), g.elbo(1)
+    assert abs(a - b) <= 1e-10 * abs(a), (a, b)
+    o.grad()
+    g.grad()
+    go, gg = o.get("grad"), g.get("grad")
+    N, p = ds.N, len(ds.nh) - 1
+    assert rel(gg, go) < 1e-8
+    assert rel(gg[N:N + p], go[N:N + p]) < 1e-8
+    _cmp(o, g, "tBeta", tol=1e-8)
+    _cmp(o, g, "C2", tol=1e-8)
+
+
+def test_mixed_precision_gradient_close_to_fp64():
+    """Mixed path (whitened basis, FP16 hi/lo Gram) vs the FP64 oracle: ELBO and
+    the whole gradient vector at the N=2,000 / G=200 / M=32 workload."""
+    from gaspacho2_amd import lib, synth
+    ds = synth.make_dataset(2000, 200, 5, 32, seed=0)
+    o, g = _pair(ds)
+    o.init_fit()
+    g.init_fit()
+    for jtr in range(2):
+        o.irls_step(jtr)
+        g.irls_step(jtr)
+    o.ql_start()
+    g.ql_start()
+    g.set_precision(lib.PREC_MIXED)
+    a, b = o.elbo(1), g.elbo(1)
+    o.grad()
+    g.grad()
+    gr = rel(g.get("grad"), o.get("grad"))
+    print(f"mixed: ELBO rel err {abs(a - b) / abs(a):.3e}, gradient rel err {gr:.3e}")
+    assert abs(a - b) <= 1e-7 * abs(a)
+    assert gr < 1e-4
