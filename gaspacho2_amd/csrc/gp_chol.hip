@@ -1,5 +1,8 @@
 // gp_chol.hip -- batched FP64 Cholesky, triangular solve and inverse.
 //
+// Inverse (solvePsi, ggplvm.cu:46-53): V = L^{-1} column by column, then the
+// symmetric product S = V^T V -- both hand-written, one workgroup per gene.
+//
 // Cholesky: left-looking, blocked by 16-column panels held in LDS; one
 // workgroup per matrix.  Input is the lower triangle (the Gram writes both
 // triangles), L overwrites lower+diag and the strict upper triangle keeps the
@@ -132,6 +135,85 @@ __global__ void k_identity(double *S, int n, int batch, size_t stride) {
     S[j * stride + e] = ((e % n) == (e / n)) ? 1.0 : 0.0;
 }
 
+
+// V_j = L_j^{-1} (lower triangle of V; the strict upper triangle is not
+// written).  Columns are produced right to left: with L = [[l, 0], [c, L2]],
+// V[j+1:, j] = -V2 c / l where V2 = V[j+1:, j+1:] is already known.
+__global__ void __launch_bounds__(CT) k_trinv(const double *__restrict__ Lall, int n,
+                                              size_t stride, double *__restrict__ Vall) {
+    extern __shared__ double lcol[];
+    const double *L = Lall + (size_t)blockIdx.x * stride;
+    double *V = Vall + (size_t)blockIdx.x * stride;
+    const int tid = threadIdx.x;
+    for (int j = n - 1; j >= 0; j--) {
+        for (int k = j + 1 + tid; k < n; k += CT) lcol[k] = L[k + (size_t)j * n];
+        __syncthreads();
+        const double vjj = 1.0 / L[j + (size_t)j * n];
+        for (int i = j + 1 + tid; i < n; i += CT) {
+            double s = 0.0;
+            for (int k = j + 1; k <= i; k++) s += V[i + (size_t)k * n] * lcol[k];
+            V[i + (size_t)j * n] = -s * vjj;
+        }
+        if (tid == 0) V[j + (size_t)j * n] = vjj;
+        __syncthreads();
+    }
+}
+
+// S_j = V_j^T V_j (full symmetric) from the lower-triangular V_j: 64x64 tiles
+// of the upper triangle, 32-row chunks of V staged in LDS, 4x4 per thread.
+// Entry (a, b), a <= b, sums rows k >= b only (V[k, b] = 0 above the diagonal).
+__global__ void __launch_bounds__(CT) k_vtv(const double *__restrict__ Vall, int n, size_t stride,
+                                            int ntile, double *__restrict__ Sall) {
+    __shared__ double As[32][65];
+    __shared__ double Bs[32][65];
+    int ta = 0, t = blockIdx.x;
+    while (t >= ntile - ta) { t -= ntile - ta; ta++; }
+    const int tb = ta + t;
+    const int a0 = ta * 64, b0 = tb * 64;
+    const double *V = Vall + (size_t)blockIdx.y * stride;
+    double *S = Sall + (size_t)blockIdx.y * stride;
+    const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
+    double acc[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[r][q] = 0.0;
+    for (int k0 = b0; k0 < n; k0 += 32) {
+        __syncthreads();
+        const int row = tid % 32, cg = tid / 32;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int col = cg * 8 + u, k = k0 + row;
+            const int ga = a0 + col, gb = b0 + col;
+            As[row][col] = (k < n && ga < n && k >= ga) ? V[k + (size_t)ga * n] : 0.0;
+            Bs[row][col] = (k < n && gb < n && k >= gb) ? V[k + (size_t)gb * n] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int kk = 0; kk < 32; kk++) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int r = 0; r < 4; r++) av[r] = As[kk][ty * 4 + r];
+#pragma unroll
+            for (int q = 0; q < 4; q++) bv[q] = Bs[kk][tx * 4 + q];
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+#pragma unroll
+                for (int q = 0; q < 4; q++) acc[r][q] += av[r] * bv[q];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int ga = a0 + ty * 4 + r, gb = b0 + tx * 4 + q;
+            if (ga < n && gb < n && ga <= gb) {
+                S[ga + (size_t)gb * n] = acc[r][q];
+                S[gb + (size_t)ga * n] = acc[r][q];
+            }
+        }
+}
+
 }  // namespace
 
 int gpk_chol_batched(gp_ctx *c, double *A, int n, int batch, size_t stride, int *info) {
@@ -153,18 +235,20 @@ int gpk_cholsl_batched(gp_ctx *c, const double *L, int n, size_t lstride, const 
 
 int gpk_chol_inverse_batched(gp_ctx *c, const double *L, int n, int batch, size_t stride,
                              double *S) {
-    const size_t tot = (size_t)n * n * batch;
-    k_identity<<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(S, n, batch, stride);
+    // V = L^{-1} into the scratch buffer, then S = V^T V
+    if ((size_t)batch * (stride ? stride : (size_t)n * n) > c->vtmp_len) {
+        if (c->Vtmp) (void)hipFree(c->Vtmp);
+        c->Vtmp = nullptr;
+        c->vtmp_len = (size_t)batch * (stride ? stride : (size_t)n * n);
+        GP_HIP(hipMalloc((void **)&c->Vtmp, c->vtmp_len * sizeof(double)));
+    }
+    const size_t st = stride ? stride : (size_t)n * n;
+    k_trinv<<<batch, CT, (size_t)n * sizeof(double), c->stream>>>(L, n, st, c->Vtmp);
     GP_HIP(hipGetLastError());
-    const double one = 1.0;
-    GP_BLAS(rocblas_dtrsm_strided_batched(c->blas, rocblas_side_left, rocblas_fill_lower,
-                                          rocblas_operation_none, rocblas_diagonal_non_unit, n, n,
-                                          &one, L, n, (rocblas_stride)stride, S, n,
-                                          (rocblas_stride)stride, batch));
-    GP_BLAS(rocblas_dtrsm_strided_batched(c->blas, rocblas_side_left, rocblas_fill_lower,
-                                          rocblas_operation_transpose, rocblas_diagonal_non_unit, n,
-                                          n, &one, L, n, (rocblas_stride)stride, S, n,
-                                          (rocblas_stride)stride, batch));
+    const int nt = (n + 63) / 64;
+    dim3 g(nt * (nt + 1) / 2, batch);
+    k_vtv<<<g, CT, 0, c->stream>>>(c->Vtmp, n, st, nt, S);
+    GP_HIP(hipGetLastError());
     return 0;
 }
 

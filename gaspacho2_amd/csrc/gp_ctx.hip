@@ -55,7 +55,8 @@ static void free_all(gp_ctx *c) {
                     c->SS, c->XX, c->lbq, c->gradDeltaTmp, c->gradLogRhoTmp, c->C1, c->C2, c->U,
                     c->ZB, c->R, c->C1a, c->AO, c->G2, c->B2, c->S2, c->Ecell, c->red, c->scal,
                     c->lkhd, c->info, c->Zh, c->Zl, c->gram_part, c->pwork, c->w32, c->wsc,
-                    c->wun, c->zun, c->Linv, c->Zw, c->DeltaW, c->DeltaWChol, c->Uw, c->tBw};
+                    c->wun, c->zun, c->Linv, c->Zw, c->DeltaW, c->DeltaWChol, c->Uw, c->tBw,
+                    c->Vtmp};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     gp_timer *ts[] = {&c->t_gram, &c->t_chol, &c->t_gc, &c->t_kuf, &c->t_elbo, &c->t_grad};

@@ -60,6 +60,8 @@ struct gp_ctx {
     double *tBw = nullptr;       // P x J  beta-tilde in the whitened basis
     int psi_whitened = 0;        // basis of the current Psi factor
     int zw_valid = 0;            // Zw / Linv match the current Kmm factor
+    double *Vtmp = nullptr;      // J x P x P scratch: L^{-1} of every factor
+    size_t vtmp_len = 0;
 
     double *W = nullptr;       // Npad x J
     double *Oinvt = nullptr;   // N
