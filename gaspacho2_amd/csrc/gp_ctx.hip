@@ -56,7 +56,8 @@ static void free_all(gp_ctx *c) {
                     c->ZB, c->R, c->C1a, c->AO, c->G2, c->B2, c->S2, c->Ecell, c->red, c->scal,
                     c->lkhd, c->info, c->Zh, c->Zl, c->gram_part, c->pwork, c->w32, c->wsc,
                     c->wun, c->zun, c->Linv, c->Zw, c->DeltaW, c->DeltaWChol, c->Uw, c->tBw,
-                    c->Vtmp};
+                    c->Vtmp, c->Sh, c->Sl, c->gc_perm, c->gc_aexp, c->gc_sexp, c->gc_bexp,
+                    c->gc_gun, c->gc_part, c->gc_zexp};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     gp_timer *ts[] = {&c->t_gram, &c->t_chol, &c->t_gc, &c->t_kuf, &c->t_elbo, &c->t_grad};
@@ -219,6 +220,34 @@ extern "C" int gp_set_precision(gp_ctx *c, int precision) {
     return 0;
 }
 
+extern "C" int gp_set_option(gp_ctx *c, const char *name, int value) {
+    if (!c || !name) { gp_set_error("NULL argument"); return 1; }
+    const std::string n(name);
+    if (n == "gram_splits") {
+        if (value < 1 || value > 256) { gp_set_error("gram_splits must be in [1, 256]"); return 1; }
+        c->gram_splits = value;
+    } else if (n == "gc_groups") {
+        if (value < 1 || value > 4096) { gp_set_error("gc_groups must be in [1, 4096]"); return 1; }
+        c->gc_groups = value;
+    } else if (n == "gc_mixed") {
+        c->gc_mixed = value ? 1 : 0;
+    } else {
+        gp_set_error("unknown option " + n);
+        return 1;
+    }
+    return 0;
+}
+
+extern "C" int gp_get_option(gp_ctx *c, const char *name, int *value) {
+    if (!c || !name || !value) { gp_set_error("NULL argument"); return 1; }
+    const std::string n(name);
+    if (n == "gram_splits") *value = c->gram_splits;
+    else if (n == "gc_groups") *value = c->gc_groups;
+    else if (n == "gc_mixed") *value = c->gc_mixed;
+    else { gp_set_error("unknown option " + n); return 1; }
+    return 0;
+}
+
 // ------------------------------------------------------------------ buffers
 namespace {
 enum Kind { DIRECT, ROWS_N, TRANS_A, W_OINVT, PSI2, DELTA_CHOL };
@@ -264,6 +293,7 @@ BufDesc describe(gp_ctx *c, const char *nm) {
     else if (is("C1")) { d.dev = c->C1; d.len = N * M; d.kind = ROWS_N; d.cols = (int)M; }
     else if (is("C2")) { d.dev = c->C2; d.len = M * M; }
     else if (is("U")) { d.dev = c->U; d.len = N * P; d.kind = ROWS_N; d.cols = (int)P; d.writable = false; }
+    else if (is("Uw")) { d.dev = c->Uw; d.len = N * P; d.kind = ROWS_N; d.cols = (int)P; d.writable = false; }
     else if (is("lkhd")) { d.dev = c->lkhd; d.len = GP_LKHD_SLOTS; }
     return d;
 }
@@ -487,7 +517,10 @@ extern "C" int gp_grad(gp_ctx *c) {
     GP_TRY(gpk_chol_inverse_batched(c, c->Psi, c->P, c->J, (size_t)c->P * c->P, c->Sinv));
     GP_TRY(gpk_grad_delta(c, w));
     gp_timer_begin(c, c->t_gc);
-    GP_TRY(gpk_gradcontract(c, w ? c->Zw : c->Z, w ? c->Uw : c->U));
+    if (w && c->precision == GP_PREC_MIXED && c->gc_mixed)
+        GP_TRY(gpk_gradcontract_mixed(c));
+    else
+        GP_TRY(gpk_gradcontract(c, w ? c->Zw : c->Z, w ? c->Uw : c->U));
     gp_timer_end(c, c->t_gc);
     if (w) GP_TRY(gpk_unwhiten_grad(c));
     GP_TRY(gpk_tn(c));

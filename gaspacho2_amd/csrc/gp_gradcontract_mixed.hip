@@ -1,0 +1,336 @@
+// gp_gradcontract_mixed.hip -- the gradient contraction of the mixed path,
+//   Uw[i,:] = sum_j o_ij Sw_j zw_i          (Sw_j = Psi~_j^{-1}, whitened basis)
+// on FP16 MFMA (v_mfma_f32_16x16x32_f16) with hi/lo operands.
+//
+// Scaling.  Per column k of Zw: 2^zc_k >= max_i |zw_ik|.  Per gene j: 2^e_j >=
+// max_i o_ij (from the weight prep) and 2^s_j >= max_{k,c} |Sw_j[k,c]| 2^zc_k.
+// kappa_j = e_j + s_j bounds the products; genes are sorted by kappa and cut
+// into groups, and each group g gets one exponent C_g = 26 - max_{j in g} kappa_j:
+//   A_ik = o_ij zw_ik 2^(12 - e_j - zc_k)        (|A| <= 2^12)
+//   B_kc = Sw_j[k,c] 2^(e_j - 12 + C_g + zc_k)    (|B| <= 2^14)
+// so sum_k A_ik B_kc = o_ij (zw_i^T Sw_j)_c 2^C_g for every gene of the group,
+// the group's FP32 partial is one sum, and k_gc_reduce adds the groups in FP64
+// times 2^-C_g.
+//
+// Work.  Workgroup = 4 waves = 128 cells x all P16 columns for one gene group.
+// Wave w owns cells i0 + 32w .. +31 (two 16-cell tiles) and keeps their zw
+// values (FP32, k = 32 ks + 8(l/16) + e) in registers.  Sw_j^hi/lo are pre-split
+// per gene into [col][k] (k contiguous, 288 k per column) and streamed through
+// LDS one 32-k step at a time (double buffered); B fragment lane l =
+// Sw[k = 8(l/16)..+7][col = l%16].
+// D lane l, r -> U[cell 16t + 4(l/16) + r][16 ct + l%16].
+#include "gp_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+namespace {
+
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+constexpr int KP = 288;      // padded k extent of the pre-split Sw (multiple of 32)
+constexpr int KCH = 32;      // k rows per LDS chunk (one MFMA k step)
+constexpr int LDK = 40;      // LDS column stride in halves (32 + 8: conflict-free b128 reads)
+constexpr int MAXT16 = 17;   // column tiles handled (P16 <= 272)
+
+// per column k of Zw: exponent zc_k with 2^zc_k >= max_i |Zw_ik| (0 beyond P)
+__global__ void __launch_bounds__(256) k_zexp(int Npad, int P, const double *__restrict__ Zw,
+                                              int *__restrict__ zexp) {
+    __shared__ double sh[4];
+    const int k = blockIdx.x;
+    double mx = 0.0;
+    if (k < P)
+        for (int i = threadIdx.x; i < Npad; i += 256) mx = fmax(mx, fabs(Zw[i + (size_t)k * Npad]));
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double m = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+        int e = 0;
+        if (m > 0.0 && isfinite(m)) (void)frexp(m, &e);
+        zexp[k] = e;
+    }
+}
+
+// per gene: exponent s_j with 2^s_j >= max_{k,c} |Sw_j[k,c]| 2^zc_k
+__global__ void __launch_bounds__(256) k_smax(int P, const double *__restrict__ S,
+                                              const int *__restrict__ zexp, int *__restrict__ sexp) {
+    __shared__ double sh[4];
+    const double *Sj = S + (size_t)blockIdx.x * P * P;
+    double mx = 0.0;
+    for (size_t t = threadIdx.x; t < (size_t)P * P; t += 256)
+        mx = fmax(mx, ldexp(fabs(Sj[t]), zexp[t % P]));
+    for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double m = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+        int e = 0;
+        if (m > 0.0 && isfinite(m)) (void)frexp(m, &e);
+        sexp[blockIdx.x] = e;
+    }
+}
+
+// Sw_j[k,c] * 2^(e_j - 12 + C_g + zc_k) split into FP16 hi/lo, layout [col < P16][k < KP]
+__global__ void k_ssplit(int P, int P16, const double *__restrict__ S, const int *__restrict__ scale_exp,
+                         const int *__restrict__ zexp, _Float16 *__restrict__ sh,
+                         _Float16 *__restrict__ sl) {
+    const int j = blockIdx.y;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;   // over P16 * KP
+    if (idx >= (size_t)P16 * KP) return;
+    const int col = (int)(idx / KP), k = (int)(idx % KP);
+    const double *Sj = S + (size_t)j * P * P;
+    const float v = (col < P && k < P) ? (float)ldexp(Sj[k + (size_t)col * P], scale_exp[j] + zexp[k]) : 0.0f;
+    const _Float16 h = (_Float16)v;
+    const size_t o = (size_t)j * P16 * KP + idx;
+    sh[o] = h;
+    sl[o] = (_Float16)(v - (float)h);
+}
+
+template <int T16>
+__global__ void __launch_bounds__(256) k_gc_mixed(int N, int Npad, int P, int gsize, int J,
+        const double *__restrict__ Zw, const float *__restrict__ w32,
+        const int *__restrict__ perm, const int *__restrict__ aexp, const int *__restrict__ zexp,
+        const _Float16 *__restrict__ shi, const _Float16 *__restrict__ slo,
+        float *__restrict__ part) {
+    constexpr int P16 = 16 * T16;
+    constexpr int KS = (P16 + 31) / 32;          // 32-k steps per gene (<= KP/32)
+    extern __shared__ __attribute__((aligned(16))) _Float16 Bs[];   // [2 buf][2 hl][P16][LDK]
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int i0 = blockIdx.x * 128;
+    const int g = blockIdx.y;
+    const int j0 = g * gsize, j1 = min(J, j0 + gsize);
+    // zw values of this lane's two cells: z[t][ks][e], k = 32 ks + 8(l/16) + e
+    float z[2][KS][8];
+    int cell[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        cell[t] = i0 + 32 * w + 16 * t + (lane & 15);
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++)
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int k = 32 * ks + 8 * (lane >> 4) + e;
+                z[t][ks][e] = (k < P) ? (float)ldexp(Zw[cell[t] + (size_t)k * Npad], -zexp[k]) : 0.0f;
+            }
+    }
+    f4 acc[2][T16];
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int ct = 0; ct < T16; ct++) acc[t][ct] = f4{0.f, 0.f, 0.f, 0.f};
+
+    // one chunk = one 32-k step: 2 (hi/lo) x P16 cols x 4 groups of 8 k, 16 B each
+    constexpr int NITEM = 2 * P16 * 4;
+    constexpr int IPT = (NITEM + 255) / 256;
+    h8 pf[IPT];
+    auto fetch = [&](int j, int ks) {
+        const _Float16 *hj = shi + (size_t)j * P16 * KP, *lj = slo + (size_t)j * P16 * KP;
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            const int it = tid + 256 * r;
+            if (it < NITEM) {
+                const int hl = it / (P16 * 4), rem = it % (P16 * 4);
+                const int col = rem >> 2, kg = rem & 3;
+                pf[r] = *(const h8 *)((hl ? lj : hj) + (size_t)col * KP + ks * KCH + 8 * kg);
+            }
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            const int it = tid + 256 * r;
+            if (it < NITEM) {
+                const int hl = it / (P16 * 4), rem = it % (P16 * 4);
+                const int col = rem >> 2, kg = rem & 3;
+                *(h8 *)&Bs[((buf * 2 + hl) * P16 + col) * LDK + 8 * kg] = pf[r];
+            }
+        }
+    };
+    if (j0 < j1) {
+        fetch(perm[j0], 0);
+        store(0);
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int jj = j0; jj < j1; jj++) {
+        const int j = perm[jj];
+        const float sa = ldexpf(1.0f, aexp[j]);
+        float o[2];
+#pragma unroll
+        for (int t = 0; t < 2; t++) o[t] = (cell[t] < N) ? w32[cell[t] + (size_t)j * Npad] * sa : 0.0f;
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const bool more = (ks + 1 < KS) || (jj + 1 < j1);
+            if (more) {
+                if (ks + 1 < KS) fetch(j, ks + 1);
+                else fetch(perm[jj + 1], 0);
+            }
+            h8 ah[2], al[2];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    // v is materialised in FP32 before the split: a fused
+                    // multiply-convert (v_fma_mix) loses accuracy in the lo
+                    // part (measured: 3e-5 instead of 2e-7 relative error)
+                    float v = z[t][ks][e] * o[t];
+                    asm volatile("" : "+v"(v));
+                    const _Float16 hv = (_Float16)v;
+                    ah[t][e] = hv;
+                    al[t][e] = (_Float16)(v - (float)hv);
+                }
+            }
+            const _Float16 *bh = Bs + ((cur * 2 + 0) * P16 + (lane & 15)) * LDK + 8 * (lane >> 4);
+            const _Float16 *bl = bh + P16 * LDK;
+#pragma unroll
+            for (int ct = 0; ct < T16; ct++) {
+                const h8 vbh = *(const h8 *)(bh + 16 * ct * LDK);
+                const h8 vbl = *(const h8 *)(bl + 16 * ct * LDK);
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    acc[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], vbh, acc[t][ct], 0, 0, 0);
+                    acc[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], vbl, acc[t][ct], 0, 0, 0);
+                    acc[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[t], vbh, acc[t][ct], 0, 0, 0);
+                }
+            }
+            if (more) store(cur ^ 1);
+            __syncthreads();
+            cur ^= 1;
+        }
+    }
+    // partial of this group: part[g][col][cell] (FP32, ld = Npad)
+    float *pg = part + (size_t)g * P16 * Npad;
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int ct = 0; ct < T16; ct++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int row = i0 + 32 * w + 16 * t + 4 * (lane >> 4) + r;
+                const int col = 16 * ct + (lane & 15);
+                pg[(size_t)col * Npad + row] = acc[t][ct][r];
+            }
+}
+
+__global__ void k_gc_reduce(int Npad, int P, int P16, int ngroup, const float *__restrict__ part,
+                            const double *__restrict__ gun, double *__restrict__ Uw) {
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (size_t)Npad * P) return;
+    double s = 0.0;
+    for (int g = 0; g < ngroup; g++) s += (double)part[(size_t)g * P16 * Npad + idx] * gun[g];
+    Uw[idx] = s;
+}
+
+}  // namespace
+
+template <int T16>
+static int launch_gc_mixed(gp_ctx *c, int gsize, int ngroup) {
+    constexpr int P16 = 16 * T16;
+    const size_t shm = (size_t)2 * 2 * P16 * LDK * sizeof(_Float16);
+    if (shm > 65536)
+        GP_HIP(hipFuncSetAttribute((const void *)k_gc_mixed<T16>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    dim3 g(c->Npad / 128, ngroup);
+    k_gc_mixed<T16><<<g, 256, shm, c->stream>>>(c->N, c->Npad, c->P, gsize, c->J, c->Zw, c->w32,
+                                                 c->gc_perm, c->gc_aexp, c->gc_zexp,
+                                                 (const _Float16 *)c->Sh,
+                                                 (const _Float16 *)c->Sl, c->gc_part);
+    GP_HIP(hipGetLastError());
+    return 0;
+}
+
+int gpk_gradcontract_mixed(gp_ctx *c) {
+    const int T16 = (c->P + 15) / 16, P16 = 16 * T16;
+    if (T16 > MAXT16 || c->Npad % 128 != 0) return gpk_gradcontract(c, c->Zw, c->Uw);
+    const int J = c->J;
+    int ngroup = std::min(c->gc_groups, J);
+    const int gsize = (J + ngroup - 1) / ngroup;
+    ngroup = (J + gsize - 1) / gsize;
+    if (!c->Sh) {
+        GP_HIP(hipMalloc(&c->Sh, (size_t)J * P16 * KP * sizeof(_Float16)));
+        GP_HIP(hipMalloc(&c->Sl, (size_t)J * P16 * KP * sizeof(_Float16)));
+        GP_HIP(hipMalloc((void **)&c->gc_perm, (size_t)J * sizeof(int)));
+        GP_HIP(hipMalloc((void **)&c->gc_aexp, (size_t)J * sizeof(int)));
+        GP_HIP(hipMalloc((void **)&c->gc_sexp, (size_t)J * sizeof(int)));
+        GP_HIP(hipMalloc((void **)&c->gc_bexp, (size_t)J * sizeof(int)));
+        GP_HIP(hipMalloc((void **)&c->gc_gun, (size_t)J * sizeof(double)));
+        GP_HIP(hipMalloc((void **)&c->gc_zexp, (size_t)KP * sizeof(int)));
+    }
+    if (c->gc_part_len < (size_t)ngroup * P16 * c->Npad) {
+        if (c->gc_part) (void)hipFree(c->gc_part);
+        c->gc_part = nullptr;
+        c->gc_part_len = (size_t)ngroup * P16 * c->Npad;
+        GP_HIP(hipMalloc((void **)&c->gc_part, c->gc_part_len * sizeof(float)));
+    }
+    // per-gene exponents: e_j from the weight prep (wun = 2^e_j), s_j from max |Sw_j|
+    GP_TRY(gpk_wprep(c, c->W));
+    k_zexp<<<KP, 256, 0, c->stream>>>(c->Npad, c->P, c->Zw, c->gc_zexp);
+    GP_HIP(hipGetLastError());
+    k_smax<<<J, 256, 0, c->stream>>>(c->P, c->Sinv, c->gc_zexp, c->gc_sexp);
+    GP_HIP(hipGetLastError());
+    std::vector<double> wun(J);
+    std::vector<int> sexp(J);
+    GP_HIP(hipMemcpyAsync(wun.data(), c->wun, J * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GP_HIP(hipMemcpyAsync(sexp.data(), c->gc_sexp, J * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    GP_HIP(hipStreamSynchronize(c->stream));
+    std::vector<int> e(J), kappa(J), perm(J), aexp(J), bexp(J);
+    for (int j = 0; j < J; j++) {
+        int ej = 0;
+        (void)std::frexp(wun[j], &ej);   // wun = 2^e exactly -> frexp gives e+1
+        e[j] = ej - 1;
+        kappa[j] = e[j] + sexp[j];
+    }
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return kappa[a] < kappa[b]; });
+    std::vector<double> gun(ngroup);
+    for (int g = 0; g < ngroup; g++) {
+        int mk = -100000;
+        for (int t = g * gsize; t < std::min(J, (g + 1) * gsize); t++) mk = std::max(mk, kappa[perm[t]]);
+        const int Cg = 26 - mk;
+        gun[g] = std::ldexp(1.0, -Cg);
+        for (int t = g * gsize; t < std::min(J, (g + 1) * gsize); t++) {
+            const int j = perm[t];
+            aexp[j] = 12 - e[j];
+            bexp[j] = e[j] - 12 + Cg;
+        }
+    }
+    GP_HIP(hipMemcpyAsync(c->gc_perm, perm.data(), J * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    GP_HIP(hipMemcpyAsync(c->gc_aexp, aexp.data(), J * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    GP_HIP(hipMemcpyAsync(c->gc_bexp, bexp.data(), J * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    GP_HIP(hipMemcpyAsync(c->gc_gun, gun.data(), ngroup * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    dim3 gs((unsigned)(((size_t)P16 * KP + 255) / 256), J);
+    k_ssplit<<<gs, 256, 0, c->stream>>>(c->P, P16, c->Sinv, c->gc_bexp, c->gc_zexp,
+                                        (_Float16 *)c->Sh, (_Float16 *)c->Sl);
+    GP_HIP(hipGetLastError());
+    int r = 0;
+    switch (T16) {
+    case 1: r = launch_gc_mixed<1>(c, gsize, ngroup); break;
+    case 2: r = launch_gc_mixed<2>(c, gsize, ngroup); break;
+    case 3: r = launch_gc_mixed<3>(c, gsize, ngroup); break;
+    case 4: r = launch_gc_mixed<4>(c, gsize, ngroup); break;
+    case 5: r = launch_gc_mixed<5>(c, gsize, ngroup); break;
+    case 6: r = launch_gc_mixed<6>(c, gsize, ngroup); break;
+    case 7: r = launch_gc_mixed<7>(c, gsize, ngroup); break;
+    case 8: r = launch_gc_mixed<8>(c, gsize, ngroup); break;
+    case 9: r = launch_gc_mixed<9>(c, gsize, ngroup); break;
+    case 10: r = launch_gc_mixed<10>(c, gsize, ngroup); break;
+    case 11: r = launch_gc_mixed<11>(c, gsize, ngroup); break;
+    case 12: r = launch_gc_mixed<12>(c, gsize, ngroup); break;
+    case 13: r = launch_gc_mixed<13>(c, gsize, ngroup); break;
+    case 14: r = launch_gc_mixed<14>(c, gsize, ngroup); break;
+    case 15: r = launch_gc_mixed<15>(c, gsize, ngroup); break;
+    case 16: r = launch_gc_mixed<16>(c, gsize, ngroup); break;
+    default: r = launch_gc_mixed<17>(c, gsize, ngroup); break;
+    }
+    if (r) return r;
+    const size_t n = (size_t)c->Npad * c->P;
+    k_gc_reduce<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(c->Npad, c->P, P16, ngroup,
+                                                                   c->gc_part, c->gc_gun, c->Uw);
+    GP_HIP(hipGetLastError());
+    return 0;
+}

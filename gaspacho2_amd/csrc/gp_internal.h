@@ -98,6 +98,15 @@ struct gp_ctx {
     size_t gram_part_bytes = 0;
     int gram_splits = 16;      // cell segments of the mixed Gram
     int zsplit_valid = 0;
+    // mixed gradient contraction: per-gene FP16 hi/lo of Sw_j, gene groups
+    void *Sh = nullptr, *Sl = nullptr;  // J x P16 x 288 halves, [col][k]
+    int *gc_perm = nullptr, *gc_aexp = nullptr, *gc_sexp = nullptr, *gc_bexp = nullptr;
+    int *gc_zexp = nullptr;    // per column k of Zw: 2^zc_k >= max |Zw[:,k]|
+    double *gc_gun = nullptr;  // per group 2^-C_g
+    float *gc_part = nullptr;  // per group FP32 partial U (P16 x Npad)
+    size_t gc_part_len = 0;
+    int gc_groups = 32;        // gene groups of the mixed contraction
+    int gc_mixed = 1;          // mixed path: contraction on FP16 MFMA (0: FP64)
 
     size_t tot_max = 0;        // N+p+(N+M+1)Q
     int timing = 0;
@@ -180,6 +189,12 @@ int gpk_gram_src(gp_ctx *c, const double *W, double *Psi, int batch, const doubl
 int gpk_split_z(gp_ctx *c, const double *Zsrc);
 int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const double *Zsrc,
                    const double *Delta);
+// w32 = W/Omega in FP32 and the per-gene power-of-two scales (all genes)
+int gpk_wprep(gp_ctx *c, const double *W);
+
+// ---------------- gp_gradcontract_mixed.hip ----------------
+// Uw = sum_j diag(W_j/Omega) Zw Sinv_j on FP16 hi/lo MFMA (whitened basis)
+int gpk_gradcontract_mixed(gp_ctx *c);
 
 // ---------------- gp_gradcontract.hip ----------------
 // U = sum_j diag(W_j/Omega) Zsrc Sinv_j  (Npad x P)

@@ -46,6 +46,8 @@ def load() -> C.CDLL:
                                 ip, ip, dp, C.c_longlong, dp, dp, dp, dp, C.c_int, C.c_int]),
         "gp_destroy": (C.c_int, [vp]),
         "gp_set_precision": (C.c_int, [vp, C.c_int]),
+        "gp_set_option": (C.c_int, [vp, C.c_char_p, C.c_int]),
+        "gp_get_option": (C.c_int, [vp, C.c_char_p, ip]),
         "gp_buffer_len": (C.c_size_t, [vp, C.c_char_p]),
         "gp_get_buffer": (C.c_int, [vp, C.c_char_p, dp, C.c_size_t]),
         "gp_set_buffer": (C.c_int, [vp, C.c_char_p, dp, C.c_size_t]),
@@ -156,6 +158,14 @@ class GPContext:
 
     def set_precision(self, precision: int) -> None:
         self._check(self._lib.gp_set_precision(self._h, int(precision)))
+
+    def set_option(self, name: str, value: int) -> None:
+        self._check(self._lib.gp_set_option(self._h, name.encode(), int(value)))
+
+    def get_option(self, name: str) -> int:
+        v = C.c_int()
+        self._check(self._lib.gp_get_option(self._h, name.encode(), C.byref(v)))
+        return v.value
 
     # -- launch groups
     def init_fit(self) -> None:

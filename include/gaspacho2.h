@@ -70,11 +70,25 @@ int gp_destroy(gp_ctx *ctx);
 /* Switch the contraction precision of an existing context. */
 int gp_set_precision(gp_ctx *ctx, int precision);
 
+/* Launch-shape options (no counterpart in the reference, whose launch shapes
+ * are fixed in ggplvm.cu; these are the knobs the autotuner sweeps):
+ *   "gram_splits" [1, 256]  cell segments of the mixed Gram (FP32 partial
+ *                           per segment, FP64 sum across segments)
+ *   "gc_groups"   [1, 4096] gene groups of the mixed gradient contraction
+ *                           (FP32 partial per group, FP64 sum across groups)
+ *   "gc_mixed"    0/1       mixed path: contraction on FP16 hi/lo MFMA (1)
+ *                           or in FP64 (0)
+ * Unknown names and out-of-range values return a status > 0. */
+int gp_set_option(gp_ctx *ctx, const char *name, int value);
+int gp_get_option(gp_ctx *ctx, const char *name, int *value);
+
 /* Named state buffers in the reference layout (host <-> device copies).
  * Names: Omega delta X T rho phi (value|backup halves where the reference
  * keeps them), Z, K, Kinv, KinvKmn, titsias, DeltaInv, W (incl. Oinvt at
  * +N*J), Nu, Psi (factor, incl. inverse at +J*P*P), Beta, tBeta, lkhds,
- * grad, grad0, ss, SS, XX, C1, C2, lkhd.  Lengths in doubles match the
+ * grad, grad0, ss, SS, XX, C1, C2, lkhd; read-only: U and Uw (the
+ * gradient contraction sum_j diag(W_j/Omega) Z Psi_j^-1 in the reference
+ * and the whitened basis, N x P).  Lengths in doubles match the
  * reference's cudaMalloc sizes (ggplvm.cu:1024-1060).
  * Replaces readParams (util_cuda.cu:520-544) / writeParams (557-580) copies. */
 size_t gp_buffer_len(gp_ctx *ctx, const char *name);
