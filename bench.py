@@ -71,6 +71,23 @@ def cpu_baseline(ds, genes=(4, 8), threads=None):
                       f"to G={ds.J}: {t_full:.0f} s per unit"}
 
 
+def copy_calibration(mib: int, reps: int = 5) -> None:
+    """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
+    counter readings give the scale factor from counter units to HBM bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"copy_mib": mib, "bytes_read": n * 4, "bytes_written": n * 4,
+                      "ms": dt * 1e3, "GBps": 2 * n * 4 / dt / 1e9}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,11 +99,19 @@ def main():
     ap.add_argument("--Q", type=int, default=10)
     ap.add_argument("--M", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", action="store_true",
+                    help="run only --precision (no FP64 comparison run)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="time a plain device copy of MIB MiB (HBM counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (--gpus {args.gpus} requested); "
               "the gene-sharded multi-GPU design is described but not built.", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     from gaspacho2_amd import lib, synth
     t0 = time.perf_counter()
@@ -128,6 +153,8 @@ def main():
 
     res = {}
     order = ["f64", "mixed"] if args.precision == "mixed" else ["f64"]
+    if args.only:
+        order = [args.precision]
     for pn in order:
         res[pn] = run(pn, args.steps, args.warmup)
     main_p = args.precision

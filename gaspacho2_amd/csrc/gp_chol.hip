@@ -252,6 +252,15 @@ int gpk_chol_inverse_batched(gp_ctx *c, const double *L, int n, int batch, size_
     return 0;
 }
 
+int gpk_vtv(gp_ctx *c, const double *V, int n, int batch, size_t stride, double *S) {
+    const size_t st = stride ? stride : (size_t)n * n;
+    const int nt = (n + 63) / 64;
+    dim3 g(nt * (nt + 1) / 2, batch);
+    k_vtv<<<g, CT, 0, c->stream>>>(V, n, st, nt, S);
+    GP_HIP(hipGetLastError());
+    return 0;
+}
+
 int gpk_tri_inverse(gp_ctx *c, const double *L, int n, double *V) {
     const size_t tot = (size_t)n * n;
     k_identity<<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(V, n, 1, 0);

@@ -57,7 +57,7 @@ static void free_all(gp_ctx *c) {
                     c->lkhd, c->info, c->Zh, c->Zl, c->gram_part, c->pwork, c->w32, c->wsc,
                     c->wun, c->zun, c->Linv, c->Zw, c->DeltaW, c->DeltaWChol, c->Uw, c->tBw,
                     c->Vtmp, c->Sh, c->Sl, c->gc_perm, c->gc_aexp, c->gc_sexp, c->gc_bexp,
-                    c->gc_gun, c->gc_part, c->gc_zexp};
+                    c->gc_gun, c->gc_part, c->gc_zexp, c->skw};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     gp_timer *ts[] = {&c->t_gram, &c->t_chol, &c->t_gc, &c->t_kuf, &c->t_elbo, &c->t_grad};
@@ -400,7 +400,9 @@ static int kernel_mats(gp_ctx *c) {
     c->zsplit_valid = 0;
     c->zw_valid = 0;
     GP_TRY(gpk_chol_batched(c, c->Kchol, c->M, 1, 0, c->info + 1));
-    GP_TRY(gpk_chol_inverse_batched(c, c->Kchol, c->M, 1, 0, c->Kinv));
+    // Kmm^-1 = L^-T L^-1; L^-1 is kept for the whitened basis
+    GP_TRY(gpk_tri_inverse(c, c->Kchol, c->M, c->Linv));
+    GP_TRY(gpk_vtv(c, c->Linv, c->M, 1, 0, c->Kinv));
     GP_TRY(gpk_titsias(c));
     gp_timer_end(c, c->t_kuf);
     return 0;

@@ -25,6 +25,9 @@
 
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+#include <vector>
+
 namespace {
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
@@ -268,7 +271,241 @@ __global__ void k_gram_reduce(int P, int nbk, int nseg, int J, const float *__re
     Pj[c + (size_t)r * P] = v;
 }
 
+
+// ---------------------------------------------------------------------------
+// Symmetric form.  With x_i = sqrt(w_ij 2^-e_j) z~_i the weighted Gram is
+// X^T X, so one operand serves both sides of every MFMA.  Workgroup =
+// (gene, cell segment), 8 waves; the upper triangle of 16x16 tiles is cut
+// into blocks over groups of 4 tiles (the last group may be narrower) and
+// each wave owns up to two blocks, balanced on the host (GramPlan).  Per
+// 32-cell chunk the workgroup builds X hi/lo for all P16 columns in LDS
+// (double buffered; the chunk's square-root weights are staged two chunks
+// ahead) and every wave runs its blocks out of LDS: a 4x4 block reads 4 + 4
+// fragments for 16 tiles, a diagonal block 4 for 10.
+constexpr int GW = 8;        // waves per workgroup
+constexpr int GSL = 2;       // block slots per wave
+constexpr int MAX_SYM_T16 = 17;  // instantiated widths of the symmetric kernel
+constexpr int LDX = 40;      // LDS column stride in halves (32 cells + 8)
+
+struct GramPlan {
+    signed char a[GW * GSL];   // row group of the slot's block (-1: empty)
+    signed char b[GW * GSL];   // column group
+};
+
+__device__ inline void xsplit8(const h8 &zh, const h8 &zl, const float *sq, h8 &hi, h8 &lo) {
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        float v = ((float)zh[e] + (float)zl[e]) * sq[e];
+        asm volatile("" : "+v"(v));
+        const _Float16 h = (_Float16)v;
+        hi[e] = h;
+        lo[e] = (_Float16)(v - (float)h);
+    }
+}
+
+__device__ inline void mfma3(f4 &acc, const h8 &ah, const h8 &al, const h8 &bh, const h8 &bl) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc, 0, 0, 0);
+}
+
+template <int T16>
+__global__ void __launch_bounds__(GW * 64) k_gram_sym(int Npad, int seg, int nseg, int J,
+        const _Float16 *__restrict__ zh, const _Float16 *__restrict__ zl,
+        const float *__restrict__ w32, const float *__restrict__ wsc, GramPlan plan,
+        float *__restrict__ part) {
+    constexpr int P16 = 16 * T16;
+    constexpr int NG = (T16 + 3) / 4;            // tile groups
+    constexpr int RL = T16 - 4 * (NG - 1);        // tiles in the last group (1..4)
+    constexpr int NT = T16 * (T16 + 1) / 2;       // upper tiles
+    constexpr int NITEM = P16 * 4;                // (column, 8-cell group) items per chunk
+    constexpr int IPT = (NITEM + GW * 64 - 1) / (GW * 64);
+    extern __shared__ __attribute__((aligned(16))) _Float16 Xs[];   // [2 buf][2 hl][P16][LDX]
+    __shared__ __attribute__((aligned(16))) float wsl[3][CH];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // workgroup -> (segment, gene); the 8 XCDs take workgroups round robin,
+    // so the workgroups resident on one XCD share cell segments in its L2
+    int sgi, j;
+    {
+        const int id = blockIdx.x;
+        if (nseg % 8 == 0) {
+            const int xcd = id % 8, loc = id / 8, nsx = nseg / 8;
+            sgi = xcd + 8 * (loc % nsx);
+            j = loc / nsx;
+        } else {
+            sgi = id % nseg;
+            j = id / nseg;
+        }
+    }
+    const int c0 = sgi * seg, c1 = min(Npad, c0 + seg);
+    const int nch = (c1 - c0) / CH;
+    const float sc = wsc[j];
+    const float *wj = w32 + (size_t)j * Npad;
+
+    f4 acc[GSL][4][4];
+#pragma unroll
+    for (int s = 0; s < GSL; s++)
+#pragma unroll
+        for (int x = 0; x < 4; x++)
+#pragma unroll
+            for (int y = 0; y < 4; y++) acc[s][x][y] = f4{0.f, 0.f, 0.f, 0.f};
+
+    h8 pzh[IPT], pzl[IPT];
+    auto fetch = [&](int cc) {
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            const int it = tid + GW * 64 * r;
+            if (it < NITEM) {
+                const size_t off = (size_t)(it >> 2) * Npad + cc + 8 * (it & 3);
+                pzh[r] = *(const h8 *)(zh + off);
+                pzl[r] = *(const h8 *)(zl + off);
+            }
+        }
+    };
+    auto store = [&](int buf, int wb) {
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            const int it = tid + GW * 64 * r;
+            if (it < NITEM) {
+                const int col = it >> 2, g = it & 3;
+                float sq[8];
+                *(f4 *)&sq[0] = *(const f4 *)&wsl[wb][8 * g];
+                *(f4 *)&sq[4] = *(const f4 *)&wsl[wb][8 * g + 4];
+                h8 hi, lo;
+                xsplit8(pzh[r], pzl[r], sq, hi, lo);
+                *(h8 *)&Xs[((buf * 2 + 0) * P16 + col) * LDX + 8 * g] = hi;
+                *(h8 *)&Xs[((buf * 2 + 1) * P16 + col) * LDX + 8 * g] = lo;
+            }
+        }
+    };
+    auto sqw = [&](int ch) { return sqrtf(wj[c0 + ch * CH + tid] * sc); };   // tid < CH
+
+    // prologue: weights of chunks 0 and 1, X of chunk 0
+    if (tid < CH) {
+        wsl[0][tid] = sqw(0);
+        if (nch > 1) wsl[1][tid] = sqw(1);
+    }
+    fetch(c0);
+    __syncthreads();
+    store(0, 0);
+    __syncthreads();
+
+    const int lrow = lane & 15, lk = 8 * (lane >> 4);
+    for (int ch = 0; ch < nch; ch++) {
+        const int buf = ch & 1;
+        const bool more = ch + 1 < nch;
+        float pw = 0.0f;
+        if (more) fetch(c0 + (ch + 1) * CH);
+        if (tid < CH && ch + 2 < nch) pw = wj[c0 + (ch + 2) * CH + tid];
+        const _Float16 *xh = Xs + (size_t)(buf * 2 + 0) * P16 * LDX;
+        const _Float16 *xl = Xs + (size_t)(buf * 2 + 1) * P16 * LDX;
+#pragma unroll
+        for (int s = 0; s < GSL; s++) {
+            const int a = plan.a[w * GSL + s], b = plan.b[w * GSL + s];
+            if (a < 0) continue;
+            h8 ah[4], al[4];
+#pragma unroll
+            for (int x = 0; x < 4; x++) {
+                const int col = (4 * a + x) * 16 + lrow;
+                if (x < RL || a < NG - 1) {
+                    ah[x] = *(const h8 *)(xh + col * LDX + lk);
+                    al[x] = *(const h8 *)(xl + col * LDX + lk);
+                }
+            }
+            if (a == b) {
+                if (a < NG - 1) {
+#pragma unroll
+                    for (int x = 0; x < 4; x++)
+#pragma unroll
+                        for (int y = x; y < 4; y++) mfma3(acc[s][x][y], ah[x], al[x], ah[y], al[y]);
+                } else {
+#pragma unroll
+                    for (int x = 0; x < RL; x++)
+#pragma unroll
+                        for (int y = x; y < RL; y++) mfma3(acc[s][x][y], ah[x], al[x], ah[y], al[y]);
+                }
+            } else if (b < NG - 1) {
+#pragma unroll
+                for (int y = 0; y < 4; y++) {
+                    const int col = (4 * b + y) * 16 + lrow;
+                    const h8 bh = *(const h8 *)(xh + col * LDX + lk);
+                    const h8 bl = *(const h8 *)(xl + col * LDX + lk);
+#pragma unroll
+                    for (int x = 0; x < 4; x++) mfma3(acc[s][x][y], ah[x], al[x], bh, bl);
+                }
+            } else {
+#pragma unroll
+                for (int y = 0; y < RL; y++) {
+                    const int col = (4 * b + y) * 16 + lrow;
+                    const h8 bh = *(const h8 *)(xh + col * LDX + lk);
+                    const h8 bl = *(const h8 *)(xl + col * LDX + lk);
+#pragma unroll
+                    for (int x = 0; x < 4; x++) mfma3(acc[s][x][y], ah[x], al[x], bh, bl);
+                }
+            }
+        }
+        if (more) store(buf ^ 1, (ch + 1) % 3);
+        if (tid < CH && ch + 2 < nch) wsl[(ch + 2) % 3][tid] = sqrtf(pw * sc);
+        __syncthreads();
+    }
+
+    // FP32 partial of this (segment, gene): tile t holds lane l, r at t*256 + 4l + r
+    float *pg = part + ((size_t)sgi * J + j) * NT * 256;
+#pragma unroll
+    for (int s = 0; s < GSL; s++) {
+        const int a = plan.a[w * GSL + s], b = plan.b[w * GSL + s];
+        if (a < 0) continue;
+        const int sa = (a < NG - 1) ? 4 : RL, sb = (b < NG - 1) ? 4 : RL;
+#pragma unroll
+        for (int x = 0; x < 4; x++)
+#pragma unroll
+            for (int y = 0; y < 4; y++) {
+                if (x >= sa || y >= sb || (a == b && y < x)) continue;
+                const int ta = 4 * a + x, tb = 4 * b + y;
+                const int t = ta * T16 - ta * (ta - 1) / 2 + (tb - ta);
+                *(f4 *)(pg + (size_t)t * 256 + 4 * lane) = acc[s][x][y];
+            }
+    }
+}
+
+// Psi_j from the symmetric-form partials: one 256-thread block per (tile, gene);
+// sums the segments in FP64, rescales by 2^e_j 2^-s_r 2^-s_c, adds the prior
+// and writes the tile and its mirror (upper-triangle values only, so Psi is
+// exactly symmetric).
+__global__ void __launch_bounds__(256) k_gram_reduce_sym(int P, int T16, int nseg, int J,
+        const float *__restrict__ part, const double *__restrict__ wun,
+        const double *__restrict__ zun, const double *__restrict__ Delta, double *__restrict__ Psi) {
+    __shared__ double tile[16][17];
+    const int t = blockIdx.x, j = blockIdx.y, tid = threadIdx.x;
+    const int NT = T16 * (T16 + 1) / 2;
+    int ta = 0, rem = t;
+    while (rem >= T16 - ta) { rem -= T16 - ta; ta++; }
+    const int tb = ta + rem;
+    // thread tid = 4*lane + r  <->  tile element (4(lane/16) + r, lane%16)
+    const int lane = tid >> 2, r = tid & 3;
+    const int rr = 4 * (lane >> 4) + r, cc = lane & 15;
+    double s = 0.0;
+    for (int g = 0; g < nseg; g++) s += (double)part[(((size_t)g * J + j) * NT + t) * 256 + tid];
+    const int row = 16 * ta + rr, col = 16 * tb + cc;
+    double v = 0.0;
+    if (row < P && col < P) v = s * (wun[j] * zun[row] * zun[col]) + Delta[row + (size_t)col * P];
+    tile[rr][cc] = v;
+    __syncthreads();
+    double *Pj = Psi + (size_t)j * P * P;
+    const int i1 = tid & 15, i2 = tid >> 4;
+    {   // Psi[16ta + i1, 16tb + i2]   (coalesced over i1)
+        const int R = 16 * ta + i1, C = 16 * tb + i2;
+        if (R < P && C < P && (ta != tb || i1 <= i2)) Pj[R + (size_t)C * P] = tile[i1][i2];
+    }
+    {   // mirror: Psi[16tb + i1, 16ta + i2] = tile[i2][i1]
+        const int R = 16 * tb + i1, C = 16 * ta + i2;
+        if (R < P && C < P && (ta != tb || i2 <= i1)) Pj[R + (size_t)C * P] = tile[i2][i1];
+    }
+}
+
 }  // namespace
+
 
 // Scratch sizes for the mixed Gram (allocated on first use).
 static int ensure_mixed(gp_ctx *c, int nseg) {
@@ -311,6 +548,47 @@ int gpk_wprep(gp_ctx *c, const double *W) {
     return 0;
 }
 
+// Blocks of the upper tile triangle (groups of 4 tiles) spread over GW waves x
+// GSL slots by longest-processing-time; false if they do not fit.
+static bool make_gram_plan(int T16, GramPlan &plan) {
+    const int NG = (T16 + 3) / 4, RL = T16 - 4 * (NG - 1);
+    struct Blk { int a, b, cost; };
+    std::vector<Blk> blks;
+    for (int a = 0; a < NG; a++)
+        for (int b = a; b < NG; b++) {
+            const int sa = a < NG - 1 ? 4 : RL, sb = b < NG - 1 ? 4 : RL;
+            blks.push_back({a, b, a == b ? sa * (sa + 1) / 2 : sa * sb});
+        }
+    if ((int)blks.size() > GW * GSL) return false;
+    std::stable_sort(blks.begin(), blks.end(), [](const Blk &x, const Blk &y) { return x.cost > y.cost; });
+    int load[GW] = {0}, used[GW] = {0};
+    for (int i = 0; i < GW * GSL; i++) plan.a[i] = plan.b[i] = -1;
+    for (const Blk &k : blks) {
+        int best = -1;
+        for (int v = 0; v < GW; v++)
+            if (used[v] < GSL && (best < 0 || load[v] < load[best])) best = v;
+        plan.a[best * GSL + used[best]] = (signed char)k.a;
+        plan.b[best * GSL + used[best]] = (signed char)k.b;
+        used[best]++;
+        load[best] += k.cost;
+    }
+    return true;
+}
+
+template <int T16>
+static int launch_gram_sym(gp_ctx *c, int seg, int nseg, int batch, const GramPlan &plan) {
+    constexpr int P16 = 16 * T16;
+    const size_t shm = (size_t)2 * 2 * P16 * LDX * sizeof(_Float16);
+    if (shm > 65536)
+        GP_HIP(hipFuncSetAttribute((const void *)k_gram_sym<T16>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    k_gram_sym<T16><<<(unsigned)(batch * nseg), GW * 64, shm, c->stream>>>(
+        c->Npad, seg, nseg, batch, (const _Float16 *)c->Zh, (const _Float16 *)c->Zl, c->w32,
+        c->wsc, plan, c->gram_part);
+    GP_HIP(hipGetLastError());
+    return 0;
+}
+
 int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const double *Zsrc,
                    const double *Delta) {
     const int T16 = (c->P + 15) / 16, P16 = 16 * T16;
@@ -325,6 +603,35 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
     nseg = (c->Npad + seg - 1) / seg;
     k_wprep<<<batch, 256, 0, c->stream>>>(c->N, c->Npad, W, c->Omega, c->w32, c->wsc, c->wun);
     GP_HIP(hipGetLastError());
+    GramPlan plan;
+    if (T16 <= MAX_SYM_T16 && make_gram_plan(T16, plan)) {
+        int r = 0;
+        switch (T16) {
+        case 1: r = launch_gram_sym<1>(c, seg, nseg, batch, plan); break;
+        case 2: r = launch_gram_sym<2>(c, seg, nseg, batch, plan); break;
+        case 3: r = launch_gram_sym<3>(c, seg, nseg, batch, plan); break;
+        case 4: r = launch_gram_sym<4>(c, seg, nseg, batch, plan); break;
+        case 5: r = launch_gram_sym<5>(c, seg, nseg, batch, plan); break;
+        case 6: r = launch_gram_sym<6>(c, seg, nseg, batch, plan); break;
+        case 7: r = launch_gram_sym<7>(c, seg, nseg, batch, plan); break;
+        case 8: r = launch_gram_sym<8>(c, seg, nseg, batch, plan); break;
+        case 9: r = launch_gram_sym<9>(c, seg, nseg, batch, plan); break;
+        case 10: r = launch_gram_sym<10>(c, seg, nseg, batch, plan); break;
+        case 11: r = launch_gram_sym<11>(c, seg, nseg, batch, plan); break;
+        case 12: r = launch_gram_sym<12>(c, seg, nseg, batch, plan); break;
+        case 13: r = launch_gram_sym<13>(c, seg, nseg, batch, plan); break;
+        case 14: r = launch_gram_sym<14>(c, seg, nseg, batch, plan); break;
+        case 15: r = launch_gram_sym<15>(c, seg, nseg, batch, plan); break;
+        case 16: r = launch_gram_sym<16>(c, seg, nseg, batch, plan); break;
+        default: r = launch_gram_sym<17>(c, seg, nseg, batch, plan); break;
+        }
+        if (r) return r;
+        dim3 gr((unsigned)(T16 * (T16 + 1) / 2), batch);
+        k_gram_reduce_sym<<<gr, 256, 0, c->stream>>>(c->P, T16, nseg, batch, c->gram_part, c->wun,
+                                                      c->zun, Delta, Psi);
+        GP_HIP(hipGetLastError());
+        return 0;
+    }
     dim3 g(nbp, batch, nseg);
     k_gram_mixed<<<g, 256, 0, c->stream>>>(c->Npad, P16, T16, nbk, seg, (const _Float16 *)c->Zh,
                                             (const _Float16 *)c->Zl, c->w32, c->wsc, c->gram_part);

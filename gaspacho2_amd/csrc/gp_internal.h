@@ -60,6 +60,8 @@ struct gp_ctx {
     double *tBw = nullptr;       // P x J  beta-tilde in the whitened basis
     int psi_whitened = 0;        // basis of the current Psi factor
     int zw_valid = 0;            // Zw / Linv match the current Kmm factor
+    double *skw = nullptr;       // split-K GEMM partials
+    size_t skw_len = 0;
     double *Vtmp = nullptr;      // J x P x P scratch: L^{-1} of every factor
     size_t vtmp_len = 0;
 
@@ -162,6 +164,10 @@ int gpk_lbfgs(gp_ctx *c, int itr, int rhoflag);
 int gpk_downdate(gp_ctx *c, int itr, double r, int rhoflag);
 int gpk_beta_from_tilde(gp_ctx *c);
 int gpk_fill(gp_ctx *c, double *p, size_t n, double v);
+// C (M x N, ld ldc) = A^T B with A K x M (lda), B K x N (ldb), K large and
+// M x N small: split over K into strided-batched GEMMs, partials summed in FP64.
+int gpk_gemm_tn_splitk(gp_ctx *c, int M, int N, int K, const double *A, int lda,
+                       const double *B, int ldb, double *C, int ldc);
 
 // ---------------- gp_chol.hip ----------------
 // Batched in-place Cholesky of `batch` n x n column-major matrices spaced
@@ -176,6 +182,8 @@ int gpk_chol_inverse_batched(gp_ctx *c, const double *L, int n, int batch,
                              size_t stride, double *S);
 // V = L^{-1} (lower triangular) of one factored n x n matrix.
 int gpk_tri_inverse(gp_ctx *c, const double *L, int n, double *V);
+// S_j = V_j^T V_j (full symmetric) for lower-triangular V_j.
+int gpk_vtv(gp_ctx *c, const double *V, int n, int batch, size_t stride, double *S);
 
 // ---------------- gp_gram.hip ----------------
 // Psi_j = DeltaInv + Z^T diag(W_j / Omega) Z, full symmetric, all genes.

@@ -5,6 +5,8 @@
 // bit-reproducible from run to run.
 #include "gp_internal.h"
 
+#include <algorithm>
+
 #include <cmath>
 
 namespace {
@@ -752,9 +754,51 @@ int gpk_r_and_sums(gp_ctx *c, const double *Zsrc) {
     k_r_sums<<<c->J, NT, 0, c->stream>>>(c->N, c->Npad, c->W, c->Nu, c->Omega, c->R, c->gq, c->gs);
     LAUNCH_CHECK();
     // b_j = Z^T R_j  (util_cuda.cu:462-487 getZtOinvjnj for every gene)
+    GP_TRY(gpk_gemm_tn_splitk(c, c->P, c->J, c->Npad, Zsrc, c->Npad, c->R, c->Npad, c->bvec, c->P));
+    return 0;
+}
+
+namespace {
+__global__ void k_sum_splits(size_t mn, int M, int ldc, int nsplit, const double *__restrict__ Wp,
+                             double *__restrict__ C) {
+    const size_t idx = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (idx >= mn) return;
+    double s = 0.0;
+    for (int q = 0; q < nsplit; q++) s += Wp[(size_t)q * mn + idx];
+    C[(idx % M) + (idx / M) * (size_t)ldc] = s;
+}
+}  // namespace
+
+int gpk_gemm_tn_splitk(gp_ctx *c, int M, int N, int K, const double *A, int lda, const double *B,
+                       int ldb, double *C, int ldc) {
     const double one = 1.0, zero = 0.0;
-    GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->P,
-                          c->J, c->Npad, &one, Zsrc, c->Npad, c->R, c->Npad, &zero, c->bvec, c->P));
+    const long long tiles = (long long)((M + 127) / 128) * ((N + 127) / 128);
+    int want = (int)((1024 + tiles - 1) / tiles);           // aim at ~1024 output tiles
+    want = std::max(1, std::min(want, K / 512));
+    if (want <= 1) {
+        GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, M, N, K,
+                              &one, A, lda, B, ldb, &zero, C, ldc));
+        return 0;
+    }
+    const int Kc = ((K + want - 1) / want + 63) / 64 * 64;
+    const int nfull = K / Kc, rem = K - nfull * Kc;
+    const int nsplit = nfull + (rem > 0);
+    const size_t mn = (size_t)M * N;
+    if (c->skw_len < (size_t)nsplit * mn) {
+        if (c->skw) (void)hipFree(c->skw);
+        c->skw = nullptr;
+        c->skw_len = (size_t)nsplit * mn;
+        GP_HIP(hipMalloc((void **)&c->skw, c->skw_len * sizeof(double)));
+    }
+    GP_BLAS(rocblas_dgemm_strided_batched(c->blas, rocblas_operation_transpose,
+                                          rocblas_operation_none, M, N, Kc, &one, A, lda, Kc, B,
+                                          ldb, Kc, &zero, c->skw, M, (rocblas_stride)mn, nfull));
+    if (rem > 0)
+        GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, M, N,
+                              rem, &one, A + (size_t)nfull * Kc, lda, B + (size_t)nfull * Kc, ldb,
+                              &zero, c->skw + (size_t)nfull * mn, M));
+    k_sum_splits<<<nblk(mn), NT, 0, c->stream>>>(mn, M, ldc, nsplit, c->skw, C);
+    LAUNCH_CHECK();
     return 0;
 }
 
@@ -764,9 +808,7 @@ int gpk_init_y0(gp_ctx *c) {
     LAUNCH_CHECK();
     k_y0_sparse<<<c->J, NT, 0, c->stream>>>(c->Npad, c->Yp, c->Yi, c->Yx, c->ls, c->R);
     LAUNCH_CHECK();
-    const double one = 1.0, zero = 0.0;
-    GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->P,
-                          c->J, c->Npad, &one, c->Z, c->Npad, c->R, c->Npad, &zero, c->bvec, c->P));
+    GP_TRY(gpk_gemm_tn_splitk(c, c->P, c->J, c->Npad, c->Z, c->Npad, c->R, c->Npad, c->bvec, c->P));
     return 0;
 }
 
@@ -873,8 +915,7 @@ int gpk_c2(gp_ctx *c, int whitened) {
     k_scale_rows<<<nblk((size_t)c->Npad * c->M), NT, 0, c->stream>>>(c->N, c->Npad, c->M, c->A,
                                                                       c->Oinvt, c->AO);
     LAUNCH_CHECK();
-    GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->M, c->M,
-                          c->Npad, &one, c->A, c->Npad, c->AO, c->Npad, &zero, c->G2, c->M));
+    GP_TRY(gpk_gemm_tn_splitk(c, c->M, c->M, c->Npad, c->A, c->Npad, c->AO, c->Npad, c->G2, c->M));
     k_scale_cols<<<nblk((size_t)c->P * c->J), NT, 0, c->stream>>>(c->P, c->J, c->tBeta, c->phi,
                                                                    c->tBphi);
     LAUNCH_CHECK();
@@ -978,7 +1019,7 @@ int gpk_beta_from_tilde(gp_ctx *c) {
 // matrix with its GP block replaced by Knm L^-T (rows of norm <= 1).
 int gpk_whiten(gp_ctx *c) {
     const double one = 1.0, zero = 0.0;
-    GP_TRY(gpk_tri_inverse(c, c->Kchol, c->M, c->Linv));
+    // c->Linv = L^-1 of the current Kmm factor (set with Kinv by the kernel-matrix step)
     GP_HIP(hipMemcpyAsync(c->Zw, c->Z, (size_t)c->Npad * c->g0 * sizeof(double),
                           hipMemcpyDeviceToDevice, c->stream));
     const double *Knm = c->Z + (size_t)c->g0 * c->Npad;

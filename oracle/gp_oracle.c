@@ -507,7 +507,7 @@ void orc_get_lkhd_pois(orc_state *s) {          /* ggplvm.cu:703-724 */
         int l = s->Yp[j], lmax = s->Yp[j + 1];
         double sum = 0.0;
         for (int i = 0; i < N; i++) {
-            if (s->Yi[l] == i && l < lmax) {
+            if (l < lmax && s->Yi[l] == i) {
                 sum += (log(Wj[i]) - dt * s->titsias[i] * s->wt) * s->Yx[l] - Wj[i];
                 l++;
             } else {

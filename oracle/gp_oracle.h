@@ -79,7 +79,7 @@ void orc_expand_delta(int p, const int *nh, int M, const double *delta,
  * nh must have room for ncov+3 entries; on return p = ncov+2 and nh[p] = P.
  * body = the covariate bytes exactly as stored in the file.
  * Returns P, or -1 on a malformed header.
- * pinned: tests/test_formats.py builds one-hot columns independently. */
+ * pinned: tests/test_oracle_pins.py builds one-hot columns independently. */
 int orc_dataframe_layout(int ncov, const int *c, const int *v, int M, int *nh);
 void orc_dataframe_expand(int N, int ncov, const int *c, const int *v,
                           const char *body, const int *nh, double *Zcov);
@@ -128,7 +128,7 @@ void orc_get_w_for_beta(orc_state *s);         /* ggplvm.cu:468-514 */
 void orc_get_psi(orc_state *s);                /* ggplvm.cu:621-680  pinned: dense Z^T D Z + DeltaInv */
 int  orc_chol_psi(orc_state *s);               /* ggplvm.cu:683-692  returns #non-PD */
 void orc_get_beta_phi(orc_state *s, int jtr);  /* ggplvm.cu:418-464  pinned: IRLS fixed point */
-void orc_get_lkhd_pois(orc_state *s);          /* ggplvm.cu:703-724  parity unpinned (source text) */
+void orc_get_lkhd_pois(orc_state *s);          /* ggplvm.cu:703-724  pinned: Poisson logpmf + titsias term */
 double orc_collect_lkhd(orc_state *s, int Nterm, int jtr);  /* ggplvm.cu:788-815 */
 void orc_get_oinvt(orc_state *s);              /* ggplvm.cu:284-294 */
 void orc_get_lkhd(orc_state *s);               /* ggplvm.cu:744-784  pinned: Woodbury dense log N */

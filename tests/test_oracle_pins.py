@@ -206,6 +206,32 @@ def test_irls_fixed_point(tiny_ds):
     assert rel(lhs, rhs) < 1e-6
 
 
+def test_glm_objective_is_poisson_loglik_with_titsias_term(tiny_ds):
+    """getLkhdPois (ggplvm.cu:703-724) per gene equals the Poisson log-pmf of
+    the counts at rate W plus log(y!), minus wt*delta_GP*sum_i t_i y_ij: the
+    titsias factor leaves the log term only (the -W term keeps it)."""
+    import scipy.special
+    import scipy.stats
+    from gaspacho2_amd.synth import dense_counts
+    o = make_oracle(tiny_ds)
+    o.init_fit()
+    o.irls_step(0)
+    o.irls_step(1)
+    N, J = o.N, o.J
+    W = o.get("W")[:N * J].reshape(N, J, order="F")
+    t = o.get("titsias")[:N]
+    dgp = o.get("delta")[len(tiny_ds.nh) - 2]
+    Y = dense_counts(tiny_ds)
+    wt = 0.1                                  # ggplvm.cu:20
+    ref = (scipy.stats.poisson.logpmf(Y, W) + scipy.special.gammaln(Y + 1)).sum(0) \
+        - wt * dgp * (t[:, None] * Y).sum(0)
+    lk = o.get("lkhds")[:J]
+    assert np.max(np.abs(lk - ref)) < 1e-10 * np.max(np.abs(ref))
+    # a plausible slip (titsias factor removed from -W too) fails the check
+    alt = ref - (W * (np.exp(-wt * dgp * t)[:, None] - 1)).sum(0)
+    assert np.max(np.abs(lk - alt)) > 1e-6 * np.max(np.abs(ref))
+
+
 # ------------------------------------------------------------------ gradient
 def _fd(o, name, idx, eps, log=False):
     v = o.get(name)
