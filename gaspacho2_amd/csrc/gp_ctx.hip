@@ -54,7 +54,7 @@ static void free_all(gp_ctx *c) {
                     c->tBeta, c->bvec, c->tBphi, c->lkhds, c->gq, c->gs, c->grad, c->grad0, c->ss,
                     c->SS, c->XX, c->lbq, c->gradDeltaTmp, c->gradLogRhoTmp, c->C1, c->C2, c->U,
                     c->ZB, c->R, c->C1a, c->AO, c->G2, c->B2, c->S2, c->Ecell, c->red, c->scal,
-                    c->lkhd, c->info, c->Zh, c->Zl, c->gram_part, c->pwork, c->w32, c->wsc,
+                    c->lkhd, c->info, c->Zh, c->Zl, c->z32, c->gram_part, c->pwork, c->w32, c->wsc,
                     c->wun, c->zun, c->Linv, c->Zw, c->DeltaW, c->DeltaWChol, c->Uw, c->tBw,
                     c->Vtmp, c->Sh, c->Sl, c->gc_perm, c->gc_aexp, c->gc_sexp, c->gc_bexp,
                     c->gc_gun, c->gc_part, c->gc_zexp, c->skw};

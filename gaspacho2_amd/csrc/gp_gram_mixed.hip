@@ -41,7 +41,8 @@ constexpr int CH = 32;       // cells per chunk (one MFMA k-step)
 // [2^13, 2^14): values down to 2^-16 of the column maximum keep a normal FP16
 // lo part, i.e. the full ~22-bit hi/lo precision.  zun[a] = 2^-s_a.
 __global__ void __launch_bounds__(256) k_split_z(int Npad, int P, const double *__restrict__ Z,
-        _Float16 *__restrict__ zh, _Float16 *__restrict__ zl, double *__restrict__ zun) {
+        _Float16 *__restrict__ zh, _Float16 *__restrict__ zl, float *__restrict__ z32,
+        double *__restrict__ zun) {
     __shared__ double sh[4];
     __shared__ float scale;
     const int a = blockIdx.x;
@@ -63,11 +64,13 @@ __global__ void __launch_bounds__(256) k_split_z(int Npad, int P, const double *
     __syncthreads();
     const float sc = scale;
     _Float16 *h = zh + (size_t)a * Npad, *l = zl + (size_t)a * Npad;
+    float *f = z32 + (size_t)a * Npad;
     for (int i = threadIdx.x; i < Npad; i += 256) {
         const float v = (a < P) ? (float)(za[i] * (double)sc) : 0.0f;
         const _Float16 hv = (_Float16)v;
         h[i] = hv;
         l[i] = (_Float16)(v - (float)hv);
+        f[i] = v;
     }
 }
 
@@ -277,30 +280,60 @@ __global__ void k_gram_reduce(int P, int nbk, int nseg, int J, const float *__re
 // X^T X, so one operand serves both sides of every MFMA.  Workgroup =
 // (gene, cell segment), 8 waves; the upper triangle of 16x16 tiles is cut
 // into blocks over groups of 4 tiles (the last group may be narrower) and
-// each wave owns up to two blocks, balanced on the host (GramPlan).  Per
-// 32-cell chunk the workgroup builds X hi/lo for all P16 columns in LDS
-// (double buffered; the chunk's square-root weights are staged two chunks
-// ahead) and every wave runs its blocks out of LDS: a 4x4 block reads 4 + 4
-// fragments for 16 tiles, a diagonal block 4 for 10.
+// each wave owns up to two blocks, balanced at compile time (longest block
+// first onto the least loaded wave).  Each wave runs its own straight-line
+// copy of the chunk loop (the blocks are template arguments), so no
+// accumulator is touched by a data-dependent branch.  Per 32-cell chunk the
+// workgroup builds X hi/lo for all P16 columns in LDS (double buffered; the
+// chunk's square-root weights are staged two chunks ahead) and every wave runs
+// its blocks out of LDS: a 4x4 block reads 4 + 4 fragments for 16 tiles, a
+// diagonal block 4 for 10.
 constexpr int GW = 8;        // waves per workgroup
 constexpr int GSL = 2;       // block slots per wave
-constexpr int MAX_SYM_T16 = 17;  // instantiated widths of the symmetric kernel
 constexpr int LDX = 40;      // LDS column stride in halves (32 cells + 8)
+constexpr int MAX_SYM_T16 = 17;  // instantiated widths of the symmetric kernel
 
 struct GramPlan {
-    signed char a[GW * GSL];   // row group of the slot's block (-1: empty)
-    signed char b[GW * GSL];   // column group
+    int a[GW * GSL];   // row group of the slot's block (-1: empty)
+    int b[GW * GSL];   // column group
 };
 
-__device__ inline void xsplit8(const h8 &zh, const h8 &zl, const float *sq, h8 &hi, h8 &lo) {
-#pragma unroll
-    for (int e = 0; e < 8; e++) {
-        float v = ((float)zh[e] + (float)zl[e]) * sq[e];
-        asm volatile("" : "+v"(v));
-        const _Float16 h = (_Float16)v;
-        hi[e] = h;
-        lo[e] = (_Float16)(v - (float)h);
+constexpr GramPlan make_gram_plan(int T16) {
+    GramPlan p{};
+    for (int i = 0; i < GW * GSL; i++) { p.a[i] = -1; p.b[i] = -1; }
+    const int NG = (T16 + 3) / 4, RL = T16 - 4 * (NG - 1);
+    int ba[GW * GSL] = {}, bb[GW * GSL] = {}, bc[GW * GSL] = {};
+    int nb = 0;
+    for (int a = 0; a < NG; a++)
+        for (int b = a; b < NG; b++) {
+            if (nb >= GW * GSL) return p;        // does not fit: caller checks plan_fits
+            const int sa = a < NG - 1 ? 4 : RL, sb = b < NG - 1 ? 4 : RL;
+            ba[nb] = a; bb[nb] = b; bc[nb] = (a == b) ? sa * (sa + 1) / 2 : sa * sb;
+            nb++;
+        }
+    // stable sort by cost, largest first
+    for (int i = 1; i < nb; i++)
+        for (int k = i; k > 0 && bc[k] > bc[k - 1]; k--) {
+            int t = ba[k]; ba[k] = ba[k - 1]; ba[k - 1] = t;
+            t = bb[k]; bb[k] = bb[k - 1]; bb[k - 1] = t;
+            t = bc[k]; bc[k] = bc[k - 1]; bc[k - 1] = t;
+        }
+    int load[GW] = {}, used[GW] = {};
+    for (int i = 0; i < nb; i++) {
+        int best = -1;
+        for (int v = 0; v < GW; v++)
+            if (used[v] < GSL && (best < 0 || load[v] < load[best])) best = v;
+        p.a[best * GSL + used[best]] = ba[i];
+        p.b[best * GSL + used[best]] = bb[i];
+        used[best]++;
+        load[best] += bc[i];
     }
+    return p;
+}
+
+constexpr bool plan_fits(int T16) {
+    const int NG = (T16 + 3) / 4;
+    return NG * (NG + 1) / 2 <= GW * GSL;
 }
 
 __device__ inline void mfma3(f4 &acc, const h8 &ah, const h8 &al, const h8 &bh, const h8 &bl) {
@@ -309,21 +342,150 @@ __device__ inline void mfma3(f4 &acc, const h8 &ah, const h8 &al, const h8 &bh, 
     acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc, 0, 0, 0);
 }
 
-template <int T16>
-__global__ void __launch_bounds__(GW * 64) k_gram_sym(int Npad, int seg, int nseg, int J,
-        const _Float16 *__restrict__ zh, const _Float16 *__restrict__ zl,
-        const float *__restrict__ w32, const float *__restrict__ wsc, GramPlan plan,
-        float *__restrict__ part) {
+// one 32-cell step of block (A, B) out of the LDS buffer (xh, xl)
+template <int T16, int A, int B>
+__device__ __forceinline__ void gram_block(f4 (&acc)[4][4], const _Float16 *xh, const _Float16 *xl,
+                                           int lrow, int lk) {
+    constexpr int NG = (T16 + 3) / 4, RL = T16 - 4 * (NG - 1);
+    constexpr int SA = A < NG - 1 ? 4 : RL, SB = B < NG - 1 ? 4 : RL;
+    h8 ah[SA], al[SA];
+#pragma unroll
+    for (int x = 0; x < SA; x++) {
+        const int col = (4 * A + x) * 16 + lrow;
+        ah[x] = *(const h8 *)(xh + col * LDX + lk);
+        al[x] = *(const h8 *)(xl + col * LDX + lk);
+    }
+    if constexpr (A == B) {
+#pragma unroll
+        for (int x = 0; x < SA; x++)
+#pragma unroll
+            for (int y = x; y < SA; y++) mfma3(acc[x][y], ah[x], al[x], ah[y], al[y]);
+    } else {
+#pragma unroll
+        for (int y = 0; y < SB; y++) {
+            const int col = (4 * B + y) * 16 + lrow;
+            const h8 bh = *(const h8 *)(xh + col * LDX + lk);
+            const h8 bl = *(const h8 *)(xl + col * LDX + lk);
+#pragma unroll
+            for (int x = 0; x < SA; x++) mfma3(acc[x][y], ah[x], al[x], bh, bl);
+        }
+    }
+}
+
+template <int T16, int A, int B>
+__device__ __forceinline__ void gram_block_store(const f4 (&acc)[4][4], float *pg, int lane) {
+    constexpr int NG = (T16 + 3) / 4, RL = T16 - 4 * (NG - 1);
+    constexpr int SA = A < NG - 1 ? 4 : RL, SB = B < NG - 1 ? 4 : RL;
+#pragma unroll
+    for (int x = 0; x < SA; x++)
+#pragma unroll
+        for (int y = (A == B ? x : 0); y < SB; y++) {
+            const int ta = 4 * A + x, tb = 4 * B + y;
+            const int t = ta * T16 - ta * (ta - 1) / 2 + (tb - ta);
+            *(f4 *)(pg + (size_t)t * 256 + 4 * lane) = acc[x][y];
+        }
+}
+
+struct GramArgs {
+    int Npad, c0, nch;
+    float sc;
+    const float *z32, *wj;
+    float *pg;
+};
+
+// the chunk loop of wave W (its blocks are compile-time constants)
+template <int T16, int W>
+__device__ __forceinline__ void gram_wave(const GramArgs &g, _Float16 *Xs, float (*wsl)[CH]) {
     constexpr int P16 = 16 * T16;
-    constexpr int NG = (T16 + 3) / 4;            // tile groups
-    constexpr int RL = T16 - 4 * (NG - 1);        // tiles in the last group (1..4)
-    constexpr int NT = T16 * (T16 + 1) / 2;       // upper tiles
+    constexpr GramPlan pl = make_gram_plan(T16);
+    constexpr int A0 = pl.a[W * GSL], B0 = pl.b[W * GSL];
+    constexpr int A1 = pl.a[W * GSL + 1], B1 = pl.b[W * GSL + 1];
     constexpr int NITEM = P16 * 4;                // (column, 8-cell group) items per chunk
     constexpr int IPT = (NITEM + GW * 64 - 1) / (GW * 64);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int lrow = lane & 15, lk = 8 * (lane >> 4);
+
+    f4 acc0[4][4], acc1[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            acc0[x][y] = f4{0.f, 0.f, 0.f, 0.f};
+            acc1[x][y] = f4{0.f, 0.f, 0.f, 0.f};
+        }
+
+    f4 pz[IPT][2];
+    auto fetch = [&](int cc) {
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            const int it = tid + GW * 64 * r;
+            if (it < NITEM) {
+                const float *src = g.z32 + (size_t)(it >> 2) * g.Npad + cc + 8 * (it & 3);
+                pz[r][0] = *(const f4 *)src;
+                pz[r][1] = *(const f4 *)(src + 4);
+            }
+        }
+    };
+    auto store = [&](int buf, int wb) {
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            const int it = tid + GW * 64 * r;
+            if (it < NITEM) {
+                const int col = it >> 2, q = it & 3;
+                const f4 s0 = *(const f4 *)&wsl[wb][8 * q];
+                const f4 s1 = *(const f4 *)&wsl[wb][8 * q + 4];
+                h8 hi, lo;
+#pragma unroll
+                for (int e = 0; e < 8; e++) {
+                    float v = (e < 4 ? pz[r][0][e] * s0[e] : pz[r][1][e - 4] * s1[e - 4]);
+                    asm volatile("" : "+v"(v));
+                    const _Float16 h = (_Float16)v;
+                    hi[e] = h;
+                    lo[e] = (_Float16)(v - (float)h);
+                }
+                *(h8 *)&Xs[((buf * 2 + 0) * P16 + col) * LDX + 8 * q] = hi;
+                *(h8 *)&Xs[((buf * 2 + 1) * P16 + col) * LDX + 8 * q] = lo;
+            }
+        }
+    };
+    auto sqw = [&](int ch) { return sqrtf(g.wj[g.c0 + ch * CH + tid] * g.sc); };   // tid < CH
+
+    // prologue: weights of chunks 0 and 1, X of chunk 0
+    if (tid < CH) {
+        wsl[0][tid] = sqw(0);
+        if (g.nch > 1) wsl[1][tid] = sqw(1);
+    }
+    fetch(g.c0);
+    __syncthreads();
+    store(0, 0);
+    __syncthreads();
+
+    for (int ch = 0; ch < g.nch; ch++) {
+        const int buf = ch & 1;
+        const bool more = ch + 1 < g.nch;
+        float pw = 0.0f;
+        if (more) fetch(g.c0 + (ch + 1) * CH);
+        if (tid < CH && ch + 2 < g.nch) pw = g.wj[g.c0 + (ch + 2) * CH + tid];
+        const _Float16 *xh = Xs + (size_t)(buf * 2 + 0) * P16 * LDX;
+        const _Float16 *xl = Xs + (size_t)(buf * 2 + 1) * P16 * LDX;
+        if constexpr (A0 >= 0) gram_block<T16, A0, B0>(acc0, xh, xl, lrow, lk);
+        if constexpr (A1 >= 0) gram_block<T16, A1, B1>(acc1, xh, xl, lrow, lk);
+        if (more) store(buf ^ 1, (ch + 1) % 3);
+        if (tid < CH && ch + 2 < g.nch) wsl[(ch + 2) % 3][tid] = sqrtf(pw * g.sc);
+        __syncthreads();
+    }
+    if constexpr (A0 >= 0) gram_block_store<T16, A0, B0>(acc0, g.pg, lane);
+    if constexpr (A1 >= 0) gram_block_store<T16, A1, B1>(acc1, g.pg, lane);
+}
+
+template <int T16>
+__global__ void __launch_bounds__(GW * 64) k_gram_sym(int Npad, int seg, int nseg, int J,
+        const float *__restrict__ z32, const float *__restrict__ w32,
+        const float *__restrict__ wsc, float *__restrict__ part) {
+    constexpr int NT = T16 * (T16 + 1) / 2;       // upper tiles
     extern __shared__ __attribute__((aligned(16))) _Float16 Xs[];   // [2 buf][2 hl][P16][LDX]
     __shared__ __attribute__((aligned(16))) float wsl[3][CH];
-    const int tid = threadIdx.x, lane = tid & 63;
-    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // workgroup -> (segment, gene); the 8 XCDs take workgroups round robin,
     // so the workgroups resident on one XCD share cell segments in its L2
     int sgi, j;
@@ -338,134 +500,23 @@ __global__ void __launch_bounds__(GW * 64) k_gram_sym(int Npad, int seg, int nse
             j = id / nseg;
         }
     }
-    const int c0 = sgi * seg, c1 = min(Npad, c0 + seg);
-    const int nch = (c1 - c0) / CH;
-    const float sc = wsc[j];
-    const float *wj = w32 + (size_t)j * Npad;
-
-    f4 acc[GSL][4][4];
-#pragma unroll
-    for (int s = 0; s < GSL; s++)
-#pragma unroll
-        for (int x = 0; x < 4; x++)
-#pragma unroll
-            for (int y = 0; y < 4; y++) acc[s][x][y] = f4{0.f, 0.f, 0.f, 0.f};
-
-    h8 pzh[IPT], pzl[IPT];
-    auto fetch = [&](int cc) {
-#pragma unroll
-        for (int r = 0; r < IPT; r++) {
-            const int it = tid + GW * 64 * r;
-            if (it < NITEM) {
-                const size_t off = (size_t)(it >> 2) * Npad + cc + 8 * (it & 3);
-                pzh[r] = *(const h8 *)(zh + off);
-                pzl[r] = *(const h8 *)(zl + off);
-            }
-        }
-    };
-    auto store = [&](int buf, int wb) {
-#pragma unroll
-        for (int r = 0; r < IPT; r++) {
-            const int it = tid + GW * 64 * r;
-            if (it < NITEM) {
-                const int col = it >> 2, g = it & 3;
-                float sq[8];
-                *(f4 *)&sq[0] = *(const f4 *)&wsl[wb][8 * g];
-                *(f4 *)&sq[4] = *(const f4 *)&wsl[wb][8 * g + 4];
-                h8 hi, lo;
-                xsplit8(pzh[r], pzl[r], sq, hi, lo);
-                *(h8 *)&Xs[((buf * 2 + 0) * P16 + col) * LDX + 8 * g] = hi;
-                *(h8 *)&Xs[((buf * 2 + 1) * P16 + col) * LDX + 8 * g] = lo;
-            }
-        }
-    };
-    auto sqw = [&](int ch) { return sqrtf(wj[c0 + ch * CH + tid] * sc); };   // tid < CH
-
-    // prologue: weights of chunks 0 and 1, X of chunk 0
-    if (tid < CH) {
-        wsl[0][tid] = sqw(0);
-        if (nch > 1) wsl[1][tid] = sqw(1);
-    }
-    fetch(c0);
-    __syncthreads();
-    store(0, 0);
-    __syncthreads();
-
-    const int lrow = lane & 15, lk = 8 * (lane >> 4);
-    for (int ch = 0; ch < nch; ch++) {
-        const int buf = ch & 1;
-        const bool more = ch + 1 < nch;
-        float pw = 0.0f;
-        if (more) fetch(c0 + (ch + 1) * CH);
-        if (tid < CH && ch + 2 < nch) pw = wj[c0 + (ch + 2) * CH + tid];
-        const _Float16 *xh = Xs + (size_t)(buf * 2 + 0) * P16 * LDX;
-        const _Float16 *xl = Xs + (size_t)(buf * 2 + 1) * P16 * LDX;
-#pragma unroll
-        for (int s = 0; s < GSL; s++) {
-            const int a = plan.a[w * GSL + s], b = plan.b[w * GSL + s];
-            if (a < 0) continue;
-            h8 ah[4], al[4];
-#pragma unroll
-            for (int x = 0; x < 4; x++) {
-                const int col = (4 * a + x) * 16 + lrow;
-                if (x < RL || a < NG - 1) {
-                    ah[x] = *(const h8 *)(xh + col * LDX + lk);
-                    al[x] = *(const h8 *)(xl + col * LDX + lk);
-                }
-            }
-            if (a == b) {
-                if (a < NG - 1) {
-#pragma unroll
-                    for (int x = 0; x < 4; x++)
-#pragma unroll
-                        for (int y = x; y < 4; y++) mfma3(acc[s][x][y], ah[x], al[x], ah[y], al[y]);
-                } else {
-#pragma unroll
-                    for (int x = 0; x < RL; x++)
-#pragma unroll
-                        for (int y = x; y < RL; y++) mfma3(acc[s][x][y], ah[x], al[x], ah[y], al[y]);
-                }
-            } else if (b < NG - 1) {
-#pragma unroll
-                for (int y = 0; y < 4; y++) {
-                    const int col = (4 * b + y) * 16 + lrow;
-                    const h8 bh = *(const h8 *)(xh + col * LDX + lk);
-                    const h8 bl = *(const h8 *)(xl + col * LDX + lk);
-#pragma unroll
-                    for (int x = 0; x < 4; x++) mfma3(acc[s][x][y], ah[x], al[x], bh, bl);
-                }
-            } else {
-#pragma unroll
-                for (int y = 0; y < RL; y++) {
-                    const int col = (4 * b + y) * 16 + lrow;
-                    const h8 bh = *(const h8 *)(xh + col * LDX + lk);
-                    const h8 bl = *(const h8 *)(xl + col * LDX + lk);
-#pragma unroll
-                    for (int x = 0; x < 4; x++) mfma3(acc[s][x][y], ah[x], al[x], bh, bl);
-                }
-            }
-        }
-        if (more) store(buf ^ 1, (ch + 1) % 3);
-        if (tid < CH && ch + 2 < nch) wsl[(ch + 2) % 3][tid] = sqrtf(pw * sc);
-        __syncthreads();
-    }
-
-    // FP32 partial of this (segment, gene): tile t holds lane l, r at t*256 + 4l + r
-    float *pg = part + ((size_t)sgi * J + j) * NT * 256;
-#pragma unroll
-    for (int s = 0; s < GSL; s++) {
-        const int a = plan.a[w * GSL + s], b = plan.b[w * GSL + s];
-        if (a < 0) continue;
-        const int sa = (a < NG - 1) ? 4 : RL, sb = (b < NG - 1) ? 4 : RL;
-#pragma unroll
-        for (int x = 0; x < 4; x++)
-#pragma unroll
-            for (int y = 0; y < 4; y++) {
-                if (x >= sa || y >= sb || (a == b && y < x)) continue;
-                const int ta = 4 * a + x, tb = 4 * b + y;
-                const int t = ta * T16 - ta * (ta - 1) / 2 + (tb - ta);
-                *(f4 *)(pg + (size_t)t * 256 + 4 * lane) = acc[s][x][y];
-            }
+    GramArgs g;
+    g.Npad = Npad;
+    g.c0 = sgi * seg;
+    g.nch = (min(Npad, g.c0 + seg) - g.c0) / CH;
+    g.sc = wsc[j];
+    g.z32 = z32;
+    g.wj = w32 + (size_t)j * Npad;
+    g.pg = part + ((size_t)sgi * J + j) * NT * 256;
+    switch (w) {
+    case 0: gram_wave<T16, 0>(g, Xs, wsl); break;
+    case 1: gram_wave<T16, 1>(g, Xs, wsl); break;
+    case 2: gram_wave<T16, 2>(g, Xs, wsl); break;
+    case 3: gram_wave<T16, 3>(g, Xs, wsl); break;
+    case 4: gram_wave<T16, 4>(g, Xs, wsl); break;
+    case 5: gram_wave<T16, 5>(g, Xs, wsl); break;
+    case 6: gram_wave<T16, 6>(g, Xs, wsl); break;
+    default: gram_wave<T16, 7>(g, Xs, wsl); break;
     }
 }
 
@@ -517,6 +568,7 @@ static int ensure_mixed(gp_ctx *c, int nseg) {
     if (!c->Zh) {
         GP_HIP(hipMalloc(&c->Zh, zbytes));
         GP_HIP(hipMalloc(&c->Zl, zbytes));
+        GP_HIP(hipMalloc((void **)&c->z32, (size_t)c->Npad * P16 * sizeof(float)));
         GP_HIP(hipMalloc((void **)&c->w32, (size_t)c->Npad * c->J * sizeof(float)));
         GP_HIP(hipMalloc((void **)&c->wsc, (size_t)c->J * sizeof(float)));
         GP_HIP(hipMalloc((void **)&c->wun, (size_t)c->J * sizeof(double)));
@@ -535,7 +587,7 @@ int gpk_split_z(gp_ctx *c, const double *Zsrc) {
     const int T16 = (c->P + 15) / 16, P16 = 16 * T16;
     GP_TRY(ensure_mixed(c, c->gram_splits));
     k_split_z<<<P16, 256, 0, c->stream>>>(c->Npad, c->P, Zsrc, (_Float16 *)c->Zh,
-                                          (_Float16 *)c->Zl, c->zun);
+                                          (_Float16 *)c->Zl, c->z32, c->zun);
     GP_HIP(hipGetLastError());
     c->zsplit_valid = 1;
     return 0;
@@ -548,43 +600,16 @@ int gpk_wprep(gp_ctx *c, const double *W) {
     return 0;
 }
 
-// Blocks of the upper tile triangle (groups of 4 tiles) spread over GW waves x
-// GSL slots by longest-processing-time; false if they do not fit.
-static bool make_gram_plan(int T16, GramPlan &plan) {
-    const int NG = (T16 + 3) / 4, RL = T16 - 4 * (NG - 1);
-    struct Blk { int a, b, cost; };
-    std::vector<Blk> blks;
-    for (int a = 0; a < NG; a++)
-        for (int b = a; b < NG; b++) {
-            const int sa = a < NG - 1 ? 4 : RL, sb = b < NG - 1 ? 4 : RL;
-            blks.push_back({a, b, a == b ? sa * (sa + 1) / 2 : sa * sb});
-        }
-    if ((int)blks.size() > GW * GSL) return false;
-    std::stable_sort(blks.begin(), blks.end(), [](const Blk &x, const Blk &y) { return x.cost > y.cost; });
-    int load[GW] = {0}, used[GW] = {0};
-    for (int i = 0; i < GW * GSL; i++) plan.a[i] = plan.b[i] = -1;
-    for (const Blk &k : blks) {
-        int best = -1;
-        for (int v = 0; v < GW; v++)
-            if (used[v] < GSL && (best < 0 || load[v] < load[best])) best = v;
-        plan.a[best * GSL + used[best]] = (signed char)k.a;
-        plan.b[best * GSL + used[best]] = (signed char)k.b;
-        used[best]++;
-        load[best] += k.cost;
-    }
-    return true;
-}
-
 template <int T16>
-static int launch_gram_sym(gp_ctx *c, int seg, int nseg, int batch, const GramPlan &plan) {
+static int launch_gram_sym(gp_ctx *c, int seg, int nseg, int batch) {
     constexpr int P16 = 16 * T16;
+    static_assert(plan_fits(T16), "block plan does not fit the waves");
     const size_t shm = (size_t)2 * 2 * P16 * LDX * sizeof(_Float16);
     if (shm > 65536)
         GP_HIP(hipFuncSetAttribute((const void *)k_gram_sym<T16>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     k_gram_sym<T16><<<(unsigned)(batch * nseg), GW * 64, shm, c->stream>>>(
-        c->Npad, seg, nseg, batch, (const _Float16 *)c->Zh, (const _Float16 *)c->Zl, c->w32,
-        c->wsc, plan, c->gram_part);
+        c->Npad, seg, nseg, batch, c->z32, c->w32, c->wsc, c->gram_part);
     GP_HIP(hipGetLastError());
     return 0;
 }
@@ -603,27 +628,26 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
     nseg = (c->Npad + seg - 1) / seg;
     k_wprep<<<batch, 256, 0, c->stream>>>(c->N, c->Npad, W, c->Omega, c->w32, c->wsc, c->wun);
     GP_HIP(hipGetLastError());
-    GramPlan plan;
-    if (T16 <= MAX_SYM_T16 && make_gram_plan(T16, plan)) {
+    if (T16 <= MAX_SYM_T16) {
         int r = 0;
         switch (T16) {
-        case 1: r = launch_gram_sym<1>(c, seg, nseg, batch, plan); break;
-        case 2: r = launch_gram_sym<2>(c, seg, nseg, batch, plan); break;
-        case 3: r = launch_gram_sym<3>(c, seg, nseg, batch, plan); break;
-        case 4: r = launch_gram_sym<4>(c, seg, nseg, batch, plan); break;
-        case 5: r = launch_gram_sym<5>(c, seg, nseg, batch, plan); break;
-        case 6: r = launch_gram_sym<6>(c, seg, nseg, batch, plan); break;
-        case 7: r = launch_gram_sym<7>(c, seg, nseg, batch, plan); break;
-        case 8: r = launch_gram_sym<8>(c, seg, nseg, batch, plan); break;
-        case 9: r = launch_gram_sym<9>(c, seg, nseg, batch, plan); break;
-        case 10: r = launch_gram_sym<10>(c, seg, nseg, batch, plan); break;
-        case 11: r = launch_gram_sym<11>(c, seg, nseg, batch, plan); break;
-        case 12: r = launch_gram_sym<12>(c, seg, nseg, batch, plan); break;
-        case 13: r = launch_gram_sym<13>(c, seg, nseg, batch, plan); break;
-        case 14: r = launch_gram_sym<14>(c, seg, nseg, batch, plan); break;
-        case 15: r = launch_gram_sym<15>(c, seg, nseg, batch, plan); break;
-        case 16: r = launch_gram_sym<16>(c, seg, nseg, batch, plan); break;
-        default: r = launch_gram_sym<17>(c, seg, nseg, batch, plan); break;
+        case 1: r = launch_gram_sym<1>(c, seg, nseg, batch); break;
+        case 2: r = launch_gram_sym<2>(c, seg, nseg, batch); break;
+        case 3: r = launch_gram_sym<3>(c, seg, nseg, batch); break;
+        case 4: r = launch_gram_sym<4>(c, seg, nseg, batch); break;
+        case 5: r = launch_gram_sym<5>(c, seg, nseg, batch); break;
+        case 6: r = launch_gram_sym<6>(c, seg, nseg, batch); break;
+        case 7: r = launch_gram_sym<7>(c, seg, nseg, batch); break;
+        case 8: r = launch_gram_sym<8>(c, seg, nseg, batch); break;
+        case 9: r = launch_gram_sym<9>(c, seg, nseg, batch); break;
+        case 10: r = launch_gram_sym<10>(c, seg, nseg, batch); break;
+        case 11: r = launch_gram_sym<11>(c, seg, nseg, batch); break;
+        case 12: r = launch_gram_sym<12>(c, seg, nseg, batch); break;
+        case 13: r = launch_gram_sym<13>(c, seg, nseg, batch); break;
+        case 14: r = launch_gram_sym<14>(c, seg, nseg, batch); break;
+        case 15: r = launch_gram_sym<15>(c, seg, nseg, batch); break;
+        case 16: r = launch_gram_sym<16>(c, seg, nseg, batch); break;
+        default: r = launch_gram_sym<17>(c, seg, nseg, batch); break;
         }
         if (r) return r;
         dim3 gr((unsigned)(T16 * (T16 + 1) / 2), batch);

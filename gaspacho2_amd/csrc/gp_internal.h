@@ -92,6 +92,7 @@ struct gp_ctx {
     int *info = nullptr;       // non-PD counters
     // mixed precision staging (fp16 hi/lo of the model matrix, tiled)
     void *Zh = nullptr, *Zl = nullptr;  // FP16 hi/lo of Z (Npad x P16)
+    float *z32 = nullptr;      // the same column-scaled model matrix in FP32
     float *w32 = nullptr;      // Npad x J  W/Omega in FP32
     float *wsc = nullptr;      // J  2^-e_j weight scale
     double *wun = nullptr;     // J  2^e_j
