@@ -12,13 +12,13 @@
 // the group's FP32 partial is one sum, and k_gc_reduce adds the groups in FP64
 // times 2^-C_g.
 //
-// Work.  Workgroup = 4 waves = 128 cells x all P16 columns for one gene group.
-// Wave w owns cells i0 + 32w .. +31 (two 16-cell tiles) and keeps their zw
+// Work.  Workgroup = 8 waves = 128 cells x all P16 columns for one gene group
+// (two waves per SIMD).  Wave w owns cells i0 + 16w .. +15 and keeps their zw
 // values (FP32, k = 32 ks + 8(l/16) + e) in registers.  Sw_j^hi/lo are pre-split
 // per gene into [col][k] (k contiguous, 288 k per column) and streamed through
 // LDS one 32-k step at a time (double buffered); B fragment lane l =
 // Sw[k = 8(l/16)..+7][col = l%16].
-// D lane l, r -> U[cell 16t + 4(l/16) + r][16 ct + l%16].
+// D lane l, r -> U[cell 4(l/16) + r][16 ct + l%16].
 #include "gp_internal.h"
 
 #include <algorithm>
@@ -35,6 +35,7 @@ constexpr int KP = 288;      // padded k extent of the pre-split Sw (multiple of
 constexpr int KCH = 32;      // k rows per LDS chunk (one MFMA k step)
 constexpr int LDK = 40;      // LDS column stride in halves (32 + 8: conflict-free b128 reads)
 constexpr int MAXT16 = 17;   // column tiles handled (P16 <= 272)
+constexpr int GCW = 8;       // waves per workgroup (16 cells each)
 
 // per column k of Zw: exponent zc_k with 2^zc_k >= max_i |Zw_ik| (0 beyond P)
 __global__ void __launch_bounds__(256) k_zexp(int Npad, int P, const double *__restrict__ Zw,
@@ -91,63 +92,53 @@ __global__ void k_ssplit(int P, int P16, const double *__restrict__ S, const int
 }
 
 template <int T16>
-__global__ void __launch_bounds__(256) k_gc_mixed(int N, int Npad, int P, int gsize, int J,
+__global__ void __launch_bounds__(GCW * 64) k_gc_mixed(int N, int Npad, int P, int gsize, int J,
         const double *__restrict__ Zw, const float *__restrict__ w32,
         const int *__restrict__ perm, const int *__restrict__ aexp, const int *__restrict__ zexp,
         const _Float16 *__restrict__ shi, const _Float16 *__restrict__ slo,
         float *__restrict__ part) {
     constexpr int P16 = 16 * T16;
     constexpr int KS = (P16 + 31) / 32;          // 32-k steps per gene (<= KP/32)
+    constexpr int NITEM = 2 * P16 * 4;           // per step: hi/lo x P16 cols x 4 groups of 8 k
+    constexpr int IPT = (NITEM + GCW * 64 - 1) / (GCW * 64);
     extern __shared__ __attribute__((aligned(16))) _Float16 Bs[];   // [2 buf][2 hl][P16][LDK]
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int i0 = blockIdx.x * 128;
     const int g = blockIdx.y;
     const int j0 = g * gsize, j1 = min(J, j0 + gsize);
-    // zw values of this lane's two cells: z[t][ks][e], k = 32 ks + 8(l/16) + e
-    float z[2][KS][8];
-    int cell[2];
+    // this lane's cell (row of the A fragment) and its zw values, k = 32 ks + 8(l/16) + e
+    const int cell = blockIdx.x * (16 * GCW) + 16 * w + (lane & 15);
+    float z[KS][8];
 #pragma unroll
-    for (int t = 0; t < 2; t++) {
-        cell[t] = i0 + 32 * w + 16 * t + (lane & 15);
+    for (int ks = 0; ks < KS; ks++)
 #pragma unroll
-        for (int ks = 0; ks < KS; ks++)
+        for (int e = 0; e < 8; e++) {
+            const int k = 32 * ks + 8 * (lane >> 4) + e;
+            z[ks][e] = (k < P) ? (float)ldexp(Zw[cell + (size_t)k * Npad], -zexp[k]) : 0.0f;
+        }
+    f4 acc[T16];
 #pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const int k = 32 * ks + 8 * (lane >> 4) + e;
-                z[t][ks][e] = (k < P) ? (float)ldexp(Zw[cell[t] + (size_t)k * Npad], -zexp[k]) : 0.0f;
-            }
-    }
-    f4 acc[2][T16];
-#pragma unroll
-    for (int t = 0; t < 2; t++)
-#pragma unroll
-        for (int ct = 0; ct < T16; ct++) acc[t][ct] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int ct = 0; ct < T16; ct++) acc[ct] = f4{0.f, 0.f, 0.f, 0.f};
 
-    // one chunk = one 32-k step: 2 (hi/lo) x P16 cols x 4 groups of 8 k, 16 B each
-    constexpr int NITEM = 2 * P16 * 4;
-    constexpr int IPT = (NITEM + 255) / 256;
     h8 pf[IPT];
     auto fetch = [&](int j, int ks) {
         const _Float16 *hj = shi + (size_t)j * P16 * KP, *lj = slo + (size_t)j * P16 * KP;
 #pragma unroll
         for (int r = 0; r < IPT; r++) {
-            const int it = tid + 256 * r;
+            const int it = tid + GCW * 64 * r;
             if (it < NITEM) {
                 const int hl = it / (P16 * 4), rem = it % (P16 * 4);
-                const int col = rem >> 2, kg = rem & 3;
-                pf[r] = *(const h8 *)((hl ? lj : hj) + (size_t)col * KP + ks * KCH + 8 * kg);
+                pf[r] = *(const h8 *)((hl ? lj : hj) + (size_t)(rem >> 2) * KP + ks * KCH + 8 * (rem & 3));
             }
         }
     };
     auto store = [&](int buf) {
 #pragma unroll
         for (int r = 0; r < IPT; r++) {
-            const int it = tid + 256 * r;
+            const int it = tid + GCW * 64 * r;
             if (it < NITEM) {
                 const int hl = it / (P16 * 4), rem = it % (P16 * 4);
-                const int col = rem >> 2, kg = rem & 3;
-                *(h8 *)&Bs[((buf * 2 + hl) * P16 + col) * LDK + 8 * kg] = pf[r];
+                *(h8 *)&Bs[((buf * 2 + hl) * P16 + (rem >> 2)) * LDK + 8 * (rem & 3)] = pf[r];
             }
         }
     };
@@ -157,46 +148,37 @@ __global__ void __launch_bounds__(256) k_gc_mixed(int N, int Npad, int P, int gs
     }
     __syncthreads();
     int cur = 0;
+    const int boff = (lane & 15) * LDK + 8 * (lane >> 4);
     for (int jj = j0; jj < j1; jj++) {
         const int j = perm[jj];
-        const float sa = ldexpf(1.0f, aexp[j]);
-        float o[2];
-#pragma unroll
-        for (int t = 0; t < 2; t++) o[t] = (cell[t] < N) ? w32[cell[t] + (size_t)j * Npad] * sa : 0.0f;
+        const float o = (cell < N) ? w32[cell + (size_t)j * Npad] * ldexpf(1.0f, aexp[j]) : 0.0f;
+        const int jn = (jj + 1 < j1) ? perm[jj + 1] : j;
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
             const bool more = (ks + 1 < KS) || (jj + 1 < j1);
-            if (more) {
-                if (ks + 1 < KS) fetch(j, ks + 1);
-                else fetch(perm[jj + 1], 0);
-            }
-            h8 ah[2], al[2];
+            if (ks + 1 < KS) fetch(j, ks + 1);
+            else if (jj + 1 < j1) fetch(jn, 0);
+            h8 ah, al;
 #pragma unroll
-            for (int t = 0; t < 2; t++) {
-#pragma unroll
-                for (int e = 0; e < 8; e++) {
-                    // v is materialised in FP32 before the split: a fused
-                    // multiply-convert (v_fma_mix) loses accuracy in the lo
-                    // part (measured: 3e-5 instead of 2e-7 relative error)
-                    float v = z[t][ks][e] * o[t];
-                    asm volatile("" : "+v"(v));
-                    const _Float16 hv = (_Float16)v;
-                    ah[t][e] = hv;
-                    al[t][e] = (_Float16)(v - (float)hv);
-                }
+            for (int e = 0; e < 8; e++) {
+                // v is materialised in FP32 before the split: a fused
+                // multiply-convert (v_fma_mix) loses accuracy in the lo
+                // part (measured: 3e-5 instead of 2e-7 relative error)
+                float v = z[ks][e] * o;
+                asm volatile("" : "+v"(v));
+                const _Float16 hv = (_Float16)v;
+                ah[e] = hv;
+                al[e] = (_Float16)(v - (float)hv);
             }
-            const _Float16 *bh = Bs + ((cur * 2 + 0) * P16 + (lane & 15)) * LDK + 8 * (lane >> 4);
-            const _Float16 *bl = bh + P16 * LDK;
+            const _Float16 *bh = Bs + (size_t)(cur * 2 + 0) * P16 * LDK + boff;
+            const _Float16 *bl = Bs + (size_t)(cur * 2 + 1) * P16 * LDK + boff;
 #pragma unroll
             for (int ct = 0; ct < T16; ct++) {
                 const h8 vbh = *(const h8 *)(bh + 16 * ct * LDK);
                 const h8 vbl = *(const h8 *)(bl + 16 * ct * LDK);
-#pragma unroll
-                for (int t = 0; t < 2; t++) {
-                    acc[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], vbh, acc[t][ct], 0, 0, 0);
-                    acc[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah[t], vbl, acc[t][ct], 0, 0, 0);
-                    acc[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al[t], vbh, acc[t][ct], 0, 0, 0);
-                }
+                acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, vbh, acc[ct], 0, 0, 0);
+                acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, vbl, acc[ct], 0, 0, 0);
+                acc[ct] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, vbh, acc[ct], 0, 0, 0);
             }
             if (more) store(cur ^ 1);
             __syncthreads();
@@ -205,16 +187,11 @@ __global__ void __launch_bounds__(256) k_gc_mixed(int N, int Npad, int P, int gs
     }
     // partial of this group: part[g][col][cell] (FP32, ld = Npad)
     float *pg = part + (size_t)g * P16 * Npad;
+    const int row0 = blockIdx.x * (16 * GCW) + 16 * w + 4 * (lane >> 4);
 #pragma unroll
-    for (int t = 0; t < 2; t++)
+    for (int ct = 0; ct < T16; ct++)
 #pragma unroll
-        for (int ct = 0; ct < T16; ct++)
-#pragma unroll
-            for (int r = 0; r < 4; r++) {
-                const int row = i0 + 32 * w + 16 * t + 4 * (lane >> 4) + r;
-                const int col = 16 * ct + (lane & 15);
-                pg[(size_t)col * Npad + row] = acc[t][ct][r];
-            }
+        for (int r = 0; r < 4; r++) pg[(size_t)(16 * ct + (lane & 15)) * Npad + row0 + r] = acc[ct][r];
 }
 
 __global__ void k_gc_reduce(int Npad, int P, int P16, int ngroup, const float *__restrict__ part,
@@ -235,8 +212,8 @@ static int launch_gc_mixed(gp_ctx *c, int gsize, int ngroup) {
     if (shm > 65536)
         GP_HIP(hipFuncSetAttribute((const void *)k_gc_mixed<T16>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    dim3 g(c->Npad / 128, ngroup);
-    k_gc_mixed<T16><<<g, 256, shm, c->stream>>>(c->N, c->Npad, c->P, gsize, c->J, c->Zw, c->w32,
+    dim3 g(c->Npad / (16 * GCW), ngroup);
+    k_gc_mixed<T16><<<g, GCW * 64, shm, c->stream>>>(c->N, c->Npad, c->P, gsize, c->J, c->Zw, c->w32,
                                                  c->gc_perm, c->gc_aexp, c->gc_zexp,
                                                  (const _Float16 *)c->Sh,
                                                  (const _Float16 *)c->Sl, c->gc_part);
