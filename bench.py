@@ -71,6 +71,23 @@ def cpu_baseline(ds, genes=(4, 8), threads=None):
                       f"to G={ds.J}: {t_full:.0f} s per unit"}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_mixed_N100K_G2K_M256.txt")
+
+
+def measured_traffic(group: str, precision: str, shape):
+    """HBM bytes per launch of the dominant kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (lines 'traffic <group> <bytes>');
+    only for the workload that summary was measured on."""
+    if precision != "mixed" or tuple(shape) != (100000, 2000, 10, 256) or not os.path.exists(TRAFFIC_FILE):
+        return None, None
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic" and parts[1] == group:
+                return float(parts[2]), os.path.relpath(TRAFFIC_FILE, ROOT)
+    return None, None
+
+
 def copy_calibration(mib: int, reps: int = 5) -> None:
     """A plain device-to-device copy of known size: its FETCH_SIZE / WRITE_SIZE
     counter readings give the scale factor from counter units to HBM bytes."""
@@ -99,6 +116,7 @@ def main():
     ap.add_argument("--Q", type=int, default=10)
     ap.add_argument("--M", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the mixed-path autotuner")
     ap.add_argument("--only", action="store_true",
                     help="run only --precision (no FP64 comparison run)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -128,8 +146,14 @@ def main():
     log(f"setup done, ELBO at start {e0:.10e}")
     P = ds.P
 
+    tuning = None
+
     def run(prec_name, steps, warmup):
+        nonlocal tuning
         g.set_precision(lib.PREC_MIXED if prec_name == "mixed" else lib.PREC_FP64)
+        if prec_name == "mixed" and not args.no_tune:
+            tuning = g.autotune()
+            log(f"[mixed] autotuned {tuning}")
         for w in range(warmup):
             t1 = time.perf_counter()
             g.elbo(1)
@@ -168,9 +192,10 @@ def main():
     peaks = {"gram": peak, "gradcontract": peak}
     dom = max(("gram", "gradcontract"), key=lambda k: kern[k])
     achieved = cands[dom]
+    traffic, tsrc = measured_traffic(dom, main_p, (ds.N, ds.J, ds.Q, ds.M))
     roofline = {"kernel": dom, "bound": "mfma", "achieved": achieved / 1e12,
                 "peak": peaks[dom] / 1e12, "unit": "TFLOP/s", "frac": achieved / peaks[dom],
-                "traffic": None,
+                "traffic": traffic, "traffic_source": tsrc,
                 "flops_per_launch": gram_flops(ds.N, ds.J, P) if dom == "gram" else gradcontract_flops(ds.N, ds.J, P),
                 "others": {k: {"achieved_tflops": v / 1e12, "frac": v / peaks[k], "avg_ms": kern[k]}
                            for k, v in cands.items()}}
@@ -196,6 +221,7 @@ def main():
         "kernels_ms": kern,
         "roofline": roofline,
         "cpu_baseline": None,
+        "tuning": tuning,
     }
     if "mixed" in res and "f64" in res:
         f, m = res["f64"], res["mixed"]

@@ -231,6 +231,8 @@ extern "C" int gp_set_option(gp_ctx *c, const char *name, int value) {
         c->gc_groups = value;
     } else if (n == "gc_mixed") {
         c->gc_mixed = value ? 1 : 0;
+    } else if (n == "autotune") {
+        c->autotune = value ? 1 : 0;
     } else {
         gp_set_error("unknown option " + n);
         return 1;
@@ -244,6 +246,7 @@ extern "C" int gp_get_option(gp_ctx *c, const char *name, int *value) {
     if (n == "gram_splits") *value = c->gram_splits;
     else if (n == "gc_groups") *value = c->gc_groups;
     else if (n == "gc_mixed") *value = c->gc_mixed;
+    else if (n == "autotune") *value = c->autotune;
     else { gp_set_error("unknown option " + n); return 1; }
     return 0;
 }
@@ -580,6 +583,10 @@ extern "C" int gp_fit(gp_ctx *c, int itrmax, int jtr_cap, int verbose, gp_outer_
         if (r > 0) return r;
         last = lk[0];
         if (verbose) printf("lkhd[%d,0]=%lf\n", itr, lk[0]);
+        if (itr == 0 && c->autotune && c->precision == GP_PREC_MIXED) {
+            r = gp_autotune(c, verbose);
+            if (r > 0) return r;
+        }
         if (itr > 20) rhoflag = 1;
         if (itr == 30) JTRMAX = 10000;
         if (jtr_cap > 0 && JTRMAX > jtr_cap) JTRMAX = jtr_cap;
@@ -604,6 +611,75 @@ extern "C" int gp_fit(gp_ctx *c, int itrmax, int jtr_cap, int verbose, gp_outer_
     }
     if (obj) *obj = last;
     return 0;
+}
+
+// ------------------------------------------------------------------ autotuner
+// Times the mixed Gram (per gram_splits) and the mixed contraction (per
+// gc_groups) on the context's current data and keeps the fastest settings.
+// Leaves Psi (factor) and Sinv recomputed at the current parameters in the
+// whitened basis, as gp_elbo + gp_grad would.
+namespace {
+int time_ms(gp_ctx *c, int (*fn)(gp_ctx *), int reps, double *ms) {
+    hipEvent_t a, b;
+    GP_HIP(hipEventCreate(&a));
+    GP_HIP(hipEventCreate(&b));
+    GP_TRY(fn(c));                       // warm-up
+    GP_HIP(hipEventRecord(a, c->stream));
+    for (int r = 0; r < reps; r++) GP_TRY(fn(c));
+    GP_HIP(hipEventRecord(b, c->stream));
+    GP_HIP(hipEventSynchronize(b));
+    float t = 0.0f;
+    GP_HIP(hipEventElapsedTime(&t, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *ms = t / reps;
+    return 0;
+}
+int tune_gram(gp_ctx *c) { return gpk_gram_mixed(c, c->W, c->Psi, c->J, c->Zw, c->DeltaW); }
+int tune_gc(gp_ctx *c) { return gpk_gradcontract_mixed(c); }
+}  // namespace
+
+extern "C" int gp_autotune(gp_ctx *c, int verbose) {
+    if (!c) { gp_set_error("NULL context"); return 1; }
+    if (c->precision != GP_PREC_MIXED) { gp_set_error("gp_autotune needs GP_PREC_MIXED"); return 1; }
+    static const int splits[] = {8, 16, 32};
+    static const int groups[] = {16, 32, 64};
+    GP_TRY(gpk_expand_delta(c));
+    if (!c->zw_valid) {
+        GP_TRY(gpk_whiten(c));
+        c->zw_valid = 1;
+        c->zsplit_valid = 0;
+    }
+    double best = 1e300;
+    int bs = c->gram_splits;
+    for (int sv : splits) {
+        c->gram_splits = sv;
+        double ms = 0.0;
+        GP_TRY(time_ms(c, tune_gram, 2, &ms));
+        if (verbose) std::printf("autotune: gram_splits %d: %.3f ms\n", sv, ms);
+        if (ms < best) { best = ms; bs = sv; }
+    }
+    c->gram_splits = bs;
+    // the contraction needs the inverses of the current factors
+    GP_TRY(clear_info(c));
+    GP_TRY(tune_gram(c));
+    GP_TRY(gpk_chol_batched(c, c->Psi, c->P, c->J, (size_t)c->P * c->P, c->info));
+    c->psi_whitened = 1;
+    GP_TRY(gpk_chol_inverse_batched(c, c->Psi, c->P, c->J, (size_t)c->P * c->P, c->Sinv));
+    best = 1e300;
+    int bg = c->gc_groups;
+    for (int gv : groups) {
+        c->gc_groups = gv;
+        double ms = 0.0;
+        GP_TRY(time_ms(c, tune_gc, 2, &ms));
+        if (verbose) std::printf("autotune: gc_groups %d: %.3f ms\n", gv, ms);
+        if (ms < best) { best = ms; bg = gv; }
+    }
+    c->gc_groups = bg;
+    GP_HIP(hipStreamSynchronize(c->stream));
+    int bad = 0;
+    GP_TRY(read_info(c, &bad));
+    return bad ? -bad : 0;
 }
 
 // ------------------------------------------------------------------ timing API

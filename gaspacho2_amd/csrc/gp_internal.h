@@ -110,6 +110,7 @@ struct gp_ctx {
     size_t gc_part_len = 0;
     int gc_groups = 32;        // gene groups of the mixed contraction
     int gc_mixed = 1;          // mixed path: contraction on FP16 MFMA (0: FP64)
+    int autotune = 0;          // gp_fit runs gp_autotune before its first quasi-likelihood phase
 
     size_t tot_max = 0;        // N+p+(N+M+1)Q
     int timing = 0;

@@ -48,6 +48,7 @@ def load() -> C.CDLL:
         "gp_set_precision": (C.c_int, [vp, C.c_int]),
         "gp_set_option": (C.c_int, [vp, C.c_char_p, C.c_int]),
         "gp_get_option": (C.c_int, [vp, C.c_char_p, ip]),
+        "gp_autotune": (C.c_int, [vp, C.c_int]),
         "gp_buffer_len": (C.c_size_t, [vp, C.c_char_p]),
         "gp_get_buffer": (C.c_int, [vp, C.c_char_p, dp, C.c_size_t]),
         "gp_set_buffer": (C.c_int, [vp, C.c_char_p, dp, C.c_size_t]),
@@ -166,6 +167,11 @@ class GPContext:
         v = C.c_int()
         self._check(self._lib.gp_get_option(self._h, name.encode(), C.byref(v)))
         return v.value
+
+    def autotune(self, verbose: bool = False) -> dict:
+        """Pick gram_splits / gc_groups by timing them on the current state."""
+        self._check(self._lib.gp_autotune(self._h, int(bool(verbose))))
+        return {k: self.get_option(k) for k in ("gram_splits", "gc_groups")}
 
     # -- launch groups
     def init_fit(self) -> None:

@@ -78,9 +78,18 @@ int gp_set_precision(gp_ctx *ctx, int precision);
  *                           (FP32 partial per group, FP64 sum across groups)
  *   "gc_mixed"    0/1       mixed path: contraction on FP16 hi/lo MFMA (1)
  *                           or in FP64 (0)
+ *   "autotune"    0/1       gp_fit calls gp_autotune before its first
+ *                           quasi-likelihood phase (mixed precision only)
  * Unknown names and out-of-range values return a status > 0. */
 int gp_set_option(gp_ctx *ctx, const char *name, int value);
 int gp_get_option(gp_ctx *ctx, const char *name, int *value);
+
+/* Autotuner (no counterpart in the reference): times the mixed-precision
+ * Gram for gram_splits in {8, 16, 32} and the mixed contraction for
+ * gc_groups in {16, 32, 64} on the context's current state and keeps the
+ * fastest of each.  Needs GP_PREC_MIXED and a state after gp_ql_start; it
+ * recomputes the per-gene factors and inverses at the current parameters. */
+int gp_autotune(gp_ctx *ctx, int verbose);
 
 /* Named state buffers in the reference layout (host <-> device copies).
  * Names: Omega delta X T rho phi (value|backup halves where the reference

@@ -297,3 +297,32 @@ def test_options_reject_bad_values(tiny_ds):
         g.set_option("no_such_option", 1)
     g.set_option("gc_groups", 7)
     assert g.get_option("gc_groups") == 7
+
+
+def test_autotune_picks_valid_settings_and_keeps_results():
+    """gp_autotune picks gram_splits / gc_groups among its candidates and the
+    mixed ELBO and gradient after tuning stay within the mixed-path error of
+    the FP64 values."""
+    from gaspacho2_amd import lib, synth
+    ds = synth.make_dataset(3000, 40, 5, 64, seed=4)
+    T0 = lib.init_ta_with_per(ds.M, ds.Q)
+    g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0,
+                      T0, precision=lib.PREC_FP64)
+    g.init_fit()
+    for jtr in range(2):
+        g.irls_step(jtr)
+    g.ql_start()
+    e64 = g.elbo(1)
+    g.grad()
+    g64 = g.get("grad")
+    g.set_precision(lib.PREC_MIXED)
+    g.set_precision(lib.PREC_FP64)
+    with pytest.raises(lib.GPError):
+        g.autotune()                      # FP64 path has nothing to tune
+    g.set_precision(lib.PREC_MIXED)
+    choice = g.autotune()
+    assert choice["gram_splits"] in (8, 16, 32) and choice["gc_groups"] in (16, 32, 64)
+    em = g.elbo(1)
+    g.grad()
+    assert abs(em - e64) <= 1e-7 * abs(e64)
+    assert rel(g.get("grad"), g64) < 1e-4
