@@ -127,14 +127,27 @@ __global__ void __launch_bounds__(CT) k_cholsl(const double *__restrict__ Lall, 
     for (int k = tid; k < n; k += CT) x[(size_t)j * n + k] = xs[k];
 }
 
-__global__ void k_identity(double *S, int n, int batch, size_t stride) {
-    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
-    const size_t nn = (size_t)n * n;
-    if (idx >= nn * batch) return;
-    const size_t j = idx / nn, e = idx % nn;
-    S[j * stride + e] = ((e % n) == (e / n)) ? 1.0 : 0.0;
-}
 
+
+// One matrix: column j of V = L^{-1} by column-oriented forward substitution
+// (one workgroup per column; the strict upper triangle of V is written as 0).
+__global__ void __launch_bounds__(CT) k_trinv_cols(const double *__restrict__ L, int n,
+                                                   double *__restrict__ V) {
+    extern __shared__ double x[];      // running right-hand side of column j
+    const int j = blockIdx.x, tid = threadIdx.x;
+    double *v = V + (size_t)j * n;
+    for (int r = tid; r < n; r += CT) {
+        x[r] = (r == j) ? 1.0 : 0.0;
+        if (r < j) v[r] = 0.0;
+    }
+    __syncthreads();
+    for (int k = j; k < n; k++) {
+        const double vk = x[k] / L[k + (size_t)k * n];
+        if (tid == 0) v[k] = vk;
+        for (int r = k + 1 + tid; r < n; r += CT) x[r] -= L[r + (size_t)k * n] * vk;
+        __syncthreads();
+    }
+}
 
 // V_j = L_j^{-1} (lower triangle of V; the strict upper triangle is not
 // written).  Columns are produced right to left: with L = [[l, 0], [c, L2]],
@@ -219,7 +232,7 @@ __global__ void __launch_bounds__(CT) k_vtv(const double *__restrict__ Vall, int
 int gpk_chol_batched(gp_ctx *c, double *A, int n, int batch, size_t stride, int *info) {
     if (n > MAXN) { gp_set_error("Cholesky size exceeds 640"); return 1; }
     const size_t shm = ((size_t)n * NB + (size_t)NB * KC) * sizeof(double);
-    if (shm > 65536)
+    if (shm > 65536 && !c->capturing)   // set before any recording: not a stream operation
         GP_HIP(hipFuncSetAttribute((const void *)k_chol, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     k_chol<<<batch, CT, shm, c->stream>>>(A, n, stride, info);
     GP_HIP(hipGetLastError());
@@ -262,11 +275,7 @@ int gpk_vtv(gp_ctx *c, const double *V, int n, int batch, size_t stride, double 
 }
 
 int gpk_tri_inverse(gp_ctx *c, const double *L, int n, double *V) {
-    const size_t tot = (size_t)n * n;
-    k_identity<<<(unsigned)((tot + 255) / 256), 256, 0, c->stream>>>(V, n, 1, 0);
+    k_trinv_cols<<<n, CT, (size_t)n * sizeof(double), c->stream>>>(L, n, V);
     GP_HIP(hipGetLastError());
-    const double one = 1.0;
-    GP_BLAS(rocblas_dtrsm(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
-                          rocblas_diagonal_non_unit, n, n, &one, L, n, V, n));
     return 0;
 }

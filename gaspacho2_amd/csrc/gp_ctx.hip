@@ -17,7 +17,7 @@ extern "C" int gp_version(void) { return 1 * 10000 + 0 * 100 + 0; }
 
 // ------------------------------------------------------------------ timing
 void gp_timer_begin(gp_ctx *c, gp_timer &t) {
-    if (!c->timing) return;
+    if (!c->timing || c->capturing) return;
     if (!t.a) { (void)hipEventCreate(&t.a); (void)hipEventCreate(&t.b); }
     (void)hipEventRecord(t.a, c->stream);
     t.open = true;
@@ -47,7 +47,9 @@ template <class T> static int dmalloc(T **p, size_t n) {
     return 0;
 }
 
+static void drop_graph(gp_ctx *c);
 static void free_all(gp_ctx *c) {
+    drop_graph(c);
     void *bufs[] = {c->Yp, c->Yi, c->Yx, c->d_nh, c->Z, c->ls, c->Omega, c->delta, c->X, c->T,
                     c->rho, c->phi, c->zeta, c->K, c->Ksym, c->Kchol, c->Kinv, c->A, c->titsias,
                     c->DeltaInv, c->DeltaChol, c->W, c->Oinvt, c->Nu, c->Psi, c->Sinv, c->Beta,
@@ -57,7 +59,7 @@ static void free_all(gp_ctx *c) {
                     c->lkhd, c->info, c->Zh, c->Zl, c->z32, c->gram_part, c->pwork, c->w32, c->wsc,
                     c->wun, c->zun, c->Linv, c->Zw, c->DeltaW, c->DeltaWChol, c->Uw, c->tBw,
                     c->Vtmp, c->Sh, c->Sl, c->gc_perm, c->gc_aexp, c->gc_sexp, c->gc_bexp,
-                    c->gc_gun, c->gc_part, c->gc_zexp, c->skw};
+                    c->gc_gun, c->gc_part, c->gc_zexp, c->skw, c->gc_kappa};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     gp_timer *ts[] = {&c->t_gram, &c->t_chol, &c->t_gc, &c->t_kuf, &c->t_elbo, &c->t_grad};
@@ -386,15 +388,14 @@ extern "C" int gp_set_buffer(gp_ctx *c, const char *name, const double *host, si
 // ------------------------------------------------------------------ groups
 static int read_info(gp_ctx *c, int *bad) {
     int h = 0;
+    *bad = 0;
+    if (c->capturing) return 0;
     GP_HIP(hipMemcpyAsync(&h, c->info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     GP_HIP(hipStreamSynchronize(c->stream));
     *bad = h;
     return 0;
 }
-static int clear_info(gp_ctx *c) {
-    GP_HIP(hipMemsetAsync(c->info, 0, sizeof(int), c->stream));
-    return 0;
-}
+static int clear_info(gp_ctx *c) { return gpk_fill_int(c, c->info, 1, 0); }
 
 // Kernel matrices: getKernelMats, cholKmm, explicit Kmm^{-1}, getKinvKmn.
 static int kernel_mats(gp_ctx *c) {
@@ -533,8 +534,57 @@ extern "C" int gp_grad(gp_ctx *c) {
     GP_TRY(gpk_c2(c, w));
     GP_TRY(gpk_grad_tau_rho(c));
     gp_timer_end(c, c->t_grad);
-    GP_HIP(hipStreamSynchronize(c->stream));
+    if (!c->capturing) GP_HIP(hipStreamSynchronize(c->stream));
     return 0;
+}
+
+// ------------------------------------------------------------------ graph
+static void drop_graph(gp_ctx *c) {
+    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+    if (c->graph) (void)hipGraphDestroy(c->graph);
+    c->graph_exec = nullptr;
+    c->graph = nullptr;
+    for (int &k : c->graph_key) k = -1;
+}
+
+// One ELBO + gradient unit (gp_elbo(slot) then gp_grad) recorded once as a
+// HIP graph and replayed: every kernel reads the parameters from device
+// memory, so a replay evaluates the unit at the current parameters.  The
+// first call (and any call after the precision, the slot or a launch-shape
+// option changed) runs the unit eagerly once -- which makes every lazy
+// allocation -- and then records it.
+extern "C" int gp_eval_graph(gp_ctx *c, int slot, double *obj) {
+    if (!c) { gp_set_error("NULL context"); return 1; }
+    if (slot < 0 || slot >= GP_LKHD_SLOTS) { gp_set_error("slot out of range"); return 1; }
+    const int key[4] = {c->precision, slot, c->gram_splits, c->gc_groups};
+    if (!c->graph_exec || !std::equal(key, key + 4, c->graph_key)) {
+        drop_graph(c);
+        double tmp = 0.0;
+        int r = gp_elbo(c, slot, &tmp);
+        if (r > 0) return r;
+        if ((r = gp_grad(c)) > 0) return r;
+        GP_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        c->capturing = 1;
+        r = gp_elbo(c, slot, nullptr);
+        if (r <= 0) r = gp_grad(c);
+        c->capturing = 0;
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (r > 0) { if (g) (void)hipGraphDestroy(g); return r; }
+        if (e != hipSuccess) {
+            gp_set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            return (int)e;
+        }
+        c->graph = g;
+        GP_HIP(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+        std::copy(key, key + 4, c->graph_key);
+    }
+    GP_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+    if (obj) GP_HIP(hipMemcpyAsync(obj, c->lkhd + slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    GP_HIP(hipStreamSynchronize(c->stream));
+    int bad = 0;
+    GP_TRY(read_info(c, &bad));
+    return bad ? -bad : 0;
 }
 
 extern "C" int gp_lbfgs_step(gp_ctx *c, int jtr, int rhoflag) {

@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(256) k_gc_mfma_f64(int N, int Npad, int J, int
 template <int T16>
 static int launch_gc(gp_ctx *c, const double *Zs, double *Uo) {
     const size_t shm = 2 * 16 * (size_t)(16 * T16) * sizeof(double);
-    if (shm > 65536)
+    if (shm > 65536 && !c->capturing)   // set before any recording: not a stream operation
         GP_HIP(hipFuncSetAttribute((const void *)k_gc_mfma_f64<T16>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     k_gc_mfma_f64<T16><<<c->Npad / 64, 256, shm, c->stream>>>(c->N, c->Npad, c->J, c->P, Zs,

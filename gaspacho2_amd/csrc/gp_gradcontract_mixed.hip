@@ -75,6 +75,46 @@ __global__ void __launch_bounds__(256) k_smax(int P, const double *__restrict__ 
     }
 }
 
+// Gene groups of the contraction, on the device (no host round trip, so the
+// unit can be captured in a graph): kappa_j = e_j + s_j with 2^e_j = wun[j];
+// genes stably sorted by kappa (counting sort by one thread: J keys in a
+// small integer range), cut into consecutive groups of gsize; group g gets
+// C_g = 26 - max kappa (its last member), gene j the operand exponents
+// aexp_j = 12 - e_j and bexp_j = e_j - 12 + C_g, and gun[g] = 2^-C_g.
+__global__ void __launch_bounds__(256) k_gc_plan(int J, int gsize, int ngroup,
+        const double *__restrict__ wun, const int *__restrict__ sexp, int *__restrict__ perm,
+        int *__restrict__ aexp, int *__restrict__ bexp, double *__restrict__ gun,
+        int *__restrict__ kappa) {
+    constexpr int NB = 1024, KOFF = 512;     // key range [-512, 511]
+    __shared__ int cnt[NB];
+    for (int b = threadIdx.x; b < NB; b += 256) cnt[b] = 0;
+    for (int j = threadIdx.x; j < J; j += 256) {
+        int e = 0;
+        (void)frexp(wun[j], &e);             // wun = 2^e_j exactly -> frexp gives e_j + 1
+        kappa[j] = min(max((e - 1) + sexp[j], -KOFF), KOFF - 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int j = 0; j < J; j++) cnt[kappa[j] + KOFF]++;
+        int run = 0;
+        for (int b = 0; b < NB; b++) { const int t = cnt[b]; cnt[b] = run; run += t; }
+        for (int j = 0; j < J; j++) perm[cnt[kappa[j] + KOFF]++] = j;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < J; t += 256) {
+        const int g = t / gsize;
+        const int last = min(J, (g + 1) * gsize) - 1;
+        const int Cg = 26 - kappa[perm[last]];
+        const int j = perm[t];
+        int e = 0;
+        (void)frexp(wun[j], &e);
+        aexp[j] = 12 - (e - 1);
+        bexp[j] = (e - 1) - 12 + Cg;
+        if (t == g * gsize) gun[g] = ldexp(1.0, -Cg);
+    }
+    (void)ngroup;
+}
+
 // Sw_j[k,c] * 2^(e_j - 12 + C_g + zc_k) split into FP16 hi/lo, layout [col < P16][k < KP]
 __global__ void k_ssplit(int P, int P16, const double *__restrict__ S, const int *__restrict__ scale_exp,
                          const int *__restrict__ zexp, _Float16 *__restrict__ sh,
@@ -209,7 +249,7 @@ template <int T16>
 static int launch_gc_mixed(gp_ctx *c, int gsize, int ngroup) {
     constexpr int P16 = 16 * T16;
     const size_t shm = (size_t)2 * 2 * P16 * LDK * sizeof(_Float16);
-    if (shm > 65536)
+    if (shm > 65536 && !c->capturing)   // set before any recording: not a stream operation
         GP_HIP(hipFuncSetAttribute((const void *)k_gc_mixed<T16>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     dim3 g(c->Npad / (16 * GCW), ngroup);
@@ -237,6 +277,7 @@ int gpk_gradcontract_mixed(gp_ctx *c) {
         GP_HIP(hipMalloc((void **)&c->gc_bexp, (size_t)J * sizeof(int)));
         GP_HIP(hipMalloc((void **)&c->gc_gun, (size_t)J * sizeof(double)));
         GP_HIP(hipMalloc((void **)&c->gc_zexp, (size_t)KP * sizeof(int)));
+        GP_HIP(hipMalloc((void **)&c->gc_kappa, (size_t)J * sizeof(int)));
     }
     if (c->gc_part_len < (size_t)ngroup * P16 * c->Npad) {
         if (c->gc_part) (void)hipFree(c->gc_part);
@@ -250,36 +291,9 @@ int gpk_gradcontract_mixed(gp_ctx *c) {
     GP_HIP(hipGetLastError());
     k_smax<<<J, 256, 0, c->stream>>>(c->P, c->Sinv, c->gc_zexp, c->gc_sexp);
     GP_HIP(hipGetLastError());
-    std::vector<double> wun(J);
-    std::vector<int> sexp(J);
-    GP_HIP(hipMemcpyAsync(wun.data(), c->wun, J * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    GP_HIP(hipMemcpyAsync(sexp.data(), c->gc_sexp, J * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    GP_HIP(hipStreamSynchronize(c->stream));
-    std::vector<int> e(J), kappa(J), perm(J), aexp(J), bexp(J);
-    for (int j = 0; j < J; j++) {
-        int ej = 0;
-        (void)std::frexp(wun[j], &ej);   // wun = 2^e exactly -> frexp gives e+1
-        e[j] = ej - 1;
-        kappa[j] = e[j] + sexp[j];
-    }
-    std::iota(perm.begin(), perm.end(), 0);
-    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return kappa[a] < kappa[b]; });
-    std::vector<double> gun(ngroup);
-    for (int g = 0; g < ngroup; g++) {
-        int mk = -100000;
-        for (int t = g * gsize; t < std::min(J, (g + 1) * gsize); t++) mk = std::max(mk, kappa[perm[t]]);
-        const int Cg = 26 - mk;
-        gun[g] = std::ldexp(1.0, -Cg);
-        for (int t = g * gsize; t < std::min(J, (g + 1) * gsize); t++) {
-            const int j = perm[t];
-            aexp[j] = 12 - e[j];
-            bexp[j] = e[j] - 12 + Cg;
-        }
-    }
-    GP_HIP(hipMemcpyAsync(c->gc_perm, perm.data(), J * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    GP_HIP(hipMemcpyAsync(c->gc_aexp, aexp.data(), J * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    GP_HIP(hipMemcpyAsync(c->gc_bexp, bexp.data(), J * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    GP_HIP(hipMemcpyAsync(c->gc_gun, gun.data(), ngroup * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    k_gc_plan<<<1, 256, 0, c->stream>>>(J, gsize, ngroup, c->wun, c->gc_sexp, c->gc_perm,
+                                        c->gc_aexp, c->gc_bexp, c->gc_gun, c->gc_kappa);
+    GP_HIP(hipGetLastError());
     dim3 gs((unsigned)(((size_t)P16 * KP + 255) / 256), J);
     k_ssplit<<<gs, 256, 0, c->stream>>>(c->P, P16, c->Sinv, c->gc_bexp, c->gc_zexp,
                                         (_Float16 *)c->Sh, (_Float16 *)c->Sl);

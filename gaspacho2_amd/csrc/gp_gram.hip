@@ -208,7 +208,7 @@ static int launch_gram_mfma(gp_ctx *c, const double *W, double *Psi, int batch, 
     const int ntot = T16 * (T16 + 1) / 2;
     const int ng = (ntot + TPG - 1) / TPG;
     const size_t shm = 2 * (size_t)KCM * (P16 + 1) * sizeof(double);
-    if (shm > 65536)
+    if (shm > 65536 && !c->capturing)   // set before any recording: not a stream operation
         GP_HIP(hipFuncSetAttribute((const void *)k_gram_mfma_f64<PF>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     dim3 g(ng, batch);

@@ -605,7 +605,7 @@ static int launch_gram_sym(gp_ctx *c, int seg, int nseg, int batch) {
     constexpr int P16 = 16 * T16;
     static_assert(plan_fits(T16), "block plan does not fit the waves");
     const size_t shm = (size_t)2 * 2 * P16 * LDX * sizeof(_Float16);
-    if (shm > 65536)
+    if (shm > 65536 && !c->capturing)   // set before any recording: not a stream operation
         GP_HIP(hipFuncSetAttribute((const void *)k_gram_sym<T16>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     k_gram_sym<T16><<<(unsigned)(batch * nseg), GW * 64, shm, c->stream>>>(

@@ -105,6 +105,8 @@ struct gp_ctx {
     void *Sh = nullptr, *Sl = nullptr;  // J x P16 x 288 halves, [col][k]
     int *gc_perm = nullptr, *gc_aexp = nullptr, *gc_sexp = nullptr, *gc_bexp = nullptr;
     int *gc_zexp = nullptr;    // per column k of Zw: 2^zc_k >= max |Zw[:,k]|
+    int *gc_kappa = nullptr;   // per gene exponent of max o * max |Sw| (group planning)
+    int capturing = 0;         // recording a graph: no host reads, syncs or allocations
     double *gc_gun = nullptr;  // per group 2^-C_g
     float *gc_part = nullptr;  // per group FP32 partial U (P16 x Npad)
     size_t gc_part_len = 0;
@@ -115,6 +117,10 @@ struct gp_ctx {
     size_t tot_max = 0;        // N+p+(N+M+1)Q
     int timing = 0;
     gp_timer t_gram, t_chol, t_gc, t_kuf, t_elbo, t_grad;
+    // recorded ELBO + gradient unit
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_key[4] = {-1, -1, -1, -1};   // precision, slot, gram_splits, gc_groups
     std::string err;
 };
 
@@ -166,6 +172,7 @@ int gpk_lbfgs(gp_ctx *c, int itr, int rhoflag);
 int gpk_downdate(gp_ctx *c, int itr, double r, int rhoflag);
 int gpk_beta_from_tilde(gp_ctx *c);
 int gpk_fill(gp_ctx *c, double *p, size_t n, double v);
+int gpk_fill_int(gp_ctx *c, int *p, size_t n, int v);   // a kernel, not a memset (graph-safe)
 // C (M x N, ld ldc) = A^T B with A K x M (lda), B K x N (ldb), K large and
 // M x N small: split over K into strided-batched GEMMs, partials summed in FP64.
 int gpk_gemm_tn_splitk(gp_ctx *c, int M, int N, int K, const double *A, int lda,

@@ -267,8 +267,9 @@ __global__ void __launch_bounds__(NT) k_beta_phi(int N, int P, int jtr,
 // ggplvm.cu:703-724: sum_nz (log W - delta t wt) y - sum_i W.
 __global__ void __launch_bounds__(NT) k_lkhd_pois(int N, int Npad, const int *Yp,
         const int *Yi, const double *Yx, const double *W, const double *tits,
-        double dt, double wt, double *lkhds) {
+        const double *__restrict__ dtp, double wt, double *lkhds) {
     __shared__ double sh[4];
+    const double dt = *dtp;
     const int j = blockIdx.x;
     const double *Wj = W + (size_t)j * Npad;
     double s = 0.0;
@@ -310,9 +311,10 @@ __global__ void __launch_bounds__(NT) k_lkhd_genes(int N, int P, const double *P
 // ---------------------------------------------------------------- collectLkhd
 // ggplvm.cu:788-815; stage 1 over cells, stage 2 adds the gene sum.
 __global__ void __launch_bounds__(NT) k_collect_cells(int N, const double *Oinvt,
-        const double *tits, const double *Omega, double wtdt, double igb, double igap1,
-        double *part) {
+        const double *tits, const double *Omega, const double *__restrict__ dtp, double wt,
+        double igb, double igap1, double *part) {
     __shared__ double sh[4];
+    const double wtdt = wt * *dtp;
     const int per = (N + RED_BLOCKS - 1) / RED_BLOCKS;
     const int lo = blockIdx.x * per, hi = min(N, lo + per);
     double s = 0.0;
@@ -429,10 +431,11 @@ __global__ void k_tn(int N, int Npad, int J, const double *W, const double *Nu,
 __global__ void k_cell_grad(int N, int Npad, int J, int M, int Q, int P, int g0, int ard,
         const double *Z, const double *U, const double *C1a, const double *A,
         const double *Omega, const double *Oinvt, const double *tits, const double *E,
-        const double *X, const double *T, const double *rho, double dt, double wt,
-        double igb, double igap1, double *gO, double *gX, double *C1) {
+        const double *X, const double *T, const double *rho, const double *__restrict__ dtp,
+        double wt, double igb, double igap1, double *gO, double *gX, double *C1) {
     const int i = blockIdx.x * NT + threadIdx.x;
     if (i >= Npad) return;
+    const double dt = *dtp;
     if (i >= N) {
         for (int k = 0; k < M; k++) C1[i + (size_t)k * Npad] = 0.0;
         return;
@@ -489,10 +492,12 @@ __global__ void k_sum_sinv(int J, int P, int M, int g0, const double *Sinv, doub
     S2[idx] = s;
 }
 // ggplvm.cu:57-100 getC2
-__global__ void k_c2(int M, int J, double wt, double dt, const double *G2, const double *B2,
-                     const double *S2, const double *Kinv, const double *K, double *C2) {
+__global__ void k_c2(int M, int J, double wt, const double *__restrict__ dtp, const double *G2,
+                     const double *B2, const double *S2, const double *Kinv, const double *K,
+                     double *C2) {
     const size_t idx = (size_t)blockIdx.x * NT + threadIdx.x;
     if (idx >= (size_t)M * M) return;
+    const double dt = *dtp;
     const int j = (int)(idx % M), k = (int)(idx / M);
     if (j > k) return;
     const size_t a = j + (size_t)k * M;
@@ -691,6 +696,19 @@ __global__ void k_fill(size_t n, double *p, double v) {
 
 #define LAUNCH_CHECK() GP_HIP(hipGetLastError())
 
+namespace {
+__global__ void k_fill_int(size_t n, int *p, int v) {
+    const size_t i = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+}  // namespace
+
+int gpk_fill_int(gp_ctx *c, int *p, size_t n, int v) {
+    k_fill_int<<<nblk(n), NT, 0, c->stream>>>(n, p, v);
+    LAUNCH_CHECK();
+    return 0;
+}
+
 int gpk_fill(gp_ctx *c, double *p, size_t n, double v) {
     if (!n) return 0;
     k_fill<<<nblk(n), NT, 0, c->stream>>>(n, p, v);
@@ -821,11 +839,8 @@ int gpk_beta_phi(gp_ctx *c, int jtr) {
 }
 
 int gpk_lkhd_pois(gp_ctx *c) {
-    double dt;
-    GP_HIP(hipMemcpyAsync(&dt, c->delta + c->p - 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    GP_HIP(hipStreamSynchronize(c->stream));
     k_lkhd_pois<<<c->J, NT, 0, c->stream>>>(c->N, c->Npad, c->Yp, c->Yi, c->Yx, c->W, c->titsias,
-                                            dt, 0.1, c->lkhds);
+                                            c->delta + c->p - 1, 0.1, c->lkhds);
     LAUNCH_CHECK();
     return 0;
 }
@@ -838,19 +853,17 @@ int gpk_lkhd_genes_with(gp_ctx *c, const double *tb, const double *Dchol) {
 }
 
 int gpk_collect(gp_ctx *c, int Nterm, int slot, double *obj) {
-    double dt;
-    GP_HIP(hipMemcpyAsync(&dt, c->delta + c->p - 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    GP_HIP(hipStreamSynchronize(c->stream));
     int npart = 0;
     if (Nterm > 0) {
         k_collect_cells<<<RED_BLOCKS, NT, 0, c->stream>>>(Nterm, c->Oinvt, c->titsias, c->Omega,
-                                                          0.1 * dt, 100.0, 100.0, c->red);
+                                                          c->delta + c->p - 1, 0.1, 100.0, 100.0,
+                                                          c->red);
         LAUNCH_CHECK();
         npart = RED_BLOCKS;
     }
     k_collect_final<<<1, NT, 0, c->stream>>>(c->J, c->lkhds, c->red, npart, c->lkhd, slot);
     LAUNCH_CHECK();
-    if (obj) {
+    if (obj && !c->capturing) {
         GP_HIP(hipMemcpyAsync(obj, c->lkhd + slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         GP_HIP(hipStreamSynchronize(c->stream));
     }
@@ -886,21 +899,16 @@ int gpk_tn(gp_ctx *c) {
 }
 
 int gpk_cell_grad(gp_ctx *c) {
-    double dt;
-    GP_HIP(hipMemcpyAsync(&dt, c->delta + c->p - 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    GP_HIP(hipStreamSynchronize(c->stream));
     k_cell_grad<<<nblk(c->Npad), NT, 0, c->stream>>>(
         c->N, c->Npad, c->J, c->M, c->Q, c->P, c->g0, c->ard_only, c->Z, c->U, c->C1a, c->A,
-        c->Omega, c->Oinvt, c->titsias, c->Ecell, c->X, c->T, c->rho, dt, 0.1, 100.0, 100.0,
+        c->Omega, c->Oinvt, c->titsias, c->Ecell, c->X, c->T, c->rho, c->delta + c->p - 1, 0.1,
+        100.0, 100.0,
         c->grad, c->grad + c->N + c->p, c->C1);
     LAUNCH_CHECK();
     return 0;
 }
 
 int gpk_c2(gp_ctx *c, int whitened) {
-    double dt;
-    GP_HIP(hipMemcpyAsync(&dt, c->delta + c->p - 1, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    GP_HIP(hipStreamSynchronize(c->stream));
     const double one = 1.0, zero = 0.0;
     k_sum_sinv<<<nblk((size_t)c->M * c->M), NT, 0, c->stream>>>(c->J, c->P, c->M, c->g0, c->Sinv,
                                                                  c->S2);
@@ -922,7 +930,7 @@ int gpk_c2(gp_ctx *c, int whitened) {
     GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, c->M, c->M,
                           c->J, &one, c->tBphi + c->g0, c->P, c->tBeta + c->g0, c->P, &zero, c->B2,
                           c->M));
-    k_c2<<<nblk((size_t)c->M * c->M), NT, 0, c->stream>>>(c->M, c->J, 0.1, dt, c->G2, c->B2, c->S2,
+    k_c2<<<nblk((size_t)c->M * c->M), NT, 0, c->stream>>>(c->M, c->J, 0.1, c->delta + c->p - 1, c->G2, c->B2, c->S2,
                                                           c->Kinv, c->K, c->C2);
     LAUNCH_CHECK();
     return 0;

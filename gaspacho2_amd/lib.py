@@ -49,6 +49,7 @@ def load() -> C.CDLL:
         "gp_set_option": (C.c_int, [vp, C.c_char_p, C.c_int]),
         "gp_get_option": (C.c_int, [vp, C.c_char_p, ip]),
         "gp_autotune": (C.c_int, [vp, C.c_int]),
+        "gp_eval_graph": (C.c_int, [vp, C.c_int, dp]),
         "gp_buffer_len": (C.c_size_t, [vp, C.c_char_p]),
         "gp_get_buffer": (C.c_int, [vp, C.c_char_p, dp, C.c_size_t]),
         "gp_set_buffer": (C.c_int, [vp, C.c_char_p, dp, C.c_size_t]),
@@ -190,6 +191,12 @@ class GPContext:
     def elbo(self, slot: int = 0) -> float:
         o = C.c_double()
         self._check(self._lib.gp_elbo(self._h, int(slot), C.byref(o)))
+        return o.value
+
+    def eval_graph(self, slot: int = 0) -> float:
+        """ELBO (returned) and gradient (device grad buffer) by replaying the recorded unit."""
+        o = C.c_double()
+        self._check(self._lib.gp_eval_graph(self._h, int(slot), C.byref(o)))
         return o.value
 
     def grad(self) -> None:

@@ -136,6 +136,14 @@ int gp_lbfgs_step(gp_ctx *ctx, int jtr, int rhoflag);
 /* ggplvm.cu:1289 Downdate<<<1,1>>> (ggplvm.cu:818-859). */
 int gp_downdate(gp_ctx *ctx, int jtr, double sr, int rhoflag);
 
+/* One ELBO + gradient unit -- gp_elbo(slot) followed by gp_grad, i.e.
+ * ggplvm.cu:1259-1278 plus 1215-1240 -- recorded once as a HIP graph and
+ * replayed (no counterpart in the reference, which launches every kernel
+ * from the host).  Replays evaluate at the current parameters; the graph is
+ * re-recorded when the precision, the slot or a launch-shape option changes.
+ * *obj = lkhd[slot]. */
+int gp_eval_graph(gp_ctx *ctx, int slot, double *obj);
+
 /* ggplvm.cu:1319-1324: expandDelta_k, getBetaFromTildeBeta. */
 int gp_end_outer(gp_ctx *ctx);
 
