@@ -50,6 +50,8 @@ def load() -> C.CDLL:
         "gp_get_option": (C.c_int, [vp, C.c_char_p, ip]),
         "gp_autotune": (C.c_int, [vp, C.c_int]),
         "gp_eval_graph": (C.c_int, [vp, C.c_int, dp]),
+        "gp_score_tests": (C.c_int, [vp, C.c_int, dp, C.c_int, ip, ip, dp]),
+        "gp_score_tests_dosage": (C.c_int, [vp, C.c_int, C.POINTER(C.c_ubyte), C.c_int, ip, ip, dp]),
         "gp_buffer_len": (C.c_size_t, [vp, C.c_char_p]),
         "gp_get_buffer": (C.c_int, [vp, C.c_char_p, dp, C.c_size_t]),
         "gp_set_buffer": (C.c_int, [vp, C.c_char_p, dp, C.c_size_t]),
@@ -216,6 +218,28 @@ class GPContext:
         self._check(self._lib.gp_fit(self._h, int(itrmax), int(jtr_cap), int(bool(verbose)),
                                      None, None, C.byref(o)))
         return o.value
+
+    # -- score tests (ggplvm_de.cu:491-532)
+    def score_tests(self, G, gene, snp) -> np.ndarray:
+        """Statistics of tests (gene[t], snp[t]); G is N x S (float: doubles,
+        uint8: 0/1/2 dosages), column-major or a 2-D array."""
+        G = np.asarray(G)
+        gene = np.ascontiguousarray(gene, dtype=np.int32)
+        snp = np.ascontiguousarray(snp, dtype=np.int32)
+        out = np.zeros(len(gene))
+        if G.dtype == np.uint8:
+            Gf = np.asfortranarray(G.reshape(self.N, -1, order="F"))
+            flat = Gf.ravel(order="F")
+            st = self._lib.gp_score_tests_dosage(self._h, Gf.shape[1],
+                                                 flat.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                                 len(gene), _ip(gene), _ip(snp), _dp(out))
+        else:
+            Gf = np.asfortranarray(G.astype(np.float64).reshape(self.N, -1, order="F"))
+            flat = Gf.ravel(order="F")
+            st = self._lib.gp_score_tests(self._h, Gf.shape[1], _dp(flat), len(gene), _ip(gene),
+                                          _ip(snp), _dp(out))
+        self._check(st)
+        return out
 
     # -- timing
     def timing(self, on: bool = True) -> None:

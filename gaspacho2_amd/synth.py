@@ -59,17 +59,29 @@ def _se_ard(X: np.ndarray, Z: np.ndarray, rho: np.ndarray) -> np.ndarray:
 
 def make_dataset(N: int, G: int, Q: int, M: int, seed: int = 0, n_numeric: int = 0,
                  n_levels: int = 0, gen_M: Optional[int] = None, pca: bool = True,
-                 block: int = 256) -> Dataset:
+                 block: int = 256, model_seed: Optional[int] = None) -> Dataset:
     """Counts from the generative model above; optional covariates
-    (n_numeric N(0,1) columns and one categorical variable with n_levels)."""
+    (n_numeric N(0,1) columns and one categorical variable with n_levels).
+
+    model_seed: draw the per-gene functions (generating inducing points,
+    lengthscales, a_g, b_g) from their own stream, so several data seeds share
+    one set of genes (new cells, size factors and counts per seed)."""
     rng = np.random.default_rng(seed)
     Mg = gen_M or M
     X = rng.standard_normal((N, Q))
-    Zg = rng.standard_normal((Mg, Q))
-    rho = rng.uniform(0.5, 2.0, Q) ** 2
-    Kxz = _se_ard(X, Zg, rho)
-    a = rng.normal(0.0, 0.5, (Mg, G))
-    b = rng.normal(-1.0, 1.0, G)
+    if model_seed is None:
+        Zg = rng.standard_normal((Mg, Q))
+        rho = rng.uniform(0.5, 2.0, Q) ** 2
+        Kxz = _se_ard(X, Zg, rho)
+        a = rng.normal(0.0, 0.5, (Mg, G))
+        b = rng.normal(-1.0, 1.0, G)
+    else:
+        mrng = np.random.default_rng([model_seed, 7919])
+        Zg = mrng.standard_normal((Mg, Q))
+        rho = mrng.uniform(0.5, 2.0, Q) ** 2
+        Kxz = _se_ard(X, Zg, rho)
+        a = mrng.normal(0.0, 0.5, (Mg, G))
+        b = mrng.normal(-1.0, 1.0, G)
     s = np.exp(rng.normal(0.0, 0.5, N))
     # covariates
     cov_cols = [np.ones(N)]
@@ -146,6 +158,7 @@ def pca_init(ds: Dataset, max_cells: int = 20000, seed: int = 1) -> np.ndarray:
     G = nu.T @ nu
     lam, V = np.linalg.eigh(G)
     lam, V = lam[::-1][:Q], V[:, ::-1][:, :Q]
+    V = V * np.where(V[np.abs(V).argmax(axis=0), np.arange(Q)] < 0, -1.0, 1.0)   # sign convention
     proj = np.asarray(S @ V) + u[:, None] * V.sum(axis=0)[None, :] - (m @ V)[None, :]
     X = proj / np.sqrt(lam) * np.sqrt(rows.size)
     return np.asfortranarray(X)

@@ -144,6 +144,19 @@ int gp_downdate(gp_ctx *ctx, int jtr, double sr, int rhoflag);
  * *obj = lkhd[slot]. */
 int gp_eval_graph(gp_ctx *ctx, int slot, double *obj);
 
+/* getScoreStats (ggplvm_de.cu:491-532) for a list of T tests: test t pairs
+ * gene gene[t] with genotype column snp[t] of G (N x S, column-major):
+ *   stat[t] = g^T Omega^-1 (y_j - W_j) / sqrt(phi_j (g^T D_j g - b^T Psi_j^-1 b)),
+ *   D_j = diag(W_j / Omega), b = Z^T D_j g,
+ * from the current W, Omega, phi and per-gene factors (call after gp_irls_step
+ * or gp_elbo).  The reference launches a (J, L) grid over all pairs with one
+ * shared G; a test list covers that (T = J L) and per-gene SNP sets.
+ * gp_score_tests_dosage takes 0/1/2 dosages as bytes. */
+int gp_score_tests(gp_ctx *ctx, int S, const double *G, int T, const int *gene,
+                   const int *snp, double *stat);
+int gp_score_tests_dosage(gp_ctx *ctx, int S, const unsigned char *dosage, int T,
+                          const int *gene, const int *snp, double *stat);
+
 /* ggplvm.cu:1319-1324: expandDelta_k, getBetaFromTildeBeta. */
 int gp_end_outer(gp_ctx *ctx);
 

@@ -467,6 +467,48 @@ static void zt_oinv_nu(const orc_state *s, const double *Wj, const double *nj, d
     }
 }
 
+/* ggplvm_de.cu:491-532 getScoreStats, for a list of tests: test t pairs gene
+ * gene[t] with genotype column snp[t] of G (N x S, column-major, doubles).
+ *   tmp  = Z^T diag(W_j/Omega) g,  tmp2 = Psi_j^{-1} tmp
+ *   gVg  = sum_i g_i^2 W_ij/Omega_i - tmp^T tmp2
+ *   ss   = sum_i g_i (y_ij - W_ij)/Omega_i   (y_ij = 0 off the non-zeros)
+ *   stat = ss / sqrt(phi_j gVg)
+ * Uses the current W, Omega, phi and the factors in Psi. */
+void orc_score_stats(orc_state *s, int S, const double *G, int T, const int *gene,
+                     const int *snp, double *stat) {
+    int N = s->N, P = s->P;
+    (void)S;
+    #pragma omp parallel
+    {
+        double *tmp = (double *)malloc(sizeof(double) * 2 * P);
+        double *tmp2 = tmp + P;
+        #pragma omp for schedule(dynamic)
+        for (int t = 0; t < T; t++) {
+            const int j = gene[t];
+            const double *Wj = s->W + (size_t)j * N;
+            const double *g = G + (size_t)snp[t] * N;
+            zt_oinv_nu(s, Wj, g, tmp);
+            orc_cholsl(s->Psi + (size_t)j * P * P, P, tmp, tmp2);
+            double gvg = 0.0;
+            for (int i = 0; i < N; i++) gvg += g[i] * g[i] * Wj[i] / s->Omega[i];
+            for (int k = 0; k < P; k++) gvg -= tmp[k] * tmp2[k];
+            int l = s->Yp[j];
+            const int lmax = s->Yp[j + 1];
+            double ss = 0.0;
+            for (int i = 0; i < N; i++) {
+                if (l < lmax && s->Yi[l] == i) {
+                    ss += g[i] / s->Omega[i] * (s->Yx[l] - Wj[i]);
+                    l++;
+                } else {
+                    ss += g[i] * Wj[i] / s->Omega[i] * (-1.0);
+                }
+            }
+            stat[t] = ss / sqrt(s->phi[j] * gvg);
+        }
+        free(tmp);
+    }
+}
+
 void orc_get_beta_phi(orc_state *s, int jtr) {   /* ggplvm.cu:418-464 */
     int N = s->N, P = s->P;
     double r = (double)(jtr + 1) / 100.0;

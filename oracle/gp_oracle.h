@@ -129,6 +129,11 @@ void orc_get_psi(orc_state *s);                /* ggplvm.cu:621-680  pinned: den
 int  orc_chol_psi(orc_state *s);               /* ggplvm.cu:683-692  returns #non-PD */
 void orc_get_beta_phi(orc_state *s, int jtr);  /* ggplvm.cu:418-464  pinned: IRLS fixed point */
 void orc_get_lkhd_pois(orc_state *s);          /* ggplvm.cu:703-724  pinned: Poisson logpmf + titsias term */
+/* ggplvm_de.cu:491-532 getScoreStats over a test list (gene[t], snp[t]) with
+ * genotype columns G (N x S column-major).  pinned: dense Woodbury form
+ * g^T (D^-1 + Z Delta Z^T)^-1 g of the variance term, dense numerator. */
+void orc_score_stats(orc_state *s, int S, const double *G, int T, const int *gene,
+                     const int *snp, double *stat);
 double orc_collect_lkhd(orc_state *s, int Nterm, int jtr);  /* ggplvm.cu:788-815 */
 void orc_get_oinvt(orc_state *s);              /* ggplvm.cu:284-294 */
 void orc_get_lkhd(orc_state *s);               /* ggplvm.cu:744-784  pinned: Woodbury dense log N */

@@ -64,6 +64,7 @@ def load() -> C.CDLL:
         "orc_get_tilde_beta": (None, [vp]), "orc_grad_log_delta": (None, [vp]),
         "orc_grad_log_omega": (None, [vp]), "orc_solve_psi": (None, [vp]),
         "orc_get_c2": (None, [vp]), "orc_grad_tau": (None, [vp]), "orc_grad_log_rho": (None, [vp]),
+        "orc_score_stats": (None, [vp, C.c_int, dp, C.c_int, ip, ip, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -161,6 +162,16 @@ class OracleState:
     def end_outer(self):
         self._lib.orc_expand_delta_k(self._h)
         self._lib.orc_beta_from_tilde_beta(self._h)
+
+    def score_stats(self, G, gene, snp) -> np.ndarray:
+        """ggplvm_de.cu:491-532 for tests (gene[t], snp[t]); G is N x S."""
+        Gf = np.asfortranarray(np.asarray(G, dtype=np.float64).reshape(self.N, -1, order="F"))
+        gene = np.ascontiguousarray(gene, dtype=np.int32)
+        snp = np.ascontiguousarray(snp, dtype=np.int32)
+        out = np.zeros(len(gene))
+        self._lib.orc_score_stats(self._h, Gf.shape[1], _dp(Gf.ravel(order="F")), len(gene),
+                                  _ip(gene), _ip(snp), _dp(out))
+        return out
 
     def fit(self, itrmax: int, jtr_cap: int = 0, verbose: bool = False) -> float:
         return self._lib.orc_fit(self._h, int(itrmax), int(jtr_cap), int(bool(verbose)))

@@ -358,3 +358,53 @@ def test_graph_replay_matches_eager(precision):
         assert rel(gr_graph, gr_eager) <= 1e-12, rep
         # a second replay at the same parameters reproduces the first
         assert g.eval_graph(3) == e_graph
+
+
+def test_score_tests_match_oracle(small_ds):
+    """getScoreStats (ggplvm_de.cu:491-532): GPU statistics against the oracle
+    after the IRLS phase (FP64 factors, reference basis), for doubles and for
+    0/1/2 dosage bytes, including a test list with repeated genes / SNPs."""
+    o, g = _pair(small_ds)
+    o.init_fit()
+    g.init_fit()
+    for jtr in range(3):
+        o.irls_step(jtr)
+        g.irls_step(jtr)
+    N, J = small_ds.N, small_ds.J
+    rng = np.random.default_rng(8)
+    S = 7
+    D = rng.integers(0, 3, (N, S)).astype(np.uint8)
+    gene = rng.integers(0, J, 60).astype(np.int32)
+    snp = rng.integers(0, S, 60).astype(np.int32)
+    so = o.score_stats(D.astype(np.float64), gene, snp)
+    sg = g.score_tests(D, gene, snp)
+    sd = g.score_tests(D.astype(np.float64), gene, snp)
+    assert np.all(np.isfinite(so))
+    assert rel(sg, so) < 1e-9
+    assert np.array_equal(sg, sd)
+
+
+def test_score_tests_mixed_basis_close_to_oracle():
+    """Score statistics from the mixed path's whitened factors (after gp_elbo)
+    against the oracle's reference-basis factors at the same W."""
+    from gaspacho2_amd import lib, synth
+    ds = synth.make_dataset(2000, 40, 5, 32, seed=2, n_numeric=1)
+    o, g = _pair(ds)
+    o.init_fit()
+    g.init_fit()
+    for jtr in range(2):
+        o.irls_step(jtr)
+        g.irls_step(jtr)
+    o.ql_start()
+    g.ql_start()
+    g.set_precision(lib.PREC_MIXED)
+    o.elbo(1)
+    g.elbo(1)
+    rng = np.random.default_rng(9)
+    D = rng.integers(0, 3, (ds.N, 10)).astype(np.uint8)
+    gene = np.repeat(np.arange(ds.J), 10).astype(np.int32)
+    snp = np.tile(np.arange(10), ds.J).astype(np.int32)
+    so = o.score_stats(D.astype(np.float64), gene, snp)
+    sg = g.score_tests(D, gene, snp)
+    print(f"mixed-basis score statistics: max rel err {rel(sg, so):.3e}")
+    assert rel(sg, so) < 1e-6

@@ -309,3 +309,37 @@ def test_init_ta_with_per_layout():
     T = O.init_ta_with_per(7, 3).reshape(7, 3, order="F")
     assert np.allclose(T[:, 0], np.arange(1, 8) / 8 * 2 * 3.14159265358979)
     assert abs(T[:, 1:].mean()) < 1.5 and T[:, 1:].std() > 0.3
+
+
+# ------------------------------------------------------------------ score tests
+def test_score_stats_match_dense_woodbury_form(tiny_ds):
+    """getScoreStats (ggplvm_de.cu:491-532): the variance term
+    g^T D g - (Z^T D g)^T Psi^-1 (Z^T D g) equals g^T (D^-1 + Z Delta Z^T)^-1 g
+    (Woodbury, Psi = Delta^-1 + Z^T D Z, D = diag(W/Omega)), checked with a dense
+    N x N solve; the numerator is g^T Omega^-1 (y - W)."""
+    from gaspacho2_amd.synth import dense_counts
+    o = make_oracle(tiny_ds)
+    o.init_fit()
+    for jtr in range(3):
+        o.irls_step(jtr)
+    N, J, P = o.N, o.J, o.P
+    rng = np.random.default_rng(2)
+    S = 4
+    G = rng.integers(0, 3, (N, S)).astype(np.float64)
+    gene = np.repeat(np.arange(J), S).astype(np.int32)
+    snp = np.tile(np.arange(S), J).astype(np.int32)
+    stat = o.score_stats(G, gene, snp)
+    Z = o.get("Z").reshape(N, P, order="F")
+    W = o.get("W")[:N * J].reshape(N, J, order="F")
+    Om = o.get("Omega")[:N]
+    phi = o.get("phi")[:J]
+    Delta = np.linalg.inv(_dense_delta(o))
+    Y = dense_counts(tiny_ds)
+    for t in range(len(gene)):
+        j, l = gene[t], snp[t]
+        g = G[:, l]
+        V = np.diag(Om / W[:, j]) + Z @ Delta @ Z.T
+        var = g @ np.linalg.solve(V, g)
+        num = g @ ((Y[:, j] - W[:, j]) / Om)
+        ref = num / np.sqrt(phi[j] * var)
+        assert abs(stat[t] - ref) <= 1e-8 * max(1.0, abs(ref)), (t, stat[t], ref)
