@@ -217,6 +217,10 @@ int gpk_gradcontract_mixed(gp_ctx *c);
 // U = sum_j diag(W_j/Omega) Zsrc Sinv_j  (Npad x P)
 int gpk_gradcontract(gp_ctx *c, const double *Zsrc, double *Uout);
 
+// ---------------- gp_batched.hip ----------------
+struct gp_bfit;
+extern "C" int gp_bfit_destroy(gp_bfit *b);
+
 // ---------------- timing ----------------
 void gp_timer_begin(gp_ctx *c, gp_timer &t);
 void gp_timer_end(gp_ctx *c, gp_timer &t);

@@ -51,6 +51,14 @@ def load() -> C.CDLL:
         "gp_autotune": (C.c_int, [vp, C.c_int]),
         "gp_eval_graph": (C.c_int, [vp, C.c_int, dp]),
         "gp_score_tests": (C.c_int, [vp, C.c_int, dp, C.c_int, ip, ip, dp]),
+        "gp_bfit_create": (C.c_int, [C.POINTER(vp), vp, C.c_int, ip, ip, ip, C.POINTER(C.c_float),
+                                     ip, ip, dp]),
+        "gp_bfit_destroy": (C.c_int, [vp]),
+        "gp_bfit_init": (C.c_int, [vp]),
+        "gp_bfit_step": (C.c_int, [vp, dp]),
+        "gp_bfit_run": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, ip, dp]),
+        "gp_bfit_get": (C.c_int, [vp, C.c_char_p, dp, C.c_size_t]),
+        "gp_bfit_dims": (C.c_int, [vp, ip, ip]),
         "gp_score_tests_dosage": (C.c_int, [vp, C.c_int, C.POINTER(C.c_ubyte), C.c_int, ip, ip, dp]),
         "gp_buffer_len": (C.c_size_t, [vp, C.c_char_p]),
         "gp_get_buffer": (C.c_int, [vp, C.c_char_p, dp, C.c_size_t]),
@@ -253,3 +261,56 @@ class GPContext:
         cnt = C.c_int()
         self._check(self._lib.gp_timing_get(self._h, name.encode(), C.byref(ms), C.byref(cnt)))
         return ms.value, cnt.value
+
+
+class BatchedFit:
+    """Per-gene Poisson GLM / Laplace fits on cell subsets (gp_bfit_*), on the
+    latent space of an initialised GPContext."""
+
+    def __init__(self, ctx: GPContext, off, cell, ncov, cov, yoff, ypos, yval):
+        self._lib = load()
+        self.ctx = ctx
+        self._keep = [np.ascontiguousarray(off, dtype=np.int32), np.ascontiguousarray(cell, dtype=np.int32),
+                      np.ascontiguousarray(ncov, dtype=np.int32), np.ascontiguousarray(cov, dtype=np.float32),
+                      np.ascontiguousarray(yoff, dtype=np.int32), np.ascontiguousarray(ypos, dtype=np.int32),
+                      np.ascontiguousarray(yval, dtype=np.float64)]
+        o, c, nc, cv, yo, yp, yv = self._keep
+        h = C.c_void_p()
+        st = self._lib.gp_bfit_create(C.byref(h), ctx._h, len(o) - 1, _ip(o), _ip(c), _ip(nc),
+                                      cv.ctypes.data_as(C.POINTER(C.c_float)), _ip(yo), _ip(yp), _dp(yv))
+        ctx._check(st)
+        self._h = h
+        J, PB = C.c_int(), C.c_int()
+        self._lib.gp_bfit_dims(h, C.byref(J), C.byref(PB))
+        self.J, self.PB = J.value, PB.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.gp_bfit_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init(self):
+        self.ctx._check(self._lib.gp_bfit_init(self._h))
+
+    def step(self) -> float:
+        o = C.c_double()
+        self.ctx._check(self._lib.gp_bfit_step(self._h, C.byref(o)))
+        return o.value
+
+    def run(self, maxit: int, tol: float = 1e-7, graph: bool = True):
+        it, o = C.c_int(), C.c_double()
+        self.ctx._check(self._lib.gp_bfit_run(self._h, int(maxit), float(tol), int(bool(graph)),
+                                              C.byref(it), C.byref(o)))
+        return it.value, o.value
+
+    def get(self, name: str) -> np.ndarray:
+        n = {"Beta": self.J * self.PB, "phi": self.J, "lkhd": self.J, "Psi": self.J * self.PB * self.PB}[name]
+        out = np.empty(n)
+        self.ctx._check(self._lib.gp_bfit_get(self._h, name.encode(), _dp(out), n))
+        return out

@@ -185,3 +185,58 @@ def write_dataframe(path: str, ds: Dataset) -> None:
 def write_doubles(path: str, x: np.ndarray) -> None:
     with open(path, "wb") as f:
         f.write(np.asarray(x, dtype="<f8").ravel(order="F").tobytes())
+
+
+@dataclass
+class BatchedData:
+    """Per-gene cell subsets with their own covariates (the uneven batched
+    GLM workload): gene g uses cells cell[off[g]:off[g+1]] (ascending), p_g =
+    ncov[g] design columns before the GP block (intercept + p_g - 1 FP32
+    N(0,1) covariates, column-major per gene in cov), and non-zero counts at
+    subset positions ypos[yoff[g]:yoff[g+1]] with values yval."""
+    off: np.ndarray
+    cell: np.ndarray
+    ncov: np.ndarray
+    cov: np.ndarray
+    yoff: np.ndarray
+    ypos: np.ndarray
+    yval: np.ndarray
+
+
+def make_batched(ds: Dataset, G: int, nmin: int, nmax: int, pmin: int = 2, pmax: int = 16,
+                 seed: int = 0, block_cells: int = 1 << 22) -> BatchedData:
+    """Subset sizes log-uniform in [nmin, nmax], p_g ~ U{pmin..pmax}; counts
+    y ~ Poisson(s_i exp(f_g(x_i) + c_i . gamma_g + b_g)) with f_g from the
+    generating SE-ARD model of ds (its true latent X)."""
+    rng = np.random.default_rng([seed, 104729])
+    N, Q = ds.N, ds.Q
+    X = ds.X_true if ds.X_true is not None else np.asarray(ds.X0)
+    Mg = 64
+    Zg = rng.standard_normal((Mg, Q))
+    rho = rng.uniform(0.5, 2.0, Q) ** 2
+    sizes = np.exp(rng.uniform(np.log(nmin), np.log(nmax), G)).astype(np.int64)
+    sizes = np.clip(sizes, 1, N)
+    ncov = rng.integers(pmin, pmax + 1, G).astype(np.int32)
+    off = np.zeros(G + 1, np.int64)
+    off[1:] = np.cumsum(sizes)
+    cells, covs, yoff, ypos, yval = [], [], [0], [], []
+    for g in range(G):
+        n = int(sizes[g])
+        c = np.sort(rng.choice(N, n, replace=False)).astype(np.int32)
+        C = rng.standard_normal((n, int(ncov[g]) - 1)).astype(np.float32)
+        a = rng.normal(0.0, 0.5, Mg)
+        gam = rng.normal(0.0, 0.2, int(ncov[g]) - 1)
+        b = rng.normal(-1.0, 1.0)
+        f = _se_ard(X[c], Zg, rho) @ a
+        eta = f + C.astype(np.float64) @ gam + b
+        y = rng.poisson(ds.ls[c] * np.exp(eta))
+        nz = np.flatnonzero(y)
+        cells.append(c)
+        covs.append(np.asfortranarray(C).ravel(order="F"))
+        ypos.append(nz.astype(np.int32))
+        yval.append(y[nz].astype(np.float64))
+        yoff.append(yoff[-1] + nz.size)
+    return BatchedData(off=off.astype(np.int32), cell=np.concatenate(cells), ncov=ncov,
+                       cov=np.concatenate(covs) if covs else np.zeros(0, np.float32),
+                       yoff=np.array(yoff, np.int32), ypos=np.concatenate(ypos),
+                       yval=np.concatenate(yval))

@@ -157,6 +157,36 @@ int gp_score_tests(gp_ctx *ctx, int S, const double *G, int T, const int *gene,
 int gp_score_tests_dosage(gp_ctx *ctx, int S, const unsigned char *dosage, int T,
                           const int *gene, const int *snp, double *stat);
 
+/* ---- Batched per-gene GLM / Laplace fits on cell subsets ----
+ * No counterpart in the reference, which fits every gene on all N cells
+ * (ggplvm.cu:487); each step restates getWForBeta / getPsi / cholPsi /
+ * getBetaPhi / getLkhdPois (ggplvm.cu:418-514, 621-724) for gene g on its
+ * own cells with design [1 | covariates | Knm], prior deltas
+ * (1.5, 0.01, 0.3) as initParams (init.cu:31-35), on the latent space of ctx
+ * (its Knm, Kmm, titsias, Omega, size factors: call after gp_init_fit).
+ *   off[J+1], cell[off[J]]: gene g's cells are cell[off[g] .. off[g+1]-1]
+ *   ncov[J]: p_g = design columns before Knm, intercept included (>= 1)
+ *   cov: per gene, n_g x (p_g - 1) FP32 column-major, genes back to back
+ *   yoff[J+1], ypos, yval: gene g's non-zero counts at positions (0-based,
+ *     within its subset, ascending) ypos[yoff[g] .. yoff[g+1]-1]
+ * p_g + M + 1 <= 160. */
+typedef struct gp_bfit gp_bfit;
+int gp_bfit_create(gp_bfit **out, gp_ctx *ctx, int J, const int *off, const int *cell,
+                   const int *ncov, const float *cov, const int *yoff, const int *ypos,
+                   const double *yval);
+int gp_bfit_destroy(gp_bfit *b);
+/* init.cu:125-221 per gene (unweighted Gram, initBeta); phi = 1, step 0. */
+int gp_bfit_init(gp_bfit *b);
+/* One IRLS step of every gene; *obj = sum of the genes' Poisson
+ * log-likelihoods at the step's starting weights (collectLkhd, N = 0). */
+int gp_bfit_step(gp_bfit *b, double *obj);
+/* Up to maxit steps, stopping when the relative objective change < tol
+ * (ggplvm.cu:1156-1181); use_graph records one step as a HIP graph. */
+int gp_bfit_run(gp_bfit *b, int maxit, double tol, int use_graph, int *iters, double *obj);
+/* "Beta" (J x PB), "phi" (J), "lkhd" (J), "Psi" (J x PB x PB factors). */
+int gp_bfit_get(gp_bfit *b, const char *name, double *host, size_t n);
+int gp_bfit_dims(gp_bfit *b, int *J, int *PB);
+
 /* ggplvm.cu:1319-1324: expandDelta_k, getBetaFromTildeBeta. */
 int gp_end_outer(gp_ctx *ctx);
 

@@ -390,7 +390,7 @@ void orc_init_beta(orc_state *s) {              /* init.cu:193-221 */
         int l = s->Yp[j], lmax = s->Yp[j + 1];
         for (int i = 0; i < N; i++) {
             double yi;
-            if (i == s->Yi[l] && l < lmax) { yi = log((s->Yx[l] + 0.05) / s->ls[i]); l++; }
+            if (l < lmax && i == s->Yi[l]) { yi = log((s->Yx[l] + 0.05) / s->ls[i]); l++; }
             else yi = log(0.05 / s->ls[i]);
             for (int m = 0; m < P; m++) tmp[m] += s->Z[i + (size_t)m * N] * yi;
         }
@@ -416,7 +416,7 @@ void orc_get_w_for_beta(orc_state *s) {         /* ggplvm.cu:468-514 */
             }
             double logW = sum + log(s->ls[i]) + dt * s->titsias[i] * s->wt;
             Wj[i] = exp(sum) * s->ls[i] * exp(dt * s->titsias[i] * s->wt);
-            if (i == s->Yi[l] && l < lmax) { nj[i] = sum2 + exp(log(s->Yx[l]) - logW) - 1.0; l++; }
+            if (l < lmax && i == s->Yi[l]) { nj[i] = sum2 + exp(log(s->Yx[l]) - logW) - 1.0; l++; }
             else nj[i] = sum2 - 1.0;
         }
     }
