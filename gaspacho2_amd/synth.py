@@ -214,6 +214,7 @@ def make_batched(ds: Dataset, G: int, nmin: int, nmax: int, pmin: int = 2, pmax:
     Mg = 64
     Zg = rng.standard_normal((Mg, Q))
     rho = rng.uniform(0.5, 2.0, Q) ** 2
+    Kxz = _se_ard(X, Zg, rho)                       # N x Mg, once
     sizes = np.exp(rng.uniform(np.log(nmin), np.log(nmax), G)).astype(np.int64)
     sizes = np.clip(sizes, 1, N)
     ncov = rng.integers(pmin, pmax + 1, G).astype(np.int32)
@@ -222,13 +223,19 @@ def make_batched(ds: Dataset, G: int, nmin: int, nmax: int, pmin: int = 2, pmax:
     cells, covs, yoff, ypos, yval = [], [], [0], [], []
     for g in range(G):
         n = int(sizes[g])
-        c = np.sort(rng.choice(N, n, replace=False)).astype(np.int32)
-        C = rng.standard_normal((n, int(ncov[g]) - 1)).astype(np.float32)
+        if 2 * n < N:     # sorted sample without replacement, no full permutation
+            c = np.unique(rng.integers(0, N, int(n * 1.3) + 16))
+            while c.size < n:
+                c = np.unique(np.concatenate([c, rng.integers(0, N, n)]))
+            c = np.sort(rng.choice(c, n, replace=False))
+        else:
+            c = np.sort(rng.choice(N, n, replace=False))
+        c = c.astype(np.int32)
+        C = rng.standard_normal((n, int(ncov[g]) - 1), dtype=np.float32)
         a = rng.normal(0.0, 0.5, Mg)
         gam = rng.normal(0.0, 0.2, int(ncov[g]) - 1)
         b = rng.normal(-1.0, 1.0)
-        f = _se_ard(X[c], Zg, rho) @ a
-        eta = f + C.astype(np.float64) @ gam + b
+        eta = Kxz[c] @ a + C.astype(np.float64) @ gam + b
         y = rng.poisson(ds.ls[c] * np.exp(eta))
         nz = np.flatnonzero(y)
         cells.append(c)
