@@ -60,6 +60,17 @@ def load() -> C.CDLL:
         "gp_bfit_get": (C.c_int, [vp, C.c_char_p, dp, C.c_size_t]),
         "gp_bfit_dims": (C.c_int, [vp, ip, ip]),
         "gp_score_tests_dosage": (C.c_int, [vp, C.c_int, C.POINTER(C.c_ubyte), C.c_int, ip, ip, dp]),
+        "gp_enc_create": (C.c_int, [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                    C.c_longlong, C.c_ulonglong]),
+        "gp_enc_destroy": (C.c_int, [vp]),
+        "gp_enc_dims": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
+        "gp_enc_add": (C.c_int, [vp, C.c_int, C.c_int, ip, ip, dp, dp, dp]),
+        "gp_enc_loss_grad": (C.c_int, [vp, C.c_int, dp]),
+        "gp_enc_adam_step": (C.c_int, [vp, C.c_double]),
+        "gp_enc_train": (C.c_int, [vp, C.c_int, C.c_double, dp]),
+        "gp_enc_predict": (C.c_int, [vp, C.c_int, C.c_int, ip, ip, dp, dp, dp, dp]),
+        "gp_enc_get": (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_float), C.c_size_t]),
+        "gp_enc_set": (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_float), C.c_size_t]),
         "gp_buffer_len": (C.c_size_t, [vp, C.c_char_p]),
         "gp_get_buffer": (C.c_int, [vp, C.c_char_p, dp, C.c_size_t]),
         "gp_set_buffer": (C.c_int, [vp, C.c_char_p, dp, C.c_size_t]),
@@ -313,4 +324,111 @@ class BatchedFit:
         n = {"Beta": self.J * self.PB, "phi": self.J, "lkhd": self.J, "Psi": self.J * self.PB * self.PB}[name]
         out = np.empty(n)
         self.ctx._check(self._lib.gp_bfit_get(self._h, name.encode(), _dp(out), n))
+        return out
+
+
+class Encoder:
+    """Amortization encoder (gp_enc_*): MLP from log1p(y / s) expression
+    features to a Gaussian over the latent position, trained on the GPU on
+    (counts, fitted X) pairs and used to warm-start X.
+
+    Parameters live in one FP32 array in the padded layout
+    [W1 (H1 x Fp) | b1 (H1) | W2 (H2 x H1) | b2 (H2) | W3 (O x H2) | b3 (O)],
+    Fp = F rounded up to 64, O = 64 (rows >= 2Q are zero)."""
+
+    def __init__(self, F: int, Q: int, H1: int = 512, H2: int = 256, batch: int = 1024,
+                 capacity: int = 0, seed: int = 0):
+        self._lib = load()
+        h = C.c_void_p()
+        self._check(self._lib.gp_enc_create(C.byref(h), int(F), int(H1), int(H2), int(Q), int(batch),
+                                            int(capacity), C.c_ulonglong(int(seed))))
+        self._h = h
+        d = (C.c_longlong * 12)()
+        self._check(self._lib.gp_enc_dims(h, d))
+        (self.F, self.Fp, self.H1, self.H2, self.Q, self.O, self.batch) = [int(x) for x in d[:7]]
+        self.n_params = int(d[8])
+
+    def _check(self, st: int) -> None:
+        if st != 0:
+            raise GPError(f"status {st}: {self._lib.gp_last_error().decode()}")
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.gp_enc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def n_cells(self) -> int:
+        d = (C.c_longlong * 12)()
+        self._check(self._lib.gp_enc_dims(self._h, d))
+        return int(d[7])
+
+    @staticmethod
+    def _counts(N, Yp, Yi, Yx, ls):
+        Yp = np.ascontiguousarray(Yp, dtype=np.int32)
+        Yi = np.ascontiguousarray(Yi, dtype=np.int32)
+        Yx = np.ascontiguousarray(Yx, dtype=np.float64)
+        ls = np.ascontiguousarray(ls, dtype=np.float64)
+        if len(ls) != N:
+            raise GPError("size factors must have N entries")
+        return Yp, Yi, Yx, ls
+
+    def add(self, N: int, Yp, Yi, Yx, ls, X) -> None:
+        """Append N training cells: gene-major CSR counts (J = F genes), size
+        factors, and target latents X (N x Q, column-major flat or 2-D)."""
+        Yp, Yi, Yx, ls = self._counts(N, Yp, Yi, Yx, ls)
+        X = np.asarray(X, dtype=np.float64)
+        X = np.ascontiguousarray(X.T if X.ndim == 2 else X.reshape(self.Q, N)).reshape(-1)
+        self._check(self._lib.gp_enc_add(self._h, int(N), len(Yp) - 1, _ip(Yp), _ip(Yi), _dp(Yx), _dp(ls),
+                                         _dp(X)))
+
+    def loss_grad(self, n: int) -> float:
+        """Loss and gradients (left on the device) of the first n loaded cells."""
+        o = C.c_double()
+        self._check(self._lib.gp_enc_loss_grad(self._h, int(n), C.byref(o)))
+        return o.value
+
+    def adam_step(self, lr: float) -> None:
+        self._check(self._lib.gp_enc_adam_step(self._h, float(lr)))
+
+    def train(self, epochs: int, lr: float = 1e-3) -> np.ndarray:
+        out = np.zeros(max(int(epochs), 1))
+        self._check(self._lib.gp_enc_train(self._h, int(epochs), float(lr), _dp(out)))
+        return out[:int(epochs)]
+
+    def predict(self, N: int, Yp, Yi, Yx, ls):
+        """(mu, log-variance), each N x Q."""
+        Yp, Yi, Yx, ls = self._counts(N, Yp, Yi, Yx, ls)
+        mu = np.zeros(N * self.Q)
+        lv = np.zeros(N * self.Q)
+        self._check(self._lib.gp_enc_predict(self._h, int(N), len(Yp) - 1, _ip(Yp), _ip(Yi), _dp(Yx), _dp(ls),
+                                             _dp(mu), _dp(lv)))
+        return mu.reshape(self.Q, N).T.copy(), lv.reshape(self.Q, N).T.copy()
+
+    def get(self, name: str) -> np.ndarray:
+        out = np.zeros(self.n_params, dtype=np.float32)
+        self._check(self._lib.gp_enc_get(self._h, name.encode(), out.ctypes.data_as(C.POINTER(C.c_float)),
+                                         self.n_params))
+        return out
+
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        self._check(self._lib.gp_enc_set(self._h, b"params", p.ctypes.data_as(C.POINTER(C.c_float)), p.size))
+
+    def unpack(self, flat: np.ndarray) -> dict:
+        """Split a parameter-layout array into W1, b1, W2, b2, W3, b3."""
+        F, Fp, H1, H2, O = self.F, self.Fp, self.H1, self.H2, self.O
+        o = 0
+        out = {}
+        for name, shape in (("W1", (H1, Fp)), ("b1", (H1,)), ("W2", (H2, H1)), ("b2", (H2,)),
+                            ("W3", (O, H2)), ("b3", (O,))):
+            n = int(np.prod(shape))
+            out[name] = flat[o:o + n].reshape(shape)
+            o += n
         return out

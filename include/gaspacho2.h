@@ -187,6 +187,42 @@ int gp_bfit_run(gp_bfit *b, int maxit, double tol, int use_graph, int *iters, do
 int gp_bfit_get(gp_bfit *b, const char *name, double *host, size_t n);
 int gp_bfit_dims(gp_bfit *b, int *J, int *PB);
 
+/* ---- Amortization encoder (warm start of X) ----
+ * No counterpart in the reference, which reads the starting X from an
+ * external PCA file (ggplvm.cu:1088-1095, argv[4]).  An MLP
+ *   a0 = log1p(y_i. / s_i) (F = genes) -> H1 -> H2 -> [mu (Q) | log-variance (Q)]
+ * with exact GELU, BF16 weight copies and activations in the products, FP32
+ * accumulation, FP32 Adam (0.9, 0.999, 1e-8) on FP32 master weights; loss =
+ * mean over cells of sum_q 1/2 [(x - mu)^2 exp(-s) + s].  H1, H2 and batch
+ * must be multiples of 64, 2Q <= 64.  Parameters: one FP32 array
+ * [W1 (H1 x Fp) | b1 | W2 (H2 x H1) | b2 | W3 (64 x H2) | b3 (64)], row-major,
+ * Fp = F rounded up to 64 (padding stays zero).
+ * dims: F, Fp, H1, H2, Q, 64, batch, cells loaded, parameter count, capacity,
+ * Adam steps, 0. */
+typedef struct gp_enc gp_enc;
+int gp_enc_create(gp_enc **out, int F, int H1, int H2, int Q, int batch, long long capacity,
+                  unsigned long long seed);
+int gp_enc_destroy(gp_enc *enc);
+int gp_enc_dims(gp_enc *enc, long long dims[12]);
+/* Append N training cells: gene-major CSR counts (J == F genes), size
+ * factors ls[N], targets X (N x Q column-major, as gp_get_buffer "X"). */
+int gp_enc_add(gp_enc *enc, int N, int J, const int *Yp, const int *Yi, const double *Yx,
+               const double *ls, const double *X);
+/* Loss of the first n loaded cells (n <= batch) and its gradient, kept on
+ * the device (gp_enc_get "grad"). */
+int gp_enc_loss_grad(gp_enc *enc, int n, double *loss);
+/* One Adam step from the current gradient. */
+int gp_enc_adam_step(gp_enc *enc, double lr);
+/* epochs passes over the loaded cells in shuffled batches; epoch_loss[e] is
+ * the cell-weighted mean of the batch losses of epoch e (may be NULL). */
+int gp_enc_train(gp_enc *enc, int epochs, double lr, double *epoch_loss);
+/* mu and logvar (N x Q column-major; logvar may be NULL) for new cells. */
+int gp_enc_predict(gp_enc *enc, int N, int J, const int *Yp, const int *Yi, const double *Yx,
+                   const double *ls, double *mu, double *logvar);
+/* "params", "grad", "adam_m", "adam_v" (n = parameter count); set: "params". */
+int gp_enc_get(gp_enc *enc, const char *name, float *host, size_t n);
+int gp_enc_set(gp_enc *enc, const char *name, const float *host, size_t n);
+
 /* ggplvm.cu:1319-1324: expandDelta_k, getBetaFromTildeBeta. */
 int gp_end_outer(gp_ctx *ctx);
 

@@ -55,3 +55,15 @@ def test_init_ta_matches_oracle():
     from gaspacho2_amd import lib
     from oracle import oracle
     assert np.array_equal(lib.init_ta_with_per(16, 4), oracle.init_ta_with_per(16, 4))
+
+
+def test_encoder_create_rejects_bad_shapes_without_device():
+    from gaspacho2_amd import lib
+    L = lib.load()
+    h = C.c_void_p()
+    for args, msg in (((2000, 500, 256, 10, 1024), b"multiples of 64"),
+                      ((2000, 512, 256, 10, 1000), b"batch"),
+                      ((2000, 512, 256, 40, 1024), b"Q <= 32")):
+        st = L.gp_enc_create(C.byref(h), *args, 0, 0)
+        assert st == 1 and msg in L.gp_last_error(), L.gp_last_error()
+        assert not h.value

@@ -343,3 +343,75 @@ def test_score_stats_match_dense_woodbury_form(tiny_ds):
         num = g @ ((Y[:, j] - W[:, j]) / Om)
         ref = num / np.sqrt(phi[j] * var)
         assert abs(stat[t] - ref) <= 1e-8 * max(1.0, abs(ref)), (t, stat[t], ref)
+
+
+# ---------------- amortization encoder oracle (oracle/encoder.py) ----------------
+
+def _enc_case(seed=0, F=10, H1=8, H2=6, Q=2, n=5):
+    from oracle import encoder as E
+    rng = np.random.default_rng(seed)
+    p = {"W1": rng.normal(0, 0.4, (H1, F)), "b1": rng.normal(0, 0.1, H1),
+         "W2": rng.normal(0, 0.4, (H2, H1)), "b2": rng.normal(0, 0.1, H2),
+         "W3": rng.normal(0, 0.4, (2 * Q, H2)), "b3": rng.normal(0, 0.1, 2 * Q)}
+    A0 = np.log1p(rng.poisson(2.0, (n, F)) / rng.lognormal(0, 0.5, n)[:, None])
+    X = rng.normal(0, 1, (n, Q))
+    return E, p, A0, X
+
+
+def test_encoder_oracle_gradient_matches_finite_differences():
+    E, p, A0, X = _enc_case()
+    _, g = E.loss_grad(p, A0, X, rnd=False)
+    rng = np.random.default_rng(1)
+    h = 1e-6
+    for name, w in p.items():
+        for _ in range(6):
+            idx = tuple(rng.integers(0, s) for s in w.shape)
+            w0 = w[idx]
+            w[idx] = w0 + h
+            lp, _ = E.loss_grad(p, A0, X, rnd=False)
+            w[idx] = w0 - h
+            lm, _ = E.loss_grad(p, A0, X, rnd=False)
+            w[idx] = w0
+            fd = (lp - lm) / (2 * h)
+            assert abs(fd - g[name][idx]) <= 1e-7 * max(1.0, abs(fd)), (name, idx, fd, g[name][idx])
+
+
+def test_encoder_oracle_gradient_matches_torch_autograd():
+    torch = pytest.importorskip("torch")
+    E, p, A0, X = _enc_case(seed=2, F=30, H1=16, H2=12, Q=3, n=9)
+    loss, g = E.loss_grad(p, A0, X, rnd=False)
+    t = {k: torch.tensor(v, dtype=torch.float64, requires_grad=True) for k, v in p.items()}
+    a0 = torch.tensor(A0)
+    x = torch.tensor(X)
+    gl = torch.nn.functional.gelu
+    z3 = gl(gl(a0 @ t["W1"].T + t["b1"]) @ t["W2"].T + t["b2"]) @ t["W3"].T + t["b3"]
+    Q = X.shape[1]
+    mu, s = z3[:, :Q], z3[:, Q:]
+    lt = 0.5 * ((x - mu) ** 2 * torch.exp(-s) + s).sum() / X.shape[0]
+    lt.backward()
+    assert abs(lt.item() - loss) <= 1e-13 * abs(loss)
+    for k in p:
+        assert np.max(np.abs(t[k].grad.numpy() - g[k])) <= 1e-12 * max(1.0, np.max(np.abs(g[k]))), k
+
+
+def test_encoder_oracle_bf16_rounding_is_round_to_nearest_even():
+    torch = pytest.importorskip("torch")
+    from oracle import encoder as E
+    one = 1.0
+    assert E.bf16(one) == 1.0
+    assert E.bf16(1.0 + 2.0 ** -8) == 1.0                         # tie -> even mantissa
+    assert E.bf16(1.0 + 3 * 2.0 ** -8) == 1.0 + 2.0 ** -6         # tie -> even mantissa
+    assert E.bf16(1.0 + 2.0 ** -8 + 2.0 ** -20) == 1.0 + 2.0 ** -7
+    x = np.random.default_rng(3).normal(0, 10, 100000).astype(np.float32)
+    ref = torch.tensor(x).to(torch.bfloat16).to(torch.float64).numpy()
+    assert np.array_equal(E.bf16(x), ref)
+
+
+def test_encoder_oracle_adam_first_step_moves_each_weight_by_lr():
+    from oracle import encoder as E
+    rng = np.random.default_rng(4)
+    w, g = rng.normal(size=50), rng.normal(size=50)
+    w1, m, v = E.adam(w, g, np.zeros(50), np.zeros(50), 1, 1e-3)
+    # first step: m/(1-b1) = g, v/(1-b2) = g^2 -> update = lr g / (|g| + eps)
+    assert np.allclose(w - w1, 1e-3 * np.sign(g) * np.abs(g) / (np.abs(g) + 1e-8), rtol=1e-9, atol=0)
+    assert np.allclose(m, 0.1 * g) and np.allclose(v, 0.001 * g * g)
