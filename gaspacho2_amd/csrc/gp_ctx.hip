@@ -3,6 +3,7 @@
 #include "../../include/gaspacho2.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -233,6 +234,8 @@ extern "C" int gp_set_option(gp_ctx *c, const char *name, int value) {
         c->gc_groups = value;
     } else if (n == "gc_mixed") {
         c->gc_mixed = value ? 1 : 0;
+    } else if (n == "irls_mixed") {
+        c->irls_mixed = value ? 1 : 0;
     } else if (n == "autotune") {
         c->autotune = value ? 1 : 0;
     } else {
@@ -248,6 +251,7 @@ extern "C" int gp_get_option(gp_ctx *c, const char *name, int *value) {
     if (n == "gram_splits") *value = c->gram_splits;
     else if (n == "gc_groups") *value = c->gc_groups;
     else if (n == "gc_mixed") *value = c->gc_mixed;
+    else if (n == "irls_mixed") *value = c->irls_mixed;
     else if (n == "autotune") *value = c->autotune;
     else { gp_set_error("unknown option " + n); return 1; }
     return 0;
@@ -461,10 +465,25 @@ extern "C" int gp_irls_step(gp_ctx *c, int jtr, double *obj) {
     if (jtr < 0 || jtr >= GP_LKHD_SLOTS) { gp_set_error("jtr out of range"); return 1; }
     GP_TRY(clear_info(c));
     GP_TRY(gpk_w_nu(c));
-    GP_TRY(gram_chol(c, GP_PREC_FP64));   // the GLM phase runs in the reference basis, FP64
-    GP_TRY(gpk_r_and_sums(c, c->Z));
-    GP_TRY(gpk_cholsl_batched(c, c->Psi, c->P, (size_t)c->P * c->P, c->bvec, c->tBphi, c->J));
-    GP_TRY(gpk_beta_phi(c, jtr));
+    if (c->precision == GP_PREC_MIXED && c->irls_mixed) {
+        // Same step in the whitened basis T = blockdiag(I, L^-T): Psi_w = T^T Psi T
+        // on the FP16 hi/lo Gram, sol_w = Psi_w^-1 Zw^T R = T^-1 sol, and the damped
+        // update (linear) and phi (tb^T Psi tb = tb_w^T Psi_w tb_w) are basis-free.
+        // beta stays in the reference basis between steps; zeta is zero (it is
+        // uninitialised in the reference and read as 0), so adding it in either
+        // basis is the same.
+        GP_TRY(gram_chol(c, GP_PREC_MIXED));
+        GP_TRY(gpk_r_and_sums(c, c->Zw));
+        GP_TRY(gpk_cholsl_batched(c, c->Psi, c->P, (size_t)c->P * c->P, c->bvec, c->tBphi, c->J));
+        GP_TRY(gpk_beta_whiten(c, c->Beta, c->tBw));
+        GP_TRY(gpk_beta_phi_in(c, jtr, c->tBw));
+        GP_TRY(gpk_beta_unwhiten(c, c->tBw, c->Beta));
+    } else {
+        GP_TRY(gram_chol(c, GP_PREC_FP64));   // the reference basis, FP64
+        GP_TRY(gpk_r_and_sums(c, c->Z));
+        GP_TRY(gpk_cholsl_batched(c, c->Psi, c->P, (size_t)c->P * c->P, c->bvec, c->tBphi, c->J));
+        GP_TRY(gpk_beta_phi(c, jtr));
+    }
     GP_TRY(gpk_lkhd_pois(c));
     GP_TRY(gpk_collect(c, 0, jtr, obj));
     int bad = 0;
@@ -613,6 +632,15 @@ extern "C" int gp_fit(gp_ctx *c, int itrmax, int jtr_cap, int verbose, gp_outer_
                       void *user, double *obj) {
     if (!c) { gp_set_error("NULL context"); return 1; }
     std::vector<double> lk(GP_LKHD_SLOTS, 0.0);
+    const auto t0 = std::chrono::steady_clock::now();
+    c->fit_obj.clear();
+    c->fit_sec.clear();
+    c->fit_kind.clear();
+    auto mark = [&](int kind, double v) {
+        c->fit_obj.push_back(v);
+        c->fit_sec.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+        c->fit_kind.push_back(kind);
+    };
     int r = gp_init_fit(c);
     if (r > 0) return r;
     int rhoflag = 0;
@@ -623,6 +651,7 @@ extern "C" int gp_fit(gp_ctx *c, int itrmax, int jtr_cap, int verbose, gp_outer_
         for (int jtr = 0; jtr < JTRMAX; jtr++) {
             r = gp_irls_step(c, jtr, &lk[jtr]);
             if (r > 0) return r;
+            mark(0, lk[jtr]);
             if (verbose) printf("lkhd[%d,%d]=%lf\n", itr, jtr, lk[jtr]);
             if (jtr > 0) {
                 if (std::isnan(lk[jtr])) { gp_set_error("GLM objective is NaN"); if (obj) *obj = NAN; return 2; }
@@ -631,6 +660,7 @@ extern "C" int gp_fit(gp_ctx *c, int itrmax, int jtr_cap, int verbose, gp_outer_
         }
         r = gp_ql_start(c, &lk[0]);
         if (r > 0) return r;
+        mark(1, lk[0]);
         last = lk[0];
         if (verbose) printf("lkhd[%d,0]=%lf\n", itr, lk[0]);
         if (itr == 0 && c->autotune && c->precision == GP_PREC_MIXED) {
@@ -654,12 +684,25 @@ extern "C" int gp_fit(gp_ctx *c, int itrmax, int jtr_cap, int verbose, gp_outer_
                 if ((r = gp_downdate(c, jtr, sr, rhoflag)) > 0) return r;
             }
             last = lk[jtr + 1];
+            mark(2, last);
             if (jtr > 0 && std::fabs(lk[jtr + 1] - lk[jtr]) < std::fabs(lk[jtr + 1]) * 1e-8) break;
         }
         if (on_outer) on_outer(c, itr, last, user);
         if ((r = gp_end_outer(c)) > 0) return r;
     }
     if (obj) *obj = last;
+    return 0;
+}
+
+extern "C" int gp_fit_trace(gp_ctx *c, double *obj, double *seconds, int *kind, int cap, int *len) {
+    if (!c || !len) { gp_set_error("NULL argument"); return 1; }
+    const int n = (int)c->fit_obj.size();
+    *len = n;
+    for (int k = 0; k < std::min(n, cap); k++) {
+        if (obj) obj[k] = c->fit_obj[k];
+        if (seconds) seconds[k] = c->fit_sec[k];
+        if (kind) kind[k] = c->fit_kind[k];
+    }
     return 0;
 }
 

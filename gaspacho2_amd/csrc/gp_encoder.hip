@@ -59,6 +59,7 @@ namespace {
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef unsigned short us8 __attribute__((ext_vector_type(8)));
 
 constexpr int ET = 256;
 
@@ -141,7 +142,8 @@ __global__ void __launch_bounds__(ET) k_gemm_nt(int K, const bf16 *__restrict__ 
 
 // Batch rows r < n of A0 (B x Fp) and A0t (Fp x B) from the feature rows
 // idx[r]; rows n..B-1 are zero.  One workgroup per 64 x 64 tile, transposed
-// through LDS.
+// through LDS.  The values are moved as raw 16-bit patterns (integer
+// vectors): element access on BF16 vectors is not used for data movement.
 __global__ void __launch_bounds__(ET) k_enc_gather(int Fp, int B, int n, const int *__restrict__ idx,
                                                    const bf16 *__restrict__ feat, bf16 *__restrict__ A0,
                                                    bf16 *__restrict__ A0t) {
@@ -149,36 +151,33 @@ __global__ void __launch_bounds__(ET) k_enc_gather(int Fp, int B, int n, const i
     const int f0 = blockIdx.x * 64, rb = blockIdx.y * 64, tid = threadIdx.x;
     {
         const int r = tid >> 2, c = (tid & 3) * 16;
-        bf8 x0, x1;
+        us8 x0 = 0, x1 = 0;
         if (rb + r < n) {
-            const bf16 *src = feat + (size_t)idx[rb + r] * Fp + f0 + c;
-            x0 = *(const bf8 *)src;
-            x1 = *(const bf8 *)(src + 8);
-        } else {
-#pragma unroll
-            for (int q = 0; q < 8; q++) { x0[q] = (bf16)0.0f; x1[q] = (bf16)0.0f; }
+            const us8 *src = (const us8 *)(feat + (size_t)idx[rb + r] * Fp + f0 + c);
+            x0 = src[0];
+            x1 = src[1];
         }
-        bf16 *dst = A0 + (size_t)(rb + r) * Fp + f0 + c;
-        *(bf8 *)dst = x0;
-        *(bf8 *)(dst + 8) = x1;
+        us8 *dst = (us8 *)(A0 + (size_t)(rb + r) * Fp + f0 + c);
+        dst[0] = x0;
+        dst[1] = x1;
 #pragma unroll
         for (int q = 0; q < 8; q++) {
-            tile[r][c + q] = __builtin_bit_cast(unsigned short, x0[q]);
-            tile[r][c + 8 + q] = __builtin_bit_cast(unsigned short, x1[q]);
+            tile[r][c + q] = x0[q];
+            tile[r][c + 8 + q] = x1[q];
         }
     }
     __syncthreads();
     {
         const int f = tid >> 2, c = (tid & 3) * 16;
-        bf8 y0, y1;
+        us8 y0, y1;
 #pragma unroll
         for (int q = 0; q < 8; q++) {
-            y0[q] = __builtin_bit_cast(bf16, tile[c + q][f]);
-            y1[q] = __builtin_bit_cast(bf16, tile[c + 8 + q][f]);
+            y0[q] = tile[c + q][f];
+            y1[q] = tile[c + 8 + q][f];
         }
-        bf16 *dst = A0t + (size_t)(f0 + f) * B + rb + c;
-        *(bf8 *)dst = y0;
-        *(bf8 *)(dst + 8) = y1;
+        us8 *dst = (us8 *)(A0t + (size_t)(f0 + f) * B + rb + c);
+        dst[0] = y0;
+        dst[1] = y1;
     }
 }
 

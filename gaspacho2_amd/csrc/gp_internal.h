@@ -112,6 +112,7 @@ struct gp_ctx {
     size_t gc_part_len = 0;
     int gc_groups = 32;        // gene groups of the mixed contraction
     int gc_mixed = 1;          // mixed path: contraction on FP16 MFMA (0: FP64)
+    int irls_mixed = 1;        // mixed path: IRLS steps on the whitened FP16 hi/lo Gram (0: FP64)
     int autotune = 0;          // gp_fit runs gp_autotune before its first quasi-likelihood phase
 
     size_t tot_max = 0;        // N+p+(N+M+1)Q
@@ -121,6 +122,9 @@ struct gp_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_key[4] = {-1, -1, -1, -1};   // precision, slot, gram_splits, gc_groups
+    // objective and elapsed seconds after every accepted step of the last gp_fit
+    std::vector<double> fit_obj, fit_sec;
+    std::vector<int> fit_kind;           // 0 IRLS step, 1 quasi-likelihood start, 2 L-BFGS step
     std::string err;
 };
 
@@ -158,6 +162,9 @@ int gpk_w_nu(gp_ctx *c);                      // IRLS W / Nu from ZB = Z*Beta
 int gpk_r_and_sums(gp_ctx *c, const double *Zsrc);  // R = W nu / Omega, gq, gs, b = Zsrc^T R
 int gpk_init_y0(gp_ctx *c);                   // R = log((y+0.05)/ls) dense
 int gpk_beta_phi(gp_ctx *c, int jtr);
+int gpk_beta_phi_in(gp_ctx *c, int jtr, double *Beta);     // damped update of Beta, phi
+int gpk_beta_whiten(gp_ctx *c, const double *B, double *Bw);
+int gpk_beta_unwhiten(gp_ctx *c, const double *Bw, double *B);
 int gpk_lkhd_pois(gp_ctx *c);
 int gpk_lkhd_genes_with(gp_ctx *c, const double *tb, const double *Dchol);  // tb = Psi_j^{-1} b_j
 int gpk_collect(gp_ctx *c, int Nterm, int slot, double *obj);

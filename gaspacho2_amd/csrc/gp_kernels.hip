@@ -830,11 +830,35 @@ int gpk_init_y0(gp_ctx *c) {
     return 0;
 }
 
-int gpk_beta_phi(gp_ctx *c, int jtr) {
+int gpk_beta_phi(gp_ctx *c, int jtr) { return gpk_beta_phi_in(c, jtr, c->Beta); }
+
+int gpk_beta_phi_in(gp_ctx *c, int jtr, double *Beta) {
     // tBphi holds Psi_j^{-1} b_j on entry (solved by the caller)
     k_beta_phi<<<c->J, NT, 0, c->stream>>>(c->N, c->P, jtr, c->Psi, c->tBphi, c->zeta, c->gq,
-                                           c->Beta, c->phi, c->pwork);
+                                           Beta, c->phi, c->pwork);
     LAUNCH_CHECK();
+    return 0;
+}
+
+// Bw = T^-1 B with T = blockdiag(I, L^-T) (Kmm = L L^T): covariate rows
+// copied, GP rows = L^T B_GP (L is the lower triangle of Kchol).
+int gpk_beta_whiten(gp_ctx *c, const double *B, double *Bw) {
+    const double one = 1.0;
+    GP_HIP(hipMemcpy2DAsync(Bw, c->P * sizeof(double), B, c->P * sizeof(double),
+                            c->g0 * sizeof(double), c->J, hipMemcpyDeviceToDevice, c->stream));
+    GP_BLAS(rocblas_dtrmm(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_transpose,
+                          rocblas_diagonal_non_unit, c->M, c->J, &one, c->Kchol, c->M, B + c->g0, c->P,
+                          Bw + c->g0, c->P));
+    return 0;
+}
+
+// B = T Bw: covariate rows copied, GP rows = L^-T Bw_GP.
+int gpk_beta_unwhiten(gp_ctx *c, const double *Bw, double *B) {
+    const double one = 1.0, zero = 0.0;
+    GP_HIP(hipMemcpy2DAsync(B, c->P * sizeof(double), Bw, c->P * sizeof(double),
+                            c->g0 * sizeof(double), c->J, hipMemcpyDeviceToDevice, c->stream));
+    GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->M, c->J, c->M,
+                          &one, c->Linv, c->M, Bw + c->g0, c->P, &zero, B + c->g0, c->P));
     return 0;
 }
 

@@ -83,6 +83,7 @@ def load() -> C.CDLL:
         "gp_downdate": (C.c_int, [vp, C.c_int, C.c_double, C.c_int]),
         "gp_end_outer": (C.c_int, [vp]),
         "gp_fit": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp, dp]),
+        "gp_fit_trace": (C.c_int, [vp, dp, dp, ip, C.c_int, ip]),
         "gp_timing_enable": (C.c_int, [vp, C.c_int]),
         "gp_timing_reset": (C.c_int, [vp]),
         "gp_timing_get": (C.c_int, [vp, C.c_char_p, dp, ip]),
@@ -237,6 +238,15 @@ class GPContext:
         self._check(self._lib.gp_fit(self._h, int(itrmax), int(jtr_cap), int(bool(verbose)),
                                      None, None, C.byref(o)))
         return o.value
+
+    def fit_trace(self):
+        """(objective, seconds, kind) after every step of the last fit; kind 0
+        IRLS, 1 quasi-likelihood start, 2 L-BFGS."""
+        n = C.c_int()
+        self._check(self._lib.gp_fit_trace(self._h, None, None, None, 0, C.byref(n)))
+        o, t, k = np.zeros(n.value), np.zeros(n.value), np.zeros(n.value, np.int32)
+        self._check(self._lib.gp_fit_trace(self._h, _dp(o), _dp(t), _ip(k), n.value, C.byref(n)))
+        return o, t, k
 
     # -- score tests (ggplvm_de.cu:491-532)
     def score_tests(self, G, gene, snp) -> np.ndarray:

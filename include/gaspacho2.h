@@ -78,6 +78,9 @@ int gp_set_precision(gp_ctx *ctx, int precision);
  *                           (FP32 partial per group, FP64 sum across groups)
  *   "gc_mixed"    0/1       mixed path: contraction on FP16 hi/lo MFMA (1)
  *                           or in FP64 (0)
+ *   "irls_mixed"  0/1       mixed path: IRLS steps on the whitened FP16
+ *                           hi/lo Gram (1) or in FP64 in the reference
+ *                           basis (0)
  *   "autotune"    0/1       gp_fit calls gp_autotune before its first
  *                           quasi-likelihood phase (mixed precision only)
  * Unknown names and out-of-range values return a status > 0. */
@@ -234,6 +237,13 @@ int gp_end_outer(gp_ctx *ctx);
 typedef void (*gp_outer_cb)(gp_ctx *ctx, int itr, double obj, void *user);
 int gp_fit(gp_ctx *ctx, int itrmax, int jtr_cap, int verbose,
            gp_outer_cb on_outer, void *user, double *obj);
+
+/* Trace of the last gp_fit (no counterpart: the reference prints lkhd[jtr]
+ * per step, ggplvm.cu:1170,1276): the objective after every IRLS step
+ * (kind 0), quasi-likelihood start (1) and accepted L-BFGS step (2), with
+ * host seconds since gp_fit began.  Copies min(*len, cap) entries; any output
+ * pointer may be NULL; *len = entries recorded. */
+int gp_fit_trace(gp_ctx *ctx, double *obj, double *seconds, int *kind, int cap, int *len);
 
 /* util.c:73-87 initTaWithPer: initial inducing points T (M x Q,
  * column-major).  Dim 0 equispaced on (0, 2*pi); dims 1..Q-1 N(0,1) from

@@ -93,17 +93,20 @@ def test_encoder_adam_step_matches_oracle():
 
 
 def test_encoder_training_learns_the_latent_map():
-    """Trained on true latents of 4,000 cells, the encoder predicts the latents
-    of 1,000 fresh cells (same gene functions, new seed) far better than
-    their mean does, and its training loss falls."""
+    """Trained on the true latents of 4,000 cells (Q = 3, gene functions on 32
+    centres), the encoder predicts the latents of 1,000 fresh cells (same gene
+    functions, new seed) much better than their mean does (a float32 torch MLP
+    of the same shape reaches R^2 0.73 with a squared loss here), and its
+    training loss falls."""
     from gaspacho2_amd import lib, synth
-    tr = synth.make_dataset(4000, F, Q, 8, seed=11, model_seed=7)
-    te = synth.make_dataset(1000, F, Q, 8, seed=12, model_seed=7)
-    enc = lib.Encoder(F, Q, H1=H1, H2=H2, batch=BATCH, capacity=tr.N, seed=3)
+    q = 3
+    tr = synth.make_dataset(4000, F, q, 8, seed=11, model_seed=7, gen_M=32, pca=False)
+    te = synth.make_dataset(1000, F, q, 8, seed=12, model_seed=7, gen_M=32, pca=False)
+    enc = lib.Encoder(F, q, H1=H1, H2=H2, batch=BATCH, capacity=tr.N, seed=3)
     enc.add(tr.N, tr.Yp, tr.Yi, tr.Yx, tr.ls, tr.X_true)
     losses = enc.train(20, lr=1e-3)
     mu, _ = enc.predict(te.N, te.Yp, te.Yi, te.Yx, te.ls)
     r2 = 1.0 - np.sum((mu - te.X_true) ** 2) / np.sum((te.X_true - te.X_true.mean(0)) ** 2)
     print(f"epoch losses {losses[0]:.3f} -> {losses[-1]:.3f}; held-out R^2 {r2:.3f}")
-    assert losses[-1] < 0.5 * losses[0]
+    assert losses[-1] < losses[0] - 1.0
     assert r2 > 0.5

@@ -252,6 +252,35 @@ def test_mixed_precision_gradient_close_to_fp64():
     assert gr < 1e-4
 
 
+@pytest.mark.parametrize("shape", [(300, 20, 3, 8, 2, 3), (2000, 200, 5, 32, 0, 0)])
+def test_mixed_irls_steps_close_to_fp64(shape):
+    """IRLS steps on the whitened FP16 hi/lo Gram (option irls_mixed, the
+    default of a mixed context) against the FP64 steps of the reference basis
+    from the same start, with covariates (first shape) and without: the GLM
+    objectives within 1e-8, beta and phi within 1e-5 of their largest entry."""
+    from gaspacho2_amd import lib, synth
+    N, G, Q, M, nnum, nlev = shape
+    ds = synth.make_dataset(N, G, Q, M, seed=1, n_numeric=nnum, n_levels=nlev)
+    T0 = lib.init_ta_with_per(ds.M, ds.Q)
+    ctx = {}
+    for prec in (lib.PREC_FP64, lib.PREC_MIXED):
+        ctx[prec] = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls,
+                                  ds.X0, T0, precision=prec)
+        ctx[prec].init_fit()
+    assert ctx[lib.PREC_MIXED].get_option("irls_mixed") == 1
+    worst = 0.0
+    for jtr in range(6):
+        a, b = ctx[lib.PREC_FP64].irls_step(jtr), ctx[lib.PREC_MIXED].irls_step(jtr)
+        worst = max(worst, abs(a - b) / abs(a))
+    rb = rel(ctx[lib.PREC_MIXED].get("Beta"), ctx[lib.PREC_FP64].get("Beta"))
+    rp = rel(ctx[lib.PREC_MIXED].get("phi"), ctx[lib.PREC_FP64].get("phi"))
+    print(f"mixed IRLS: objective rel err {worst:.2e}, beta {rb:.2e}, phi {rp:.2e}")
+    assert worst < 1e-8
+    assert rb < 1e-5 and rp < 1e-5
+    for g in ctx.values():
+        g.close()
+
+
 def test_mixed_gradient_contraction_at_m256():
     """Mixed path at P = 257 (M = 256, the widest column-tile variant of both
     FP16 MFMA kernels).  The FP16 hi/lo contraction against the FP64
