@@ -165,18 +165,6 @@ def _trace_summary(o, sec, kind):
             "lbfgs_s": spent[2], "lbfgs_steps": counts[2]}
 
 
-def time_to_reach(o, sec, kind, target, rtol=1e-6):
-    """Seconds (and L-BFGS step count) at which a fit trace first reaches
-    target within rtol, or (None, None)."""
-    nst = 0
-    for v, t, k in zip(o, sec, kind):
-        if k == 2:
-            nst += 1
-            if v >= target - rtol * abs(target):
-                return t, nst
-    return None, None
-
-
 def main_fit(args):
     """End-to-end fit (gp_fit: init, IRLS phase, quasi-likelihood start,
     L-BFGS phase with its line search, per outer iteration) from the PCA
@@ -204,7 +192,8 @@ def main_fit(args):
             f"{res[prec]['irls_steps']} IRLS steps ({res[prec]['irls_s']:.2f} s)")
         g.close()
     f64, mx = res["f64"], res["mixed"]
-    t_hit, n_hit = time_to_reach(*mx["trace"], f64["final"])
+    from gaspacho2_amd.warmstart import steps_to_reach
+    n_hit, t_hit = steps_to_reach(mx["trace"], f64["final"])
     gap = (mx["final"] - f64["final"]) / abs(f64["final"])
     out = {"metric": "end-to-end fit seconds at N=100K,G=2K,M=256: mixed vs FP64 plain",
            "value": mx["seconds"], "unit": "s", "n_gpus": 1, "steps": mx["lbfgs_steps"], "warmup": 0,
@@ -219,6 +208,77 @@ def main_fit(args):
            "mixed_reaches_fp64_final": {"seconds": t_hit, "lbfgs_steps": n_hit,
                                         "speedup_vs_fp64_fit": (f64["seconds"] / t_hit) if t_hit else None},
            "speedup_same_schedule": f64["seconds"] / mx["seconds"]}
+    print(json.dumps(out), flush=True)
+
+
+def _make_ds(args_tuple):
+    from gaspacho2_amd import synth
+    N, G, Q, M, seed, model_seed = args_tuple
+    return seed, synth.make_dataset(N, G, Q, M, seed=seed, model_seed=model_seed)
+
+
+def main_warmstart(args):
+    """Amortization-encoder warm start (gaspacho2_amd.warmstart).  Fits
+    (mixed precision, PCA initial latent) on seeds 0..S-1 that share one set
+    of gene functions give (counts, fitted X) pairs; an encoder is trained on
+    them for each target in --enc-target (the first is the headline), and a
+    fresh seed is fitted with the same schedule from the PCA latent and from
+    each encoder's warm start.  Reports the objectives and the L-BFGS steps
+    the warm start needs to reach the PCA-started fit's final objective."""
+    from multiprocessing import get_context
+    from gaspacho2_amd import lib, warmstart
+    S = args.train_seeds
+    targets = args.enc_target.split(",")
+    t0 = time.perf_counter()
+    with get_context("fork").Pool(min(S + 1, 8)) as pool:
+        data = dict(pool.imap_unordered(_make_ds, [(args.N, args.G, args.Q, args.M, s, 1000)
+                                                   for s in range(S + 1)]))
+    log(f"data: {S + 1} seeds of N={args.N} G={args.G} ({time.perf_counter() - t0:.1f} s)")
+    T0 = lib.init_ta_with_per(args.M, args.Q)
+    train = []
+    for s in range(S):
+        X, obj, dt, _ = warmstart.fit_from(data[s], data[s].X0, T0, args.fit_itr, args.fit_cap)
+        train.append((data[s], X))
+        log(f"seed {s}: PCA-started fit {dt:.1f} s, objective {obj:.10e}, "
+            f"rms(X - X_pca) {np.sqrt(np.mean((X - data[s].X0) ** 2)):.3f}")
+    ds = data[S]
+    Xp, obj_p, dt_p, tr_p = warmstart.fit_from(ds, ds.X0, T0, args.fit_itr, args.fit_cap)
+    n_pca = int(np.sum(tr_p[2] == 2))
+    first = lambda tr: float(tr[0][np.argmax(tr[2] == 1)])
+    log(f"fresh seed, PCA start: {obj_p:.10e} after {n_pca} L-BFGS steps ({dt_p:.1f} s)")
+    variants = {}
+    for target in targets:
+        t1 = time.perf_counter()
+        enc, losses = warmstart.train_encoder(train, target, epochs=args.enc_epochs, log=log)
+        t_train = time.perf_counter() - t1
+        X0e = warmstart.warm_start(enc, ds, target)
+        enc.close()
+        log(f"{target}: encoder {S * args.N} cells, {args.enc_epochs} epochs, {t_train:.1f} s, "
+            f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+        _, obj_e, dt_e, tr_e = warmstart.fit_from(ds, X0e, T0, args.fit_itr, args.fit_cap)
+        n_hit, t_hit = warmstart.steps_to_reach(tr_e, obj_p)
+        log(f"{target}: warm-started fit {obj_e:.10e} ({dt_e:.1f} s), reaches the PCA-started final after "
+            f"{n_hit} of {n_pca} L-BFGS steps")
+        variants[target] = {
+            "steps_saved": (n_pca - n_hit) if n_hit else None,
+            "final_objective": obj_e, "rel_gap_vs_pca_start": (obj_e - obj_p) / abs(obj_p),
+            "objective_after_ql_start": first(tr_e), "fit_seconds": dt_e,
+            "reaches_pca_final": {"lbfgs_steps": n_hit, "seconds": t_hit},
+            "encoder_train_seconds": t_train, "encoder_epoch_loss": [float(losses[0]), float(losses[-1])],
+            "warm_start_rms_to_pca_fit": float(np.sqrt(np.mean((X0e - Xp) ** 2)))}
+    head = variants[targets[0]]
+    out = {"metric": "L-BFGS steps saved by the encoder warm start (fresh seed, mixed fit)",
+           "value": head["steps_saved"], "unit": "steps", "n_gpus": 1,
+           "steps": n_pca, "warmup": 0, "ms_per_step": head["fit_seconds"] / max(n_pca, 1) * 1e3,
+           "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+           "dtype": "bf16/f32 MFMA encoder, fp16x3/f32 + f64 fit", "data": f"synthetic, seeds 0..{S}",
+           "config": {"workload": f"encoder warm start: {S} training seeds x N={args.N}, G={args.G}, "
+                                  f"Q={args.Q}, M={args.M}; MLP {args.G}-512-256-{2 * args.Q}; "
+                                  f"{args.fit_itr} outer iteration(s), L-BFGS cap {args.fit_cap}",
+                      "target": targets[0]},
+           "pca_start": {"final_objective": obj_p, "objective_after_ql_start": first(tr_p), "fit_seconds": dt_p,
+                         "pca_rms_to_pca_fit": float(np.sqrt(np.mean((ds.X0 - Xp) ** 2)))},
+           "variants": variants}
     print(json.dumps(out), flush=True)
 
 
@@ -239,7 +299,11 @@ def main():
     ap.add_argument("--fit-itr", type=int, default=1, help="fit workload: outer iterations")
     ap.add_argument("--verbose", action="store_true", help="fit workload: print every objective")
     ap.add_argument("--fit-cap", type=int, default=100, help="fit workload: L-BFGS steps per outer iteration")
-    ap.add_argument("--workload", choices=["elbo", "batched", "fit"], default="elbo",
+    ap.add_argument("--train-seeds", type=int, default=8, help="warmstart workload: training fits")
+    ap.add_argument("--enc-epochs", type=int, default=30, help="warmstart workload: encoder epochs")
+    ap.add_argument("--enc-target", default="residual,aligned",
+                    help="warmstart workload: encoder targets, comma-separated (first = headline)")
+    ap.add_argument("--workload", choices=["elbo", "batched", "fit", "warmstart"], default="elbo",
                     help="elbo: the ELBO+gradient unit (default); batched: uneven per-gene GLM fits")
     ap.add_argument("--nmin", type=int, default=500)
     ap.add_argument("--nmax", type=int, default=50000)
@@ -263,6 +327,9 @@ def main():
         return
     if args.workload == "fit":
         main_fit(args)
+        return
+    if args.workload == "warmstart":
+        main_warmstart(args)
         return
 
     from gaspacho2_amd import lib, synth

@@ -7,7 +7,8 @@
 //   z1 = W1 a0 + b1, h1 = gelu(z1)             (H1)
 //   z2 = W2 h1 + b2, h2 = gelu(z2)             (H2)
 //   z3 = W3 h2 + b3 = [mu (Q) | log-variance (Q)]
-//   loss = (1/n) sum_i sum_q 1/2 [ (x_iq - mu_iq)^2 exp(-s_iq) + s_iq ]
+//   loss = (1/n) sum_i sum_q 1/2 [ (x_iq - mu_iq)^2 exp(-s_iq) + s_iq ],
+//   s = log-variance clamped to [-10, 10]
 //
 // gelu is the exact form z Phi(z).  Weights are FP32 masters with BF16 copies
 // for the products; activations and backward signals enter the products as
@@ -183,7 +184,11 @@ __global__ void __launch_bounds__(ET) k_enc_gather(int Fp, int B, int n, const i
 
 // Gaussian negative log-likelihood of the targets (without the 1/2 log 2 pi
 // constant) and its gradient with respect to z3, over batch rows r < n
-// (rows >= n and columns >= 2Q get zero).  One thread per row.
+// (rows >= n and columns >= 2Q get zero).  The log-variance is clamped to
+// [-LV_CLAMP, LV_CLAMP] (zero gradient outside) so that exp(-s) stays finite
+// in FP32.  One thread per row.
+constexpr float LV_CLAMP = 10.0f;
+
 __global__ void __launch_bounds__(ET) k_enc_loss(int B, int n, int Q, int O, const int *__restrict__ idx,
                                                  const float *__restrict__ tgt, const float *__restrict__ Z3,
                                                  float *__restrict__ dZ3, bf16 *__restrict__ dZ3b,
@@ -198,12 +203,13 @@ __global__ void __launch_bounds__(ET) k_enc_loss(int B, int n, int Q, int O, con
             if (r < n && q < 2 * Q) {
                 const int qq = q < Q ? q : q - Q;
                 const float x = tgt[(size_t)idx[r] * Q + qq];
-                const float mu = Z3[(size_t)r * O + qq], s = Z3[(size_t)r * O + Q + qq];
+                const float mu = Z3[(size_t)r * O + qq], sr = Z3[(size_t)r * O + Q + qq];
+                const float s = fminf(fmaxf(sr, -LV_CLAMP), LV_CLAMP);
                 const float res = x - mu, es = expf(-s);
                 if (q < Q) {
                     d = -res * es * inv_n;
                     ls += 0.5 * ((double)res * res * es + s);
-                } else {
+                } else if (sr > -LV_CLAMP && sr < LV_CLAMP) {
                     d = 0.5f * (1.0f - res * res * es) * inv_n;
                 }
             }
@@ -617,6 +623,10 @@ extern "C" int gp_enc_train(gp_enc *en, int epochs, double lr, double *epoch_los
         }
         // per-batch losses are batch means: the epoch value is their cell-weighted mean
         if (rc == 0 && epoch_loss) epoch_loss[ep] = s / (double)n;
+        if (rc == 0 && !std::isfinite(s)) {
+            gp_set_error("encoder training diverged: non-finite loss in epoch " + std::to_string(ep));
+            rc = 3;
+        }
     }
     (void)hipStreamSynchronize(en->stream);
     (void)hipFree(perm_d);

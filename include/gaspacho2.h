@@ -196,7 +196,8 @@ int gp_bfit_dims(gp_bfit *b, int *J, int *PB);
  *   a0 = log1p(y_i. / s_i) (F = genes) -> H1 -> H2 -> [mu (Q) | log-variance (Q)]
  * with exact GELU, BF16 weight copies and activations in the products, FP32
  * accumulation, FP32 Adam (0.9, 0.999, 1e-8) on FP32 master weights; loss =
- * mean over cells of sum_q 1/2 [(x - mu)^2 exp(-s) + s].  H1, H2 and batch
+ * mean over cells of sum_q 1/2 [(x - mu)^2 exp(-s) + s], s = log-variance
+ * clamped to [-10, 10] (zero gradient outside).  H1, H2 and batch
  * must be multiples of 64, 2Q <= 64.  Parameters: one FP32 array
  * [W1 (H1 x Fp) | b1 | W2 (H2 x H1) | b2 | W3 (64 x H2) | b3 (64)], row-major,
  * Fp = F rounded up to 64 (padding stays zero).
@@ -217,7 +218,8 @@ int gp_enc_loss_grad(gp_enc *enc, int n, double *loss);
 /* One Adam step from the current gradient. */
 int gp_enc_adam_step(gp_enc *enc, double lr);
 /* epochs passes over the loaded cells in shuffled batches; epoch_loss[e] is
- * the cell-weighted mean of the batch losses of epoch e (may be NULL). */
+ * the cell-weighted mean of the batch losses of epoch e (may be NULL).
+ * Returns 3 if an epoch's loss is not finite. */
 int gp_enc_train(gp_enc *enc, int epochs, double lr, double *epoch_loss);
 /* mu and logvar (N x Q column-major; logvar may be NULL) for new cells. */
 int gp_enc_predict(gp_enc *enc, int N, int J, const int *Yp, const int *Yi, const double *Yx,

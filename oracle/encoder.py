@@ -10,7 +10,7 @@ off its gradient is checked against central finite differences of its own
 loss and against torch.autograd in float64 (tests/test_oracle_pins.py), and
 Adam against the closed form of its first step (|update| = lr per entry).
 
-With bf16=True it rounds exactly where the GPU path rounds to BF16 (round to
+With rnd=True it rounds exactly where the GPU path rounds to BF16 (round to
 nearest even): the features, the weight copies, the hidden activations fed
 to the next product and the backward signals fed to the products; it keeps
 the GPU's FP32 values (pre-activations, bias gradients, loss) unrounded.  The
@@ -19,7 +19,8 @@ products themselves are float64 here and FP32-accumulated on the GPU.
 Model:
   a0 = log1p(y / s); z1 = W1 a0 + b1; h1 = gelu(z1); z2 = W2 h1 + b2;
   h2 = gelu(z2); z3 = W3 h2 + b3 = [mu | logvar];
-  loss = mean_i sum_q 1/2 [(x_iq - mu_iq)^2 exp(-logvar_iq) + logvar_iq].
+  loss = mean_i sum_q 1/2 [(x_iq - mu_iq)^2 exp(-s_iq) + s_iq],
+  s = logvar clamped to [-10, 10] (zero gradient outside).
 """
 from __future__ import annotations
 
@@ -27,6 +28,7 @@ import numpy as np
 from scipy.special import erf
 
 _SQRT1_2 = 0.70710678118654752
+LV_CLAMP = 10.0      # log-variance clamp of the loss (zero gradient outside)
 _INV_SQRT_2PI = 0.39894228040143268
 
 
@@ -83,11 +85,13 @@ def loss_grad(p: dict, A0: np.ndarray, X: np.ndarray, rnd: bool = True):
     r = bf16 if rnd else (lambda v: v)
     n, Q = X.shape
     f = forward(p, A0, rnd)
-    mu, s = f["z3"][:, :Q], f["z3"][:, Q:]
+    mu, sr = f["z3"][:, :Q], f["z3"][:, Q:]
+    s = np.clip(sr, -LV_CLAMP, LV_CLAMP)
     res = X - mu
     es = np.exp(-s)
     loss = 0.5 * np.sum(res * res * es + s) / n
-    dz3 = np.concatenate([-res * es, 0.5 * (1.0 - res * res * es)], axis=1) / n
+    inside = (sr > -LV_CLAMP) & (sr < LV_CLAMP)
+    dz3 = np.concatenate([-res * es, np.where(inside, 0.5 * (1.0 - res * res * es), 0.0)], axis=1) / n
     g = {}
     d3 = r(dz3)
     g["W3"] = d3.T @ f["h2"]

@@ -376,6 +376,29 @@ def test_encoder_oracle_gradient_matches_finite_differences():
             assert abs(fd - g[name][idx]) <= 1e-7 * max(1.0, abs(fd)), (name, idx, fd, g[name][idx])
 
 
+def test_encoder_oracle_log_variance_clamp_has_zero_gradient_outside():
+    E, p, A0, X = _enc_case(seed=5)
+    Q = X.shape[1]
+    p["b3"][Q] = 14.0            # first log-variance output above the clamp
+    p["b3"][Q + 1] = -14.0       # second below it
+    loss, g = E.loss_grad(p, A0, X, rnd=False)
+    for k in (Q, Q + 1):
+        h = 1e-6
+        p["b3"][k] += h
+        lp, _ = E.loss_grad(p, A0, X, rnd=False)
+        p["b3"][k] -= 2 * h
+        lm, _ = E.loss_grad(p, A0, X, rnd=False)
+        p["b3"][k] += h
+        assert lp == lm == loss and g["b3"][k] == 0.0
+    # the clamped value enters the loss: 1/2 (r^2 e^-10 + 10) for the first, e^+10 for the second
+    f = E.forward(p, A0, rnd=False)["z3"]
+    r0, r1 = X[:, 0] - f[:, 0], X[:, 1] - f[:, 1]
+    rest = 0.5 * np.sum((X[:, 2:] - f[:, 2:Q]) ** 2 * np.exp(-f[:, Q + 2:]) + f[:, Q + 2:])
+    expect = (0.5 * np.sum(r0 ** 2 * np.exp(-10.0) + 10.0) + 0.5 * np.sum(r1 ** 2 * np.exp(10.0) - 10.0)
+              + rest) / X.shape[0]
+    assert abs(loss - expect) <= 1e-12 * abs(expect)
+
+
 def test_encoder_oracle_gradient_matches_torch_autograd():
     torch = pytest.importorskip("torch")
     E, p, A0, X = _enc_case(seed=2, F=30, H1=16, H2=12, Q=3, n=9)
