@@ -232,6 +232,9 @@ extern "C" int gp_set_option(gp_ctx *c, const char *name, int value) {
     } else if (n == "gc_groups") {
         if (value < 1 || value > 4096) { gp_set_error("gc_groups must be in [1, 4096]"); return 1; }
         c->gc_groups = value;
+    } else if (n == "gram_waves") {
+        if (value != 4 && value != 8) { gp_set_error("gram_waves must be 4 or 8"); return 1; }
+        c->gram_waves = value;
     } else if (n == "gc_mixed") {
         c->gc_mixed = value ? 1 : 0;
     } else if (n == "irls_mixed") {
@@ -251,6 +254,7 @@ extern "C" int gp_get_option(gp_ctx *c, const char *name, int *value) {
     if (n == "gram_splits") *value = c->gram_splits;
     else if (n == "gc_groups") *value = c->gc_groups;
     else if (n == "gc_mixed") *value = c->gc_mixed;
+    else if (n == "gram_waves") *value = c->gram_waves;
     else if (n == "irls_mixed") *value = c->irls_mixed;
     else if (n == "autotune") *value = c->autotune;
     else { gp_set_error("unknown option " + n); return 1; }
@@ -575,8 +579,8 @@ static void drop_graph(gp_ctx *c) {
 extern "C" int gp_eval_graph(gp_ctx *c, int slot, double *obj) {
     if (!c) { gp_set_error("NULL context"); return 1; }
     if (slot < 0 || slot >= GP_LKHD_SLOTS) { gp_set_error("slot out of range"); return 1; }
-    const int key[4] = {c->precision, slot, c->gram_splits, c->gc_groups};
-    if (!c->graph_exec || !std::equal(key, key + 4, c->graph_key)) {
+    const int key[5] = {c->precision, slot, c->gram_splits, c->gc_groups, c->gram_waves};
+    if (!c->graph_exec || !std::equal(key, key + 5, c->graph_key)) {
         drop_graph(c);
         double tmp = 0.0;
         int r = gp_elbo(c, slot, &tmp);
@@ -596,7 +600,7 @@ extern "C" int gp_eval_graph(gp_ctx *c, int slot, double *obj) {
         }
         c->graph = g;
         GP_HIP(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
-        std::copy(key, key + 4, c->graph_key);
+        std::copy(key, key + 5, c->graph_key);
     }
     GP_HIP(hipGraphLaunch(c->graph_exec, c->stream));
     if (obj) GP_HIP(hipMemcpyAsync(obj, c->lkhd + slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -737,6 +741,7 @@ extern "C" int gp_autotune(gp_ctx *c, int verbose) {
     if (c->precision != GP_PREC_MIXED) { gp_set_error("gp_autotune needs GP_PREC_MIXED"); return 1; }
     static const int splits[] = {8, 16, 32};
     static const int groups[] = {16, 32, 64};
+    static const int waves[] = {8, 4};
     GP_TRY(gpk_expand_delta(c));
     if (!c->zw_valid) {
         GP_TRY(gpk_whiten(c));
@@ -744,15 +749,18 @@ extern "C" int gp_autotune(gp_ctx *c, int verbose) {
         c->zsplit_valid = 0;
     }
     double best = 1e300;
-    int bs = c->gram_splits;
-    for (int sv : splits) {
-        c->gram_splits = sv;
-        double ms = 0.0;
-        GP_TRY(time_ms(c, tune_gram, 2, &ms));
-        if (verbose) std::printf("autotune: gram_splits %d: %.3f ms\n", sv, ms);
-        if (ms < best) { best = ms; bs = sv; }
-    }
+    int bs = c->gram_splits, bw = c->gram_waves;
+    for (int wv : waves)
+        for (int sv : splits) {
+            c->gram_splits = sv;
+            c->gram_waves = wv;
+            double ms = 0.0;
+            GP_TRY(time_ms(c, tune_gram, 2, &ms));
+            if (verbose) std::printf("autotune: gram_waves %d gram_splits %d: %.3f ms\n", wv, sv, ms);
+            if (ms < best) { best = ms; bs = sv; bw = wv; }
+        }
     c->gram_splits = bs;
+    c->gram_waves = bw;
     // the contraction needs the inverses of the current factors
     GP_TRY(clear_info(c));
     GP_TRY(tune_gram(c));

@@ -189,10 +189,15 @@ __global__ void __launch_bounds__(GCW * 64) k_gc_mixed(int N, int Npad, int P, i
     __syncthreads();
     int cur = 0;
     const int boff = (lane & 15) * LDK + 8 * (lane >> 4);
+    // the cell's weight for the group's first gene; later genes' weights are
+    // loaded one gene ahead, so the load's latency is not paid at each gene start
+    const int cellc = min(cell, N - 1);
+    float wnext = (j0 < j1) ? w32[cellc + (size_t)perm[j0] * Npad] : 0.0f;
     for (int jj = j0; jj < j1; jj++) {
         const int j = perm[jj];
-        const float o = (cell < N) ? w32[cell + (size_t)j * Npad] * ldexpf(1.0f, aexp[j]) : 0.0f;
         const int jn = (jj + 1 < j1) ? perm[jj + 1] : j;
+        const float o = (cell < N) ? wnext * ldexpf(1.0f, aexp[j]) : 0.0f;
+        wnext = w32[cellc + (size_t)jn * Npad];
 #pragma unroll
         for (int ks = 0; ks < KS; ks++) {
             const bool more = (ks + 1 < KS) || (jj + 1 < j1);

@@ -414,38 +414,39 @@ __device__ __forceinline__ void gram_wave(const GramArgs &g, _Float16 *Xs, float
             acc1[x][y] = f4{0.f, 0.f, 0.f, 0.f};
         }
 
+    // The prefetch is straight-line code (item index and chunk clamped, no
+    // branches around the loads) and is pinned ahead of the MFMAs by a
+    // scheduling barrier, so its wait lands at its first use in store(),
+    // after this chunk's MFMAs: the L2/HBM latency hides behind them.
     f4 pz[IPT][2];
     auto fetch = [&](int cc) {
 #pragma unroll
         for (int r = 0; r < IPT; r++) {
-            const int it = tid + GW * 64 * r;
-            if (it < NITEM) {
-                const float *src = g.z32 + (size_t)(it >> 2) * g.Npad + cc + 8 * (it & 3);
-                pz[r][0] = *(const f4 *)src;
-                pz[r][1] = *(const f4 *)(src + 4);
-            }
+            const int it = min(tid + GW * 64 * r, NITEM - 1);
+            const float *src = g.z32 + (size_t)(it >> 2) * g.Npad + cc + 8 * (it & 3);
+            pz[r][0] = *(const f4 *)src;
+            pz[r][1] = *(const f4 *)(src + 4);
         }
     };
     auto store = [&](int buf, int wb) {
+        // items past the end repeat the last item (same values, same place)
 #pragma unroll
         for (int r = 0; r < IPT; r++) {
-            const int it = tid + GW * 64 * r;
-            if (it < NITEM) {
-                const int col = it >> 2, q = it & 3;
-                const f4 s0 = *(const f4 *)&wsl[wb][8 * q];
-                const f4 s1 = *(const f4 *)&wsl[wb][8 * q + 4];
-                h8 hi, lo;
+            const int it = min(tid + GW * 64 * r, NITEM - 1);
+            const int col = it >> 2, q = it & 3;
+            const f4 s0 = *(const f4 *)&wsl[wb][8 * q];
+            const f4 s1 = *(const f4 *)&wsl[wb][8 * q + 4];
+            h8 hi, lo;
 #pragma unroll
-                for (int e = 0; e < 8; e++) {
-                    float v = (e < 4 ? pz[r][0][e] * s0[e] : pz[r][1][e - 4] * s1[e - 4]);
-                    asm volatile("" : "+v"(v));
-                    const _Float16 h = (_Float16)v;
-                    hi[e] = h;
-                    lo[e] = (_Float16)(v - (float)h);
-                }
-                *(h8 *)&Xs[((buf * 2 + 0) * P16 + col) * LDX + 8 * q] = hi;
-                *(h8 *)&Xs[((buf * 2 + 1) * P16 + col) * LDX + 8 * q] = lo;
+            for (int e = 0; e < 8; e++) {
+                float v = (e < 4 ? pz[r][0][e] * s0[e] : pz[r][1][e - 4] * s1[e - 4]);
+                asm volatile("" : "+v"(v));
+                const _Float16 h = (_Float16)v;
+                hi[e] = h;
+                lo[e] = (_Float16)(v - (float)h);
             }
+            *(h8 *)&Xs[((buf * 2 + 0) * P16 + col) * LDX + 8 * q] = hi;
+            *(h8 *)&Xs[((buf * 2 + 1) * P16 + col) * LDX + 8 * q] = lo;
         }
     };
     auto sqw = [&](int ch) { return sqrtf(g.wj[g.c0 + ch * CH + tid] * g.sc); };   // tid < CH
@@ -463,15 +464,22 @@ __device__ __forceinline__ void gram_wave(const GramArgs &g, _Float16 *Xs, float
     for (int ch = 0; ch < g.nch; ch++) {
         const int buf = ch & 1;
         const bool more = ch + 1 < g.nch;
-        float pw = 0.0f;
-        if (more) fetch(g.c0 + (ch + 1) * CH);
-        if (tid < CH && ch + 2 < g.nch) pw = g.wj[g.c0 + (ch + 2) * CH + tid];
+        const float pw = g.wj[g.c0 + min(ch + 2, g.nch - 1) * CH + (tid & (CH - 1))];
+        fetch(g.c0 + min(ch + 1, g.nch - 1) * CH);
+        __builtin_amdgcn_sched_barrier(0);   // issue the prefetch ahead of this chunk's MFMAs
         const _Float16 *xh = Xs + (size_t)(buf * 2 + 0) * P16 * LDX;
         const _Float16 *xl = Xs + (size_t)(buf * 2 + 1) * P16 * LDX;
         if constexpr (A0 >= 0) gram_block<T16, A0, B0>(acc0, xh, xl, lrow, lk);
         if constexpr (A1 >= 0) gram_block<T16, A1, B1>(acc1, xh, xl, lrow, lk);
-        if (more) store(buf ^ 1, (ch + 1) % 3);
-        if (tid < CH && ch + 2 < g.nch) wsl[(ch + 2) % 3][tid] = sqrtf(pw * g.sc);
+        // unconditional, so the prefetch's only use is in this block and it is
+        // not sunk into a branch (the last chunk's extra store is never read)
+        store(buf ^ 1, (ch + 1) % 3);
+        // every thread writes (32 distinct values, duplicates identical) so the
+        // weight load is not sunk into a branch either; past the last chunk the
+        // slot holds the last chunk's weights and is never read.  Kept after
+        // the MFMAs so that its wait does not stall them.
+        __builtin_amdgcn_sched_barrier(0);
+        wsl[(ch + 2) % 3][tid & (CH - 1)] = sqrtf(pw * g.sc);
         __syncthreads();
     }
     if constexpr (A0 >= 0) gram_block_store<T16, A0, B0>(acc0, g.pg, lane);
@@ -517,6 +525,235 @@ __global__ void __launch_bounds__(GW * 64) k_gram_sym(int Npad, int seg, int nse
     case 5: gram_wave<T16, 5>(g, Xs, wsl); break;
     case 6: gram_wave<T16, 6>(g, Xs, wsl); break;
     default: gram_wave<T16, 7>(g, Xs, wsl); break;
+    }
+}
+
+// ---- four-wave form: one wave per SIMD.  Each wave owns about a quarter of
+// the upper tiles as blocks over 4-tile column groups and loads the fragments
+// of every group it touches once per chunk, so blocks of one wave share them
+// (at P16 = 272: 102 fragment reads per chunk instead of 170).  Blocks are
+// dealt largest first to the least loaded wave, ties going to a wave that
+// already touches one of the block's groups.  A single wave per SIMD hides
+// latency by interleaving its own LDS reads, staging arithmetic and MFMAs.
+constexpr int GW4 = 4;       // waves per workgroup
+constexpr int MAXB4 = 8;     // blocks per wave
+
+struct GramPlan4 {
+    int nb[GW4];
+    int a[GW4][MAXB4];
+    int b[GW4][MAXB4];
+};
+
+constexpr int gram_blk_cost(int T16, int a, int b) {
+    const int NG = (T16 + 3) / 4, RL = T16 - 4 * (NG - 1);
+    const int sa = a < NG - 1 ? 4 : RL, sb = b < NG - 1 ? 4 : RL;
+    return a == b ? sa * (sa + 1) / 2 : sa * sb;
+}
+
+constexpr GramPlan4 make_gram_plan4(int T16) {
+    GramPlan4 p{};
+    const int NG = (T16 + 3) / 4;
+    int ba[16] = {}, bb[16] = {}, bc[16] = {};
+    int nb = 0;
+    for (int a = 0; a < NG; a++)
+        for (int b = a; b < NG; b++) {
+            ba[nb] = a; bb[nb] = b; bc[nb] = gram_blk_cost(T16, a, b);
+            nb++;
+        }
+    for (int i = 1; i < nb; i++)
+        for (int k = i; k > 0 && bc[k] > bc[k - 1]; k--) {
+            int t = ba[k]; ba[k] = ba[k - 1]; ba[k - 1] = t;
+            t = bb[k]; bb[k] = bb[k - 1]; bb[k - 1] = t;
+            t = bc[k]; bc[k] = bc[k - 1]; bc[k - 1] = t;
+        }
+    int load[GW4] = {};
+    unsigned gm[GW4] = {};
+    for (int i = 0; i < nb; i++) {
+        const unsigned need = (1u << ba[i]) | (1u << bb[i]);
+        int best = -1;
+        for (int v = 0; v < GW4; v++) {
+            if (p.nb[v] >= MAXB4) continue;
+            if (best < 0 || load[v] < load[best] ||
+                (load[v] == load[best] && (gm[v] & need) && !(gm[best] & need)))
+                best = v;
+        }
+        p.a[best][p.nb[best]] = ba[i];
+        p.b[best][p.nb[best]] = bb[i];
+        p.nb[best]++;
+        load[best] += bc[i];
+        gm[best] |= need;
+    }
+    return p;
+}
+
+constexpr unsigned gram_group_mask4(int T16, int W) {
+    const GramPlan4 p = make_gram_plan4(T16);
+    unsigned m = 0;
+    for (int i = 0; i < p.nb[W]; i++) m |= (1u << p.a[W][i]) | (1u << p.b[W][i]);
+    return m;
+}
+
+// one 32-cell step of block (A, B) from fragments already in registers
+template <int T16, int A, int B>
+__device__ __forceinline__ void gram_block4(f4 (&acc)[4][4], const h8 (&ah)[4], const h8 (&al)[4],
+                                            const h8 (&bh)[4], const h8 (&bl)[4]) {
+    constexpr int NG = (T16 + 3) / 4, RL = T16 - 4 * (NG - 1);
+    constexpr int SA = A < NG - 1 ? 4 : RL, SB = B < NG - 1 ? 4 : RL;
+    if constexpr (A == B) {
+#pragma unroll
+        for (int x = 0; x < SA; x++)
+#pragma unroll
+            for (int y = x; y < SA; y++) mfma3(acc[x][y], ah[x], al[x], ah[y], al[y]);
+    } else {
+#pragma unroll
+        for (int y = 0; y < SB; y++)
+#pragma unroll
+            for (int x = 0; x < SA; x++) mfma3(acc[x][y], ah[x], al[x], bh[y], bl[y]);
+    }
+}
+
+template <int T16, int W, int I>
+__device__ __forceinline__ void gram_run4(f4 (&acc)[MAXB4][4][4], const h8 (&fh)[5][4],
+                                          const h8 (&fl)[5][4]) {
+    constexpr GramPlan4 pl = make_gram_plan4(T16);
+    if constexpr (I < pl.nb[W]) {
+        constexpr int A = pl.a[W][I], B = pl.b[W][I];
+        gram_block4<T16, A, B>(acc[I], fh[A], fl[A], fh[B], fl[B]);
+        gram_run4<T16, W, I + 1>(acc, fh, fl);
+    }
+}
+
+template <int T16, int W, int I>
+__device__ __forceinline__ void gram_store4(const f4 (&acc)[MAXB4][4][4], float *pg, int lane) {
+    constexpr GramPlan4 pl = make_gram_plan4(T16);
+    if constexpr (I < pl.nb[W]) {
+        constexpr int A = pl.a[W][I], B = pl.b[W][I];
+        gram_block_store<T16, A, B>(acc[I], pg, lane);
+        gram_store4<T16, W, I + 1>(acc, pg, lane);
+    }
+}
+
+template <int T16, int W>
+__device__ __forceinline__ void gram_wave4(const GramArgs &g, _Float16 *Xs, float (*wsl)[CH]) {
+    constexpr int P16 = 16 * T16, NG = (T16 + 3) / 4, RL = T16 - 4 * (NG - 1);
+    constexpr unsigned GM = gram_group_mask4(T16, W);
+    constexpr int NITEM = P16 * 4;                // (column, 8-cell group) items per chunk
+    constexpr int IPT = (NITEM + GW4 * 64 - 1) / (GW4 * 64);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int lrow = lane & 15, lk = 8 * (lane >> 4);
+
+    f4 acc[MAXB4][4][4];
+#pragma unroll
+    for (int i = 0; i < MAXB4; i++)
+#pragma unroll
+        for (int x = 0; x < 4; x++)
+#pragma unroll
+            for (int y = 0; y < 4; y++) acc[i][x][y] = f4{0.f, 0.f, 0.f, 0.f};
+
+    // staging as in the eight-wave form: straight-line, clamped items
+    f4 pz[IPT][2];
+    auto fetch = [&](int cc) {
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            const int it = min(tid + GW4 * 64 * r, NITEM - 1);
+            const float *src = g.z32 + (size_t)(it >> 2) * g.Npad + cc + 8 * (it & 3);
+            pz[r][0] = *(const f4 *)src;
+            pz[r][1] = *(const f4 *)(src + 4);
+        }
+    };
+    auto store = [&](int buf, int wb) {
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            const int it = min(tid + GW4 * 64 * r, NITEM - 1);
+            const int col = it >> 2, q = it & 3;
+            const f4 s0 = *(const f4 *)&wsl[wb][8 * q];
+            const f4 s1 = *(const f4 *)&wsl[wb][8 * q + 4];
+            h8 hi, lo;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                float v = (e < 4 ? pz[r][0][e] * s0[e] : pz[r][1][e - 4] * s1[e - 4]);
+                asm volatile("" : "+v"(v));
+                const _Float16 h = (_Float16)v;
+                hi[e] = h;
+                lo[e] = (_Float16)(v - (float)h);
+            }
+            *(h8 *)&Xs[((buf * 2 + 0) * P16 + col) * LDX + 8 * q] = hi;
+            *(h8 *)&Xs[((buf * 2 + 1) * P16 + col) * LDX + 8 * q] = lo;
+        }
+    };
+    auto sqw = [&](int ch) { return sqrtf(g.wj[g.c0 + ch * CH + (tid & (CH - 1))] * g.sc); };
+
+    wsl[0][tid & (CH - 1)] = sqw(0);
+    wsl[1][tid & (CH - 1)] = sqw(min(1, g.nch - 1));
+    fetch(g.c0);
+    __syncthreads();
+    store(0, 0);
+    __syncthreads();
+
+    for (int ch = 0; ch < g.nch; ch++) {
+        const int buf = ch & 1;
+        const float pw = g.wj[g.c0 + min(ch + 2, g.nch - 1) * CH + (tid & (CH - 1))];
+        fetch(g.c0 + min(ch + 1, g.nch - 1) * CH);
+        __builtin_amdgcn_sched_barrier(0);
+        const _Float16 *xh = Xs + (size_t)(buf * 2 + 0) * P16 * LDX;
+        const _Float16 *xl = Xs + (size_t)(buf * 2 + 1) * P16 * LDX;
+        h8 fh[5][4], fl[5][4];
+#pragma unroll
+        for (int gi = 0; gi < NG; gi++) {
+            if ((GM >> gi) & 1) {
+                const int SG = gi < NG - 1 ? 4 : RL;
+#pragma unroll
+                for (int x = 0; x < 4; x++) {
+                    if (x < SG) {
+                        const int col = (4 * gi + x) * 16 + lrow;
+                        fh[gi][x] = *(const h8 *)(xh + col * LDX + lk);
+                        fl[gi][x] = *(const h8 *)(xl + col * LDX + lk);
+                    }
+                }
+            }
+        }
+        gram_run4<T16, W, 0>(acc, fh, fl);
+        store(buf ^ 1, (ch + 1) % 3);   // past the last chunk: a copy never read
+        __builtin_amdgcn_sched_barrier(0);
+        wsl[(ch + 2) % 3][tid & (CH - 1)] = sqrtf(pw * g.sc);
+        __syncthreads();
+    }
+    gram_store4<T16, W, 0>(acc, g.pg, lane);
+}
+
+template <int T16>
+__global__ void __launch_bounds__(GW4 * 64) k_gram_sym4(int Npad, int seg, int nseg, int J,
+        const float *__restrict__ z32, const float *__restrict__ w32,
+        const float *__restrict__ wsc, float *__restrict__ part) {
+    constexpr int NT = T16 * (T16 + 1) / 2;
+    extern __shared__ __attribute__((aligned(16))) _Float16 Xs[];   // [2 buf][2 hl][P16][LDX]
+    __shared__ __attribute__((aligned(16))) float wsl[3][CH];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int sgi, j;
+    {
+        const int id = blockIdx.x;
+        if (nseg % 8 == 0) {
+            const int xcd = id % 8, loc = id / 8, nsx = nseg / 8;
+            sgi = xcd + 8 * (loc % nsx);
+            j = loc / nsx;
+        } else {
+            sgi = id % nseg;
+            j = id / nseg;
+        }
+    }
+    GramArgs g;
+    g.Npad = Npad;
+    g.c0 = sgi * seg;
+    g.nch = (min(Npad, g.c0 + seg) - g.c0) / CH;
+    g.sc = wsc[j];
+    g.z32 = z32;
+    g.wj = w32 + (size_t)j * Npad;
+    g.pg = part + ((size_t)sgi * J + j) * NT * 256;
+    switch (w) {
+    case 0: gram_wave4<T16, 0>(g, Xs, wsl); break;
+    case 1: gram_wave4<T16, 1>(g, Xs, wsl); break;
+    case 2: gram_wave4<T16, 2>(g, Xs, wsl); break;
+    default: gram_wave4<T16, 3>(g, Xs, wsl); break;
     }
 }
 
@@ -605,11 +842,19 @@ static int launch_gram_sym(gp_ctx *c, int seg, int nseg, int batch) {
     constexpr int P16 = 16 * T16;
     static_assert(plan_fits(T16), "block plan does not fit the waves");
     const size_t shm = (size_t)2 * 2 * P16 * LDX * sizeof(_Float16);
-    if (shm > 65536 && !c->capturing)   // set before any recording: not a stream operation
-        GP_HIP(hipFuncSetAttribute((const void *)k_gram_sym<T16>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    k_gram_sym<T16><<<(unsigned)(batch * nseg), GW * 64, shm, c->stream>>>(
-        c->Npad, seg, nseg, batch, c->z32, c->w32, c->wsc, c->gram_part);
+    if (c->gram_waves == 4) {
+        if (shm > 65536 && !c->capturing)   // set before any recording: not a stream operation
+            GP_HIP(hipFuncSetAttribute((const void *)k_gram_sym4<T16>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        k_gram_sym4<T16><<<(unsigned)(batch * nseg), GW4 * 64, shm, c->stream>>>(
+            c->Npad, seg, nseg, batch, c->z32, c->w32, c->wsc, c->gram_part);
+    } else {
+        if (shm > 65536 && !c->capturing)
+            GP_HIP(hipFuncSetAttribute((const void *)k_gram_sym<T16>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        k_gram_sym<T16><<<(unsigned)(batch * nseg), GW * 64, shm, c->stream>>>(
+            c->Npad, seg, nseg, batch, c->z32, c->w32, c->wsc, c->gram_part);
+    }
     GP_HIP(hipGetLastError());
     return 0;
 }

@@ -100,6 +100,7 @@ struct gp_ctx {
     float *gram_part = nullptr;   // per-segment FP32 partial Gram blocks
     size_t gram_part_bytes = 0;
     int gram_splits = 16;      // cell segments of the mixed Gram
+    int gram_waves = 8;        // waves per workgroup of the symmetric mixed Gram (8 or 4)
     int zsplit_valid = 0;
     // mixed gradient contraction: per-gene FP16 hi/lo of Sw_j, gene groups
     void *Sh = nullptr, *Sl = nullptr;  // J x P16 x 288 halves, [col][k]
@@ -121,7 +122,7 @@ struct gp_ctx {
     // recorded ELBO + gradient unit
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    int graph_key[4] = {-1, -1, -1, -1};   // precision, slot, gram_splits, gc_groups
+    int graph_key[5] = {-1, -1, -1, -1, -1};   // precision, slot, gram_splits, gc_groups, gram_waves
     // objective and elapsed seconds after every accepted step of the last gp_fit
     std::vector<double> fit_obj, fit_sec;
     std::vector<int> fit_kind;           // 0 IRLS step, 1 quasi-likelihood start, 2 L-BFGS step

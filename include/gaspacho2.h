@@ -74,6 +74,10 @@ int gp_set_precision(gp_ctx *ctx, int precision);
  * are fixed in ggplvm.cu; these are the knobs the autotuner sweeps):
  *   "gram_splits" [1, 256]  cell segments of the mixed Gram (FP32 partial
  *                           per segment, FP64 sum across segments)
+ *   "gram_waves"  4 or 8    waves per workgroup of the mixed Gram: 8 (two
+ *                           blocks of tiles per wave, two waves per SIMD) or
+ *                           4 (a quarter of the tiles per wave, fragments
+ *                           shared across its blocks, one wave per SIMD)
  *   "gc_groups"   [1, 4096] gene groups of the mixed gradient contraction
  *                           (FP32 partial per group, FP64 sum across groups)
  *   "gc_mixed"    0/1       mixed path: contraction on FP16 hi/lo MFMA (1)
@@ -88,7 +92,7 @@ int gp_set_option(gp_ctx *ctx, const char *name, int value);
 int gp_get_option(gp_ctx *ctx, const char *name, int *value);
 
 /* Autotuner (no counterpart in the reference): times the mixed-precision
- * Gram for gram_splits in {8, 16, 32} and the mixed contraction for
+ * Gram for gram_waves in {8, 4} x gram_splits in {8, 16, 32} and the mixed contraction for
  * gc_groups in {16, 32, 64} on the context's current state and keeps the
  * fastest of each.  Needs GP_PREC_MIXED and a state after gp_ql_start; it
  * recomputes the per-gene factors and inverses at the current parameters. */

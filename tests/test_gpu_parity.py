@@ -326,6 +326,35 @@ def test_options_reject_bad_values(tiny_ds):
         g.set_option("no_such_option", 1)
     g.set_option("gc_groups", 7)
     assert g.get_option("gc_groups") == 7
+    with pytest.raises(lib.GPError):
+        g.set_option("gram_waves", 6)
+    assert g.get_option("gram_waves") == 8
+
+
+@pytest.mark.parametrize("shape", [(2000, 40, 5, 32), (3000, 24, 10, 256), (3000, 24, 10, 240)])
+def test_gram_four_wave_form_matches_eight_wave_form(shape):
+    """The four-wave mixed Gram (fragments shared across a wave's blocks) sums
+    every tile in the same order as the eight-wave form, so the factors and
+    the ELBO agree to the last bit; P = 33, 257 and 241 (last column group
+    narrower than 4 tiles)."""
+    from gaspacho2_amd import lib, synth
+    N, G, Q, M = shape
+    ds = synth.make_dataset(N, G, Q, M, seed=4)
+    T0 = lib.init_ta_with_per(ds.M, ds.Q)
+    g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0, T0,
+                      precision=lib.PREC_FP64)
+    g.init_fit()
+    g.irls_step(0)
+    g.ql_start()
+    g.set_precision(lib.PREC_MIXED)
+    out = {}
+    for w in (8, 4):
+        g.set_option("gram_waves", w)
+        e = g.elbo(1)
+        out[w] = (e, g.get("Psi").copy())
+    assert out[4][0] == out[8][0]
+    assert np.array_equal(out[4][1], out[8][1])
+    g.close()
 
 
 def test_autotune_picks_valid_settings_and_keeps_results():
