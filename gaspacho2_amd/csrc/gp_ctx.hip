@@ -235,6 +235,9 @@ extern "C" int gp_set_option(gp_ctx *c, const char *name, int value) {
     } else if (n == "gram_waves") {
         if (value != 4 && value != 8) { gp_set_error("gram_waves must be 4 or 8"); return 1; }
         c->gram_waves = value;
+    } else if (n == "gc_form") {
+        if (value != 0 && value != 1) { gp_set_error("gc_form must be 0 or 1"); return 1; }
+        c->gc_form = value;
     } else if (n == "gc_mixed") {
         c->gc_mixed = value ? 1 : 0;
     } else if (n == "irls_mixed") {
@@ -255,6 +258,7 @@ extern "C" int gp_get_option(gp_ctx *c, const char *name, int *value) {
     else if (n == "gc_groups") *value = c->gc_groups;
     else if (n == "gc_mixed") *value = c->gc_mixed;
     else if (n == "gram_waves") *value = c->gram_waves;
+    else if (n == "gc_form") *value = c->gc_form;
     else if (n == "irls_mixed") *value = c->irls_mixed;
     else if (n == "autotune") *value = c->autotune;
     else { gp_set_error("unknown option " + n); return 1; }
@@ -579,8 +583,8 @@ static void drop_graph(gp_ctx *c) {
 extern "C" int gp_eval_graph(gp_ctx *c, int slot, double *obj) {
     if (!c) { gp_set_error("NULL context"); return 1; }
     if (slot < 0 || slot >= GP_LKHD_SLOTS) { gp_set_error("slot out of range"); return 1; }
-    const int key[5] = {c->precision, slot, c->gram_splits, c->gc_groups, c->gram_waves};
-    if (!c->graph_exec || !std::equal(key, key + 5, c->graph_key)) {
+    const int key[6] = {c->precision, slot, c->gram_splits, c->gc_groups, c->gram_waves, c->gc_form};
+    if (!c->graph_exec || !std::equal(key, key + 6, c->graph_key)) {
         drop_graph(c);
         double tmp = 0.0;
         int r = gp_elbo(c, slot, &tmp);
@@ -600,7 +604,7 @@ extern "C" int gp_eval_graph(gp_ctx *c, int slot, double *obj) {
         }
         c->graph = g;
         GP_HIP(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
-        std::copy(key, key + 5, c->graph_key);
+        std::copy(key, key + 6, c->graph_key);
     }
     GP_HIP(hipGraphLaunch(c->graph_exec, c->stream));
     if (obj) GP_HIP(hipMemcpyAsync(obj, c->lkhd + slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -742,6 +746,7 @@ extern "C" int gp_autotune(gp_ctx *c, int verbose) {
     static const int splits[] = {8, 16, 32};
     static const int groups[] = {16, 32, 64};
     static const int waves[] = {8, 4};
+    static const int forms[] = {1, 0};
     GP_TRY(gpk_expand_delta(c));
     if (!c->zw_valid) {
         GP_TRY(gpk_whiten(c));
@@ -768,15 +773,18 @@ extern "C" int gp_autotune(gp_ctx *c, int verbose) {
     c->psi_whitened = 1;
     GP_TRY(gpk_chol_inverse_batched(c, c->Psi, c->P, c->J, (size_t)c->P * c->P, c->Sinv));
     best = 1e300;
-    int bg = c->gc_groups;
-    for (int gv : groups) {
-        c->gc_groups = gv;
-        double ms = 0.0;
-        GP_TRY(time_ms(c, tune_gc, 2, &ms));
-        if (verbose) std::printf("autotune: gc_groups %d: %.3f ms\n", gv, ms);
-        if (ms < best) { best = ms; bg = gv; }
-    }
+    int bg = c->gc_groups, bf = c->gc_form;
+    for (int fv : forms)
+        for (int gv : groups) {
+            c->gc_groups = gv;
+            c->gc_form = fv;
+            double ms = 0.0;
+            GP_TRY(time_ms(c, tune_gc, 2, &ms));
+            if (verbose) std::printf("autotune: gc_form %d gc_groups %d: %.3f ms\n", fv, gv, ms);
+            if (ms < best) { best = ms; bg = gv; bf = fv; }
+        }
     c->gc_groups = bg;
+    c->gc_form = bf;
     GP_HIP(hipStreamSynchronize(c->stream));
     int bad = 0;
     GP_TRY(read_info(c, &bad));

@@ -239,6 +239,149 @@ __global__ void __launch_bounds__(GCW * 64) k_gc_mixed(int N, int Npad, int P, i
         for (int r = 0; r < 4; r++) pg[(size_t)(16 * ct + (lane & 15)) * Npad + row0 + r] = acc[ct][r];
 }
 
+// ---- row-block form: a workgroup owns 128 cells (8 row tiles) of one gene
+// group.  Wave w owns the column tiles 2w and 2w+1; with an odd tile count
+// the last column tile is shared out by row tile (wave w takes row tiles
+// r = w mod NW), so every wave carries the same load.  Each wave keeps its
+// slice of S_j (hi/lo, streamed one 32-k step ahead from L2) in registers,
+// while A = o z (hi/lo) for the 128 cells is staged once per gene in LDS as
+// [cell][k].  Per 32-k step a wave reads 2 A fragments per row tile for 6
+// MFMAs (0.33 LDS fragments per MFMA, half of the 16-cell form).  z (scaled
+// by 2^-zexp) stays in registers for the whole gene group.  Each accumulator
+// sees the same sequence of MFMAs as in the 16-cell form, so both forms give
+// the same bits.  (Refilling each 32-k slot for the next gene as soon as it
+// is free, to overlap the staging with the MFMAs, measured slower: it needs a
+// barrier per step.)
+constexpr int GRC = 128;     // cells per workgroup
+
+constexpr int gc_rows_waves(int T16) { return T16 / 2 > 0 ? T16 / 2 : 1; }
+
+template <int T16>
+__global__ void __launch_bounds__(gc_rows_waves(T16) * 64) k_gc_rows(int N, int Npad, int P, int gsize, int J,
+        const double *__restrict__ Zw, const float *__restrict__ w32,
+        const int *__restrict__ perm, const int *__restrict__ aexp, const int *__restrict__ zexp,
+        const _Float16 *__restrict__ shi, const _Float16 *__restrict__ slo,
+        float *__restrict__ part) {
+    constexpr int P16 = 16 * T16;
+    constexpr int NPAIR = T16 / 2, ODD = T16 % 2;
+    constexpr int NW = gc_rows_waves(T16);
+    constexpr int KS = (P16 + 31) / 32;
+    constexpr int LDA = 32 * KS + 8;             // halves per cell row (stride 16 B multiple)
+    constexpr int NITEM = GRC * KS * 4;          // (cell, 8-k group) items
+    constexpr int IPT = (NITEM + NW * 64 - 1) / (NW * 64);
+    constexpr int NX = (8 + NW - 1) / NW;        // row tiles of the odd column per wave
+    extern __shared__ __attribute__((aligned(16))) _Float16 As[];   // [2 hl][GRC][LDA]
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g = blockIdx.y;
+    const int j0 = g * gsize, j1 = min(J, j0 + gsize);
+    const int cb = blockIdx.x * GRC;
+    const int ct0 = 2 * w;
+
+    // z of this thread's items (cell = it % GRC, k = 8 (it / GRC) + e), constant over the group
+    float zr[IPT][8];
+#pragma unroll
+    for (int r = 0; r < IPT; r++) {
+        const int it = min(tid + NW * 64 * r, NITEM - 1);
+        const int cl = it % GRC, k0 = 8 * (it / GRC);
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int k = k0 + e, cell = cb + cl;
+            zr[r][e] = (k < P && cell < N) ? (float)ldexp(Zw[cell + (size_t)k * Npad], -zexp[k]) : 0.0f;
+        }
+    }
+    f4 acc[8][2], accx[NX];
+#pragma unroll
+    for (int r = 0; r < 8; r++) { acc[r][0] = f4{0.f, 0.f, 0.f, 0.f}; acc[r][1] = f4{0.f, 0.f, 0.f, 0.f}; }
+#pragma unroll
+    for (int x = 0; x < NX; x++) accx[x] = f4{0.f, 0.f, 0.f, 0.f};
+
+    const int lk = (lane & 15), lq = 8 * (lane >> 4);
+    const int bcol0 = (16 * min(ct0, T16 - 1) + lk) * KP + lq;
+    const int bcol1 = (16 * min(ct0 + 1, T16 - 1) + lk) * KP + lq;
+    const int bcolx = (16 * (T16 - 1) + lk) * KP + lq;
+    const int arow = lk * LDA + lq;
+    _Float16 *Ah = As, *Al = As + (size_t)GRC * LDA;
+
+    for (int jj = j0; jj < j1; jj++) {
+        const int j = perm[jj];
+        const float sc = ldexpf(1.0f, aexp[j]);
+        const float *wj = w32 + (size_t)j * Npad + cb;
+        // stage A = o z, split hi/lo (items past the end repeat the last one)
+#pragma unroll
+        for (int r = 0; r < IPT; r++) {
+            const int it = min(tid + NW * 64 * r, NITEM - 1);
+            const int cl = it % GRC, k0 = 8 * (it / GRC);
+            const float o = (cb + cl < N) ? wj[cl] * sc : 0.0f;
+            h8 hi, lo;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                float v = zr[r][e] * o;
+                asm volatile("" : "+v"(v));
+                const _Float16 h = (_Float16)v;
+                hi[e] = h;
+                lo[e] = (_Float16)(v - (float)h);
+            }
+            *(h8 *)&Ah[cl * LDA + k0] = hi;
+            *(h8 *)&Al[cl * LDA + k0] = lo;
+        }
+        __syncthreads();
+        const _Float16 *hj = shi + (size_t)j * P16 * KP, *lj = slo + (size_t)j * P16 * KP;
+        h8 bh0 = *(const h8 *)(hj + bcol0), bl0 = *(const h8 *)(lj + bcol0);
+        h8 bh1 = *(const h8 *)(hj + bcol1), bl1 = *(const h8 *)(lj + bcol1);
+        h8 bhx = *(const h8 *)(hj + bcolx), blx = *(const h8 *)(lj + bcolx);
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const int kn = 32 * min(ks + 1, KS - 1);
+            const h8 nh0 = *(const h8 *)(hj + bcol0 + kn), nl0 = *(const h8 *)(lj + bcol0 + kn);
+            const h8 nh1 = *(const h8 *)(hj + bcol1 + kn), nl1 = *(const h8 *)(lj + bcol1 + kn);
+            const h8 nhx = *(const h8 *)(hj + bcolx + kn), nlx = *(const h8 *)(lj + bcolx + kn);
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const h8 ah = *(const h8 *)(Ah + 16 * r * LDA + arow + 32 * ks);
+                const h8 al = *(const h8 *)(Al + 16 * r * LDA + arow + 32 * ks);
+                if constexpr (NPAIR > 0) {
+                    acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh0, acc[r][0], 0, 0, 0);
+                    acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl0, acc[r][0], 0, 0, 0);
+                    acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh0, acc[r][0], 0, 0, 0);
+                    acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh1, acc[r][1], 0, 0, 0);
+                    acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl1, acc[r][1], 0, 0, 0);
+                    acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh1, acc[r][1], 0, 0, 0);
+                }
+                if constexpr (ODD) {
+                    if (r % NW == w) {       // wave-uniform
+                        f4 &ax = accx[r / NW];
+                        ax = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bhx, ax, 0, 0, 0);
+                        ax = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, blx, ax, 0, 0, 0);
+                        ax = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bhx, ax, 0, 0, 0);
+                    }
+                }
+            }
+            bh0 = nh0; bl0 = nl0; bh1 = nh1; bl1 = nl1; bhx = nhx; blx = nlx;
+        }
+        __syncthreads();
+    }
+    // partial of this group: part[g][col][cell]
+    float *pg = part + (size_t)g * P16 * Npad;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const int row0 = cb + 16 * r + 4 * (lane >> 4);
+        if constexpr (NPAIR > 0) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                pg[(size_t)(16 * ct0 + lk) * Npad + row0 + q] = acc[r][0][q];
+                pg[(size_t)(16 * (ct0 + 1) + lk) * Npad + row0 + q] = acc[r][1][q];
+            }
+        }
+        if constexpr (ODD) {
+            if (r % NW == w) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) pg[(size_t)(16 * (T16 - 1) + lk) * Npad + row0 + q] = accx[r / NW][q];
+            }
+        }
+    }
+}
+
 __global__ void k_gc_reduce(int Npad, int P, int P16, int ngroup, const float *__restrict__ part,
                             const double *__restrict__ gun, double *__restrict__ Uw) {
     const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -249,6 +392,30 @@ __global__ void k_gc_reduce(int Npad, int P, int P16, int ngroup, const float *_
 }
 
 }  // namespace
+
+template <int T16>
+static int launch_gc_mixed(gp_ctx *c, int gsize, int ngroup);
+
+template <int T16>
+static int launch_gc_rows(gp_ctx *c, int gsize, int ngroup) {
+    constexpr int P16 = 16 * T16, KS = (P16 + 31) / 32, NW = gc_rows_waves(T16);
+    const size_t shm = (size_t)2 * GRC * (32 * KS + 8) * sizeof(_Float16);
+    if (shm > 65536 && !c->capturing)   // set before any recording: not a stream operation
+        GP_HIP(hipFuncSetAttribute((const void *)k_gc_rows<T16>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    dim3 g(c->Npad / GRC, ngroup);
+    k_gc_rows<T16><<<g, NW * 64, shm, c->stream>>>(c->N, c->Npad, c->P, gsize, c->J, c->Zw, c->w32,
+                                                c->gc_perm, c->gc_aexp, c->gc_zexp,
+                                                (const _Float16 *)c->Sh, (const _Float16 *)c->Sl,
+                                                c->gc_part);
+    GP_HIP(hipGetLastError());
+    return 0;
+}
+
+template <int T16>
+static int launch_gc(gp_ctx *c, int gsize, int ngroup) {
+    return c->gc_form == 1 ? launch_gc_rows<T16>(c, gsize, ngroup) : launch_gc_mixed<T16>(c, gsize, ngroup);
+}
 
 template <int T16>
 static int launch_gc_mixed(gp_ctx *c, int gsize, int ngroup) {
@@ -305,23 +472,23 @@ int gpk_gradcontract_mixed(gp_ctx *c) {
     GP_HIP(hipGetLastError());
     int r = 0;
     switch (T16) {
-    case 1: r = launch_gc_mixed<1>(c, gsize, ngroup); break;
-    case 2: r = launch_gc_mixed<2>(c, gsize, ngroup); break;
-    case 3: r = launch_gc_mixed<3>(c, gsize, ngroup); break;
-    case 4: r = launch_gc_mixed<4>(c, gsize, ngroup); break;
-    case 5: r = launch_gc_mixed<5>(c, gsize, ngroup); break;
-    case 6: r = launch_gc_mixed<6>(c, gsize, ngroup); break;
-    case 7: r = launch_gc_mixed<7>(c, gsize, ngroup); break;
-    case 8: r = launch_gc_mixed<8>(c, gsize, ngroup); break;
-    case 9: r = launch_gc_mixed<9>(c, gsize, ngroup); break;
-    case 10: r = launch_gc_mixed<10>(c, gsize, ngroup); break;
-    case 11: r = launch_gc_mixed<11>(c, gsize, ngroup); break;
-    case 12: r = launch_gc_mixed<12>(c, gsize, ngroup); break;
-    case 13: r = launch_gc_mixed<13>(c, gsize, ngroup); break;
-    case 14: r = launch_gc_mixed<14>(c, gsize, ngroup); break;
-    case 15: r = launch_gc_mixed<15>(c, gsize, ngroup); break;
-    case 16: r = launch_gc_mixed<16>(c, gsize, ngroup); break;
-    default: r = launch_gc_mixed<17>(c, gsize, ngroup); break;
+    case 1: r = launch_gc<1>(c, gsize, ngroup); break;
+    case 2: r = launch_gc<2>(c, gsize, ngroup); break;
+    case 3: r = launch_gc<3>(c, gsize, ngroup); break;
+    case 4: r = launch_gc<4>(c, gsize, ngroup); break;
+    case 5: r = launch_gc<5>(c, gsize, ngroup); break;
+    case 6: r = launch_gc<6>(c, gsize, ngroup); break;
+    case 7: r = launch_gc<7>(c, gsize, ngroup); break;
+    case 8: r = launch_gc<8>(c, gsize, ngroup); break;
+    case 9: r = launch_gc<9>(c, gsize, ngroup); break;
+    case 10: r = launch_gc<10>(c, gsize, ngroup); break;
+    case 11: r = launch_gc<11>(c, gsize, ngroup); break;
+    case 12: r = launch_gc<12>(c, gsize, ngroup); break;
+    case 13: r = launch_gc<13>(c, gsize, ngroup); break;
+    case 14: r = launch_gc<14>(c, gsize, ngroup); break;
+    case 15: r = launch_gc<15>(c, gsize, ngroup); break;
+    case 16: r = launch_gc<16>(c, gsize, ngroup); break;
+    default: r = launch_gc<17>(c, gsize, ngroup); break;
     }
     if (r) return r;
     const size_t n = (size_t)c->Npad * c->P;

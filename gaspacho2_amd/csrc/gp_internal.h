@@ -113,6 +113,7 @@ struct gp_ctx {
     size_t gc_part_len = 0;
     int gc_groups = 32;        // gene groups of the mixed contraction
     int gc_mixed = 1;          // mixed path: contraction on FP16 MFMA (0: FP64)
+    int gc_form = 1;           // mixed contraction: 0 = 16-cell waves, 1 = 128-cell row blocks
     int irls_mixed = 1;        // mixed path: IRLS steps on the whitened FP16 hi/lo Gram (0: FP64)
     int autotune = 0;          // gp_fit runs gp_autotune before its first quasi-likelihood phase
 
@@ -122,7 +123,7 @@ struct gp_ctx {
     // recorded ELBO + gradient unit
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    int graph_key[5] = {-1, -1, -1, -1, -1};   // precision, slot, gram_splits, gc_groups, gram_waves
+    int graph_key[6] = {-1, -1, -1, -1, -1, -1};   // precision, slot, gram_splits, gc_groups, gram_waves, gc_form
     // objective and elapsed seconds after every accepted step of the last gp_fit
     std::vector<double> fit_obj, fit_sec;
     std::vector<int> fit_kind;           // 0 IRLS step, 1 quasi-likelihood start, 2 L-BFGS step

@@ -192,9 +192,10 @@ class GPContext:
         return v.value
 
     def autotune(self, verbose: bool = False) -> dict:
-        """Pick gram_splits / gc_groups by timing them on the current state."""
+        """Pick the mixed Gram form and segments and the contraction form and
+        gene groups by timing them on the current state."""
         self._check(self._lib.gp_autotune(self._h, int(bool(verbose))))
-        return {k: self.get_option(k) for k in ("gram_splits", "gc_groups")}
+        return {k: self.get_option(k) for k in ("gram_waves", "gram_splits", "gc_form", "gc_groups")}
 
     # -- launch groups
     def init_fit(self) -> None:

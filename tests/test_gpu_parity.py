@@ -357,6 +357,34 @@ def test_gram_four_wave_form_matches_eight_wave_form(shape):
     g.close()
 
 
+@pytest.mark.parametrize("shape", [(2000, 24, 5, 32), (3000, 24, 10, 256), (3000, 16, 10, 240),
+                                   (3000, 16, 10, 224), (700, 8, 3, 8)])
+def test_contraction_row_block_form_matches_16_cell_form(shape):
+    """The 128-cell row-block contraction (gc_form 1) feeds every accumulator
+    the same MFMA sequence as the 16-cell form (gc_form 0): the gradient
+    agrees to the last bit; P = 33, 257, 241, 225 and 9 (odd and even
+    column-tile counts, N not a multiple of 128)."""
+    from gaspacho2_amd import lib, synth
+    N, G, Q, M = shape
+    ds = synth.make_dataset(N, G, Q, M, seed=6)
+    T0 = lib.init_ta_with_per(ds.M, ds.Q)
+    g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0, T0,
+                      precision=lib.PREC_FP64)
+    g.init_fit()
+    g.irls_step(0)
+    g.ql_start()
+    g.set_precision(lib.PREC_MIXED)
+    out = {}
+    for form in (0, 1):
+        g.set_option("gc_form", form)
+        g.elbo(1)
+        g.grad()
+        out[form] = (g.get("grad").copy(), g.get("C1").copy())
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    g.close()
+
+
 def test_autotune_picks_valid_settings_and_keeps_results():
     """gp_autotune picks gram_splits / gc_groups among its candidates and the
     mixed ELBO and gradient after tuning stay within the mixed-path error of
@@ -380,6 +408,7 @@ def test_autotune_picks_valid_settings_and_keeps_results():
     g.set_precision(lib.PREC_MIXED)
     choice = g.autotune()
     assert choice["gram_splits"] in (8, 16, 32) and choice["gc_groups"] in (16, 32, 64)
+    assert choice["gram_waves"] in (4, 8) and choice["gc_form"] in (0, 1)
     em = g.elbo(1)
     g.grad()
     assert abs(em - e64) <= 1e-7 * abs(e64)
