@@ -799,7 +799,8 @@ extern "C" int gp_timing_enable(gp_ctx *c, int on) {
 }
 extern "C" int gp_timing_reset(gp_ctx *c) {
     if (!c) { gp_set_error("NULL context"); return 1; }
-    gp_timer *ts[] = {&c->t_gram, &c->t_chol, &c->t_gc, &c->t_kuf, &c->t_elbo, &c->t_grad};
+    gp_timer *ts[] = {&c->t_gram, &c->t_chol, &c->t_gc, &c->t_kuf, &c->t_elbo, &c->t_grad,
+                      &c->t_gramk, &c->t_gck};
     for (gp_timer *t : ts) { t->total_ms = 0.0; t->count = 0; }
     return 0;
 }
@@ -812,6 +813,8 @@ extern "C" int gp_timing_get(gp_ctx *c, const char *name, double *avg_ms, int *c
     else if (!std::strcmp(name, "kuf")) t = &c->t_kuf;
     else if (!std::strcmp(name, "elbo")) t = &c->t_elbo;
     else if (!std::strcmp(name, "grad")) t = &c->t_grad;
+    else if (!std::strcmp(name, "gram_kernel")) t = &c->t_gramk;
+    else if (!std::strcmp(name, "gradcontract_kernel")) t = &c->t_gck;
     if (!t) { gp_set_error(std::string("unknown timer ") + name); return 1; }
     if (avg_ms) *avg_ms = t->count ? t->total_ms / t->count : 0.0;
     if (count) *count = t->count;

@@ -185,9 +185,11 @@ static int launch_gc(gp_ctx *c, const double *Zs, double *Uo) {
     if (shm > 65536 && !c->capturing)   // set before any recording: not a stream operation
         GP_HIP(hipFuncSetAttribute((const void *)k_gc_mfma_f64<T16>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    gp_timer_begin(c, c->t_gck);
     k_gc_mfma_f64<T16><<<c->Npad / 64, 256, shm, c->stream>>>(c->N, c->Npad, c->J, c->P, Zs,
                                                                c->W, c->Omega, c->Sinv, Uo);
     GP_HIP(hipGetLastError());
+    gp_timer_end(c, c->t_gck);
     return 0;
 }
 

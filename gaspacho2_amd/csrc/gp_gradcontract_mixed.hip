@@ -471,6 +471,7 @@ int gpk_gradcontract_mixed(gp_ctx *c) {
                                         (_Float16 *)c->Sh, (_Float16 *)c->Sl);
     GP_HIP(hipGetLastError());
     int r = 0;
+    gp_timer_begin(c, c->t_gck);
     switch (T16) {
     case 1: r = launch_gc<1>(c, gsize, ngroup); break;
     case 2: r = launch_gc<2>(c, gsize, ngroup); break;
@@ -491,6 +492,7 @@ int gpk_gradcontract_mixed(gp_ctx *c) {
     default: r = launch_gc<17>(c, gsize, ngroup); break;
     }
     if (r) return r;
+    gp_timer_end(c, c->t_gck);
     const size_t n = (size_t)c->Npad * c->P;
     k_gc_reduce<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(c->Npad, c->P, P16, ngroup,
                                                                    c->gc_part, c->gc_gun, c->Uw);

@@ -212,9 +212,11 @@ static int launch_gram_mfma(gp_ctx *c, const double *W, double *Psi, int batch, 
         GP_HIP(hipFuncSetAttribute((const void *)k_gram_mfma_f64<PF>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     dim3 g(ng, batch);
+    gp_timer_begin(c, c->t_gramk);
     k_gram_mfma_f64<PF><<<g, 256, shm, c->stream>>>(c->N, c->Npad, c->P, P16, T16, ntot, Zs, W,
                                                      c->Omega, Dl, Psi);
     GP_HIP(hipGetLastError());
+    gp_timer_end(c, c->t_gramk);
     return 0;
 }
 

@@ -875,6 +875,7 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
     GP_HIP(hipGetLastError());
     if (T16 <= MAX_SYM_T16) {
         int r = 0;
+        gp_timer_begin(c, c->t_gramk);
         switch (T16) {
         case 1: r = launch_gram_sym<1>(c, seg, nseg, batch); break;
         case 2: r = launch_gram_sym<2>(c, seg, nseg, batch); break;
@@ -895,6 +896,7 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
         default: r = launch_gram_sym<17>(c, seg, nseg, batch); break;
         }
         if (r) return r;
+        gp_timer_end(c, c->t_gramk);
         dim3 gr((unsigned)(T16 * (T16 + 1) / 2), batch);
         k_gram_reduce_sym<<<gr, 256, 0, c->stream>>>(c->P, T16, nseg, batch, c->gram_part, c->wun,
                                                       c->zun, Delta, Psi);

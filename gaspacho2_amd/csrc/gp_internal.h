@@ -99,7 +99,7 @@ struct gp_ctx {
     double *zun = nullptr;     // P16  2^-s_a column unscale of the FP16 model matrix
     float *gram_part = nullptr;   // per-segment FP32 partial Gram blocks
     size_t gram_part_bytes = 0;
-    int gram_splits = 16;      // cell segments of the mixed Gram
+    int gram_splits = 8;       // cell segments of the mixed Gram
     int gram_waves = 8;        // waves per workgroup of the symmetric mixed Gram (8 or 4)
     int zsplit_valid = 0;
     // mixed gradient contraction: per-gene FP16 hi/lo of Sw_j, gene groups
@@ -111,7 +111,7 @@ struct gp_ctx {
     double *gc_gun = nullptr;  // per group 2^-C_g
     float *gc_part = nullptr;  // per group FP32 partial U (P16 x Npad)
     size_t gc_part_len = 0;
-    int gc_groups = 32;        // gene groups of the mixed contraction
+    int gc_groups = 16;        // gene groups of the mixed contraction
     int gc_mixed = 1;          // mixed path: contraction on FP16 MFMA (0: FP64)
     int gc_form = 1;           // mixed contraction: 0 = 16-cell waves, 1 = 128-cell row blocks
     int irls_mixed = 1;        // mixed path: IRLS steps on the whitened FP16 hi/lo Gram (0: FP64)
@@ -120,6 +120,7 @@ struct gp_ctx {
     size_t tot_max = 0;        // N+p+(N+M+1)Q
     int timing = 0;
     gp_timer t_gram, t_chol, t_gc, t_kuf, t_elbo, t_grad;
+    gp_timer t_gramk, t_gck;   // the main Gram / contraction kernel alone
     // recorded ELBO + gradient unit
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;

@@ -260,10 +260,12 @@ int gp_fit_trace(gp_ctx *ctx, double *obj, double *seconds, int *kind, int cap, 
  * Host-only; needs no device. */
 int gp_init_ta_with_per(int M, int Q, double *T);
 
-/* Average device time (ms) of the last launch of a named kernel group and
- * its launch count since gp_timing_reset: "gram", "chol", "gradcontract",
- * "kuf", "elbo", "grad".  Replaces the cudaEvent printf timers of
- * ggplvm.cu:1155-1276. */
+/* Average device time (ms) per launch of a named kernel group since
+ * gp_timing_reset, and the launch count: "gram", "chol", "gradcontract",
+ * "kuf", "elbo", "grad", and "gram_kernel" / "gradcontract_kernel" (the main
+ * Gram / contraction kernel alone, without the preparation and reduction
+ * kernels of its group).  Each timed group synchronizes the stream.
+ * Replaces the cudaEvent printf timers of ggplvm.cu:1155-1276. */
 int gp_timing_enable(gp_ctx *ctx, int on);
 int gp_timing_reset(gp_ctx *ctx);
 int gp_timing_get(gp_ctx *ctx, const char *name, double *avg_ms, int *count);
