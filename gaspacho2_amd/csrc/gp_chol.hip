@@ -172,6 +172,106 @@ __global__ void __launch_bounds__(CT) k_trinv(const double *__restrict__ Lall, i
     }
 }
 
+// Blocked V_j = L_j^{-1} (lower triangle; the strict upper triangle is not
+// written), 32 x 32 blocks.  From L V = I, block row I > J of block column J:
+//   L_II V_IJ = -sum_{K=J}^{I-1} L_IK V_KJ   =>   V_IJ = -V_II T_IJ,
+// so with the diagonal inverses V_II known, every block column is
+// independent: one workgroup per (block column, gene) walks I downwards
+// keeping the column's finished blocks in LDS.  L is read once per block
+// product (from L2), V is written once.
+constexpr int TB = 32;
+
+// V_II = L_II^{-1} for every diagonal block: lane c solves column c by
+// forward substitution in LDS.
+__global__ void __launch_bounds__(64) k_trinv_diag(const double *__restrict__ Lall, int n, size_t stride,
+                                                   double *__restrict__ Vall) {
+    __shared__ double Ld[TB][TB + 1];
+    __shared__ double Vd[TB][TB + 1];
+    const double *L = Lall + (size_t)blockIdx.y * stride;
+    double *V = Vall + (size_t)blockIdx.y * stride;
+    const int r0 = TB * blockIdx.x, bs = min(TB, n - r0), tid = threadIdx.x;
+    for (int idx = tid; idx < TB * TB; idx += 64) {
+        const int r = idx % TB, c = idx / TB;
+        Ld[r][c] = (r < bs && c < bs && r >= c) ? L[(r0 + r) + (size_t)(r0 + c) * n] : 0.0;
+    }
+    __syncthreads();
+    if (tid < bs) {
+        const int c = tid;
+        for (int r = c; r < bs; r++) {
+            double sum = (r == c) ? 1.0 : 0.0;
+            for (int k = c; k < r; k++) sum -= Ld[r][k] * Vd[k][c];
+            Vd[r][c] = sum / Ld[r][r];
+        }
+    }
+    __syncthreads();
+    for (int idx = tid; idx < TB * TB; idx += 64) {
+        const int r = idx % TB, c = idx / TB;
+        if (r < bs && c < bs && r >= c) V[(r0 + r) + (size_t)(r0 + c) * n] = Vd[r][c];
+    }
+}
+
+// Off-diagonal blocks of block column J = blockIdx.x (all but the last block
+// column).  Thread t owns the 4 entries (t / 8, 4 (t % 8) + q) of a block.
+__global__ void __launch_bounds__(CT) k_trinv_panel(const double *__restrict__ Lall, int n, size_t stride,
+                                                    double *__restrict__ Vall) {
+    extern __shared__ double sm[];
+    double (*Lb)[TB + 1] = (double (*)[TB + 1])sm;                  // L_IK
+    double (*Vjj)[TB + 1] = (double (*)[TB + 1])(sm + TB * (TB + 1));   // V_JJ
+    double (*Tt)[TB + 1] = (double (*)[TB + 1])(sm + 2 * TB * (TB + 1)); // T, then V_II
+    double (*Vp)[TB + 1] = (double (*)[TB + 1])(sm + 3 * TB * (TB + 1)); // finished V_KJ, K > J
+    const double *L = Lall + (size_t)blockIdx.y * stride;
+    double *V = Vall + (size_t)blockIdx.y * stride;
+    const int NB = (n + TB - 1) / TB, J = blockIdx.x, c0 = TB * J, tid = threadIdx.x;
+    const int tr = tid / 8, tc = 4 * (tid % 8);
+    for (int idx = tid; idx < TB * TB; idx += CT) {
+        const int r = idx % TB, c = idx / TB;
+        Vjj[r][c] = (r >= c) ? V[(c0 + r) + (size_t)(c0 + c) * n] : 0.0;   // block J is full (J < NB - 1)
+    }
+    for (int I = J + 1; I < NB; I++) {
+        const int rI = TB * I, bsI = min(TB, n - rI);
+        double acc[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int K = J; K < I; K++) {
+            __syncthreads();                      // Lb free (and Vjj / Vp writes visible)
+            for (int idx = tid; idx < TB * TB; idx += CT) {
+                const int r = idx % TB, k = idx / TB;
+                Lb[r][k] = (r < bsI) ? L[(rI + r) + (size_t)(TB * K + k) * n] : 0.0;   // K < I: full width
+            }
+            __syncthreads();
+            const double (*Bk)[TB + 1] = (K == J) ? (const double (*)[TB + 1])Vjj
+                                                  : (const double (*)[TB + 1])&Vp[TB * (K - J - 1)];
+#pragma unroll 8
+            for (int k = 0; k < TB; k++) {
+                const double l = Lb[tr][k];
+#pragma unroll
+                for (int q = 0; q < 4; q++) acc[q] += l * Bk[k][tc + q];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; q++) Tt[tr][tc + q] = acc[q];
+        // V_II into Lb (lower triangle, zero elsewhere)
+        for (int idx = tid; idx < TB * TB; idx += CT) {
+            const int r = idx % TB, k = idx / TB;
+            Lb[r][k] = (r < bsI && k < bsI && r >= k) ? V[(rI + r) + (size_t)(rI + k) * n] : 0.0;
+        }
+        __syncthreads();
+        double v[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 8
+        for (int k = 0; k < TB; k++) {
+            const double l = Lb[tr][k];
+#pragma unroll
+            for (int q = 0; q < 4; q++) v[q] -= l * Tt[k][tc + q];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) Vp[TB * (I - J - 1) + tr][tc + q] = v[q];
+        __syncthreads();
+        for (int idx = tid; idx < TB * TB; idx += CT) {
+            const int r = idx % TB, c = idx / TB;
+            if (r < bsI) V[(rI + r) + (size_t)(c0 + c) * n] = Vp[TB * (I - J - 1) + r][c];
+        }
+    }
+}
+
 // S_j = V_j^T V_j (full symmetric) from the lower-triangular V_j: 64x64 tiles
 // of the upper triangle, 32-row chunks of V staged in LDS, 4x4 per thread.
 // Entry (a, b), a <= b, sums rows k >= b only (V[k, b] = 0 above the diagonal).
@@ -256,8 +356,22 @@ int gpk_chol_inverse_batched(gp_ctx *c, const double *L, int n, int batch, size_
         GP_HIP(hipMalloc((void **)&c->Vtmp, c->vtmp_len * sizeof(double)));
     }
     const size_t st = stride ? stride : (size_t)n * n;
-    k_trinv<<<batch, CT, (size_t)n * sizeof(double), c->stream>>>(L, n, st, c->Vtmp);
-    GP_HIP(hipGetLastError());
+    const int nb = (n + TB - 1) / TB;
+    const size_t shm = (size_t)(3 + (nb - 1)) * TB * (TB + 1) * sizeof(double);
+    if (shm > 160 * 1024) {   // the block column does not fit in LDS: column-by-column form
+        k_trinv<<<batch, CT, (size_t)n * sizeof(double), c->stream>>>(L, n, st, c->Vtmp);
+        GP_HIP(hipGetLastError());
+    } else {
+        k_trinv_diag<<<dim3(nb, batch), 64, 0, c->stream>>>(L, n, st, c->Vtmp);
+        GP_HIP(hipGetLastError());
+        if (nb > 1) {
+            if (shm > 65536 && !c->capturing)   // set before any recording: not a stream operation
+                GP_HIP(hipFuncSetAttribute((const void *)k_trinv_panel,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+            k_trinv_panel<<<dim3(nb - 1, batch), CT, shm, c->stream>>>(L, n, st, c->Vtmp);
+            GP_HIP(hipGetLastError());
+        }
+    }
     const int nt = (n + 63) / 64;
     dim3 g(nt * (nt + 1) / 2, batch);
     k_vtv<<<g, CT, 0, c->stream>>>(c->Vtmp, n, st, nt, S);
