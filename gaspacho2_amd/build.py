@@ -18,11 +18,12 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgaspacho2.so")
 DRIVER = os.path.join(HERE, "ggplvm")
+PCAINIT = os.path.join(HERE, "pcainit")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HIP_SOURCES = ["gp_ctx.hip", "gp_kernels.hip", "gp_chol.hip", "gp_gram.hip",
                "gp_gram_mixed.hip", "gp_gradcontract.hip", "gp_gradcontract_mixed.hip",
-               "gp_score.hip", "gp_batched.hip", "gp_encoder.hip"]
+               "gp_score.hip", "gp_batched.hip", "gp_encoder.hip", "gp_pca.hip"]
 # per-source extra flags: the Gram kernel keeps its MFMA accumulators in
 # VGPRs (no per-chunk AGPR<->VGPR copies, two waves per SIMD fit)
 EXTRA_FLAGS = {"gp_gram.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
@@ -64,7 +65,8 @@ def build_library(jobs: int = 8) -> str:
     objs.append(io_obj)
     tmp = LIB + ".tmp"
     _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs,
-          "-L" + os.path.join(ROCM, "lib"), "-lrocblas", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+          "-L" + os.path.join(ROCM, "lib"), "-lrocblas", "-lrocsolver",
+          "-Wl,-rpath," + os.path.join(ROCM, "lib")])
     os.replace(tmp, LIB)
     return LIB
 
@@ -79,6 +81,16 @@ def build_driver() -> str:
     return DRIVER
 
 
+def build_pcainit() -> str:
+    hipcc = _hipcc()
+    tmp = PCAINIT + ".tmp"
+    _run([hipcc, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"),
+          os.path.join(CSRC, "pcainit_main.cpp"),
+          "-o", tmp, "-L" + HERE, "-lgaspacho2", "-Wl,-rpath,$ORIGIN"])
+    os.replace(tmp, PCAINIT)
+    return PCAINIT
+
+
 def build_oracle() -> str:
     odir = os.path.join(ROOT, "oracle")
     _run(["make", "-s", "-C", odir, "-B", "liboracle.so"])
@@ -89,6 +101,8 @@ def build_all() -> None:
     build_library()
     if os.path.exists(os.path.join(CSRC, "ggplvm_main.cpp")):
         build_driver()
+    if os.path.exists(os.path.join(CSRC, "pcainit_main.cpp")):
+        build_pcainit()
     build_oracle()
 
 

@@ -60,6 +60,7 @@ def load() -> C.CDLL:
         "gp_bfit_get": (C.c_int, [vp, C.c_char_p, dp, C.c_size_t]),
         "gp_bfit_dims": (C.c_int, [vp, ip, ip]),
         "gp_score_tests_dosage": (C.c_int, [vp, C.c_int, C.POINTER(C.c_ubyte), C.c_int, ip, ip, dp]),
+        "gp_pca_init": (C.c_int, [C.c_int, C.c_int, ip, ip, dp, dp, C.c_int, dp, C.c_int, dp, dp]),
         "gp_enc_create": (C.c_int, [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_longlong, C.c_ulonglong]),
         "gp_enc_destroy": (C.c_int, [vp]),
@@ -112,6 +113,25 @@ def init_ta_with_per(M: int, Q: int) -> np.ndarray:
     if lib.gp_init_ta_with_per(M, Q, _dp(T)) != 0:
         raise GPError(lib.gp_last_error().decode())
     return T
+
+
+def pca_init(N: int, Yp, Yi, Yx, ls, Zcov, Q: int):
+    """PCA initial latent on the GPU (gp_pca_init, init.cu:304-357): returns
+    (X as N x Q, eigenvalues largest first).  Zcov: N x p0, intercept first."""
+    lib = load()
+    Yp = np.ascontiguousarray(Yp, dtype=np.int32)
+    Yi = np.ascontiguousarray(Yi, dtype=np.int32)
+    Yx = np.ascontiguousarray(Yx, dtype=np.float64)
+    ls = np.ascontiguousarray(ls, dtype=np.float64)
+    Z = np.asfortranarray(np.asarray(Zcov, dtype=np.float64).reshape(N, -1, order="F"))
+    zf = Z.ravel(order="F")
+    X = np.zeros(N * Q)
+    ev = np.zeros(Q)
+    st = lib.gp_pca_init(int(N), len(Yp) - 1, _ip(Yp), _ip(Yi), _dp(Yx), _dp(ls), Z.shape[1], _dp(zf), int(Q),
+                         _dp(X), _dp(ev))
+    if st != 0:
+        raise GPError(f"status {st}: {lib.gp_last_error().decode()}")
+    return X.reshape(Q, N).T.copy(), ev
 
 
 class GPContext:

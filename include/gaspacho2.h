@@ -197,6 +197,22 @@ int gp_bfit_run(gp_bfit *b, int maxit, double tol, int use_graph, int *iters, do
 int gp_bfit_get(gp_bfit *b, const char *name, double *host, size_t n);
 int gp_bfit_dims(gp_bfit *b, int *J, int *PB);
 
+/* ---- PCA initial latent ----
+ * The reference's fit starts from an X read from a file (ggplvm.cu:1088-1095)
+ * whose recipe is the helpers getNu0 / getNuTNu / getX (init.cu:304-357):
+ *   nu0[i, j] = log((y_ij + 0.5) / s_i) - z_i . beta_j,
+ *   top-Q eigenpairs (lambda_k, v_k) of nu0^T nu0, largest first,
+ *   X[:, k] = nu0 v_k / sqrt(lambda_k) * sqrt(N).
+ * z_i = Zcov row i (N x p0 column-major, intercept first); beta_j is the
+ * least-squares fit of log((y_j + 0.5) / s) on Zcov (columns dependent on
+ * earlier ones are dropped).  v_k is signed so that
+ * its entry of largest magnitude is positive.  Gene-major CSR counts (row
+ * indices increasing within a gene).  X: N x Q column-major (the Xinit.bin
+ * layout of the driver); eval (may be NULL): the Q eigenvalues.  Standalone:
+ * no context; runs on the current device. */
+int gp_pca_init(int N, int J, const int *Yp, const int *Yi, const double *Yx, const double *ls,
+                int p0, const double *Zcov, int Q, double *X, double *eval);
+
 /* ---- Amortization encoder (warm start of X) ----
  * No counterpart in the reference, which reads the starting X from an
  * external PCA file (ggplvm.cu:1088-1095, argv[4]).  An MLP

@@ -552,3 +552,40 @@ def test_batched_subset_glm_matches_oracle(graph):
         assert rel(B[k, :P], o.get("Beta")[:P]) < 1e-8, k
         assert abs(phi[k] - o.get("phi")[0]) <= 1e-8 * abs(phi[k]), k
     bf.close()
+
+
+@pytest.mark.parametrize("case", ["covariates", "intercept"])
+def test_pca_init_matches_oracle(case, tmp_path):
+    """gp_pca_init (dense nu0, covariate fit, SYRK, rocSOLVER syevd, projection)
+    against the numpy restatement of init.cu:304-357: eigenvalues within 1e-10,
+    X within 1e-8 of its largest entry.  The covariate case has a full set of
+    one-hot levels next to the intercept (dependent columns).  The pcainit
+    tool writes the same X as Xinit.bin."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    from gaspacho2_amd import lib, synth
+    from oracle import pca
+    if case == "covariates":
+        ds = synth.make_dataset(600, 40, 4, 8, seed=7, n_numeric=2, n_levels=3)
+    else:
+        ds = synth.make_dataset(800, 60, 5, 8, seed=8)
+    N, Q = ds.N, ds.Q
+    ncol = int(ds.nh[-2])
+    Z = np.asarray(ds.Zcov).reshape(N, -1, order="F")[:, :ncol]
+    Xo, lo = pca.pca_init(N, ds.Yp, ds.Yi, ds.Yx, ds.ls, Z, Q)
+    Xg, lg = lib.pca_init(N, ds.Yp, ds.Yi, ds.Yx, ds.ls, Z, Q)
+    gaps = np.diff(lo[::-1]) / lo[0]
+    print(f"{case}: eigenvalues {lo}, smallest relative gap {gaps.min():.2e}; "
+          f"X rel err {rel(Xg, Xo):.2e}, eigenvalue rel err {rel(lg, lo):.2e}")
+    assert rel(lg, lo) < 1e-10
+    assert rel(Xg, Xo) < 1e-8
+    for f, name in ((synth.write_sparsemat, "Y.bin"), (synth.write_dataframe, "DF.bin")):
+        f(str(tmp_path / name), ds)
+    synth.write_doubles(str(tmp_path / "ls.bin"), ds.ls)
+    tool = os.path.join(ROOT, "gaspacho2_amd", "pcainit")
+    p = subprocess.run([tool, "Y.bin", "DF.bin", "ls.bin", str(Q), "Xinit.bin"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    Xt = np.frombuffer(open(tmp_path / "Xinit.bin", "rb").read(), dtype="<f8").reshape(Q, N).T
+    assert np.array_equal(Xt, Xg)

@@ -438,3 +438,31 @@ def test_encoder_oracle_adam_first_step_moves_each_weight_by_lr():
     # first step: m/(1-b1) = g, v/(1-b2) = g^2 -> update = lr g / (|g| + eps)
     assert np.allclose(w - w1, 1e-3 * np.sign(g) * np.abs(g) / (np.abs(g) + 1e-8), rtol=1e-9, atol=0)
     assert np.allclose(m, 0.1 * g) and np.allclose(v, 0.001 * g * g)
+
+
+# ---------------- PCA initial latent (oracle/pca.py) ----------------
+
+def test_pca_oracle_closed_forms(small_ds):
+    """X^T X = N I; X spans the top-Q left singular vectors of nu0 (numpy SVD)
+    with the eigenvalues the squared singular values; nu0 is orthogonal to the
+    covariate columns (least-squares residual)."""
+    from oracle import pca
+    ds = small_ds
+    N, Q = ds.N, 3
+    ncol = int(ds.nh[-2])
+    Z = np.asarray(ds.Zcov).reshape(N, -1, order="F")[:, :ncol]
+    X, lam = pca.pca_init(N, ds.Yp, ds.Yi, ds.Yx, ds.ls, Z, Q)
+    assert np.allclose(X.T @ X, N * np.eye(Q), atol=1e-8 * N)
+    nu = pca.nu0(N, ds.Yp, ds.Yi, ds.Yx, ds.ls, Z)
+    assert np.max(np.abs(Z.T @ nu)) < 1e-9 * np.max(np.abs(nu)) * N
+    U, sv, _ = np.linalg.svd(nu, full_matrices=False)
+    assert np.allclose(lam, sv[:Q] ** 2, rtol=1e-10)
+    # same subspace, columns equal up to sign
+    for k in range(Q):
+        u = U[:, k] * np.sqrt(N)
+        assert min(np.max(np.abs(X[:, k] - u)), np.max(np.abs(X[:, k] + u))) < 1e-7 * np.sqrt(N)
+    # the sign convention: the projection direction's largest entry is positive
+    V = nu.T @ X                                     # proportional to the eigenvectors
+    for k in range(Q):
+        v = V[:, k]
+        assert v[np.argmax(np.abs(v))] > 0
