@@ -46,8 +46,10 @@ constexpr int BT = 10;                 // 16-column tiles of the augmented desig
 constexpr int BPA = 16 * BT;
 constexpr int BCH = 32;                // cells per chunk
 constexpr int BLD = 176;               // LDS row stride in doubles (rows land 32 banks apart)
-constexpr int BSLOT = (BT * (BT + 1) / 2 + 3) / 4;   // upper tiles per wave (14)
-constexpr int BPRE = (BCH * BPA + 255) / 256;        // prefetched values per thread (20)
+constexpr int BTH = 512;               // threads of the pass (8 waves)
+constexpr int BSLOT = (BT * (BT + 1) / 2 + BTH / 64 - 1) / (BTH / 64);   // upper tiles per wave (7)
+constexpr int KPL = (BPA + 63) / 64;   // Knm values per lane and cell (M < BPA)
+constexpr int CVR = (BPA + BTH / BCH - 1) / (BTH / BCH);   // covariate columns per thread (p_g < BPA)
 constexpr double WT = 0.1;             // ggplvm.cu:20
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -58,6 +60,18 @@ __device__ inline double bsum(double v, double *sh) {
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
     const double r = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return r;
+}
+
+// block sum over BTH threads
+__device__ inline double bsum8(double v, double *sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = 0.0;
+    for (int k = 0; k < BTH / 64; k++) r += sh[k];
     __syncthreads();
     return r;
 }
@@ -74,7 +88,13 @@ __device__ inline double prior(int a, int b, int pg, int M, const double *K, con
 
 // mode 0: initPsi/initBeta pass (unit weights, nu = log((y+0.05)/ls));
 // mode 1: IRLS pass at the current beta (also the Poisson log-likelihood).
-__global__ void __launch_bounds__(256) k_bf_pass(int mode, int J, int M, int PB,
+// Eight waves: per 32-cell chunk, wave w gathers the Knm rows of cells
+// 4w .. 4w+3 (one 1 KB row per cell, lanes over columns; prefetched one chunk
+// ahead in registers), thread t gathers the covariate value (cell t % 32,
+// column t / 32) and the intercept; 16 threads per cell form eta, W, nu;
+// the upper 16x16 tiles of the (P_g+1)-wide Gram are dealt round robin to
+// the eight waves (at most BSLOT tiles each).
+__global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
         const int *__restrict__ order, int *__restrict__ queue,
         const int *__restrict__ off, const int *__restrict__ cell, const int *__restrict__ ncov,
         const long long *__restrict__ covoff, const float *__restrict__ cov,
@@ -86,8 +106,9 @@ __global__ void __launch_bounds__(256) k_bf_pass(int mode, int J, int M, int PB,
     __shared__ double Xs[BCH * BLD];
     __shared__ double bs[BPA];
     __shared__ double sq[BCH], nuv[BCH];
-    __shared__ double red[4];
+    __shared__ double red[BTH / 64];
     __shared__ int gsh;
+    __shared__ unsigned char tta[BT * (BT + 1) / 2], ttb[BT * (BT + 1) / 2];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const double dt = delta[2];
     for (;;) {
@@ -102,48 +123,70 @@ __global__ void __launch_bounds__(256) k_bf_pass(int mode, int J, int M, int PB,
         const int o0 = off[g], n = off[g + 1] - o0, pg = ncov[g], P = pg + M;
         const int PA = P + 1, TA = (PA + 15) / 16, NTL = TA * (TA + 1) / 2;
         const float *cg = cov + covoff[g];
-        for (int k = tid; k < BPA; k += 256) bs[k] = (mode == 1 && k < P) ? Beta[(size_t)g * PB + k] : 0.0;
+        for (int k = tid; k < BPA; k += BTH) bs[k] = (mode == 1 && k < P) ? Beta[(size_t)g * PB + k] : 0.0;
+        // upper tile t -> (ta, tb), once per gene
+        if (tid < NTL) {
+            int ta = 0, rem = tid;
+            while (rem >= TA - ta) { rem -= TA - ta; ta++; }
+            tta[tid] = (unsigned char)ta;
+            ttb[tid] = (unsigned char)(ta + rem);
+        }
         d4 acc[BSLOT];
 #pragma unroll
         for (int s = 0; s < BSLOT; s++) acc[s] = d4{0.0, 0.0, 0.0, 0.0};
         double lks = 0.0;
-        // raw design values of one chunk: item it -> (cell it / PA, column it % PA)
-        double pre[BPRE];
+        // prefetch registers: Knm values of this wave's 4 cells, one covariate value
+        double kv[4][KPL];
+        float cvv[CVR];
+        const int cci = tid & (BCH - 1), ccol = tid / BCH;     // covariate items: columns ccol + 16 r
         auto fetch = [&](int c0) {
 #pragma unroll
-            for (int r = 0; r < BPRE; r++) {
-                const int it = tid + 256 * r;
-                const int ci = it / PA, col = it - ci * PA;
-                double v = 0.0;
-                if (ci < BCH && c0 + ci < n) {
-                    const int pos = o0 + c0 + ci;
-                    if (col == 0) v = 1.0;
-                    else if (col < pg) v = (double)cg[(size_t)(col - 1) * n + c0 + ci];
-                    else if (col < P) v = Krm[(size_t)cell[pos] * M + (col - pg)];
+            for (int u = 0; u < 4; u++) {
+                const int ci = 4 * w + u;
+                const bool ok = c0 + ci < n;
+                const double *row = Krm + (size_t)(ok ? cell[o0 + c0 + ci] : 0) * M;
+#pragma unroll
+                for (int e = 0; e < KPL; e++) {
+                    const int col = lane + 64 * e;
+                    kv[u][e] = (ok && col < M) ? row[col] : 0.0;
                 }
-                pre[r] = v;
+            }
+#pragma unroll
+            for (int r = 0; r < CVR; r++) {
+                const int col = ccol + (BTH / BCH) * r;
+                cvv[r] = (col >= 1 && col < pg && c0 + cci < n) ? cg[(size_t)(col - 1) * n + c0 + cci] : 0.0f;
             }
         };
         fetch(0);
         for (int c0 = 0; c0 < n; c0 += BCH) {
-            __syncthreads();
+            __syncthreads();                          // the previous chunk's MFMAs are done with Xs
 #pragma unroll
-            for (int r = 0; r < BPRE; r++) {
-                const int it = tid + 256 * r;
-                const int ci = it / PA, col = it - ci * PA;
-                if (ci < BCH) Xs[ci * BLD + col] = pre[r];
+            for (int u = 0; u < 4; u++) {
+                const int ci = 4 * w + u;
+#pragma unroll
+                for (int e = 0; e < KPL; e++) {
+                    const int col = lane + 64 * e;
+                    if (col < M) Xs[ci * BLD + pg + col] = kv[u][e];
+                }
+            }
+            if (ccol == 0) Xs[cci * BLD] = (c0 + cci < n) ? 1.0 : 0.0;
+#pragma unroll
+            for (int r = 0; r < CVR; r++) {
+                const int col = ccol + (BTH / BCH) * r;
+                if (col >= 1 && col < pg) Xs[cci * BLD + col] = (double)cvv[r];
             }
             if (c0 + BCH < n) fetch(c0 + BCH);
             __syncthreads();
-            // per cell: eta, W, nu, weight (8 threads per cell)
+            // per cell: eta, W, nu, weight (16 threads per cell)
             {
-                const int ci = tid >> 3, part = tid & 7;
+                const int ci = tid >> 4, part = tid & 15;
                 double eta = 0.0;
                 if (mode == 1)
-                    for (int k = part; k < P; k += 8) eta += Xs[ci * BLD + k] * bs[k];
+                    for (int k = part; k < P; k += 16) eta += Xs[ci * BLD + k] * bs[k];
                 eta += __shfl_xor(eta, 1, 64);
                 eta += __shfl_xor(eta, 2, 64);
                 eta += __shfl_xor(eta, 4, 64);
+                eta += __shfl_xor(eta, 8, 64);
                 if (part == 0) {
                     double s = 0.0, nu = 0.0;
                     if (c0 + ci < n) {
@@ -167,19 +210,19 @@ __global__ void __launch_bounds__(256) k_bf_pass(int mode, int J, int M, int PB,
                 }
             }
             __syncthreads();
-            for (int it = tid; it < BCH * PA; it += 256) {
-                const int ci = it / PA, col = it - ci * PA;
-                Xs[ci * BLD + col] = (col < P) ? Xs[ci * BLD + col] * sq[ci] : sq[ci] * nuv[ci];
+            {
+                const int ci = tid >> 4, part = tid & 15;
+                const double sc = sq[ci];
+                for (int col = part; col < P; col += 16) Xs[ci * BLD + col] *= sc;
+                if (part == 0) Xs[ci * BLD + P] = sc * nuv[ci];
             }
             __syncthreads();
-            // X^T X over the chunk: tile t = w + 4 s of the upper triangle
+            // X^T X over the chunk: tile t = w + 8 s of the upper triangle
 #pragma unroll
             for (int s = 0; s < BSLOT; s++) {
-                const int t = w + 4 * s;
+                const int t = w + (BTH / 64) * s;
                 if (t < NTL) {
-                    int ta = 0, rem = t;
-                    while (rem >= TA - ta) { rem -= TA - ta; ta++; }
-                    const int tb = ta + rem;
+                    const int ta = tta[t], tb = ttb[t];
 #pragma unroll
                     for (int k0 = 0; k0 < BCH; k0 += 4) {
                         const double a = Xs[(k0 + (lane >> 4)) * BLD + 16 * ta + (lane & 15)];
@@ -193,11 +236,9 @@ __global__ void __launch_bounds__(256) k_bf_pass(int mode, int J, int M, int PB,
         double *Pg = Psi + (size_t)g * PB * PB;
 #pragma unroll
         for (int s = 0; s < BSLOT; s++) {
-            const int t = w + 4 * s;
+            const int t = w + (BTH / 64) * s;
             if (t < NTL) {
-                int ta = 0, rem = t;
-                while (rem >= TA - ta) { rem -= TA - ta; ta++; }
-                const int tb = ta + rem;
+                const int ta = tta[t], tb = ttb[t];
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int row = 16 * ta + 4 * r + (lane >> 4), col = 16 * tb + (lane & 15);
@@ -214,12 +255,12 @@ __global__ void __launch_bounds__(256) k_bf_pass(int mode, int J, int M, int PB,
                 }
             }
         }
-        for (int e = tid; e < PB * PB; e += 256) {
+        for (int e = tid; e < PB * PB; e += BTH) {
             const int r = e % PB, cc = e / PB;
             if (r >= P || cc >= P) Pg[e] = (r == cc) ? 1.0 : 0.0;
         }
-        for (int k = P + tid; k < PB; k += 256) bvec[(size_t)g * PB + k] = 0.0;
-        lks = bsum(lks, red);
+        for (int k = P + tid; k < PB; k += BTH) bvec[(size_t)g * PB + k] = 0.0;
+        lks = bsum8(lks, red);
         if (tid == 0 && mode == 1) lk[g] = lks;
     }
 }
@@ -322,7 +363,7 @@ int bf_pass(gp_bfit *b, int mode) {
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     const int grid = std::min(b->J, 2 * ncu);
-    k_bf_pass<<<grid, 256, 0, c->stream>>>(mode, b->J, b->M, b->PB, b->order, b->queue, b->off,
+    k_bf_pass<<<grid, BTH, 0, c->stream>>>(mode, b->J, b->M, b->PB, b->order, b->queue, b->off,
                                           b->cell, b->ncov, b->covoff, b->cov, b->yd, b->Krm,
                                           c->ls, c->Omega, c->titsias, c->K, b->delta, b->Beta,
                                           b->Psi, b->bvec, b->sov2, b->lk);
