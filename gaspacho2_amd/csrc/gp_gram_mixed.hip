@@ -298,38 +298,34 @@ struct GramPlan {
     int b[GW * GSL];   // column group
 };
 
-constexpr GramPlan make_gram_plan(int T16) {
-    GramPlan p{};
-    for (int i = 0; i < GW * GSL; i++) { p.a[i] = -1; p.b[i] = -1; }
-    const int NG = (T16 + 3) / 4, RL = T16 - 4 * (NG - 1);
-    int ba[GW * GSL] = {}, bb[GW * GSL] = {}, bc[GW * GSL] = {};
-    int nb = 0;
-    for (int a = 0; a < NG; a++)
-        for (int b = a; b < NG; b++) {
-            if (nb >= GW * GSL) return p;        // does not fit: caller checks plan_fits
-            const int sa = a < NG - 1 ? 4 : RL, sb = b < NG - 1 ? 4 : RL;
-            ba[nb] = a; bb[nb] = b; bc[nb] = (a == b) ? sa * (sa + 1) / 2 : sa * sb;
-            nb++;
-        }
-    // stable sort by cost, largest first
-    for (int i = 1; i < nb; i++)
-        for (int k = i; k > 0 && bc[k] > bc[k - 1]; k--) {
-            int t = ba[k]; ba[k] = ba[k - 1]; ba[k - 1] = t;
-            t = bb[k]; bb[k] = bb[k - 1]; bb[k - 1] = t;
-            t = bc[k]; bc[k] = bc[k - 1]; bc[k - 1] = t;
-        }
-    int load[GW] = {}, used[GW] = {};
-    for (int i = 0; i < nb; i++) {
-        int best = -1;
-        for (int v = 0; v < GW; v++)
-            if (used[v] < GSL && (best < 0 || load[v] < load[best])) best = v;
-        p.a[best * GSL + used[best]] = ba[i];
-        p.b[best * GSL + used[best]] = bb[i];
-        used[best]++;
-        load[best] += bc[i];
-    }
-    return p;
-}
+// Block plans for T16 = 1..17: wave w runs on SIMD w mod 4 and owns the
+// blocks in slots 2w, 2w+1 (row group a, column group b; -1 = empty).  Each
+// plan was chosen, among random deals of the blocks with at most two per
+// wave, for the smallest largest MFMA load of a SIMD (tiles of its two waves)
+// and then the fewest LDS fragment reads per chunk (a wave reads each 4-tile
+// group it touches once: at P16 = 272, 120 reads instead of 170 when every
+// block reads its own).
+constexpr GramPlan kGramPlans[MAX_SYM_T16] = {
+    {{0, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}, {0, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}},   // T16 = 1: max SIMD load 1 tiles, 2 fragment reads
+    {{0, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}, {0, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}},   // T16 = 2: max SIMD load 3 tiles, 4 fragment reads
+    {{0, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}, {0, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}},   // T16 = 3: max SIMD load 6 tiles, 6 fragment reads
+    {{0, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}, {0, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}},   // T16 = 4: max SIMD load 10 tiles, 8 fragment reads
+    {{0, -1, 0, -1, 1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}, {0, -1, 1, -1, 1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}},   // T16 = 5: max SIMD load 10 tiles, 20 fragment reads
+    {{0, -1, 0, -1, 1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}, {1, -1, 0, -1, 1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}},   // T16 = 6: max SIMD load 10 tiles, 24 fragment reads
+    {{1, -1, 0, -1, 0, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}, {1, -1, 0, -1, 1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}},   // T16 = 7: max SIMD load 12 tiles, 28 fragment reads
+    {{1, -1, 0, -1, 0, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}, {1, -1, 1, -1, 0, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1}},   // T16 = 8: max SIMD load 16 tiles, 32 fragment reads
+    {{1, -1, 0, -1, 0, -1, 0, -1, 1, -1, 2, -1, -1, -1, -1, -1}, {2, -1, 2, -1, 0, -1, 1, -1, 1, -1, 2, -1, -1, -1, -1, -1}},   // T16 = 9: max SIMD load 16 tiles, 54 fragment reads
+    {{0, -1, 2, -1, 0, -1, 1, -1, 1, -1, 0, -1, -1, -1, -1, -1}, {2, -1, 2, -1, 1, -1, 1, -1, 2, -1, 0, -1, -1, -1, -1, -1}},   // T16 = 10: max SIMD load 16 tiles, 60 fragment reads
+    {{2, -1, 0, -1, 0, -1, 0, -1, 1, -1, 1, -1, -1, -1, -1, -1}, {2, -1, 0, -1, 2, -1, 1, -1, 2, -1, 1, -1, -1, -1, -1, -1}},   // T16 = 11: max SIMD load 20 tiles, 66 fragment reads
+    {{2, -1, 0, -1, 0, -1, 1, -1, 0, -1, 1, -1, -1, -1, -1, -1}, {2, -1, 1, -1, 0, -1, 2, -1, 2, -1, 1, -1, -1, -1, -1, -1}},   // T16 = 12: max SIMD load 26 tiles, 72 fragment reads
+    {{0, 0, 1, -1, 1, -1, 0, -1, 1, -1, 3, -1, 0, -1, 2, 2}, {3, 1, 2, -1, 1, -1, 0, -1, 3, -1, 3, -1, 2, -1, 3, 2}},   // T16 = 13: max SIMD load 26 tiles, 88 fragment reads
+    {{0, -1, 2, 0, 0, -1, 2, -1, 0, -1, 3, -1, 1, 1, 1, -1}, {0, -1, 3, 2, 3, -1, 2, -1, 1, -1, 3, -1, 1, 3, 2, -1}},   // T16 = 14: max SIMD load 27 tiles, 96 fragment reads
+    {{0, -1, 1, -1, 1, -1, 0, 1, 2, 2, 0, -1, 0, -1, 3, -1}, {0, -1, 3, -1, 2, -1, 1, 1, 3, 2, 2, -1, 3, -1, 3, -1}},   // T16 = 15: max SIMD load 32 tiles, 104 fragment reads
+    {{3, -1, 0, -1, 0, -1, 2, -1, 1, 1, 1, -1, 0, 2, 0, -1}, {3, -1, 3, -1, 0, -1, 3, -1, 1, 2, 3, -1, 2, 2, 1, -1}},   // T16 = 16: max SIMD load 36 tiles, 112 fragment reads
+    {{4, 1, 2, 0, 1, 2, 3, -1, 2, 1, 0, 3, 0, 0, 1, 0}, {4, 4, 4, 2, 2, 2, 3, -1, 3, 3, 3, 4, 0, 4, 1, 1}},   // T16 = 17: max SIMD load 40 tiles, 120 fragment reads
+};
+
+constexpr GramPlan make_gram_plan(int T16) { return kGramPlans[T16 - 1]; }
 
 constexpr bool plan_fits(int T16) {
     const int NG = (T16 + 3) / 4;
