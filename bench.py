@@ -413,7 +413,7 @@ def main():
     traffic, tsrc = measured_traffic(dom + "_kernel", main_p, (ds.N, ds.J, ds.Q, ds.M))
     if main_p == "mixed":
         kname = {"gram": "k_gram_sym4" if g.get_option("gram_waves") == 4 else "k_gram_sym",
-                 "gradcontract": "k_gc_rows" if g.get_option("gc_form") == 1 else "k_gc_mixed"}[dom]
+                 "gradcontract": {0: "k_gc_mixed", 1: "k_gc_rows", 2: "k_gc_rows2"}[g.get_option("gc_form")]}[dom]
     else:
         kname = {"gram": "k_gram_mfma_f64", "gradcontract": "k_gc_mfma_f64"}[dom]
     roofline = {"kernel": f"{kname}<{(P + 15) // 16}> ({dom})", "bound": "mfma", "achieved": achieved / 1e12,

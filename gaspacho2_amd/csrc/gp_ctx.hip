@@ -236,7 +236,7 @@ extern "C" int gp_set_option(gp_ctx *c, const char *name, int value) {
         if (value != 4 && value != 8) { gp_set_error("gram_waves must be 4 or 8"); return 1; }
         c->gram_waves = value;
     } else if (n == "gc_form") {
-        if (value != 0 && value != 1) { gp_set_error("gc_form must be 0 or 1"); return 1; }
+        if (value < 0 || value > 2) { gp_set_error("gc_form must be 0, 1 or 2"); return 1; }
         c->gc_form = value;
     } else if (n == "gc_mixed") {
         c->gc_mixed = value ? 1 : 0;
@@ -746,7 +746,7 @@ extern "C" int gp_autotune(gp_ctx *c, int verbose) {
     static const int splits[] = {8, 16, 32};
     static const int groups[] = {16, 32, 64};
     static const int waves[] = {8, 4};
-    static const int forms[] = {1, 0};
+    static const int forms[] = {2, 1, 0};
     GP_TRY(gpk_expand_delta(c));
     if (!c->zw_valid) {
         GP_TRY(gpk_whiten(c));

@@ -382,6 +382,155 @@ __global__ void __launch_bounds__(gc_rows_waves(T16) * 64) k_gc_rows(int N, int 
     }
 }
 
+// ---- factored row-block form (gc_form 2): the cell weight o is a per-row
+// scale of A = o z, so it moves onto the accumulator: per gene the workgroup
+// forms C = Z_I S_j with a fresh accumulator and adds o_i C[i, :] into U.
+// A = z (hi/lo) is then the same for every gene of the group and is staged
+// in LDS once; per gene only the 128 weights are staged (double buffered,
+// one barrier per gene).  Same MFMA tiling as the row-block form; the
+// rounding differs (o multiplies the FP32 gene sum instead of each operand).
+template <int T16>
+__global__ void __launch_bounds__(gc_rows_waves(T16) * 64) k_gc_rows2(int N, int Npad, int P, int gsize, int J,
+        const double *__restrict__ Zw, const float *__restrict__ w32,
+        const int *__restrict__ perm, const int *__restrict__ aexp, const int *__restrict__ zexp,
+        const _Float16 *__restrict__ shi, const _Float16 *__restrict__ slo,
+        float *__restrict__ part) {
+    constexpr int P16 = 16 * T16;
+    constexpr int NPAIR = T16 / 2, ODD = T16 % 2;
+    constexpr int NW = gc_rows_waves(T16);
+    constexpr int KS = (P16 + 31) / 32;
+    constexpr int LDA = 32 * KS + 8;
+    constexpr int NITEM = GRC * KS * 4;
+    constexpr int IPT = (NITEM + NW * 64 - 1) / (NW * 64);
+    constexpr int NX = (8 + NW - 1) / NW;
+    extern __shared__ __attribute__((aligned(16))) _Float16 As[];   // [2 hl][GRC][LDA]
+    __shared__ float os[2][GRC];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g = blockIdx.y;
+    const int j0 = g * gsize, j1 = min(J, j0 + gsize);
+    const int cb = blockIdx.x * GRC;
+    const int ct0 = 2 * w;
+    _Float16 *Ah = As, *Al = As + (size_t)GRC * LDA;
+    // A = z (scaled by 2^-zexp), split hi/lo, once for the group
+#pragma unroll
+    for (int r = 0; r < IPT; r++) {
+        const int it = min(tid + NW * 64 * r, NITEM - 1);
+        const int cl = it % GRC, k0 = 8 * (it / GRC);
+        h8 hi, lo;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int k = k0 + e, cell = cb + cl;
+            float v = (k < P && cell < N) ? (float)ldexp(Zw[cell + (size_t)k * Npad], -zexp[k]) : 0.0f;
+            asm volatile("" : "+v"(v));
+            const _Float16 h = (_Float16)v;
+            hi[e] = h;
+            lo[e] = (_Float16)(v - (float)h);
+        }
+        *(h8 *)&Ah[cl * LDA + k0] = hi;
+        *(h8 *)&Al[cl * LDA + k0] = lo;
+    }
+    auto weights = [&](int jj, int buf) {
+        const int j = perm[min(jj, j1 - 1)];
+        const float sc = ldexpf(1.0f, aexp[j]);
+        for (int t = tid; t < GRC; t += NW * 64)   // fewer than 128 threads when T16 < 4
+            os[buf][t] = (cb + t < N) ? w32[cb + t + (size_t)j * Npad] * sc : 0.0f;
+    };
+    if (j0 < j1) weights(j0, 0);
+    __syncthreads();
+
+    f4 U[8][2], Ux[NX];
+#pragma unroll
+    for (int r = 0; r < 8; r++) { U[r][0] = f4{0.f, 0.f, 0.f, 0.f}; U[r][1] = f4{0.f, 0.f, 0.f, 0.f}; }
+#pragma unroll
+    for (int x = 0; x < NX; x++) Ux[x] = f4{0.f, 0.f, 0.f, 0.f};
+
+    const int lk = (lane & 15), lq = 8 * (lane >> 4);
+    const int bcol0 = (16 * min(ct0, T16 - 1) + lk) * KP + lq;
+    const int bcol1 = (16 * min(ct0 + 1, T16 - 1) + lk) * KP + lq;
+    const int bcolx = (16 * (T16 - 1) + lk) * KP + lq;
+    const int arow = lk * LDA + lq;
+    const int orow = 4 * (lane >> 4);             // this lane's accumulator rows within a tile
+
+    for (int jj = j0; jj < j1; jj++) {
+        const int j = perm[jj];
+        const int buf = (jj - j0) & 1;
+        weights(jj + 1, buf ^ 1);                 // the next gene's weights (read after this gene's barrier)
+        const _Float16 *hj = shi + (size_t)j * P16 * KP, *lj = slo + (size_t)j * P16 * KP;
+        h8 bh0 = *(const h8 *)(hj + bcol0), bl0 = *(const h8 *)(lj + bcol0);
+        h8 bh1 = *(const h8 *)(hj + bcol1), bl1 = *(const h8 *)(lj + bcol1);
+        h8 bhx = *(const h8 *)(hj + bcolx), blx = *(const h8 *)(lj + bcolx);
+        f4 acc[8][2], accx[NX];
+#pragma unroll
+        for (int r = 0; r < 8; r++) { acc[r][0] = f4{0.f, 0.f, 0.f, 0.f}; acc[r][1] = f4{0.f, 0.f, 0.f, 0.f}; }
+#pragma unroll
+        for (int x = 0; x < NX; x++) accx[x] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const int kn = 32 * min(ks + 1, KS - 1);
+            const h8 nh0 = *(const h8 *)(hj + bcol0 + kn), nl0 = *(const h8 *)(lj + bcol0 + kn);
+            const h8 nh1 = *(const h8 *)(hj + bcol1 + kn), nl1 = *(const h8 *)(lj + bcol1 + kn);
+            const h8 nhx = *(const h8 *)(hj + bcolx + kn), nlx = *(const h8 *)(lj + bcolx + kn);
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const h8 ah = *(const h8 *)(Ah + 16 * r * LDA + arow + 32 * ks);
+                const h8 al = *(const h8 *)(Al + 16 * r * LDA + arow + 32 * ks);
+                if constexpr (NPAIR > 0) {
+                    acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh0, acc[r][0], 0, 0, 0);
+                    acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl0, acc[r][0], 0, 0, 0);
+                    acc[r][0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh0, acc[r][0], 0, 0, 0);
+                    acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh1, acc[r][1], 0, 0, 0);
+                    acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl1, acc[r][1], 0, 0, 0);
+                    acc[r][1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh1, acc[r][1], 0, 0, 0);
+                }
+                if constexpr (ODD) {
+                    if (r % NW == w) {
+                        f4 &ax = accx[r / NW];
+                        ax = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bhx, ax, 0, 0, 0);
+                        ax = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, blx, ax, 0, 0, 0);
+                        ax = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bhx, ax, 0, 0, 0);
+                    }
+                }
+            }
+            bh0 = nh0; bl0 = nl0; bh1 = nh1; bl1 = nl1; bhx = nhx; blx = nlx;
+        }
+        // U += o (rows) * C
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const float o = os[buf][16 * r + orow + q];
+                if constexpr (NPAIR > 0) {
+                    U[r][0][q] += o * acc[r][0][q];
+                    U[r][1][q] += o * acc[r][1][q];
+                }
+                if constexpr (ODD) {
+                    if (r % NW == w) Ux[r / NW][q] += o * accx[r / NW][q];
+                }
+            }
+        }
+        __syncthreads();                          // os[buf ^ 1] written; os[buf] free
+    }
+    float *pg = part + (size_t)g * P16 * Npad;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const int row0 = cb + 16 * r + orow;
+        if constexpr (NPAIR > 0) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                pg[(size_t)(16 * ct0 + lk) * Npad + row0 + q] = U[r][0][q];
+                pg[(size_t)(16 * (ct0 + 1) + lk) * Npad + row0 + q] = U[r][1][q];
+            }
+        }
+        if constexpr (ODD) {
+            if (r % NW == w) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) pg[(size_t)(16 * (T16 - 1) + lk) * Npad + row0 + q] = Ux[r / NW][q];
+            }
+        }
+    }
+}
+
 __global__ void k_gc_reduce(int Npad, int P, int P16, int ngroup, const float *__restrict__ part,
                             const double *__restrict__ gun, double *__restrict__ Uw) {
     const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -413,7 +562,24 @@ static int launch_gc_rows(gp_ctx *c, int gsize, int ngroup) {
 }
 
 template <int T16>
+static int launch_gc_rows2(gp_ctx *c, int gsize, int ngroup) {
+    constexpr int P16 = 16 * T16, KS = (P16 + 31) / 32, NW = gc_rows_waves(T16);
+    const size_t shm = (size_t)2 * GRC * (32 * KS + 8) * sizeof(_Float16);
+    if (shm > 65536 && !c->capturing)   // set before any recording: not a stream operation
+        GP_HIP(hipFuncSetAttribute((const void *)k_gc_rows2<T16>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    dim3 g(c->Npad / GRC, ngroup);
+    k_gc_rows2<T16><<<g, NW * 64, shm, c->stream>>>(c->N, c->Npad, c->P, gsize, c->J, c->Zw, c->w32,
+                                                 c->gc_perm, c->gc_aexp, c->gc_zexp,
+                                                 (const _Float16 *)c->Sh, (const _Float16 *)c->Sl,
+                                                 c->gc_part);
+    GP_HIP(hipGetLastError());
+    return 0;
+}
+
+template <int T16>
 static int launch_gc(gp_ctx *c, int gsize, int ngroup) {
+    if (c->gc_form == 2) return launch_gc_rows2<T16>(c, gsize, ngroup);
     return c->gc_form == 1 ? launch_gc_rows<T16>(c, gsize, ngroup) : launch_gc_mixed<T16>(c, gsize, ngroup);
 }
 

@@ -113,7 +113,7 @@ struct gp_ctx {
     size_t gc_part_len = 0;
     int gc_groups = 16;        // gene groups of the mixed contraction
     int gc_mixed = 1;          // mixed path: contraction on FP16 MFMA (0: FP64)
-    int gc_form = 1;           // mixed contraction: 0 = 16-cell waves, 1 = 128-cell row blocks
+    int gc_form = 2;           // mixed contraction: 0 = 16-cell waves, 1 = 128-cell row blocks, 2 = factored row blocks
     int irls_mixed = 1;        // mixed path: IRLS steps on the whitened FP16 hi/lo Gram (0: FP64)
     int autotune = 0;          // gp_fit runs gp_autotune before its first quasi-likelihood phase
 

@@ -80,9 +80,12 @@ int gp_set_precision(gp_ctx *ctx, int precision);
  *                           shared across its blocks, one wave per SIMD)
  *   "gc_groups"   [1, 4096] gene groups of the mixed gradient contraction
  *                           (FP32 partial per group, FP64 sum across groups)
- *   "gc_form"     0/1       mixed contraction: 0 = 16-cell waves with S_j
+ *   "gc_form"     0/1/2     mixed contraction: 0 = 16-cell waves with S_j
  *                           staged in LDS; 1 = 128-cell row blocks with
- *                           each wave's S_j columns in registers (same bits)
+ *                           each wave's S_j columns in registers (same bits
+ *                           as 0); 2 = row blocks with the cell weights
+ *                           applied to each gene's FP32 product, so A is
+ *                           staged once per gene group
  *   "gc_mixed"    0/1       mixed path: contraction on FP16 hi/lo MFMA (1)
  *                           or in FP64 (0)
  *   "irls_mixed"  0/1       mixed path: IRLS steps on the whitened FP16
@@ -96,7 +99,7 @@ int gp_get_option(gp_ctx *ctx, const char *name, int *value);
 
 /* Autotuner (no counterpart in the reference): times the mixed-precision
  * Gram for gram_waves in {8, 4} x gram_splits in {8, 16, 32} and the mixed
- * contraction for gc_form in {1, 0} x gc_groups in {16, 32, 64} on the context's current state and keeps the
+ * contraction for gc_form in {2, 1, 0} x gc_groups in {16, 32, 64} on the context's current state and keeps the
  * fastest of each.  Needs GP_PREC_MIXED and a state after gp_ql_start; it
  * recomputes the per-gene factors and inverses at the current parameters. */
 int gp_autotune(gp_ctx *ctx, int verbose);

@@ -362,8 +362,9 @@ def test_gram_four_wave_form_matches_eight_wave_form(shape):
 def test_contraction_row_block_form_matches_16_cell_form(shape):
     """The 128-cell row-block contraction (gc_form 1) feeds every accumulator
     the same MFMA sequence as the 16-cell form (gc_form 0): the gradient
-    agrees to the last bit; P = 33, 257, 241, 225 and 9 (odd and even
-    column-tile counts, N not a multiple of 128)."""
+    agrees to the last bit; the factored form (gc_form 2) within 1e-6 on the
+    contraction Uw and 1e-5 on the gradient; P = 33, 257, 241, 225 and 9 (odd
+    and even column-tile counts, N not a multiple of 128)."""
     from gaspacho2_amd import lib, synth
     N, G, Q, M = shape
     ds = synth.make_dataset(N, G, Q, M, seed=6)
@@ -375,13 +376,17 @@ def test_contraction_row_block_form_matches_16_cell_form(shape):
     g.ql_start()
     g.set_precision(lib.PREC_MIXED)
     out = {}
-    for form in (0, 1):
+    for form in (0, 1, 2):
         g.set_option("gc_form", form)
         g.elbo(1)
         g.grad()
-        out[form] = (g.get("grad").copy(), g.get("C1").copy())
+        out[form] = (g.get("grad").copy(), g.get("C1").copy(), g.get("Uw").copy())
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
+    # form 2 rounds differently (the weight multiplies each gene's FP32 sum)
+    ru, rg = rel(out[2][2], out[0][2]), rel(out[2][0], out[0][0])
+    print(f"gc_form 2 vs 0: Uw rel err {ru:.2e}, gradient rel err {rg:.2e}")
+    assert ru < 1e-6 and rg < 1e-5
     g.close()
 
 
@@ -408,7 +413,7 @@ def test_autotune_picks_valid_settings_and_keeps_results():
     g.set_precision(lib.PREC_MIXED)
     choice = g.autotune()
     assert choice["gram_splits"] in (8, 16, 32) and choice["gc_groups"] in (16, 32, 64)
-    assert choice["gram_waves"] in (4, 8) and choice["gc_form"] in (0, 1)
+    assert choice["gram_waves"] in (4, 8) and choice["gc_form"] in (0, 1, 2)
     em = g.elbo(1)
     g.grad()
     assert abs(em - e64) <= 1e-7 * abs(e64)
