@@ -177,7 +177,9 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
             }
             if (c0 + BCH < n) fetch(c0 + BCH);
             __syncthreads();
-            // per cell: eta, W, nu, weight (16 threads per cell)
+            // per cell: eta (16 threads per cell), then W, nu, weight and the
+            // likelihood term by one lane per cell of the first wave (the FP64
+            // exp / log / sqrt run once per cell, not once per wave)
             {
                 const int ci = tid >> 4, part = tid & 15;
                 double eta = 0.0;
@@ -187,27 +189,31 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
                 eta += __shfl_xor(eta, 2, 64);
                 eta += __shfl_xor(eta, 4, 64);
                 eta += __shfl_xor(eta, 8, 64);
-                if (part == 0) {
-                    double s = 0.0, nu = 0.0;
-                    if (c0 + ci < n) {
-                        const int pos = o0 + c0 + ci, i = cell[pos];
-                        const double y = (double)yd[pos];
-                        if (mode == 0) {
-                            s = 1.0;
-                            nu = log((y + 0.05) / ls[i]);
-                        } else {
-                            // ggplvm.cu:468-514 (zeta = 0)
-                            const double logW = eta + log(ls[i]) + dt * tits[i] * WT;
-                            const double Wi = exp(eta) * ls[i] * exp(dt * tits[i] * WT);
-                            nu = (y > 0.0) ? eta + exp(log(y) - logW) - 1.0 : eta - 1.0;
-                            s = sqrt(Wi / Omega[i]);
-                            // ggplvm.cu:703-724
-                            lks += (y > 0.0 ? (log(Wi) - dt * tits[i] * WT) * y : 0.0) - Wi;
-                        }
+                if (part == 0) nuv[ci] = eta;
+            }
+            __syncthreads();
+            if (tid < BCH) {
+                const int ci = tid;
+                const double eta = nuv[ci];
+                double s = 0.0, nu = 0.0;
+                if (c0 + ci < n) {
+                    const int pos = o0 + c0 + ci, i = cell[pos];
+                    const double y = (double)yd[pos];
+                    if (mode == 0) {
+                        s = 1.0;
+                        nu = log((y + 0.05) / ls[i]);
+                    } else {
+                        // ggplvm.cu:468-514 (zeta = 0)
+                        const double logW = eta + log(ls[i]) + dt * tits[i] * WT;
+                        const double Wi = exp(eta) * ls[i] * exp(dt * tits[i] * WT);
+                        nu = (y > 0.0) ? eta + exp(log(y) - logW) - 1.0 : eta - 1.0;
+                        s = sqrt(Wi / Omega[i]);
+                        // ggplvm.cu:703-724
+                        lks += (y > 0.0 ? (log(Wi) - dt * tits[i] * WT) * y : 0.0) - Wi;
                     }
-                    sq[ci] = s;
-                    nuv[ci] = nu;
                 }
+                sq[ci] = s;
+                nuv[ci] = nu;
             }
             __syncthreads();
             {
