@@ -410,25 +410,31 @@ __device__ __forceinline__ void gram_wave(const GramArgs &g, _Float16 *Xs, float
             acc1[x][y] = f4{0.f, 0.f, 0.f, 0.f};
         }
 
-    // The prefetch is straight-line code (item index and chunk clamped, no
-    // branches around the loads) and is pinned ahead of the MFMAs by a
-    // scheduling barrier, so its wait lands at its first use in store(),
-    // after this chunk's MFMAs: the L2/HBM latency hides behind them.
+    // The prefetch is straight-line code (chunk clamped, no branches around
+    // the loads) and is pinned ahead of the MFMAs by a scheduling barrier, so
+    // its wait lands at its first use in store(), after this chunk's MFMAs:
+    // the L2/HBM latency hides behind them.  Item round r of wave W holds
+    // items 64 (W + GW r) + lane; NITEM = 64 T16, so a round is wholly inside
+    // or wholly past the end, decided at compile time (the low waves take
+    // the last, partial round; at T16 = 17 that is wave 0 alone, whose plan
+    // has the lightest MFMA load, 5 tiles).
     f4 pz[IPT][2];
     auto fetch = [&](int cc) {
 #pragma unroll
         for (int r = 0; r < IPT; r++) {
-            const int it = min(tid + GW * 64 * r, NITEM - 1);
-            const float *src = g.z32 + (size_t)(it >> 2) * g.Npad + cc + 8 * (it & 3);
-            pz[r][0] = *(const f4 *)src;
-            pz[r][1] = *(const f4 *)(src + 4);
+            if (W + GW * r < T16) {
+                const int it = tid + GW * 64 * r;
+                const float *src = g.z32 + (size_t)(it >> 2) * g.Npad + cc + 8 * (it & 3);
+                pz[r][0] = *(const f4 *)src;
+                pz[r][1] = *(const f4 *)(src + 4);
+            }
         }
     };
     auto store = [&](int buf, int wb) {
-        // items past the end repeat the last item (same values, same place)
 #pragma unroll
         for (int r = 0; r < IPT; r++) {
-            const int it = min(tid + GW * 64 * r, NITEM - 1);
+            if (!(W + GW * r < T16)) continue;
+            const int it = tid + GW * 64 * r;
             const int col = it >> 2, q = it & 3;
             const f4 s0 = *(const f4 *)&wsl[wb][8 * q];
             const f4 s1 = *(const f4 *)&wsl[wb][8 * q + 4];
@@ -459,8 +465,8 @@ __device__ __forceinline__ void gram_wave(const GramArgs &g, _Float16 *Xs, float
 
     for (int ch = 0; ch < g.nch; ch++) {
         const int buf = ch & 1;
-        const bool more = ch + 1 < g.nch;
-        const float pw = g.wj[g.c0 + min(ch + 2, g.nch - 1) * CH + (tid & (CH - 1))];
+        float pw = 0.0f;
+        if constexpr (W == 0) pw = g.wj[g.c0 + min(ch + 2, g.nch - 1) * CH + (lane & (CH - 1))];
         fetch(g.c0 + min(ch + 1, g.nch - 1) * CH);
         __builtin_amdgcn_sched_barrier(0);   // issue the prefetch ahead of this chunk's MFMAs
         const _Float16 *xh = Xs + (size_t)(buf * 2 + 0) * P16 * LDX;
@@ -470,12 +476,12 @@ __device__ __forceinline__ void gram_wave(const GramArgs &g, _Float16 *Xs, float
         // unconditional, so the prefetch's only use is in this block and it is
         // not sunk into a branch (the last chunk's extra store is never read)
         store(buf ^ 1, (ch + 1) % 3);
-        // every thread writes (32 distinct values, duplicates identical) so the
-        // weight load is not sunk into a branch either; past the last chunk the
-        // slot holds the last chunk's weights and is never read.  Kept after
-        // the MFMAs so that its wait does not stall them.
+        // wave 0 writes the weights two chunks ahead (lanes 32-63 repeat lanes
+        // 0-31); past the last chunk the slot holds the last chunk's weights
+        // and is never read.  Kept after the MFMAs so that its wait does not
+        // stall them.
         __builtin_amdgcn_sched_barrier(0);
-        wsl[(ch + 2) % 3][tid & (CH - 1)] = sqrtf(pw * g.sc);
+        if constexpr (W == 0) wsl[(ch + 2) % 3][lane & (CH - 1)] = sqrtf(pw * g.sc);
         __syncthreads();
     }
     if constexpr (A0 >= 0) gram_block_store<T16, A0, B0>(acc0, g.pg, lane);
@@ -646,21 +652,24 @@ __device__ __forceinline__ void gram_wave4(const GramArgs &g, _Float16 *Xs, floa
 #pragma unroll
             for (int y = 0; y < 4; y++) acc[i][x][y] = f4{0.f, 0.f, 0.f, 0.f};
 
-    // staging as in the eight-wave form: straight-line, clamped items
+    // staging as in the eight-wave form: straight-line, whole item rounds
     f4 pz[IPT][2];
     auto fetch = [&](int cc) {
 #pragma unroll
         for (int r = 0; r < IPT; r++) {
-            const int it = min(tid + GW4 * 64 * r, NITEM - 1);
-            const float *src = g.z32 + (size_t)(it >> 2) * g.Npad + cc + 8 * (it & 3);
-            pz[r][0] = *(const f4 *)src;
-            pz[r][1] = *(const f4 *)(src + 4);
+            if (W + GW4 * r < T16) {
+                const int it = tid + GW4 * 64 * r;
+                const float *src = g.z32 + (size_t)(it >> 2) * g.Npad + cc + 8 * (it & 3);
+                pz[r][0] = *(const f4 *)src;
+                pz[r][1] = *(const f4 *)(src + 4);
+            }
         }
     };
     auto store = [&](int buf, int wb) {
 #pragma unroll
         for (int r = 0; r < IPT; r++) {
-            const int it = min(tid + GW4 * 64 * r, NITEM - 1);
+            if (!(W + GW4 * r < T16)) continue;
+            const int it = tid + GW4 * 64 * r;
             const int col = it >> 2, q = it & 3;
             const f4 s0 = *(const f4 *)&wsl[wb][8 * q];
             const f4 s1 = *(const f4 *)&wsl[wb][8 * q + 4];
