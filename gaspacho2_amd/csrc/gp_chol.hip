@@ -1,7 +1,8 @@
 // gp_chol.hip -- batched FP64 Cholesky, triangular solve and inverse.
 //
-// Inverse (solvePsi, ggplvm.cu:46-53): V = L^{-1} column by column, then the
-// symmetric product S = V^T V -- both hand-written, one workgroup per gene.
+// Inverse (solvePsi, ggplvm.cu:46-53): V = L^{-1} in 32 x 32 blocks (diagonal
+// inverses, then the block columns on FP64 MFMA), then the symmetric product
+// S = V^T V -- all hand-written.
 //
 // Cholesky: left-looking, blocked by 16-column panels held in LDS; one
 // workgroup per matrix.  Input is the lower triangle (the Gram writes both
@@ -149,36 +150,11 @@ __global__ void __launch_bounds__(CT) k_trinv_cols(const double *__restrict__ L,
     }
 }
 
-// V_j = L_j^{-1} (lower triangle of V; the strict upper triangle is not
-// written).  Columns are produced right to left: with L = [[l, 0], [c, L2]],
-// V[j+1:, j] = -V2 c / l where V2 = V[j+1:, j+1:] is already known.
-__global__ void __launch_bounds__(CT) k_trinv(const double *__restrict__ Lall, int n,
-                                              size_t stride, double *__restrict__ Vall) {
-    extern __shared__ double lcol[];
-    const double *L = Lall + (size_t)blockIdx.x * stride;
-    double *V = Vall + (size_t)blockIdx.x * stride;
-    const int tid = threadIdx.x;
-    for (int j = n - 1; j >= 0; j--) {
-        for (int k = j + 1 + tid; k < n; k += CT) lcol[k] = L[k + (size_t)j * n];
-        __syncthreads();
-        const double vjj = 1.0 / L[j + (size_t)j * n];
-        for (int i = j + 1 + tid; i < n; i += CT) {
-            double s = 0.0;
-            for (int k = j + 1; k <= i; k++) s += V[i + (size_t)k * n] * lcol[k];
-            V[i + (size_t)j * n] = -s * vjj;
-        }
-        if (tid == 0) V[j + (size_t)j * n] = vjj;
-        __syncthreads();
-    }
-}
-
 // Blocked V_j = L_j^{-1} (lower triangle; the strict upper triangle is not
 // written), 32 x 32 blocks.  From L V = I, block row I > J of block column J:
 //   L_II V_IJ = -sum_{K=J}^{I-1} L_IK V_KJ   =>   V_IJ = -V_II T_IJ,
 // so with the diagonal inverses V_II known, every block column is
-// independent: one workgroup per (block column, gene) walks I downwards
-// keeping the column's finished blocks in LDS.  L is read once per block
-// product (from L2), V is written once.
+// independent: one workgroup per (block column, gene) walks I downwards.
 constexpr int TB = 32;
 
 // V_II = L_II^{-1} for every diagonal block: lane c solves column c by
@@ -211,64 +187,63 @@ __global__ void __launch_bounds__(64) k_trinv_diag(const double *__restrict__ La
 }
 
 // Off-diagonal blocks of block column J = blockIdx.x (all but the last block
-// column).  Thread t owns the 4 entries (t / 8, 4 (t % 8) + q) of a block.
-__global__ void __launch_bounds__(CT) k_trinv_panel(const double *__restrict__ Lall, int n, size_t stride,
-                                                    double *__restrict__ Vall) {
-    extern __shared__ double sm[];
-    double (*Lb)[TB + 1] = (double (*)[TB + 1])sm;                  // L_IK
-    double (*Vjj)[TB + 1] = (double (*)[TB + 1])(sm + TB * (TB + 1));   // V_JJ
-    double (*Tt)[TB + 1] = (double (*)[TB + 1])(sm + 2 * TB * (TB + 1)); // T, then V_II
-    double (*Vp)[TB + 1] = (double (*)[TB + 1])(sm + 3 * TB * (TB + 1)); // finished V_KJ, K > J
+// column) on v_mfma_f64_16x16x4f64: 4 waves, wave w owns the 16 x 16 tile
+// (w / 2, w % 2) of each 32 x 32 block.  Per block row I: T = sum_K L_IK V_KJ
+// with both fragments read from global memory (L from the factorization; V_KJ
+// from k_trinv_diag for K = J, else written by this workgroup in an earlier
+// step and visible after its barrier), T through LDS, then V_IJ = -V_II T.
+// No LDS beyond T, so several workgroups share a CU and hide the load
+// latency of each other's chains.  Fragment layouts (lane l):
+//   A[m = l % 16][k = l / 16],  B[k = l / 16][n = l % 16],  D[m = 4 r + l / 16][n = l % 16].
+typedef double tri_d4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(CT) k_trinv_mfma(const double *__restrict__ Lall, int n, size_t stride,
+                                                   double *__restrict__ Vall) {
+    __shared__ double Tt[TB][TB + 1];
     const double *L = Lall + (size_t)blockIdx.y * stride;
     double *V = Vall + (size_t)blockIdx.y * stride;
-    const int NB = (n + TB - 1) / TB, J = blockIdx.x, c0 = TB * J, tid = threadIdx.x;
-    const int tr = tid / 8, tc = 4 * (tid % 8);
-    for (int idx = tid; idx < TB * TB; idx += CT) {
-        const int r = idx % TB, c = idx / TB;
-        Vjj[r][c] = (r >= c) ? V[(c0 + r) + (size_t)(c0 + c) * n] : 0.0;   // block J is full (J < NB - 1)
-    }
+    const int NB = (n + TB - 1) / TB, J = blockIdx.x, c0 = TB * J;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int tr = w >> 1, tc = w & 1;
+    const int lm = lane & 15, lk = lane >> 4;
+    const int bcol = 16 * tc + lm;                        // column of V_.J in the block (J < NB - 1: full)
+    double *vcol = V + (size_t)(c0 + bcol) * n;
     for (int I = J + 1; I < NB; I++) {
         const int rI = TB * I, bsI = min(TB, n - rI);
-        double acc[4] = {0.0, 0.0, 0.0, 0.0};
-        for (int K = J; K < I; K++) {
-            __syncthreads();                      // Lb free (and Vjj / Vp writes visible)
-            for (int idx = tid; idx < TB * TB; idx += CT) {
-                const int r = idx % TB, k = idx / TB;
-                Lb[r][k] = (r < bsI) ? L[(rI + r) + (size_t)(TB * K + k) * n] : 0.0;   // K < I: full width
-            }
-            __syncthreads();
-            const double (*Bk)[TB + 1] = (K == J) ? (const double (*)[TB + 1])Vjj
-                                                  : (const double (*)[TB + 1])&Vp[TB * (K - J - 1)];
-#pragma unroll 8
-            for (int k = 0; k < TB; k++) {
-                const double l = Lb[tr][k];
+        const int am = 16 * tr + lm;                      // A row within block row I
+        const bool arow = am < bsI;
+        const double *lrow = L + (rI + min(am, bsI - 1));
+        tri_d4 acc = tri_d4{0.0, 0.0, 0.0, 0.0};
+        for (int K = J; K < I; K++) {                     // K < I <= NB - 1: block K is full
 #pragma unroll
-                for (int q = 0; q < 4; q++) acc[q] += l * Bk[k][tc + q];
+            for (int s = 0; s < TB / 4; s++) {
+                const int kk = 4 * s + lk, k = TB * K + kk;
+                const double lv = lrow[(size_t)k * n];
+                const double vv = vcol[k];
+                const double a = arow ? lv : 0.0;
+                const double b = (K > J || kk >= bcol) ? vv : 0.0;   // V_JJ: lower triangle only
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
             }
         }
-        __syncthreads();
 #pragma unroll
-        for (int q = 0; q < 4; q++) Tt[tr][tc + q] = acc[q];
-        // V_II into Lb (lower triangle, zero elsewhere)
-        for (int idx = tid; idx < TB * TB; idx += CT) {
-            const int r = idx % TB, k = idx / TB;
-            Lb[r][k] = (r < bsI && k < bsI && r >= k) ? V[(rI + r) + (size_t)(rI + k) * n] : 0.0;
-        }
+        for (int r = 0; r < 4; r++) Tt[16 * tr + 4 * r + lk][16 * tc + lm] = acc[r];
         __syncthreads();
-        double v[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll 8
-        for (int k = 0; k < TB; k++) {
-            const double l = Lb[tr][k];
+        tri_d4 v = tri_d4{0.0, 0.0, 0.0, 0.0};
+        const double *vii = V + (rI + min(am, bsI - 1)) + (size_t)rI * n;
 #pragma unroll
-            for (int q = 0; q < 4; q++) v[q] -= l * Tt[k][tc + q];
+        for (int s = 0; s < TB / 4; s++) {
+            const int kk = 4 * s + lk;
+            const double x = vii[(size_t)min(kk, bsI - 1) * n];
+            const double a = (arow && kk <= am) ? x : 0.0;    // V_II: lower triangle, bsI x bsI
+            v = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Tt[kk][16 * tc + lm], v, 0, 0, 0);
         }
 #pragma unroll
-        for (int q = 0; q < 4; q++) Vp[TB * (I - J - 1) + tr][tc + q] = v[q];
-        __syncthreads();
-        for (int idx = tid; idx < TB * TB; idx += CT) {
-            const int r = idx % TB, c = idx / TB;
-            if (r < bsI) V[(rI + r) + (size_t)(c0 + c) * n] = Vp[TB * (I - J - 1) + r][c];
+        for (int r = 0; r < 4; r++) {
+            const int row = 16 * tr + 4 * r + lk;
+            if (row < bsI) vcol[rI + row] = -v[r];
         }
+        __syncthreads();                                  // V_IJ visible; Tt free
     }
 }
 
@@ -357,20 +332,11 @@ int gpk_chol_inverse_batched(gp_ctx *c, const double *L, int n, int batch, size_
     }
     const size_t st = stride ? stride : (size_t)n * n;
     const int nb = (n + TB - 1) / TB;
-    const size_t shm = (size_t)(3 + (nb - 1)) * TB * (TB + 1) * sizeof(double);
-    if (shm > 160 * 1024) {   // the block column does not fit in LDS: column-by-column form
-        k_trinv<<<batch, CT, (size_t)n * sizeof(double), c->stream>>>(L, n, st, c->Vtmp);
+    k_trinv_diag<<<dim3(nb, batch), 64, 0, c->stream>>>(L, n, st, c->Vtmp);
+    GP_HIP(hipGetLastError());
+    if (nb > 1) {
+        k_trinv_mfma<<<dim3(nb - 1, batch), CT, 0, c->stream>>>(L, n, st, c->Vtmp);
         GP_HIP(hipGetLastError());
-    } else {
-        k_trinv_diag<<<dim3(nb, batch), 64, 0, c->stream>>>(L, n, st, c->Vtmp);
-        GP_HIP(hipGetLastError());
-        if (nb > 1) {
-            if (shm > 65536 && !c->capturing)   // set before any recording: not a stream operation
-                GP_HIP(hipFuncSetAttribute((const void *)k_trinv_panel,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-            k_trinv_panel<<<dim3(nb - 1, batch), CT, shm, c->stream>>>(L, n, st, c->Vtmp);
-            GP_HIP(hipGetLastError());
-        }
     }
     const int nt = (n + 63) / 64;
     dim3 g(nt * (nt + 1) / 2, batch);
