@@ -218,6 +218,7 @@ extern "C" int gp_destroy(gp_ctx *c) {
 
 extern "C" int gp_set_precision(gp_ctx *c, int precision) {
     if (!c) { gp_set_error("NULL context"); return 1; }
+    c->lkhd_fresh = 0;
     if (precision < GP_PREC_FP64 || precision > GP_PREC_FP64_WHITENED) { gp_set_error("bad precision"); return 1; }
     c->precision = precision;
     return 0;
@@ -225,6 +226,7 @@ extern "C" int gp_set_precision(gp_ctx *c, int precision) {
 
 extern "C" int gp_set_option(gp_ctx *c, const char *name, int value) {
     if (!c || !name) { gp_set_error("NULL argument"); return 1; }
+    c->lkhd_fresh = 0;
     const std::string n(name);
     if (n == "gram_splits") {
         if (value < 1 || value > 256) { gp_set_error("gram_splits must be in [1, 256]"); return 1; }
@@ -367,6 +369,7 @@ extern "C" int gp_get_buffer(gp_ctx *c, const char *name, double *host, size_t n
 
 extern "C" int gp_set_buffer(gp_ctx *c, const char *name, const double *host, size_t n) {
     if (!c || !name || !host) { gp_set_error("NULL argument"); return 1; }
+    c->lkhd_fresh = 0;
     BufDesc d = describe(c, name);
     if (!d.dev) { gp_set_error(std::string("unknown buffer ") + name); return 1; }
     if (!d.writable) { gp_set_error(std::string("buffer is read-only: ") + name); return 1; }
@@ -454,6 +457,7 @@ static int gram_chol(gp_ctx *c, int precision) {
 
 extern "C" int gp_init_fit(gp_ctx *c) {
     if (!c) { gp_set_error("NULL context"); return 1; }
+    c->lkhd_fresh = 0;
     GP_TRY(clear_info(c));
     GP_TRY(kernel_mats(c));
     GP_TRY(gpk_expand_delta(c));
@@ -470,6 +474,7 @@ extern "C" int gp_init_fit(gp_ctx *c) {
 
 extern "C" int gp_irls_step(gp_ctx *c, int jtr, double *obj) {
     if (!c) { gp_set_error("NULL context"); return 1; }
+    c->lkhd_fresh = 0;
     if (jtr < 0 || jtr >= GP_LKHD_SLOTS) { gp_set_error("jtr out of range"); return 1; }
     GP_TRY(clear_info(c));
     GP_TRY(gpk_w_nu(c));
@@ -505,11 +510,13 @@ static int lkhd_genes(gp_ctx *c) {
     GP_TRY(gpk_r_and_sums(c, w ? c->Zw : c->Z));
     GP_TRY(gpk_cholsl_batched(c, c->Psi, c->P, (size_t)c->P * c->P, c->bvec, c->pwork, c->J));
     GP_TRY(gpk_lkhd_genes_with(c, c->pwork, w ? c->DeltaWChol : c->DeltaChol));
+    c->lkhd_fresh = 1;
     return 0;
 }
 
 extern "C" int gp_ql_start(gp_ctx *c, double *obj) {
     if (!c) { gp_set_error("NULL context"); return 1; }
+    c->lkhd_fresh = 0;   // set again by the likelihood evaluation below
     GP_TRY(clear_info(c));
     GP_TRY(gpk_oinvt(c));
     GP_TRY(gpk_chol_batched(c, c->DeltaChol, c->P, 1, 0, c->info + 2));
@@ -520,6 +527,7 @@ extern "C" int gp_ql_start(gp_ctx *c, double *obj) {
 
 extern "C" int gp_elbo(gp_ctx *c, int slot, double *obj) {
     if (!c) { gp_set_error("NULL context"); return 1; }
+    c->lkhd_fresh = 0;   // set again by the likelihood evaluation below
     if (slot < 0 || slot >= GP_LKHD_SLOTS) { gp_set_error("slot out of range"); return 1; }
     gp_timer_begin(c, c->t_elbo);
     GP_TRY(clear_info(c));
@@ -540,12 +548,19 @@ extern "C" int gp_grad(gp_ctx *c) {
     if (!c) { gp_set_error("NULL context"); return 1; }
     gp_timer_begin(c, c->t_grad);
     GP_TRY(gpk_expand_delta(c));
-    GP_TRY(gpk_oinvt(c));
     const bool w = c->psi_whitened;
-    // getTildeBeta (in the basis of the current factor)
-    GP_TRY(gpk_r_and_sums(c, w ? c->Zw : c->Z));
-    GP_TRY(gpk_cholsl_batched(c, c->Psi, c->P, (size_t)c->P * c->P, c->bvec,
-                              w ? c->tBw : c->tBeta, c->J));
+    double *tB = w ? c->tBw : c->tBeta;
+    // getTildeBeta (in the basis of the current factor): the likelihood
+    // evaluation at these parameters already solved it into pwork (with R,
+    // bvec and Oinvt current); otherwise solve it here
+    if (c->lkhd_fresh) {
+        GP_HIP(hipMemcpyAsync(tB, c->pwork, (size_t)c->P * c->J * sizeof(double),
+                              hipMemcpyDeviceToDevice, c->stream));
+    } else {
+        GP_TRY(gpk_oinvt(c));
+        GP_TRY(gpk_r_and_sums(c, w ? c->Zw : c->Z));
+        GP_TRY(gpk_cholsl_batched(c, c->Psi, c->P, (size_t)c->P * c->P, c->bvec, tB, c->J));
+    }
     // solvePsi (explicit inverses, needed by the delta gradient, U and C2)
     GP_TRY(gpk_chol_inverse_batched(c, c->Psi, c->P, c->J, (size_t)c->P * c->P, c->Sinv));
     GP_TRY(gpk_grad_delta(c, w));
@@ -607,6 +622,7 @@ extern "C" int gp_eval_graph(gp_ctx *c, int slot, double *obj) {
         std::copy(key, key + 6, c->graph_key);
     }
     GP_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+    c->lkhd_fresh = 1;   // the replayed unit ran the likelihood at the current parameters
     if (obj) GP_HIP(hipMemcpyAsync(obj, c->lkhd + slot, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     GP_HIP(hipStreamSynchronize(c->stream));
     int bad = 0;
@@ -616,6 +632,7 @@ extern "C" int gp_eval_graph(gp_ctx *c, int slot, double *obj) {
 
 extern "C" int gp_lbfgs_step(gp_ctx *c, int jtr, int rhoflag) {
     if (!c) { gp_set_error("NULL context"); return 1; }
+    c->lkhd_fresh = 0;
     if (jtr < 0) { gp_set_error("jtr < 0"); return 1; }
     GP_TRY(gpk_lbfgs(c, jtr, rhoflag ? 1 : 0));
     return 0;
@@ -623,12 +640,14 @@ extern "C" int gp_lbfgs_step(gp_ctx *c, int jtr, int rhoflag) {
 
 extern "C" int gp_downdate(gp_ctx *c, int jtr, double sr, int rhoflag) {
     if (!c) { gp_set_error("NULL context"); return 1; }
+    c->lkhd_fresh = 0;
     GP_TRY(gpk_downdate(c, jtr, sr, rhoflag ? 1 : 0));
     return 0;
 }
 
 extern "C" int gp_end_outer(gp_ctx *c) {
     if (!c) { gp_set_error("NULL context"); return 1; }
+    c->lkhd_fresh = 0;
     GP_TRY(gpk_expand_delta(c));
     GP_TRY(gpk_beta_from_tilde(c));
     GP_HIP(hipStreamSynchronize(c->stream));
@@ -742,6 +761,7 @@ int tune_gc(gp_ctx *c) { return gpk_gradcontract_mixed(c); }
 
 extern "C" int gp_autotune(gp_ctx *c, int verbose) {
     if (!c) { gp_set_error("NULL context"); return 1; }
+    c->lkhd_fresh = 0;
     if (c->precision != GP_PREC_MIXED) { gp_set_error("gp_autotune needs GP_PREC_MIXED"); return 1; }
     static const int splits[] = {8, 16, 32};
     static const int groups[] = {16, 32, 64};

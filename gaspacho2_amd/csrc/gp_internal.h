@@ -116,6 +116,11 @@ struct gp_ctx {
     int gc_form = 2;           // mixed contraction: 0 = 16-cell waves, 1 = 128-cell row blocks, 2 = factored row blocks
     int irls_mixed = 1;        // mixed path: IRLS steps on the whitened FP16 hi/lo Gram (0: FP64)
     int autotune = 0;          // gp_fit runs gp_autotune before its first quasi-likelihood phase
+    // 1 while R, bvec, Oinvt and pwork = Psi^-1 bvec hold the likelihood
+    // evaluation (gp_elbo / gp_ql_start) at the current parameters and factors,
+    // so gp_grad takes beta-tilde from pwork instead of solving again; every
+    // other state-changing entry point clears it
+    int lkhd_fresh = 0;
 
     size_t tot_max = 0;        // N+p+(N+M+1)Q
     int timing = 0;

@@ -421,6 +421,32 @@ def test_autotune_picks_valid_settings_and_keeps_results():
 
 
 @pytest.mark.parametrize("precision", [0, 1])
+def test_gradient_reuses_likelihood_solve(precision):
+    """gp_grad right after gp_elbo takes beta-tilde from the likelihood's own
+    solve; after any state-changing call (here gp_set_option) it solves again.
+    Same inputs, same kernels: the two gradients agree to the last bit."""
+    from gaspacho2_amd import lib, synth
+    ds = synth.make_dataset(1500, 30, 4, 48, seed=7, n_numeric=1)
+    T0 = lib.init_ta_with_per(ds.M, ds.Q)
+    g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0,
+                      T0, precision=lib.PREC_FP64)
+    g.init_fit()
+    for jtr in range(2):
+        g.irls_step(jtr)
+    g.ql_start()
+    g.set_precision(precision)
+    g.elbo(2)
+    g.grad()
+    reused = g.get("grad").copy()
+    tb = g.get("tBeta").copy()
+    g.set_option("gc_groups", g.get_option("gc_groups"))   # no-op value, clears the reuse
+    g.grad()
+    assert np.array_equal(g.get("grad"), reused)
+    assert np.array_equal(g.get("tBeta"), tb)
+    g.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
 def test_graph_replay_matches_eager(precision):
     """gp_eval_graph (the ELBO + gradient unit recorded as a HIP graph) gives
     the eager results, replay after replay, with eager work and parameter
