@@ -18,6 +18,7 @@ constexpr int CT = 256;   // threads per workgroup
 constexpr int NB = 16;    // panel width
 constexpr int KC = 64;    // k-chunk of previous columns staged in LDS
 constexpr int MAXN = 640; // largest n handled (panel rows in LDS)
+static_assert(KC % NB == 0, "k-chunks hold whole panels");
 
 __global__ void __launch_bounds__(CT) k_chol(double *__restrict__ Aall, int n, size_t stride,
                                              int *__restrict__ info) {
@@ -54,10 +55,16 @@ __global__ void __launch_bounds__(CT) k_chol(double *__restrict__ Aall, int n, s
                 }
                 __syncthreads();
                 if (r < rows) {
-                    for (int kk = 0; kk < kn; kk++) {
-                        const double lr = a[(kb + r) + (size_t)(k0 + kk) * n];
+                    // kn is a multiple of NB (kb and KC are): NB independent
+                    // loads of row r in flight per step instead of one
+                    for (int kk = 0; kk < kn; kk += NB) {
+                        double lr[NB];
 #pragma unroll
-                        for (int cc = 0; cc < NB; cc++) acc[cc] += lr * lc[cc * KC + kk];
+                        for (int u = 0; u < NB; u++) lr[u] = a[(kb + r) + (size_t)(k0 + kk + u) * n];
+#pragma unroll
+                        for (int u = 0; u < NB; u++)
+#pragma unroll
+                            for (int cc = 0; cc < NB; cc++) acc[cc] += lr[u] * lc[cc * KC + kk + u];
                     }
                 }
             }
@@ -68,27 +75,57 @@ __global__ void __launch_bounds__(CT) k_chol(double *__restrict__ Aall, int n, s
             }
         }
         __syncthreads();
-        // 3. factor the panel in LDS
-        for (int cc = 0; cc < nbk; cc++) {
-            const double piv = pan[cc * NB + cc];
-            if (piv <= 0.0) {           // same test as chol.cu:26
-                if (tid == 0) bad = 1;
-                break;
+        // 3. factor the panel: the nbk x nbk diagonal block on wave 0 (lane r
+        //    holds row r, pivots and columns broadcast by shuffles), then each
+        //    row below by forward substitution against it -- the same
+        //    operations in the same order as column-by-column elimination,
+        //    with two barriers instead of two per column
+        if (tid < 64) {
+            const int r = tid;
+            double x[NB];
+#pragma unroll
+            for (int cc = 0; cc < NB; cc++) x[cc] = (r < nbk) ? pan[min(r, nbk - 1) * NB + cc] : 0.0;
+            int fail = 0;
+#pragma unroll
+            for (int cc = 0; cc < NB; cc++) {
+                if (cc >= nbk || fail) break;
+                const double piv = __shfl(x[cc], cc);
+                if (piv <= 0.0) { fail = 1; break; }   // same test as chol.cu:26
+                const double d = sqrt(piv);
+                if (r == cc) x[cc] = d;
+                else if (r > cc) x[cc] /= d;
+#pragma unroll
+                for (int c2 = cc + 1; c2 < NB; c2++) {
+                    const double l2 = __shfl(x[cc], c2);
+                    if (r > cc && r >= c2 && c2 < nbk) x[c2] -= x[cc] * l2;
+                }
             }
-            const double d = sqrt(piv);
-            __syncthreads();
-            for (int r = cc + 1 + tid; r < rows; r += CT) pan[r * NB + cc] /= d;
-            if (tid == 0) pan[cc * NB + cc] = d;
-            __syncthreads();
-            for (int r = cc + 1 + tid; r < rows; r += CT) {
-                const double lrc = pan[r * NB + cc];
-                const int cmax = min(nbk - 1, r);
-                for (int c2 = cc + 1; c2 <= cmax; c2++) pan[r * NB + c2] -= lrc * pan[c2 * NB + cc];
+            if (r < nbk) {
+#pragma unroll
+                for (int cc = 0; cc < NB; cc++)
+                    if (cc <= r) pan[r * NB + cc] = x[cc];
             }
-            __syncthreads();
+            if (tid == 0 && fail) bad = 1;
         }
         __syncthreads();
         if (bad) break;
+        for (int r = nbk + tid; r < rows; r += CT) {
+            double x[NB];
+#pragma unroll
+            for (int cc = 0; cc < NB; cc++) x[cc] = pan[r * NB + cc];
+#pragma unroll
+            for (int cc = 0; cc < NB; cc++) {
+                if (cc < nbk) {
+                    double v = x[cc];
+#pragma unroll
+                    for (int c = 0; c < cc; c++) v -= x[c] * pan[cc * NB + c];
+                    x[cc] = v / pan[cc * NB + cc];
+                }
+            }
+#pragma unroll
+            for (int cc = 0; cc < NB; cc++) pan[r * NB + cc] = x[cc];
+        }
+        __syncthreads();
         // 4. write the factored panel back
         for (int t = tid; t < rows * nbk; t += CT) {
             const int r = t % rows, cc = t / rows;
