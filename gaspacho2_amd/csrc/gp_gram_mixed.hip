@@ -283,14 +283,19 @@ __global__ void k_gram_reduce(int P, int nbk, int nseg, int J, const float *__re
 // each wave owns up to two blocks, balanced at compile time (longest block
 // first onto the least loaded wave).  Each wave runs its own straight-line
 // copy of the chunk loop (the blocks are template arguments), so no
-// accumulator is touched by a data-dependent branch.  Per 32-cell chunk the
-// workgroup builds X hi/lo for all P16 columns in LDS (double buffered; the
-// chunk's square-root weights are staged two chunks ahead) and every wave runs
-// its blocks out of LDS: a 4x4 block reads 4 + 4 fragments for 16 tiles, a
-// diagonal block 4 for 10.
+// accumulator is touched by a data-dependent branch.  Per 64-cell chunk (two
+// MFMA k-steps) the workgroup builds X hi/lo for all P16 columns in LDS
+// (double buffered; the chunk's square-root weights are staged two chunks
+// ahead) and every wave runs its blocks out of LDS: per k-step a 4x4 block
+// reads 4 + 4 fragments for 16 tiles, a diagonal block 4 for 10.  Two k-steps
+// per chunk halve the barriers and the LDS-read ramp at each chunk start per
+// MFMA; at P16 = 272 the two buffers take 153 KB of the 160 KB LDS.
 constexpr int GW = 8;        // waves per workgroup
 constexpr int GSL = 2;       // block slots per wave
-constexpr int LDX = 40;      // LDS column stride in halves (32 cells + 8)
+constexpr int LDX = 40;      // LDS column stride in halves, four-wave form (32 cells + 8)
+constexpr int GKC = 2;       // MFMA k-steps per chunk, eight-wave form
+constexpr int GCH = 32 * GKC;    // cells per chunk, eight-wave form
+constexpr int GLDX = GCH + 8;    // LDS column stride in halves, eight-wave form
 constexpr int MAX_SYM_T16 = 17;  // instantiated widths of the symmetric kernel
 
 struct GramPlan {
@@ -332,14 +337,52 @@ constexpr bool plan_fits(int T16) {
     return NG * (NG + 1) / 2 <= GW * GSL;
 }
 
+// tiles of block (a, b) of the 4-tile column groups
+constexpr int gram_blk_cost(int T16, int a, int b) {
+    const int NG = (T16 + 3) / 4, RL = T16 - 4 * (NG - 1);
+    const int sa = a < NG - 1 ? 4 : RL, sb = b < NG - 1 ? 4 : RL;
+    return a == b ? sa * (sa + 1) / 2 : sa * sb;
+}
+
+// Staging rounds (64 items each, T16 * GKC per chunk) dealt to the waves of
+// the eight-wave form: wave W stages rounds r0[W] .. r0[W] + nr[W] - 1.  The
+// two waves of a SIMD share its matrix core but each issues its own VALU work,
+// so the rounds go to the waves with the least MFMA work (a round is about 200
+// VALU cycles against 96 MFMA cycles per tile per chunk): each round to the
+// wave whose MFMA plus staging cycles are smallest.  At T16 = 17 the heaviest
+// wave (32 tiles) stages nothing and the lightest (5 tiles) a third.
+struct GramRounds {
+    int r0[GW];
+    int nr[GW];
+};
+
+constexpr GramRounds make_gram_rounds(int T16) {
+    GramRounds g{};
+    const GramPlan pl = make_gram_plan(T16);
+    int t[GW] = {};
+    for (int w = 0; w < GW; w++)
+        for (int k = 0; k < GSL; k++)
+            if (pl.a[w * GSL + k] >= 0) t[w] += 96 * gram_blk_cost(T16, pl.a[w * GSL + k], pl.b[w * GSL + k]);
+    for (int r = 0; r < T16 * GKC; r++) {
+        int best = 0;
+        for (int w = 1; w < GW; w++)
+            if (t[w] < t[best]) best = w;
+        t[best] += 200;
+        g.nr[best]++;
+    }
+    for (int w = 1; w < GW; w++) g.r0[w] = g.r0[w - 1] + g.nr[w - 1];
+    return g;
+}
+
 __device__ inline void mfma3(f4 &acc, const h8 &ah, const h8 &al, const h8 &bh, const h8 &bl) {
     acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh, acc, 0, 0, 0);
 }
 
-// one 32-cell step of block (A, B) out of the LDS buffer (xh, xl)
-template <int T16, int A, int B>
+// one 32-cell k-step of block (A, B) out of the LDS buffer (xh, xl), column
+// stride LD halves; lk includes the k-step's offset
+template <int T16, int A, int B, int LD>
 __device__ __forceinline__ void gram_block(f4 (&acc)[4][4], const _Float16 *xh, const _Float16 *xl,
                                            int lrow, int lk) {
     constexpr int NG = (T16 + 3) / 4, RL = T16 - 4 * (NG - 1);
@@ -348,8 +391,8 @@ __device__ __forceinline__ void gram_block(f4 (&acc)[4][4], const _Float16 *xh, 
 #pragma unroll
     for (int x = 0; x < SA; x++) {
         const int col = (4 * A + x) * 16 + lrow;
-        ah[x] = *(const h8 *)(xh + col * LDX + lk);
-        al[x] = *(const h8 *)(xl + col * LDX + lk);
+        ah[x] = *(const h8 *)(xh + col * LD + lk);
+        al[x] = *(const h8 *)(xl + col * LD + lk);
     }
     if constexpr (A == B) {
 #pragma unroll
@@ -360,8 +403,8 @@ __device__ __forceinline__ void gram_block(f4 (&acc)[4][4], const _Float16 *xh, 
 #pragma unroll
         for (int y = 0; y < SB; y++) {
             const int col = (4 * B + y) * 16 + lrow;
-            const h8 bh = *(const h8 *)(xh + col * LDX + lk);
-            const h8 bl = *(const h8 *)(xl + col * LDX + lk);
+            const h8 bh = *(const h8 *)(xh + col * LD + lk);
+            const h8 bl = *(const h8 *)(xl + col * LD + lk);
 #pragma unroll
             for (int x = 0; x < SA; x++) mfma3(acc[x][y], ah[x], al[x], bh, bl);
         }
@@ -391,13 +434,14 @@ struct GramArgs {
 
 // the chunk loop of wave W (its blocks are compile-time constants)
 template <int T16, int W>
-__device__ __forceinline__ void gram_wave(const GramArgs &g, _Float16 *Xs, float (*wsl)[CH]) {
+__device__ __forceinline__ void gram_wave(const GramArgs &g, _Float16 *Xs, float (*wsl)[GCH]) {
     constexpr int P16 = 16 * T16;
     constexpr GramPlan pl = make_gram_plan(T16);
     constexpr int A0 = pl.a[W * GSL], B0 = pl.b[W * GSL];
     constexpr int A1 = pl.a[W * GSL + 1], B1 = pl.b[W * GSL + 1];
-    constexpr int NITEM = P16 * 4;                // (column, 8-cell group) items per chunk
-    constexpr int IPT = (NITEM + GW * 64 - 1) / (GW * 64);
+    constexpr int QC = 4 * GKC;                   // 8-cell groups per chunk
+    constexpr GramRounds rp = make_gram_rounds(T16);
+    constexpr int RB = rp.r0[W], IPT = rp.nr[W];  // this wave's staging rounds
     const int tid = threadIdx.x, lane = tid & 63;
     const int lrow = lane & 15, lk = 8 * (lane >> 4);
 
@@ -413,29 +457,27 @@ __device__ __forceinline__ void gram_wave(const GramArgs &g, _Float16 *Xs, float
     // The prefetch is straight-line code (chunk clamped, no branches around
     // the loads) and is pinned ahead of the MFMAs by a scheduling barrier, so
     // its wait lands at its first use in store(), after this chunk's MFMAs:
-    // the L2/HBM latency hides behind them.  Item round r of wave W holds
-    // items 64 (W + GW r) + lane; NITEM = 64 T16, so a round is wholly inside
-    // or wholly past the end, decided at compile time (the low waves take
-    // the last, partial round; at T16 = 17 that is wave 0 alone, whose plan
-    // has the lightest MFMA load, 5 tiles).
-    f4 pz[IPT][2];
-    auto fetch = [&](int cc) {
+    // the L2/HBM latency hides behind them.  The P16 x QC (column, 8-cell
+    // group) items of a chunk come in rounds of 64; this wave's rounds
+    // (compile time) hold items 64 (RB + r) + lane, r < IPT.  They are fetched
+    // and stored in GKC parts, part s after the MFMAs of k-steps s - 1 and s,
+    // so that only one part is held in registers.
+    constexpr int IPS = (IPT + GKC - 1) / GKC;    // rounds per part
+    f4 pz[IPT > 0 ? IPT : 1][2];
+    auto fetch = [&](int cc, int r0, int r1) {
 #pragma unroll
-        for (int r = 0; r < IPT; r++) {
-            if (W + GW * r < T16) {
-                const int it = tid + GW * 64 * r;
-                const float *src = g.z32 + (size_t)(it >> 2) * g.Npad + cc + 8 * (it & 3);
-                pz[r][0] = *(const f4 *)src;
-                pz[r][1] = *(const f4 *)(src + 4);
-            }
+        for (int r = r0; r < r1; r++) {
+            const int it = 64 * (RB + r) + lane;
+            const float *src = g.z32 + (size_t)(it / QC) * g.Npad + cc + 8 * (it % QC);
+            pz[r][0] = *(const f4 *)src;
+            pz[r][1] = *(const f4 *)(src + 4);
         }
     };
-    auto store = [&](int buf, int wb) {
+    auto store = [&](int buf, int wb, int r0, int r1) {
 #pragma unroll
-        for (int r = 0; r < IPT; r++) {
-            if (!(W + GW * r < T16)) continue;
-            const int it = tid + GW * 64 * r;
-            const int col = it >> 2, q = it & 3;
+        for (int r = r0; r < r1; r++) {
+            const int it = 64 * (RB + r) + lane;
+            const int col = it / QC, q = it % QC;
             const f4 s0 = *(const f4 *)&wsl[wb][8 * q];
             const f4 s1 = *(const f4 *)&wsl[wb][8 * q + 4];
             h8 hi, lo;
@@ -447,41 +489,46 @@ __device__ __forceinline__ void gram_wave(const GramArgs &g, _Float16 *Xs, float
                 hi[e] = h;
                 lo[e] = (_Float16)(v - (float)h);
             }
-            *(h8 *)&Xs[((buf * 2 + 0) * P16 + col) * LDX + 8 * q] = hi;
-            *(h8 *)&Xs[((buf * 2 + 1) * P16 + col) * LDX + 8 * q] = lo;
+            *(h8 *)&Xs[((buf * 2 + 0) * P16 + col) * GLDX + 8 * q] = hi;
+            *(h8 *)&Xs[((buf * 2 + 1) * P16 + col) * GLDX + 8 * q] = lo;
         }
     };
-    auto sqw = [&](int ch) { return sqrtf(g.wj[g.c0 + ch * CH + tid] * g.sc); };   // tid < CH
+    auto sqw = [&](int ch) { return sqrtf(g.wj[g.c0 + ch * GCH + tid] * g.sc); };   // tid < GCH
 
     // prologue: weights of chunks 0 and 1, X of chunk 0
-    if (tid < CH) {
+    if (tid < GCH) {
         wsl[0][tid] = sqw(0);
         if (g.nch > 1) wsl[1][tid] = sqw(1);
     }
-    fetch(g.c0);
+    fetch(g.c0, 0, IPT);
     __syncthreads();
-    store(0, 0);
+    store(0, 0, 0, IPT);
     __syncthreads();
 
     for (int ch = 0; ch < g.nch; ch++) {
         const int buf = ch & 1;
+        const int cn = g.c0 + min(ch + 1, g.nch - 1) * GCH;
         float pw = 0.0f;
-        if constexpr (W == 0) pw = g.wj[g.c0 + min(ch + 2, g.nch - 1) * CH + (lane & (CH - 1))];
-        fetch(g.c0 + min(ch + 1, g.nch - 1) * CH);
+        if constexpr (W == 0) pw = g.wj[g.c0 + min(ch + 2, g.nch - 1) * GCH + (lane & (GCH - 1))];
+        fetch(cn, 0, min(IPS, IPT));
         __builtin_amdgcn_sched_barrier(0);   // issue the prefetch ahead of this chunk's MFMAs
-        const _Float16 *xh = Xs + (size_t)(buf * 2 + 0) * P16 * LDX;
-        const _Float16 *xl = Xs + (size_t)(buf * 2 + 1) * P16 * LDX;
-        if constexpr (A0 >= 0) gram_block<T16, A0, B0>(acc0, xh, xl, lrow, lk);
-        if constexpr (A1 >= 0) gram_block<T16, A1, B1>(acc1, xh, xl, lrow, lk);
-        // unconditional, so the prefetch's only use is in this block and it is
-        // not sunk into a branch (the last chunk's extra store is never read)
-        store(buf ^ 1, (ch + 1) % 3);
-        // wave 0 writes the weights two chunks ahead (lanes 32-63 repeat lanes
-        // 0-31); past the last chunk the slot holds the last chunk's weights
-        // and is never read.  Kept after the MFMAs so that its wait does not
-        // stall them.
-        __builtin_amdgcn_sched_barrier(0);
-        if constexpr (W == 0) wsl[(ch + 2) % 3][lane & (CH - 1)] = sqrtf(pw * g.sc);
+        const _Float16 *xh = Xs + (size_t)(buf * 2 + 0) * P16 * GLDX;
+        const _Float16 *xl = Xs + (size_t)(buf * 2 + 1) * P16 * GLDX;
+#pragma unroll
+        for (int s = 0; s < GKC; s++) {
+            if constexpr (A0 >= 0) gram_block<T16, A0, B0, GLDX>(acc0, xh, xl, lrow, lk + 32 * s);
+            if constexpr (A1 >= 0) gram_block<T16, A1, B1, GLDX>(acc1, xh, xl, lrow, lk + 32 * s);
+            // unconditional, so the prefetch's only use is in this block and it
+            // is not sunk into a branch (the last chunk's extra store is never
+            // read); then the next part's loads, ahead of the next k-step
+            store(buf ^ 1, (ch + 1) % 3, min(s * IPS, IPT), min((s + 1) * IPS, IPT));
+            if (s + 1 < GKC) fetch(cn, min((s + 1) * IPS, IPT), min((s + 2) * IPS, IPT));
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // wave 0 writes the weights two chunks ahead; past the last chunk the
+        // slot holds the last chunk's weights and is never read.  Kept after
+        // the MFMAs so that its wait does not stall them.
+        if constexpr (W == 0) wsl[(ch + 2) % 3][lane & (GCH - 1)] = sqrtf(pw * g.sc);
         __syncthreads();
     }
     if constexpr (A0 >= 0) gram_block_store<T16, A0, B0>(acc0, g.pg, lane);
@@ -493,8 +540,8 @@ __global__ void __launch_bounds__(GW * 64) k_gram_sym(int Npad, int seg, int nse
         const float *__restrict__ z32, const float *__restrict__ w32,
         const float *__restrict__ wsc, float *__restrict__ part) {
     constexpr int NT = T16 * (T16 + 1) / 2;       // upper tiles
-    extern __shared__ __attribute__((aligned(16))) _Float16 Xs[];   // [2 buf][2 hl][P16][LDX]
-    __shared__ __attribute__((aligned(16))) float wsl[3][CH];
+    extern __shared__ __attribute__((aligned(16))) _Float16 Xs[];   // [2 buf][2 hl][P16][GLDX]
+    __shared__ __attribute__((aligned(16))) float wsl[3][GCH];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     // workgroup -> (segment, gene); the 8 XCDs take workgroups round robin,
     // so the workgroups resident on one XCD share cell segments in its L2
@@ -513,7 +560,7 @@ __global__ void __launch_bounds__(GW * 64) k_gram_sym(int Npad, int seg, int nse
     GramArgs g;
     g.Npad = Npad;
     g.c0 = sgi * seg;
-    g.nch = (min(Npad, g.c0 + seg) - g.c0) / CH;
+    g.nch = (min(Npad, g.c0 + seg) - g.c0) / GCH;
     g.sc = wsc[j];
     g.z32 = z32;
     g.wj = w32 + (size_t)j * Npad;
@@ -545,12 +592,6 @@ struct GramPlan4 {
     int a[GW4][MAXB4];
     int b[GW4][MAXB4];
 };
-
-constexpr int gram_blk_cost(int T16, int a, int b) {
-    const int NG = (T16 + 3) / 4, RL = T16 - 4 * (NG - 1);
-    const int sa = a < NG - 1 ? 4 : RL, sb = b < NG - 1 ? 4 : RL;
-    return a == b ? sa * (sa + 1) / 2 : sa * sb;
-}
 
 constexpr GramPlan4 make_gram_plan4(int T16) {
     GramPlan4 p{};
@@ -846,14 +887,15 @@ template <int T16>
 static int launch_gram_sym(gp_ctx *c, int seg, int nseg, int batch) {
     constexpr int P16 = 16 * T16;
     static_assert(plan_fits(T16), "block plan does not fit the waves");
-    const size_t shm = (size_t)2 * 2 * P16 * LDX * sizeof(_Float16);
     if (c->gram_waves == 4) {
+        const size_t shm = (size_t)2 * 2 * P16 * LDX * sizeof(_Float16);
         if (shm > 65536 && !c->capturing)   // set before any recording: not a stream operation
             GP_HIP(hipFuncSetAttribute((const void *)k_gram_sym4<T16>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         k_gram_sym4<T16><<<(unsigned)(batch * nseg), GW4 * 64, shm, c->stream>>>(
             c->Npad, seg, nseg, batch, c->z32, c->w32, c->wsc, c->gram_part);
     } else {
+        const size_t shm = (size_t)2 * 2 * P16 * GLDX * sizeof(_Float16);
         if (shm > 65536 && !c->capturing)
             GP_HIP(hipFuncSetAttribute((const void *)k_gram_sym<T16>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
@@ -872,9 +914,10 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
     int nseg = c->gram_splits > 0 ? c->gram_splits : 1;
     GP_TRY(ensure_mixed(c, nseg));
     if (!c->zsplit_valid) GP_TRY(gpk_split_z(c, Zsrc));
-    // segment length: multiple of the chunk, covering Npad
+    // segment length: multiple of the chunk of either symmetric form (GCH is a
+    // multiple of CH; Npad is a multiple of 256), covering Npad
     int seg = (c->Npad + nseg - 1) / nseg;
-    seg = (seg + CH - 1) / CH * CH;
+    seg = (seg + GCH - 1) / GCH * GCH;
     nseg = (c->Npad + seg - 1) / seg;
     k_wprep<<<batch, 256, 0, c->stream>>>(c->N, c->Npad, W, c->Omega, c->w32, c->wsc, c->wun);
     GP_HIP(hipGetLastError());
