@@ -519,7 +519,6 @@ extern "C" int gp_ql_start(gp_ctx *c, double *obj) {
     c->lkhd_fresh = 0;   // set again by the likelihood evaluation below
     GP_TRY(clear_info(c));
     GP_TRY(gpk_oinvt(c));
-    GP_TRY(gpk_chol_batched(c, c->DeltaChol, c->P, 1, 0, c->info + 2));
     GP_TRY(lkhd_genes(c));
     GP_TRY(gpk_collect(c, c->N, 0, obj));
     return 0;
@@ -535,7 +534,6 @@ extern "C" int gp_elbo(gp_ctx *c, int slot, double *obj) {
     GP_TRY(kernel_mats(c));
     GP_TRY(gpk_expand_delta(c));
     GP_TRY(gram_chol(c, c->precision));
-    GP_TRY(gpk_chol_batched(c, c->DeltaChol, c->P, 1, 0, c->info + 2));
     GP_TRY(lkhd_genes(c));
     GP_TRY(gpk_collect(c, c->N, slot, obj));
     gp_timer_end(c, c->t_elbo);

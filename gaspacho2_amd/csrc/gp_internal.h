@@ -163,7 +163,7 @@ void gp_set_error(const std::string &msg);
 
 // ---------------- launch wrappers (gp_kernels.hip) ----------------
 int gpk_kuf(gp_ctx *c);                       // Knm into Z, K, Ksym, Kchol
-int gpk_expand_delta(gp_ctx *c);              // DeltaInv + DeltaChol copy
+int gpk_expand_delta(gp_ctx *c);              // DeltaInv and its factor DeltaChol (from Kchol)
 int gpk_oinvt(gp_ctx *c);
 int gpk_titsias(gp_ctx *c);                   // A = Knm Kinv (rocBLAS) + titsias
 int gpk_w_nu(gp_ctx *c);                      // IRLS W / Nu from ZB = Z*Beta

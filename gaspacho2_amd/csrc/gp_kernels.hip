@@ -104,9 +104,13 @@ __global__ void k_kmm(int M, int Q, const double *T, const double *rho, int ard,
 }
 
 // ---------------------------------------------------------------- DeltaInv
-// util_cuda.cu:249-269 expandDelta_k; DeltaChol gets the full symmetric copy.
+// util_cuda.cu:249-269 expandDelta_k.  DeltaChol is the factor choldc would
+// leave for DeltaInv (chol.cu:19-43: L on and below the diagonal, the matrix
+// above it): DeltaInv = blockdiag(diag(1/delta_l), K/delta_{p-1}), so its
+// factor is blockdiag(diag(sqrt(1/delta_l)), Kchol/sqrt(delta_{p-1})) with
+// Kchol the factor of K -- formed here instead of factoring again.
 __global__ void k_expand_delta(int P, int p, int M, int g0, const int *nh,
-                               const double *delta, const double *K,
+                               const double *delta, const double *K, const double *Kchol,
                                double *DeltaInv, double *DeltaChol, double *DeltaW,
                                double *DeltaWChol) {
     const size_t idx = (size_t)blockIdx.x * NT + threadIdx.x;
@@ -122,7 +126,12 @@ __global__ void k_expand_delta(int P, int p, int M, int g0, const int *nh,
         sym = K[(lo - g0) + (size_t)(hi - g0) * M] / delta[p - 1];
     }
     DeltaInv[idx] = (r <= c) ? sym : 0.0;
-    DeltaChol[idx] = sym;
+    double fac = sym;                                  // above the diagonal: the matrix
+    if (r >= c) {                                      // on and below: the factor
+        if (c >= g0) fac = Kchol[(r - g0) + (size_t)(c - g0) * M] / sqrt(delta[p - 1]);
+        else fac = (r == c) ? sqrt(sym) : 0.0;
+    }
+    DeltaChol[idx] = fac;
     if (DeltaW) {
         // whitened prior: diag(1/delta_l) (the GP block K/delta becomes I/delta)
         double dw = 0.0;
@@ -729,8 +738,8 @@ int gpk_kuf(gp_ctx *c) {
 
 int gpk_expand_delta(gp_ctx *c) {
     k_expand_delta<<<nblk((size_t)c->P * c->P), NT, 0, c->stream>>>(
-        c->P, c->p, c->M, c->g0, c->d_nh, c->delta, c->K, c->DeltaInv, c->DeltaChol, c->DeltaW,
-        c->DeltaWChol);
+        c->P, c->p, c->M, c->g0, c->d_nh, c->delta, c->K, c->Kchol, c->DeltaInv, c->DeltaChol,
+        c->DeltaW, c->DeltaWChol);
     LAUNCH_CHECK();
     return 0;
 }
