@@ -56,7 +56,7 @@ static void free_all(gp_ctx *c) {
                     c->DeltaInv, c->DeltaChol, c->W, c->Oinvt, c->Nu, c->Psi, c->Sinv, c->Beta,
                     c->tBeta, c->bvec, c->tBphi, c->lkhds, c->gq, c->gs, c->grad, c->grad0, c->ss,
                     c->SS, c->XX, c->lbq, c->gradDeltaTmp, c->gradLogRhoTmp, c->C1, c->C2, c->U,
-                    c->ZB, c->R, c->C1a, c->AO, c->G2, c->B2, c->S2, c->Ecell, c->red, c->scal,
+                    c->ZB, c->R, c->C1a, c->AO, c->G2, c->B2, c->S2, c->Ecell, c->Epart, c->ci1, c->red, c->scal,
                     c->lkhd, c->info, c->Zh, c->Zl, c->z32, c->gram_part, c->pwork, c->w32, c->wsc,
                     c->wun, c->zun, c->Linv, c->Zw, c->DeltaW, c->DeltaWChol, c->Uw, c->tBw,
                     c->Vtmp, c->Sh, c->Sl, c->gc_perm, c->gc_aexp, c->gc_sexp, c->gc_bexp,
@@ -173,6 +173,8 @@ extern "C" int gp_create(gp_ctx **out, int N, int J, int M, int Q, int p, const 
     ALLOC(c->B2, (size_t)M * M);
     ALLOC(c->S2, (size_t)M * M);
     ALLOC(c->Ecell, (size_t)N);
+    ALLOC(c->Epart, (size_t)GP_TN_SPLITS * Np);
+    ALLOC(c->ci1, (size_t)N);
     ALLOC(c->red, 4096);
     ALLOC(c->scal, 64);
     ALLOC(c->lkhd, GP_LKHD_SLOTS);

@@ -21,6 +21,8 @@ struct gp_timer {
     bool open = false;
 };
 
+constexpr int GP_TN_SPLITS = 4;   // gene ranges of the per-cell tn / E kernel
+
 struct gp_ctx {
     int dev = 0;
     hipStream_t stream = nullptr;
@@ -86,6 +88,8 @@ struct gp_ctx {
     double *AO = nullptr;      // Npad x M  diag(Oinvt) A
     double *G2 = nullptr, *B2 = nullptr, *S2 = nullptr;  // M x M pieces of C2
     double *Ecell = nullptr;   // N
+    double *Epart = nullptr;   // GP_TN_SPLITS x Npad partial E sums of k_tn
+    double *ci1 = nullptr;     // N: per-cell row sums of C1 (latent gradient)
     double *red = nullptr;     // reduction scratch
     double *scal = nullptr;    // device scalars (L-BFGS alphas etc.)
     double *lkhd = nullptr;    // GP_LKHD_SLOTS

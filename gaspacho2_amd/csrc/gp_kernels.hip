@@ -153,7 +153,15 @@ __global__ void k_oinvt(int N, int Npad, int J, const double *W, const double *O
     if (i >= N) return;
     const double om = Omega[i];
     double o = 0.0;
-    for (int j = 0; j < J; j++) o += W[i + (size_t)j * Npad] / om;
+    int j = 0;
+    for (; j + 8 <= J; j += 8) {          // 8 loads in flight, same summation order
+        double w[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) w[u] = W[i + (size_t)(j + u) * Npad];
+#pragma unroll
+        for (int u = 0; u < 8; u++) o += w[u] / om;
+    }
+    for (; j < J; j++) o += W[i + (size_t)j * Npad] / om;
     Oinvt[i] = o;
 }
 
@@ -393,45 +401,70 @@ __global__ void __launch_bounds__(NT) k_grad_delta_gene(int P, int p, int M, int
     }
 }
 
-// collectGradLogDelta ggplvm.cu:256-281
-__global__ void __launch_bounds__(NT) k_grad_delta_collect(int N, int J, int P, int p,
+// collectGradLogDelta ggplvm.cu:256-281: group l of delta sums the per-gene
+// terms of its columns and, for the GP group, the Titsias term over cells.
+// Block (l, sp) sums genes [J sp / GDS, J (sp + 1) / GDS) and cells
+// [N sp / GDS, N (sp + 1) / GDS) into part[l GDS + sp]; k_grad_delta_sum adds
+// the GDS parts in order.
+constexpr int GDS = 16;
+__global__ void __launch_bounds__(NT) k_grad_delta_part(int N, int J, int P, int p,
         const int *nh, const double *gdt, const double *Oinvt, const double *tits,
-        const double *delta, double wt, double *gd) {
+        const double *delta, double wt, double *part) {
     __shared__ double sh[4];
-    const int l = blockIdx.x;
-    const int c0 = nh[l], c1 = nh[l + 1], w = c1 - c0;
+    const int l = blockIdx.x, sp = blockIdx.y;
+    const int c0 = nh[l], c1 = nh[l + 1];
+    const int j0 = (int)((long long)J * sp / GDS), j1 = (int)((long long)J * (sp + 1) / GDS);
     double s = 0.0;
-    for (size_t t = threadIdx.x; t < (size_t)J * w; t += NT) {
-        const int j = (int)(t / w), k = c0 + (int)(t % w);
-        s += gdt[k + (size_t)j * P];
+    for (int j = j0; j < j1; j++)
+        for (int k = c0 + threadIdx.x; k < c1; k += NT) s += gdt[k + (size_t)j * P];
+    if (l == p - 1) {
+        const int i0 = (int)((long long)N * sp / GDS), i1 = (int)((long long)N * (sp + 1) / GDS);
+        for (int i = i0 + threadIdx.x; i < i1; i += NT) s -= wt * Oinvt[i] * tits[i] * delta[p - 1];
     }
-    if (l == p - 1)
-        for (int i = threadIdx.x; i < N; i += NT) s -= wt * Oinvt[i] * tits[i] * delta[p - 1];
     s = block_sum256(s, sh);
-    if (threadIdx.x == 0) gd[l] = s;
+    if (threadIdx.x == 0) part[l * GDS + sp] = s;
+}
+__global__ void k_grad_delta_sum(int p, const double *part, double *gd) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= p) return;
+    double s = 0.0;
+    for (int sp = 0; sp < GDS; sp++) s += part[l * GDS + sp];
+    gd[l] = s;
 }
 
 // ---------------------------------------------------------------- tn / E
 // ggplvm.cu:335-345 per-cell pieces: tn_ij = o nu - o z^T tBeta_j,
-// E_i = sum_j tn^2 / W / phi / 2, TN_ij = tn_ij / phi_j.
+// E_i = sum_j tn^2 / W / phi / 2, TN_ij = tn_ij / phi_j.  The genes are cut
+// into TN_SPLITS ranges (blockIdx.y) for occupancy; Epart holds the partial
+// sums of E, added in order by k_tn_sum.
+constexpr int TN_SPLITS = GP_TN_SPLITS;
 __global__ void k_tn(int N, int Npad, int J, const double *W, const double *Nu,
                      const double *ZB, const double *Omega, const double *phi,
-                     double *TN, double *E) {
+                     double *TN, double *Epart) {
     const int i = blockIdx.x * NT + threadIdx.x;
+    const int j0 = (int)((long long)J * blockIdx.y / TN_SPLITS);
+    const int j1 = (int)((long long)J * (blockIdx.y + 1) / TN_SPLITS);
     if (i >= Npad) return;
     if (i >= N) {
-        for (int j = 0; j < J; j++) TN[i + (size_t)j * Npad] = 0.0;
+        for (int j = j0; j < j1; j++) TN[i + (size_t)j * Npad] = 0.0;
         return;
     }
     const double om = Omega[i];
     double e = 0.0;
-    for (int j = 0; j < J; j++) {
+    for (int j = j0; j < j1; j++) {
         const size_t a = i + (size_t)j * Npad;
         const double w = W[a], o = w / om;
         const double tn = o * Nu[a] - o * ZB[a];
         e += tn * tn / w / phi[j] / 2.0;
         TN[a] = tn / phi[j];
     }
+    Epart[(size_t)blockIdx.y * Npad + i] = e;
+}
+__global__ void k_tn_sum(int N, int Npad, const double *Epart, double *E) {
+    const int i = blockIdx.x * NT + threadIdx.x;
+    if (i >= N) return;
+    double e = 0.0;
+    for (int s = 0; s < TN_SPLITS; s++) e += Epart[(size_t)s * Npad + i];
     E[i] = e;
 }
 
@@ -441,7 +474,7 @@ __global__ void k_cell_grad(int N, int Npad, int J, int M, int Q, int P, int g0,
         const double *Z, const double *U, const double *C1a, const double *A,
         const double *Omega, const double *Oinvt, const double *tits, const double *E,
         const double *X, const double *T, const double *rho, const double *__restrict__ dtp,
-        double wt, double igb, double igap1, double *gO, double *gX, double *C1) {
+        double wt, double igb, double igap1, double *gO, double *gX, double *C1, double *ci1out) {
     const int i = blockIdx.x * NT + threadIdx.x;
     if (i >= Npad) return;
     const double dt = *dtp;
@@ -469,13 +502,18 @@ __global__ void k_cell_grad(int N, int Npad, int J, int M, int Q, int P, int g0,
             gx0 -= c * sin(t) * cos(t) / rho[0];
         }
     }
-    int q0 = 0;
-    if (!ard) { gX[i] = gx0; q0 = 1; }
-    for (int q = q0; q < Q; q++) {
-        double s = 0.0;
-        for (int k = 0; k < M; k++) s += C1[i + (size_t)k * Npad] * T[k + (size_t)q * M];
-        gX[i + (size_t)q * N] = (s - ci1 * X[i + (size_t)q * N]) / rho[q];
-    }
+    if (!ard) gX[i] = gx0;
+    // the other columns: gX[:, q] = (C1 T[:, q] - ci1 X[:, q]) / rho_q, the
+    // product on rocBLAS and the rest in k_gx_finish
+    ci1out[i] = ci1;
+}
+__global__ void k_gx_finish(int N, int q0, const double *ci1, const double *X, const double *rho,
+                            double *gX) {
+    const int i = blockIdx.x * NT + threadIdx.x;
+    const int q = q0 + blockIdx.y;
+    if (i >= N) return;
+    const size_t a = i + (size_t)q * N;
+    gX[a] = (gX[a] - ci1[i] * X[a]) / rho[q];
 }
 
 // ---------------------------------------------------------------- C2
@@ -496,8 +534,18 @@ __global__ void k_sum_sinv(int J, int P, int M, int g0, const double *Sinv, doub
     const size_t idx = (size_t)blockIdx.x * NT + threadIdx.x;
     if (idx >= (size_t)M * M) return;
     const int a = (int)(idx % M), b = (int)(idx / M);
+    const double *src = Sinv + (g0 + a) + (size_t)(g0 + b) * P;
+    const size_t PP = (size_t)P * P;
     double s = 0.0;
-    for (int j = 0; j < J; j++) s += Sinv[(size_t)j * P * P + (g0 + a) + (size_t)(g0 + b) * P];
+    int j = 0;
+    for (; j + 8 <= J; j += 8) {          // 8 loads in flight, same order
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = src[(size_t)(j + u) * PP];
+#pragma unroll
+        for (int u = 0; u < 8; u++) s += v[u];
+    }
+    for (; j < J; j++) s += src[(size_t)j * PP];
     S2[idx] = s;
 }
 // ggplvm.cu:57-100 getC2
@@ -520,14 +568,14 @@ __global__ void k_c2(int M, int J, double wt, const double *__restrict__ dtp, co
 }
 
 // ---------------------------------------------------------------- getGradTau / rho
-// ggplvm.cu:103-145 and 165-205: one block per inducing point m.
+// ggplvm.cu:103-145 and 165-205: one block per (inducing point m, dimension q).
 __global__ void __launch_bounds__(NT) k_grad_tau_rho(int N, int Npad, int M, int Q, int ard,
         const double *C1, const double *C2, const double *X, const double *T,
         const double *rho, double *gT, double *grt) {
     __shared__ double sh[4];
-    const int m = blockIdx.x;
+    const int m = blockIdx.x, q = blockIdx.y;   // one block per (inducing point, dimension)
     const double *c1 = C1 + (size_t)m * Npad;
-    for (int q = 0; q < Q; q++) {
+    {
         const double tm = T[m + (size_t)q * M];
         const double rq = rho[q];
         double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, r1 = 0.0, r2 = 0.0;
@@ -909,9 +957,14 @@ int gpk_grad_delta(gp_ctx *c, int whitened) {
                                                   c->Ksym, c->Sinv, whitened ? c->tBw : c->tBeta,
                                                   c->phi, c->gradDeltaTmp, whitened);
     LAUNCH_CHECK();
-    k_grad_delta_collect<<<c->p, NT, 0, c->stream>>>(c->N, c->J, c->P, c->p, c->d_nh,
-                                                     c->gradDeltaTmp, c->Oinvt, c->titsias,
-                                                     c->delta, 0.1, c->grad + c->N);
+    if ((size_t)c->p * GDS > 4096) { gp_set_error("too many delta groups"); return 1; }
+    // c->red is free here (the likelihood collection and the L-BFGS dots use
+    // it within their own calls)
+    k_grad_delta_part<<<dim3(c->p, GDS), NT, 0, c->stream>>>(c->N, c->J, c->P, c->p, c->d_nh,
+                                                             c->gradDeltaTmp, c->Oinvt, c->titsias,
+                                                             c->delta, 0.1, c->red);
+    LAUNCH_CHECK();
+    k_grad_delta_sum<<<(c->p + 63) / 64, 64, 0, c->stream>>>(c->p, c->red, c->grad + c->N);
     LAUNCH_CHECK();
     return 0;
 }
@@ -921,8 +974,11 @@ int gpk_tn(gp_ctx *c) {
     const double one = 1.0, zero = 0.0;
     GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, c->Npad,
                           c->J, c->P, &one, c->Z, c->Npad, c->tBeta, c->P, &zero, c->ZB, c->Npad));
-    k_tn<<<nblk(c->Npad), NT, 0, c->stream>>>(c->N, c->Npad, c->J, c->W, c->Nu, c->ZB, c->Omega,
-                                              c->phi, c->R, c->Ecell);
+    k_tn<<<dim3(nblk(c->Npad), TN_SPLITS), NT, 0, c->stream>>>(c->N, c->Npad, c->J, c->W, c->Nu,
+                                                                c->ZB, c->Omega, c->phi, c->R,
+                                                                c->Epart);
+    LAUNCH_CHECK();
+    k_tn_sum<<<nblk(c->N), NT, 0, c->stream>>>(c->N, c->Npad, c->Epart, c->Ecell);
     LAUNCH_CHECK();
     // C1a = (TN / phi) * tBeta_GP^T
     GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, c->Npad,
@@ -936,8 +992,19 @@ int gpk_cell_grad(gp_ctx *c) {
         c->N, c->Npad, c->J, c->M, c->Q, c->P, c->g0, c->ard_only, c->Z, c->U, c->C1a, c->A,
         c->Omega, c->Oinvt, c->titsias, c->Ecell, c->X, c->T, c->rho, c->delta + c->p - 1, 0.1,
         100.0, 100.0,
-        c->grad, c->grad + c->N + c->p, c->C1);
+        c->grad, c->grad + c->N + c->p, c->C1, c->ci1);
     LAUNCH_CHECK();
+    const int q0 = c->ard_only ? 0 : 1;
+    if (c->Q > q0) {
+        const double one = 1.0, zero = 0.0;
+        double *gX = c->grad + c->N + c->p;
+        GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, c->N,
+                              c->Q - q0, c->M, &one, c->C1, c->Npad, c->T + (size_t)q0 * c->M,
+                              c->M, &zero, gX + (size_t)q0 * c->N, c->N));
+        k_gx_finish<<<dim3(nblk(c->N), c->Q - q0), NT, 0, c->stream>>>(c->N, q0, c->ci1, c->X,
+                                                                        c->rho, gX);
+        LAUNCH_CHECK();
+    }
     return 0;
 }
 
@@ -972,7 +1039,7 @@ int gpk_c2(gp_ctx *c, int whitened) {
 int gpk_grad_tau_rho(gp_ctx *c) {
     double *gT = c->grad + c->N + c->p + (size_t)c->N * c->Q;
     double *gr = gT + (size_t)c->M * c->Q;
-    k_grad_tau_rho<<<c->M, NT, 0, c->stream>>>(c->N, c->Npad, c->M, c->Q, c->ard_only, c->C1,
+    k_grad_tau_rho<<<dim3(c->M, c->Q), NT, 0, c->stream>>>(c->N, c->Npad, c->M, c->Q, c->ard_only, c->C1,
                                                 c->C2, c->X, c->T, c->rho, gT, c->gradLogRhoTmp);
     LAUNCH_CHECK();
     k_collect_rho<<<1, 64, 0, c->stream>>>(c->M, c->Q, c->gradLogRhoTmp, gr);
