@@ -240,7 +240,7 @@ extern "C" int gp_set_option(gp_ctx *c, const char *name, int value) {
         if (value != 4 && value != 8) { gp_set_error("gram_waves must be 4 or 8"); return 1; }
         c->gram_waves = value;
     } else if (n == "gc_form") {
-        if (value < 0 || value > 2) { gp_set_error("gc_form must be 0, 1 or 2"); return 1; }
+        if (value < 0 || value > 3) { gp_set_error("gc_form must be 0, 1, 2 or 3"); return 1; }
         c->gc_form = value;
     } else if (n == "gc_mixed") {
         c->gc_mixed = value ? 1 : 0;

@@ -531,6 +531,174 @@ __global__ void __launch_bounds__(gc_rows_waves(T16) * 64) k_gc_rows2(int N, int
     }
 }
 
+// ---- factored row-block form on one wave per SIMD (gc_form 3): the tiling of
+// form 2 with 4 waves of 512 registers instead of 8 of 256.  Wave w keeps
+// CT = T16 / 4 column tiles of S_j (columns 16 (w CT + i)) for all 8 row
+// tiles, so each A fragment read from LDS feeds 3 CT MFMAs instead of 6
+// (half the LDS reads per MFMA); the T16 mod 4 leftover column tiles are
+// dealt by row tile (r mod 4 == w).  The B fragments of the next k-step
+// are loaded during the current one.  Every accumulator sees form 2's
+// MFMA sequence and update, so the two forms give the same bits.
+constexpr int GW3 = 4;       // waves per workgroup, one per SIMD
+
+template <int T16>
+__global__ void __launch_bounds__(GW3 * 64, 1) k_gc_rows4(int N, int Npad, int P, int gsize, int J,
+        const double *__restrict__ Zw, const float *__restrict__ w32,
+        const int *__restrict__ perm, const int *__restrict__ aexp, const int *__restrict__ zexp,
+        const _Float16 *__restrict__ shi, const _Float16 *__restrict__ slo,
+        float *__restrict__ part) {
+    constexpr int P16 = 16 * T16;
+    constexpr int CT = T16 / GW3, ODD = T16 % GW3;
+    constexpr int NX = ODD * (8 / GW3);           // leftover (row tile, column tile) pairs per wave
+    constexpr int KS = (P16 + 31) / 32;
+    constexpr int LDA = 32 * KS + 8;
+    constexpr int NITEM = GRC * KS * 4;
+    constexpr int IPT = (NITEM + GW3 * 64 - 1) / (GW3 * 64);
+    extern __shared__ __attribute__((aligned(16))) _Float16 As[];   // [2 hl][GRC][LDA]
+    __shared__ float os[2][GRC];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g = blockIdx.y;
+    const int j0 = g * gsize, j1 = min(J, j0 + gsize);
+    const int cb = blockIdx.x * GRC;
+    _Float16 *Ah = As, *Al = As + (size_t)GRC * LDA;
+    // A = z (scaled by 2^-zexp), split hi/lo, once for the group
+#pragma unroll
+    for (int r = 0; r < IPT; r++) {
+        const int it = min(tid + GW3 * 64 * r, NITEM - 1);
+        const int cl = it % GRC, k0 = 8 * (it / GRC);
+        h8 hi, lo;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int k = k0 + e, cell = cb + cl;
+            float v = (k < P && cell < N) ? (float)ldexp(Zw[cell + (size_t)k * Npad], -zexp[k]) : 0.0f;
+            asm volatile("" : "+v"(v));
+            const _Float16 h = (_Float16)v;
+            hi[e] = h;
+            lo[e] = (_Float16)(v - (float)h);
+        }
+        *(h8 *)&Ah[cl * LDA + k0] = hi;
+        *(h8 *)&Al[cl * LDA + k0] = lo;
+    }
+    auto weights = [&](int jj, int buf) {
+        const int j = perm[min(jj, j1 - 1)];
+        const float sc = ldexpf(1.0f, aexp[j]);
+        for (int t = tid; t < GRC; t += GW3 * 64)
+            os[buf][t] = (cb + t < N) ? w32[cb + t + (size_t)j * Npad] * sc : 0.0f;
+    };
+    if (j0 < j1) weights(j0, 0);
+    __syncthreads();
+
+    f4 U[8][CT > 0 ? CT : 1], Ux[NX > 0 ? NX : 1];
+#pragma unroll
+    for (int r = 0; r < 8; r++)
+#pragma unroll
+        for (int i = 0; i < CT; i++) U[r][i] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int x = 0; x < NX; x++) Ux[x] = f4{0.f, 0.f, 0.f, 0.f};
+
+    const int lk = (lane & 15), lq = 8 * (lane >> 4);
+    int bcol[CT > 0 ? CT : 1], bxc[ODD > 0 ? ODD : 1];
+#pragma unroll
+    for (int i = 0; i < CT; i++) bcol[i] = (16 * (w * CT + i) + lk) * KP + lq;
+#pragma unroll
+    for (int e = 0; e < ODD; e++) bxc[e] = (16 * (GW3 * CT + e) + lk) * KP + lq;
+    const int arow = lk * LDA + lq;
+    const int orow = 4 * (lane >> 4);
+
+    for (int jj = j0; jj < j1; jj++) {
+        const int j = perm[jj];
+        const int buf = (jj - j0) & 1;
+        weights(jj + 1, buf ^ 1);                 // the next gene's weights (read after this gene's barrier)
+        const _Float16 *hj = shi + (size_t)j * P16 * KP, *lj = slo + (size_t)j * P16 * KP;
+        h8 bh[CT > 0 ? CT : 1], bl[CT > 0 ? CT : 1], xh[ODD > 0 ? ODD : 1], xl[ODD > 0 ? ODD : 1];
+#pragma unroll
+        for (int i = 0; i < CT; i++) { bh[i] = *(const h8 *)(hj + bcol[i]); bl[i] = *(const h8 *)(lj + bcol[i]); }
+#pragma unroll
+        for (int e = 0; e < ODD; e++) { xh[e] = *(const h8 *)(hj + bxc[e]); xl[e] = *(const h8 *)(lj + bxc[e]); }
+        f4 acc[8][CT > 0 ? CT : 1], accx[NX > 0 ? NX : 1];
+#pragma unroll
+        for (int r = 0; r < 8; r++)
+#pragma unroll
+            for (int i = 0; i < CT; i++) acc[r][i] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int x = 0; x < NX; x++) accx[x] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            const int kn = 32 * min(ks + 1, KS - 1);
+            h8 nbh[CT > 0 ? CT : 1], nbl[CT > 0 ? CT : 1], nxh[ODD > 0 ? ODD : 1], nxl[ODD > 0 ? ODD : 1];
+#pragma unroll
+            for (int i = 0; i < CT; i++) {
+                nbh[i] = *(const h8 *)(hj + bcol[i] + kn);
+                nbl[i] = *(const h8 *)(lj + bcol[i] + kn);
+            }
+#pragma unroll
+            for (int e = 0; e < ODD; e++) {
+                nxh[e] = *(const h8 *)(hj + bxc[e] + kn);
+                nxl[e] = *(const h8 *)(lj + bxc[e] + kn);
+            }
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const h8 ah = *(const h8 *)(Ah + 16 * r * LDA + arow + 32 * ks);
+                const h8 al = *(const h8 *)(Al + 16 * r * LDA + arow + 32 * ks);
+#pragma unroll
+                for (int i = 0; i < CT; i++) {
+                    acc[r][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[i], acc[r][i], 0, 0, 0);
+                    acc[r][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[i], acc[r][i], 0, 0, 0);
+                    acc[r][i] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[i], acc[r][i], 0, 0, 0);
+                }
+                if (r % GW3 == w) {
+#pragma unroll
+                    for (int e = 0; e < ODD; e++) {
+                        f4 &ax = accx[e * (8 / GW3) + r / GW3];
+                        ax = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, xh[e], ax, 0, 0, 0);
+                        ax = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, xl[e], ax, 0, 0, 0);
+                        ax = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, xh[e], ax, 0, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < CT; i++) { bh[i] = nbh[i]; bl[i] = nbl[i]; }
+#pragma unroll
+            for (int e = 0; e < ODD; e++) { xh[e] = nxh[e]; xl[e] = nxl[e]; }
+        }
+        // U += o (rows) * C
+#pragma unroll
+        for (int r = 0; r < 8; r++) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const float o = os[buf][16 * r + orow + q];
+#pragma unroll
+                for (int i = 0; i < CT; i++) U[r][i][q] += o * acc[r][i][q];
+                if (r % GW3 == w) {
+#pragma unroll
+                    for (int e = 0; e < ODD; e++) {
+                        const int x = e * (8 / GW3) + r / GW3;
+                        Ux[x][q] += o * accx[x][q];
+                    }
+                }
+            }
+        }
+        __syncthreads();                          // os[buf ^ 1] written; os[buf] free
+    }
+    float *pg = part + (size_t)g * P16 * Npad;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const int row0 = cb + 16 * r + orow;
+#pragma unroll
+        for (int i = 0; i < CT; i++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) pg[(size_t)(16 * (w * CT + i) + lk) * Npad + row0 + q] = U[r][i][q];
+        if (r % GW3 == w) {
+#pragma unroll
+            for (int e = 0; e < ODD; e++)
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    pg[(size_t)(16 * (GW3 * CT + e) + lk) * Npad + row0 + q] = Ux[e * (8 / GW3) + r / GW3][q];
+        }
+    }
+}
+
 __global__ void k_gc_reduce(int Npad, int P, int P16, int ngroup, const float *__restrict__ part,
                             const double *__restrict__ gun, double *__restrict__ Uw) {
     const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
@@ -578,7 +746,24 @@ static int launch_gc_rows2(gp_ctx *c, int gsize, int ngroup) {
 }
 
 template <int T16>
+static int launch_gc_rows4(gp_ctx *c, int gsize, int ngroup) {
+    constexpr int P16 = 16 * T16, KS = (P16 + 31) / 32;
+    const size_t shm = (size_t)2 * GRC * (32 * KS + 8) * sizeof(_Float16);
+    if (shm > 65536 && !c->capturing)   // set before any recording: not a stream operation
+        GP_HIP(hipFuncSetAttribute((const void *)k_gc_rows4<T16>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    dim3 g(c->Npad / GRC, ngroup);
+    k_gc_rows4<T16><<<g, GW3 * 64, shm, c->stream>>>(c->N, c->Npad, c->P, gsize, c->J, c->Zw, c->w32,
+                                                  c->gc_perm, c->gc_aexp, c->gc_zexp,
+                                                  (const _Float16 *)c->Sh, (const _Float16 *)c->Sl,
+                                                  c->gc_part);
+    GP_HIP(hipGetLastError());
+    return 0;
+}
+
+template <int T16>
 static int launch_gc(gp_ctx *c, int gsize, int ngroup) {
+    if (c->gc_form == 3) return launch_gc_rows4<T16>(c, gsize, ngroup);
     if (c->gc_form == 2) return launch_gc_rows2<T16>(c, gsize, ngroup);
     return c->gc_form == 1 ? launch_gc_rows<T16>(c, gsize, ngroup) : launch_gc_mixed<T16>(c, gsize, ngroup);
 }

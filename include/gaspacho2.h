@@ -80,12 +80,13 @@ int gp_set_precision(gp_ctx *ctx, int precision);
  *                           shared across its blocks, one wave per SIMD)
  *   "gc_groups"   [1, 4096] gene groups of the mixed gradient contraction
  *                           (FP32 partial per group, FP64 sum across groups)
- *   "gc_form"     0/1/2     mixed contraction: 0 = 16-cell waves with S_j
+ *   "gc_form"     0/1/2/3   mixed contraction: 0 = 16-cell waves with S_j
  *                           staged in LDS; 1 = 128-cell row blocks with
  *                           each wave's S_j columns in registers (same bits
  *                           as 0); 2 = row blocks with the cell weights
  *                           applied to each gene's FP32 product, so A is
- *                           staged once per gene group
+ *                           staged once per gene group; 3 = form 2 on four
+ *                           waves of 512 registers (same bits as 2)
  *   "gc_mixed"    0/1       mixed path: contraction on FP16 hi/lo MFMA (1)
  *                           or in FP64 (0)
  *   "irls_mixed"  0/1       mixed path: IRLS steps on the whitened FP16

@@ -329,6 +329,10 @@ def test_options_reject_bad_values(tiny_ds):
     with pytest.raises(lib.GPError):
         g.set_option("gram_waves", 6)
     assert g.get_option("gram_waves") == 8
+    with pytest.raises(lib.GPError):
+        g.set_option("gc_form", 4)
+    g.set_option("gc_form", 3)
+    assert g.get_option("gc_form") == 3
 
 
 @pytest.mark.parametrize("shape", [(2000, 40, 5, 32), (3000, 24, 10, 256), (3000, 24, 10, 240)])
@@ -363,8 +367,10 @@ def test_contraction_row_block_form_matches_16_cell_form(shape):
     """The 128-cell row-block contraction (gc_form 1) feeds every accumulator
     the same MFMA sequence as the 16-cell form (gc_form 0): the gradient
     agrees to the last bit; the factored form (gc_form 2) within 1e-6 on the
-    contraction Uw and 1e-5 on the gradient; P = 33, 257, 241, 225 and 9 (odd
-    and even column-tile counts, N not a multiple of 128)."""
+    contraction Uw and 1e-5 on the gradient, and its one-wave-per-SIMD tiling
+    (gc_form 3) to the last bit of form 2; P = 33, 257, 241, 225 and 9
+    (column-tile counts 3, 17, 16, 15 and 1: every remainder mod 4, N not a
+    multiple of 128)."""
     from gaspacho2_amd import lib, synth
     N, G, Q, M = shape
     ds = synth.make_dataset(N, G, Q, M, seed=6)
@@ -376,7 +382,7 @@ def test_contraction_row_block_form_matches_16_cell_form(shape):
     g.ql_start()
     g.set_precision(lib.PREC_MIXED)
     out = {}
-    for form in (0, 1, 2):
+    for form in (0, 1, 2, 3):
         g.set_option("gc_form", form)
         g.elbo(1)
         g.grad()
@@ -387,6 +393,8 @@ def test_contraction_row_block_form_matches_16_cell_form(shape):
     ru, rg = rel(out[2][2], out[0][2]), rel(out[2][0], out[0][0])
     print(f"gc_form 2 vs 0: Uw rel err {ru:.2e}, gradient rel err {rg:.2e}")
     assert ru < 1e-6 and rg < 1e-5
+    assert np.array_equal(out[3][2], out[2][2])
+    assert np.array_equal(out[3][0], out[2][0])
     g.close()
 
 
@@ -413,7 +421,7 @@ def test_autotune_picks_valid_settings_and_keeps_results():
     g.set_precision(lib.PREC_MIXED)
     choice = g.autotune()
     assert choice["gram_splits"] in (8, 16, 32) and choice["gc_groups"] in (16, 32, 64)
-    assert choice["gram_waves"] in (4, 8) and choice["gc_form"] in (0, 1, 2)
+    assert choice["gram_waves"] in (4, 8) and choice["gc_form"] in (0, 1, 2, 3)
     em = g.elbo(1)
     g.grad()
     assert abs(em - e64) <= 1e-7 * abs(e64)
