@@ -47,9 +47,7 @@ constexpr int BPA = 16 * BT;
 constexpr int BCH = 32;                // cells per chunk
 constexpr int BLD = 176;               // LDS row stride in doubles (rows land 32 banks apart)
 constexpr int BTH = 512;               // threads of the pass (8 waves)
-constexpr int BSLOT = (BT * (BT + 1) / 2 + BTH / 64 - 1) / (BTH / 64);   // upper tiles per wave (7)
 constexpr int KPL = (BPA + 63) / 64;   // Knm values per lane and cell (M < BPA)
-constexpr int CVR = (BPA + BTH / BCH - 1) / (BTH / BCH);   // covariate columns per thread (p_g < BPA)
 constexpr double WT = 0.1;             // ggplvm.cu:20
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -88,12 +86,19 @@ __device__ inline double prior(int a, int b, int pg, int M, const double *K, con
 
 // mode 0: initPsi/initBeta pass (unit weights, nu = log((y+0.05)/ls));
 // mode 1: IRLS pass at the current beta (also the Poisson log-likelihood).
-// Eight waves: per 32-cell chunk, wave w gathers the Knm rows of cells
-// 4w .. 4w+3 (one 1 KB row per cell, lanes over columns; prefetched one chunk
-// ahead in registers), thread t gathers the covariate value (cell t % 32,
-// column t / 32) and the intercept; 16 threads per cell form eta, W, nu;
-// the upper 16x16 tiles of the (P_g+1)-wide Gram are dealt round robin to
-// the eight waves (at most BSLOT tiles each).
+// Eight waves in two roles, one of each per SIMD, over a double-buffered
+// 32-cell chunk: while waves 0-3 run the MFMAs of chunk c, waves 4-7 build
+// chunk c + 1 in the other buffer -- producer wave p owns cells 8p .. 8p+7:
+// it writes their design rows (Knm rows prefetched a chunk ahead in
+// registers, covariates, intercept), forms eta with 8 lanes per cell, W, nu,
+// the weight and the likelihood term on one lane per cell, and scales the
+// rows; no step crosses waves, so one barrier per chunk suffices.  The
+// upper 16x16 tiles of the (P_g+1)-wide Gram are dealt round robin to the
+// four MFMA waves (at most BSLOT4 each).
+constexpr int BSLOT4 = (BT * (BT + 1) / 2 + 3) / 4;    // upper tiles per MFMA wave (14)
+constexpr int BCV = 2;                                   // prefetched covariate values per producer lane
+static_assert(BSLOT4 <= 14 && BCH == 32, "pass tiling");
+
 __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
         const int *__restrict__ order, int *__restrict__ queue,
         const int *__restrict__ off, const int *__restrict__ cell, const int *__restrict__ ncov,
@@ -103,13 +108,16 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
         const double *__restrict__ K, const double *__restrict__ delta,
         const double *__restrict__ Beta, double *__restrict__ Psi, double *__restrict__ bvec,
         double *__restrict__ sov2, double *__restrict__ lk) {
-    __shared__ double Xs[BCH * BLD];
+    __shared__ double Xs[2][BCH * BLD];
     __shared__ double bs[BPA];
-    __shared__ double sq[BCH], nuv[BCH];
     __shared__ double red[BTH / 64];
     __shared__ int gsh;
     __shared__ unsigned char tta[BT * (BT + 1) / 2], ttb[BT * (BT + 1) / 2];
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const bool producer = w >= 4;                 // waves 4-7 build chunks, 0-3 run the MFMAs
+    const int pw = w - 4;                         // producer index: cells 8 pw .. 8 pw + 7
+    const int cu = lane >> 3, part = lane & 7;    // producer lane: cell 8 pw + cu, part of it
     const double dt = delta[2];
     for (;;) {
         if (tid == 0) {
@@ -124,25 +132,27 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
         const int PA = P + 1, TA = (PA + 15) / 16, NTL = TA * (TA + 1) / 2;
         const float *cg = cov + covoff[g];
         for (int k = tid; k < BPA; k += BTH) bs[k] = (mode == 1 && k < P) ? Beta[(size_t)g * PB + k] : 0.0;
-        // upper tile t -> (ta, tb), once per gene
-        if (tid < NTL) {
+        if (tid < NTL) {                          // upper tile t -> (ta, tb), once per gene
             int ta = 0, rem = tid;
             while (rem >= TA - ta) { rem -= TA - ta; ta++; }
             tta[tid] = (unsigned char)ta;
             ttb[tid] = (unsigned char)(ta + rem);
         }
-        d4 acc[BSLOT];
-#pragma unroll
-        for (int s = 0; s < BSLOT; s++) acc[s] = d4{0.0, 0.0, 0.0, 0.0};
+        __syncthreads();                          // bs, tta, ttb visible to both roles
         double lks = 0.0;
-        // prefetch registers: Knm values of this wave's 4 cells, one covariate value
-        double kv[4][KPL];
-        float cvv[CVR];
-        const int cci = tid & (BCH - 1), ccol = tid / BCH;     // covariate items: columns ccol + 16 r
+        const int nch = (n + BCH - 1) / BCH;
+        double *Pg = Psi + (size_t)g * PB * PB;
+        // The two roles run separate loops with the same barriers (one per
+        // chunk), so the accumulators and the prefetch registers are never
+        // live together.
+        if (producer) {
+        // prefetch registers: the Knm rows of its 8 cells and its covariate values
+        double kv[8][KPL];
+        float cvv[BCV];
         auto fetch = [&](int c0) {
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int ci = 4 * w + u;
+            for (int u = 0; u < 8; u++) {
+                const int ci = 8 * pw + u;
                 const bool ok = c0 + ci < n;
                 const double *row = Krm + (size_t)(ok ? cell[o0 + c0 + ci] : 0) * M;
 #pragma unroll
@@ -151,116 +161,128 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
                     kv[u][e] = (ok && col < M) ? row[col] : 0.0;
                 }
             }
+            const bool okc = c0 + 8 * pw + cu < n;
 #pragma unroll
-            for (int r = 0; r < CVR; r++) {
-                const int col = ccol + (BTH / BCH) * r;
-                cvv[r] = (col >= 1 && col < pg && c0 + cci < n) ? cg[(size_t)(col - 1) * n + c0 + cci] : 0.0f;
+            for (int r = 0; r < BCV; r++) {
+                const int col = 1 + part + 8 * r;
+                cvv[r] = (okc && col < pg) ? cg[(size_t)(col - 1) * n + c0 + 8 * pw + cu] : 0.0f;
             }
         };
-        fetch(0);
-        for (int c0 = 0; c0 < n; c0 += BCH) {
-            __syncthreads();                          // the previous chunk's MFMAs are done with Xs
+        // build chunk c0 into buffer nb (producer waves only; wave-local)
+        auto build = [&](int c0, int nb) {
+            double *X = Xs[nb];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int ci = 4 * w + u;
+            for (int u = 0; u < 8; u++) {
+                const int ci = 8 * pw + u;
 #pragma unroll
                 for (int e = 0; e < KPL; e++) {
                     const int col = lane + 64 * e;
-                    if (col < M) Xs[ci * BLD + pg + col] = kv[u][e];
+                    if (col < M) X[ci * BLD + pg + col] = kv[u][e];
                 }
             }
-            if (ccol == 0) Xs[cci * BLD] = (c0 + cci < n) ? 1.0 : 0.0;
+            const int ci = 8 * pw + cu;
+            const bool ok = c0 + ci < n;
+            if (part == 0) X[ci * BLD] = ok ? 1.0 : 0.0;
 #pragma unroll
-            for (int r = 0; r < CVR; r++) {
-                const int col = ccol + (BTH / BCH) * r;
-                if (col >= 1 && col < pg) Xs[cci * BLD + col] = (double)cvv[r];
+            for (int r = 0; r < BCV; r++) {
+                const int col = 1 + part + 8 * r;
+                if (col < pg) X[ci * BLD + col] = (double)cvv[r];
             }
-            if (c0 + BCH < n) fetch(c0 + BCH);
-            __syncthreads();
-            // per cell: eta (16 threads per cell), then W, nu, weight and the
-            // likelihood term by one lane per cell of the first wave (the FP64
-            // exp / log / sqrt run once per cell, not once per wave)
-            {
-                const int ci = tid >> 4, part = tid & 15;
-                double eta = 0.0;
-                if (mode == 1)
-                    for (int k = part; k < P; k += 16) eta += Xs[ci * BLD + k] * bs[k];
-                eta += __shfl_xor(eta, 1, 64);
-                eta += __shfl_xor(eta, 2, 64);
-                eta += __shfl_xor(eta, 4, 64);
-                eta += __shfl_xor(eta, 8, 64);
-                if (part == 0) nuv[ci] = eta;
-            }
-            __syncthreads();
-            if (tid < BCH) {
-                const int ci = tid;
-                const double eta = nuv[ci];
-                double s = 0.0, nu = 0.0;
-                if (c0 + ci < n) {
-                    const int pos = o0 + c0 + ci, i = cell[pos];
-                    const double y = (double)yd[pos];
-                    if (mode == 0) {
-                        s = 1.0;
-                        nu = log((y + 0.05) / ls[i]);
-                    } else {
-                        // ggplvm.cu:468-514 (zeta = 0)
-                        const double logW = eta + log(ls[i]) + dt * tits[i] * WT;
-                        const double Wi = exp(eta) * ls[i] * exp(dt * tits[i] * WT);
-                        nu = (y > 0.0) ? eta + exp(log(y) - logW) - 1.0 : eta - 1.0;
-                        s = sqrt(Wi / Omega[i]);
-                        // ggplvm.cu:703-724
-                        lks += (y > 0.0 ? (log(Wi) - dt * tits[i] * WT) * y : 0.0) - Wi;
-                    }
+            // covariates beyond the prefetched 8 BCV (wider designs), read here
+            for (int col = 1 + part + 8 * BCV; col < pg; col += 8)
+                X[ci * BLD + col] = ok ? (double)cg[(size_t)(col - 1) * n + min(c0 + ci, n - 1)] : 0.0;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            double eta = 0.0;
+            if (mode == 1)
+                for (int k = part; k < P; k += 8) eta += X[ci * BLD + k] * bs[k];
+            eta += __shfl_xor(eta, 1, 64);
+            eta += __shfl_xor(eta, 2, 64);
+            eta += __shfl_xor(eta, 4, 64);
+            double s = 0.0, nu = 0.0;
+            if (part == 0 && ok) {
+                const int pos = o0 + c0 + ci, i = cell[pos];
+                const double y = (double)yd[pos];
+                if (mode == 0) {
+                    s = 1.0;
+                    nu = log((y + 0.05) / ls[i]);
+                } else {
+                    // ggplvm.cu:468-514 (zeta = 0)
+                    const double logW = eta + log(ls[i]) + dt * tits[i] * WT;
+                    const double Wi = exp(eta) * ls[i] * exp(dt * tits[i] * WT);
+                    nu = (y > 0.0) ? eta + exp(log(y) - logW) - 1.0 : eta - 1.0;
+                    s = sqrt(Wi / Omega[i]);
+                    // ggplvm.cu:703-724
+                    lks += (y > 0.0 ? (log(Wi) - dt * tits[i] * WT) * y : 0.0) - Wi;
                 }
-                sq[ci] = s;
-                nuv[ci] = nu;
             }
-            __syncthreads();
-            {
-                const int ci = tid >> 4, part = tid & 15;
-                const double sc = sq[ci];
-                for (int col = part; col < P; col += 16) Xs[ci * BLD + col] *= sc;
-                if (part == 0) Xs[ci * BLD + P] = sc * nuv[ci];
+            s = __shfl(s, lane & ~7, 64);
+            nu = __shfl(nu, lane & ~7, 64);
+            for (int col = part; col < P; col += 8) X[ci * BLD + col] *= s;
+            if (part == 0) X[ci * BLD + P] = s * nu;
+        };
+        fetch(0);
+        build(0, 0);
+        if (nch > 1) fetch(BCH);
+        __syncthreads();                          // chunk 0 built
+        for (int ch = 0; ch < nch; ch++) {
+            if (ch + 1 < nch) {
+                build((ch + 1) * BCH, (ch & 1) ^ 1);
+                if (ch + 2 < nch) fetch((ch + 2) * BCH);
             }
-            __syncthreads();
-            // X^T X over the chunk: tile t = w + 8 s of the upper triangle
+            __syncthreads();                      // chunk ch + 1 built; buffer ch & 1 free
+        }
+        } else {
+        d4 acc[BSLOT4];
 #pragma unroll
-            for (int s = 0; s < BSLOT; s++) {
-                const int t = w + (BTH / 64) * s;
+        for (int s = 0; s < BSLOT4; s++) acc[s] = d4{0.0, 0.0, 0.0, 0.0};
+        __syncthreads();                          // chunk 0 built
+        for (int ch = 0; ch < nch; ch++) {
+            // X^T X over chunk ch: tile t = w + 4 s of the upper triangle
+            const double *X = Xs[ch & 1];
+#pragma unroll
+            for (int s = 0; s < BSLOT4; s++) {
+                const int t = w + 4 * s;
                 if (t < NTL) {
                     const int ta = tta[t], tb = ttb[t];
 #pragma unroll
                     for (int k0 = 0; k0 < BCH; k0 += 4) {
-                        const double a = Xs[(k0 + (lane >> 4)) * BLD + 16 * ta + (lane & 15)];
-                        const double b = Xs[(k0 + (lane >> 4)) * BLD + 16 * tb + (lane & 15)];
+                        const double a = X[(k0 + (lane >> 4)) * BLD + 16 * ta + (lane & 15)];
+                        const double b = X[(k0 + (lane >> 4)) * BLD + 16 * tb + (lane & 15)];
                         acc[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[s], 0, 0, 0);
                     }
                 }
+                __builtin_amdgcn_sched_barrier(0);   // one tile's fragments live at a time
             }
+            __syncthreads();                      // chunk ch + 1 built; buffer ch & 1 free
         }
-        // write Psi_g (+ prior, mirrored, identity padding), b_g, nu^T D nu
-        double *Pg = Psi + (size_t)g * PB * PB;
+        // write Psi_g (+ prior, mirrored), b_g, nu^T D nu
+        {
 #pragma unroll
-        for (int s = 0; s < BSLOT; s++) {
-            const int t = w + (BTH / 64) * s;
-            if (t < NTL) {
-                const int ta = tta[t], tb = ttb[t];
+            for (int s = 0; s < BSLOT4; s++) {
+                const int t = w + 4 * s;
+                if (t < NTL) {
+                    const int ta = tta[t], tb = ttb[t];
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    const int row = 16 * ta + 4 * r + (lane >> 4), col = 16 * tb + (lane & 15);
-                    const double v = acc[s][r];
-                    if (row > col) continue;
-                    if (col < P) {
-                        const double pv = v + prior(row, col, pg, M, K, delta);
-                        Pg[row + (size_t)col * PB] = pv;
-                        Pg[col + (size_t)row * PB] = pv;
-                    } else if (col == P) {
-                        if (row < P) bvec[(size_t)g * PB + row] = v;
-                        else sov2[g] = v;
+                    for (int r = 0; r < 4; r++) {
+                        const int row = 16 * ta + 4 * r + (lane >> 4), col = 16 * tb + (lane & 15);
+                        const double v = acc[s][r];
+                        if (row > col) continue;
+                        if (col < P) {
+                            const double pv = v + prior(row, col, pg, M, K, delta);
+                            Pg[row + (size_t)col * PB] = pv;
+                            Pg[col + (size_t)row * PB] = pv;
+                        } else if (col == P) {
+                            if (row < P) bvec[(size_t)g * PB + row] = v;
+                            else sov2[g] = v;
+                        }
                     }
                 }
             }
         }
+        }
+        // identity padding
         for (int e = tid; e < PB * PB; e += BTH) {
             const int r = e % PB, cc = e / PB;
             if (r >= P || cc >= P) Pg[e] = (r == cc) ? 1.0 : 0.0;
