@@ -34,6 +34,7 @@ struct gp_bfit {
     long long *covoff = nullptr;
     float *cov = nullptr, *yd = nullptr;
     double *Krm = nullptr, *Beta = nullptr, *Psi = nullptr, *bvec = nullptr, *sov2 = nullptr;
+    double *cellc = nullptr;                  // N x 3 per-cell constants of a pass (k_bf_cellc)
     double *phi = nullptr, *lk = nullptr, *obj = nullptr, *delta = nullptr;
     int *info = nullptr, *jtr = nullptr;
     hipGraph_t graph = nullptr;
@@ -96,6 +97,18 @@ __device__ inline double prior(int a, int b, int pg, int M, const double *K, con
 // upper 16x16 tiles of the (P_g+1)-wide Gram are dealt round robin to the
 // four MFMA waves (at most BSLOT4 each).
 constexpr int BSLOT4 = (BT * (BT + 1) / 2 + 3) / 4;    // upper tiles per MFMA wave (14)
+
+// Per-cell constants of the W / nu formulas (ggplvm.cu:468-514) for a pass:
+// cc[3i] = log(ls_i), cc[3i+1] = dt tits_i WT, cc[3i+2] = exp(dt tits_i WT).
+__global__ void k_bf_cellc(int N, const double *__restrict__ ls, const double *__restrict__ tits,
+                           const double *__restrict__ delta, double *__restrict__ cc) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    const double c = delta[2] * tits[i] * WT;
+    cc[3 * (size_t)i] = log(ls[i]);
+    cc[3 * (size_t)i + 1] = c;
+    cc[3 * (size_t)i + 2] = exp(c);
+}
 constexpr int BCV = 2;                                   // prefetched covariate values per producer lane
 static_assert(BSLOT4 <= 14 && BCH == 32, "pass tiling");
 
@@ -104,7 +117,7 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
         const int *__restrict__ off, const int *__restrict__ cell, const int *__restrict__ ncov,
         const long long *__restrict__ covoff, const float *__restrict__ cov,
         const float *__restrict__ yd, const double *__restrict__ Krm, const double *__restrict__ ls,
-        const double *__restrict__ Omega, const double *__restrict__ tits,
+        const double *__restrict__ Omega, const double *__restrict__ cellc,
         const double *__restrict__ K, const double *__restrict__ delta,
         const double *__restrict__ Beta, double *__restrict__ Psi, double *__restrict__ bvec,
         double *__restrict__ sov2, double *__restrict__ lk) {
@@ -118,7 +131,6 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
     const bool producer = w >= 4;                 // waves 4-7 build chunks, 0-3 run the MFMAs
     const int pw = w - 4;                         // producer index: cells 8 pw .. 8 pw + 7
     const int cu = lane >> 3, part = lane & 7;    // producer lane: cell 8 pw + cu, part of it
-    const double dt = delta[2];
     for (;;) {
         if (tid == 0) {
             const int q = atomicAdd(queue, 1);
@@ -195,31 +207,66 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             double eta = 0.0;
-            if (mode == 1)
-                for (int k = part; k < P; k += 8) eta += X[ci * BLD + k] * bs[k];
+            if (mode == 1) {                      // 4 loads of each stream in flight, same order
+                int k = part;
+                for (; k + 24 < P; k += 32) {
+                    double xv[4], bv[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) { xv[u] = X[ci * BLD + k + 8 * u]; bv[u] = bs[k + 8 * u]; }
+#pragma unroll
+                    for (int u = 0; u < 4; u++) eta += xv[u] * bv[u];
+                }
+                for (; k < P; k += 8) eta += X[ci * BLD + k] * bs[k];
+            }
             eta += __shfl_xor(eta, 1, 64);
             eta += __shfl_xor(eta, 2, 64);
             eta += __shfl_xor(eta, 4, 64);
+            // W, nu, the weight and the likelihood term of the cell, the
+            // transcendentals spread over the cell's lanes in two rounds
+            // (the operations and their order are those of one lane)
+            const int pos = o0 + c0 + min(ci, n - 1 - c0), i = cell[pos];
+            const double y = (double)yd[pos];
             double s = 0.0, nu = 0.0;
-            if (part == 0 && ok) {
-                const int pos = o0 + c0 + ci, i = cell[pos];
-                const double y = (double)yd[pos];
-                if (mode == 0) {
-                    s = 1.0;
-                    nu = log((y + 0.05) / ls[i]);
-                } else {
-                    // ggplvm.cu:468-514 (zeta = 0)
-                    const double logW = eta + log(ls[i]) + dt * tits[i] * WT;
-                    const double Wi = exp(eta) * ls[i] * exp(dt * tits[i] * WT);
-                    nu = (y > 0.0) ? eta + exp(log(y) - logW) - 1.0 : eta - 1.0;
-                    s = sqrt(Wi / Omega[i]);
-                    // ggplvm.cu:703-724
-                    lks += (y > 0.0 ? (log(Wi) - dt * tits[i] * WT) * y : 0.0) - Wi;
-                }
+            if (mode == 0) {
+                s = 1.0;
+                if (part == 0) nu = log((y + 0.05) / ls[i]);
+            } else {
+                const double l1 = cellc[3 * (size_t)i], c = cellc[3 * (size_t)i + 1];
+                const double e2 = cellc[3 * (size_t)i + 2];
+                // round 1: exp(eta) (part 0), log(y) (part 1)
+                double r1 = 0.0;
+                if (part == 0) r1 = exp(eta);
+                else if (part == 1 && y > 0.0) r1 = log(y);
+                const double eeta = __shfl(r1, lane & ~7, 64);
+                const double ly = __shfl(r1, (lane & ~7) + 1, 64);
+                const double logW = eta + l1 + c;
+                const double Wi = eeta * ls[i] * e2;
+                // round 2: exp(log y - logW) (part 0), sqrt(W / Omega) (part 1), log(W) (part 2)
+                double r2 = 0.0;
+                if (part == 0 && y > 0.0) r2 = exp(ly - logW);
+                else if (part == 1) r2 = sqrt(Wi / Omega[i]);
+                else if (part == 2 && y > 0.0) r2 = log(Wi);
+                const double ex = __shfl(r2, lane & ~7, 64);
+                s = __shfl(r2, (lane & ~7) + 1, 64);
+                const double lw = __shfl(r2, (lane & ~7) + 2, 64);
+                nu = (y > 0.0) ? eta + ex - 1.0 : eta - 1.0;
+                // ggplvm.cu:703-724
+                if (part == 0 && ok) lks += (y > 0.0 ? (lw - c) * y : 0.0) - Wi;
             }
+            if (!ok) s = 0.0;
             s = __shfl(s, lane & ~7, 64);
             nu = __shfl(nu, lane & ~7, 64);
-            for (int col = part; col < P; col += 8) X[ci * BLD + col] *= s;
+            {                                     // scale the row: loads first, then stores
+                int col = part;
+                for (; col + 24 < P; col += 32) {
+                    double v[4];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) v[u] = X[ci * BLD + col + 8 * u];
+#pragma unroll
+                    for (int u = 0; u < 4; u++) X[ci * BLD + col + 8 * u] = v[u] * s;
+                }
+                for (; col < P; col += 8) X[ci * BLD + col] *= s;
+            }
             if (part == 0) X[ci * BLD + P] = s * nu;
         };
         fetch(0);
@@ -387,13 +434,15 @@ int bf_pass(gp_bfit *b, int mode) {
     gp_ctx *c = b->c;
     k_bf_fill_int<<<1, 256, 0, c->stream>>>(1, b->queue, 0);
     GP_HIP(hipGetLastError());
+    k_bf_cellc<<<(c->N + 255) / 256, 256, 0, c->stream>>>(c->N, c->ls, c->titsias, b->delta, b->cellc);
+    GP_HIP(hipGetLastError());
     int dev = 0, ncu = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     const int grid = std::min(b->J, 2 * ncu);
     k_bf_pass<<<grid, BTH, 0, c->stream>>>(mode, b->J, b->M, b->PB, b->order, b->queue, b->off,
                                           b->cell, b->ncov, b->covoff, b->cov, b->yd, b->Krm,
-                                          c->ls, c->Omega, c->titsias, c->K, b->delta, b->Beta,
+                                          c->ls, c->Omega, b->cellc, c->K, b->delta, b->Beta,
                                           b->Psi, b->bvec, b->sov2, b->lk);
     GP_HIP(hipGetLastError());
     GP_TRY(gpk_chol_batched(c, b->Psi, b->PB, b->J, (size_t)b->PB * b->PB, b->info));
@@ -467,6 +516,7 @@ extern "C" int gp_bfit_create(gp_bfit **out, gp_ctx *c, int J, const int *off, c
     BF_ALLOC(b->cov, std::max<long long>(ncovtot, 1));
     BF_ALLOC(b->yd, std::max<long long>(L, 1));
     BF_ALLOC(b->Krm, (size_t)c->N * M);
+    BF_ALLOC(b->cellc, (size_t)c->N * 3);
     BF_ALLOC(b->Beta, (size_t)J * b->PB);
     BF_ALLOC(b->Psi, (size_t)J * b->PB * b->PB);
     BF_ALLOC(b->bvec, (size_t)J * b->PB);
@@ -526,7 +576,8 @@ extern "C" int gp_bfit_destroy(gp_bfit *b) {
     if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     void *bufs[] = {b->off, b->cell, b->ncov, b->order, b->queue, b->covoff, b->cov, b->yd, b->Krm,
-                    b->Beta, b->Psi, b->bvec, b->sov2, b->phi, b->lk, b->obj, b->delta, b->info, b->jtr};
+                    b->Beta, b->Psi, b->bvec, b->sov2, b->phi, b->lk, b->obj, b->delta, b->info, b->jtr,
+                    b->cellc};
     for (void *p : bufs)
         if (p) (void)hipFree(p);
     delete b;
