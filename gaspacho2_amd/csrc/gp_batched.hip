@@ -158,19 +158,27 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
         // chunk), so the accumulators and the prefetch registers are never
         // live together.
         if (producer) {
-        // prefetch registers: the Knm rows of its 8 cells and its covariate values
+        // prefetch registers, one chunk ahead: the Knm rows of its 8 cells,
+        // its covariate values and the lane's cell data (count, size factor,
+        // Omega, the per-cell constants); the cell indices come one chunk
+        // earlier still, so no load waits on another within a chunk
         double kv[8][KPL];
         float cvv[BCV];
-        auto fetch = [&](int c0) {
+        double yv = 0.0, lsv = 1.0, omv = 1.0, l1v = 0.0, cvn = 0.0, e2v = 1.0;
+        int cn[8];
+        auto cells = [&](int c0) {                // cell indices of chunk c0 (clamped)
+#pragma unroll
+            for (int u = 0; u < 8; u++) cn[u] = cell[o0 + min(c0 + 8 * pw + u, n - 1)];
+        };
+        auto fetch = [&](int c0) {                // uses cn of chunk c0, then loads the next chunk's
 #pragma unroll
             for (int u = 0; u < 8; u++) {
-                const int ci = 8 * pw + u;
-                const bool ok = c0 + ci < n;
-                const double *row = Krm + (size_t)(ok ? cell[o0 + c0 + ci] : 0) * M;
+                const bool ok = c0 + 8 * pw + u < n;
+                const double *row = Krm + (size_t)cn[u] * M;
 #pragma unroll
                 for (int e = 0; e < KPL; e++) {
                     const int col = lane + 64 * e;
-                    kv[u][e] = (ok && col < M) ? row[col] : 0.0;
+                    kv[u][e] = (ok && col < M) ? row[min(col, M - 1)] : 0.0;
                 }
             }
             const bool okc = c0 + 8 * pw + cu < n;
@@ -179,6 +187,16 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
                 const int col = 1 + part + 8 * r;
                 cvv[r] = (okc && col < pg) ? cg[(size_t)(col - 1) * n + c0 + 8 * pw + cu] : 0.0f;
             }
+            int ic = cn[0];
+#pragma unroll
+            for (int u = 1; u < 8; u++) ic = (cu == u) ? cn[u] : ic;
+            yv = (double)yd[o0 + min(c0 + 8 * pw + cu, n - 1)];
+            lsv = ls[ic];
+            omv = Omega[ic];
+            l1v = cellc[3 * (size_t)ic];
+            cvn = cellc[3 * (size_t)ic + 1];
+            e2v = cellc[3 * (size_t)ic + 2];
+            cells(c0 + BCH);
         };
         // build chunk c0 into buffer nb (producer waves only; wave-local)
         auto build = [&](int c0, int nb) {
@@ -224,15 +242,13 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
             // W, nu, the weight and the likelihood term of the cell, the
             // transcendentals spread over the cell's lanes in two rounds
             // (the operations and their order are those of one lane)
-            const int pos = o0 + c0 + min(ci, n - 1 - c0), i = cell[pos];
-            const double y = (double)yd[pos];
+            const double y = yv;
             double s = 0.0, nu = 0.0;
             if (mode == 0) {
                 s = 1.0;
-                if (part == 0) nu = log((y + 0.05) / ls[i]);
+                if (part == 0) nu = log((y + 0.05) / lsv);
             } else {
-                const double l1 = cellc[3 * (size_t)i], c = cellc[3 * (size_t)i + 1];
-                const double e2 = cellc[3 * (size_t)i + 2];
+                const double l1 = l1v, c = cvn, e2 = e2v;
                 // round 1: exp(eta) (part 0), log(y) (part 1)
                 double r1 = 0.0;
                 if (part == 0) r1 = exp(eta);
@@ -240,11 +256,11 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
                 const double eeta = __shfl(r1, lane & ~7, 64);
                 const double ly = __shfl(r1, (lane & ~7) + 1, 64);
                 const double logW = eta + l1 + c;
-                const double Wi = eeta * ls[i] * e2;
+                const double Wi = eeta * lsv * e2;
                 // round 2: exp(log y - logW) (part 0), sqrt(W / Omega) (part 1), log(W) (part 2)
                 double r2 = 0.0;
                 if (part == 0 && y > 0.0) r2 = exp(ly - logW);
-                else if (part == 1) r2 = sqrt(Wi / Omega[i]);
+                else if (part == 1) r2 = sqrt(Wi / omv);
                 else if (part == 2 && y > 0.0) r2 = log(Wi);
                 const double ex = __shfl(r2, lane & ~7, 64);
                 s = __shfl(r2, (lane & ~7) + 1, 64);
@@ -269,6 +285,7 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
             }
             if (part == 0) X[ci * BLD + P] = s * nu;
         };
+        cells(0);
         fetch(0);
         build(0, 0);
         if (nch > 1) fetch(BCH);
