@@ -133,13 +133,19 @@ int gp_ql_start(gp_ctx *ctx, double *obj);
 
 /* ggplvm.cu:1259-1278: one ELBO evaluation at the current parameters
  * (getOinvt, getKernelMats, expandDelta_k, cholKmm, getKinvKmn, getPsi,
- * cholPsi, cholDeltaInv, getLkhd, collectLkhd); stored at lkhd[slot]. */
+ * cholPsi, cholDeltaInv, getLkhd, collectLkhd); stored at lkhd[slot].  The
+ * factor of DeltaInv = blockdiag(diag 1/delta, Kmm/delta_GP) is formed from
+ * the Kmm factor (cholDeltaInv's result up to rounding). */
 int gp_elbo(gp_ctx *ctx, int slot, double *obj);
 
 /* ggplvm.cu:1215-1240: one gradient evaluation (expandDelta_k, getOinvt,
  * getTildeBeta, getGradLogDelta, collectGradLogDelta, getGradLogOmega,
  * solvePsi, getC2, getGradTau, getGradLogRho, collectGradLogRho) into the
- * device grad buffer [gOmega N | gdelta p | gX NQ | gT MQ | grho Q]. */
+ * device grad buffer [gOmega N | gdelta p | gX NQ | gT MQ | grho Q].
+ * As in the reference, it uses the factors of the last gp_elbo /
+ * gp_ql_start, so it belongs at that call's parameters; when no other
+ * state-changing call came in between, getTildeBeta is taken from that
+ * call's own solve instead of being repeated (same bits). */
 int gp_grad(gp_ctx *ctx);
 
 /* ggplvm.cu:1251 LBFGS<<<1,1>>> (ggplvm.cu:865-934, util_cuda.cu:187-232):
