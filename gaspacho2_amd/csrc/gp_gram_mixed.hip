@@ -803,7 +803,9 @@ __global__ void __launch_bounds__(GW4 * 64) k_gram_sym4(int Npad, int seg, int n
     }
 }
 
-// Psi_j from the symmetric-form partials: one 256-thread block per (tile, gene);
+constexpr int RSG = 4;       // genes per block of the symmetric reduction
+
+// Psi_j from the symmetric-form partials: one 256-thread block per (tile, RSG genes);
 // sums the segments in FP64, rescales by 2^e_j 2^-s_r 2^-s_c, adds the prior
 // and writes the tile and its mirror (upper-triangle values only, so Psi is
 // exactly symmetric).
@@ -811,7 +813,7 @@ __global__ void __launch_bounds__(256) k_gram_reduce_sym(int P, int T16, int nse
         const float *__restrict__ part, const double *__restrict__ wun,
         const double *__restrict__ zun, const double *__restrict__ Delta, double *__restrict__ Psi) {
     __shared__ double tile[16][17];
-    const int t = blockIdx.x, j = blockIdx.y, tid = threadIdx.x;
+    const int t = blockIdx.x, tid = threadIdx.x;
     const int NT = T16 * (T16 + 1) / 2;
     int ta = 0, rem = t;
     while (rem >= T16 - ta) { rem -= T16 - ta; ta++; }
@@ -819,22 +821,26 @@ __global__ void __launch_bounds__(256) k_gram_reduce_sym(int P, int T16, int nse
     // thread tid = 4*lane + r  <->  tile element (4(lane/16) + r, lane%16)
     const int lane = tid >> 2, r = tid & 3;
     const int rr = 4 * (lane >> 4) + r, cc = lane & 15;
-    double s = 0.0;
-    for (int g = 0; g < nseg; g++) s += (double)part[(((size_t)g * J + j) * NT + t) * 256 + tid];
     const int row = 16 * ta + rr, col = 16 * tb + cc;
-    double v = 0.0;
-    if (row < P && col < P) v = s * (wun[j] * zun[row] * zun[col]) + Delta[row + (size_t)col * P];
-    tile[rr][cc] = v;
-    __syncthreads();
-    double *Pj = Psi + (size_t)j * P * P;
     const int i1 = tid & 15, i2 = tid >> 4;
-    {   // Psi[16ta + i1, 16tb + i2]   (coalesced over i1)
-        const int R = 16 * ta + i1, C = 16 * tb + i2;
-        if (R < P && C < P && (ta != tb || i1 <= i2)) Pj[R + (size_t)C * P] = tile[i1][i2];
-    }
-    {   // mirror: Psi[16tb + i1, 16ta + i2] = tile[i2][i1]
-        const int R = 16 * tb + i1, C = 16 * ta + i2;
-        if (R < P && C < P && (ta != tb || i2 <= i1)) Pj[R + (size_t)C * P] = tile[i2][i1];
+    const int j0 = blockIdx.y * RSG, j1 = min(J, j0 + RSG);
+    for (int j = j0; j < j1; j++) {
+        if (j > j0) __syncthreads();              // the previous gene's tile reads are done
+        double s = 0.0;
+        for (int g = 0; g < nseg; g++) s += (double)part[(((size_t)g * J + j) * NT + t) * 256 + tid];
+        double v = 0.0;
+        if (row < P && col < P) v = s * (wun[j] * zun[row] * zun[col]) + Delta[row + (size_t)col * P];
+        tile[rr][cc] = v;
+        __syncthreads();
+        double *Pj = Psi + (size_t)j * P * P;
+        {   // Psi[16ta + i1, 16tb + i2]   (coalesced over i1)
+            const int R = 16 * ta + i1, C = 16 * tb + i2;
+            if (R < P && C < P && (ta != tb || i1 <= i2)) Pj[R + (size_t)C * P] = tile[i1][i2];
+        }
+        {   // mirror: Psi[16tb + i1, 16ta + i2] = tile[i2][i1]
+            const int R = 16 * tb + i1, C = 16 * ta + i2;
+            if (R < P && C < P && (ta != tb || i2 <= i1)) Pj[R + (size_t)C * P] = tile[i2][i1];
+        }
     }
 }
 
@@ -945,7 +951,7 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
         }
         if (r) return r;
         gp_timer_end(c, c->t_gramk);
-        dim3 gr((unsigned)(T16 * (T16 + 1) / 2), batch);
+        dim3 gr((unsigned)(T16 * (T16 + 1) / 2), (unsigned)((batch + RSG - 1) / RSG));
         k_gram_reduce_sym<<<gr, 256, 0, c->stream>>>(c->P, T16, nseg, batch, c->gram_part, c->wun,
                                                       c->zun, Delta, Psi);
         GP_HIP(hipGetLastError());
