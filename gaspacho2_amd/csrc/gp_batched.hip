@@ -47,7 +47,10 @@ constexpr int BT = 10;                 // 16-column tiles of the augmented desig
 constexpr int BPA = 16 * BT;
 constexpr int BCH = 32;                // cells per chunk
 constexpr int BLD = 176;               // LDS row stride in doubles (rows land 32 banks apart)
-constexpr int BTH = 512;               // threads of the pass (8 waves)
+constexpr int BNP = 4;                 // producer waves of the pass (8 measured slower: spills at 3 waves per SIMD)
+constexpr int BTH = 64 * (4 + BNP);     // threads of the pass (4 MFMA waves + the producers)
+constexpr int BCPW = 32 / BNP;          // cells per producer wave and chunk
+constexpr int BLPC = 64 / BCPW;         // producer lanes per cell
 constexpr int KPL = (BPA + 63) / 64;   // Knm values per lane and cell (M < BPA)
 constexpr double WT = 0.1;             // ggplvm.cu:20
 
@@ -87,13 +90,14 @@ __device__ inline double prior(int a, int b, int pg, int M, const double *K, con
 
 // mode 0: initPsi/initBeta pass (unit weights, nu = log((y+0.05)/ls));
 // mode 1: IRLS pass at the current beta (also the Poisson log-likelihood).
-// Eight waves in two roles, one of each per SIMD, over a double-buffered
-// 32-cell chunk: while waves 0-3 run the MFMAs of chunk c, waves 4-7 build
-// chunk c + 1 in the other buffer -- producer wave p owns cells 8p .. 8p+7:
+// Two roles over a double-buffered 32-cell chunk: while waves 0-3 (one per
+// SIMD) run the MFMAs of chunk c, the BNP producer waves build chunk c + 1 in
+// the other buffer -- producer wave p owns cells BCPW p .. BCPW p + BCPW - 1:
 // it writes their design rows (Knm rows prefetched a chunk ahead in
-// registers, covariates, intercept), forms eta with 8 lanes per cell, W, nu,
-// the weight and the likelihood term on one lane per cell, and scales the
-// rows; no step crosses waves, so one barrier per chunk suffices.  The
+// registers, covariates, intercept), forms eta with BLPC lanes per cell, W,
+// nu, the weight and the likelihood term (the per-cell transcendentals spread
+// over the cell's lanes), and scales the rows; no step crosses waves, so one
+// barrier per chunk suffices.  The
 // upper 16x16 tiles of the (P_g+1)-wide Gram are dealt round robin to the
 // four MFMA waves (at most BSLOT4 each).
 constexpr int BSLOT4 = (BT * (BT + 1) / 2 + 3) / 4;    // upper tiles per MFMA wave (14)
@@ -109,7 +113,7 @@ __global__ void k_bf_cellc(int N, const double *__restrict__ ls, const double *_
     cc[3 * (size_t)i + 1] = c;
     cc[3 * (size_t)i + 2] = exp(c);
 }
-constexpr int BCV = 2;                                   // prefetched covariate values per producer lane
+constexpr int BCV = (16 + BLPC - 1) / BLPC;             // prefetched covariate values per producer lane
 static_assert(BSLOT4 <= 14 && BCH == 32, "pass tiling");
 
 __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
@@ -128,9 +132,10 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
     __shared__ unsigned char tta[BT * (BT + 1) / 2], ttb[BT * (BT + 1) / 2];
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const bool producer = w >= 4;                 // waves 4-7 build chunks, 0-3 run the MFMAs
-    const int pw = w - 4;                         // producer index: cells 8 pw .. 8 pw + 7
-    const int cu = lane >> 3, part = lane & 7;    // producer lane: cell 8 pw + cu, part of it
+    const bool producer = w >= 4;                 // waves 4.. build chunks, 0-3 run the MFMAs
+    const int pw = w - 4;                         // producer index: cells BCPW pw .. BCPW pw + BCPW - 1
+    const int cu = lane / BLPC, part = lane % BLPC;   // producer lane: cell BCPW pw + cu, part of it
+    const int cb0 = lane & ~(BLPC - 1);           // first lane of this lane's cell
     for (;;) {
         if (tid == 0) {
             const int q = atomicAdd(queue, 1);
@@ -158,22 +163,22 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
         // chunk), so the accumulators and the prefetch registers are never
         // live together.
         if (producer) {
-        // prefetch registers, one chunk ahead: the Knm rows of its 8 cells,
+        // prefetch registers, one chunk ahead: the Knm rows of its BCPW cells,
         // its covariate values and the lane's cell data (count, size factor,
         // Omega, the per-cell constants); the cell indices come one chunk
         // earlier still, so no load waits on another within a chunk
-        double kv[8][KPL];
+        double kv[BCPW][KPL];
         float cvv[BCV];
         double yv = 0.0, lsv = 1.0, omv = 1.0, l1v = 0.0, cvn = 0.0, e2v = 1.0;
-        int cn[8];
+        int cn[BCPW];
         auto cells = [&](int c0) {                // cell indices of chunk c0 (clamped)
 #pragma unroll
-            for (int u = 0; u < 8; u++) cn[u] = cell[o0 + min(c0 + 8 * pw + u, n - 1)];
+            for (int u = 0; u < BCPW; u++) cn[u] = cell[o0 + min(c0 + BCPW * pw + u, n - 1)];
         };
         auto fetch = [&](int c0) {                // uses cn of chunk c0, then loads the next chunk's
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const bool ok = c0 + 8 * pw + u < n;
+            for (int u = 0; u < BCPW; u++) {
+                const bool ok = c0 + BCPW * pw + u < n;
                 const double *row = Krm + (size_t)cn[u] * M;
 #pragma unroll
                 for (int e = 0; e < KPL; e++) {
@@ -181,16 +186,16 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
                     kv[u][e] = (ok && col < M) ? row[min(col, M - 1)] : 0.0;
                 }
             }
-            const bool okc = c0 + 8 * pw + cu < n;
+            const bool okc = c0 + BCPW * pw + cu < n;
 #pragma unroll
             for (int r = 0; r < BCV; r++) {
-                const int col = 1 + part + 8 * r;
-                cvv[r] = (okc && col < pg) ? cg[(size_t)(col - 1) * n + c0 + 8 * pw + cu] : 0.0f;
+                const int col = 1 + part + BLPC * r;
+                cvv[r] = (okc && col < pg) ? cg[(size_t)(col - 1) * n + c0 + BCPW * pw + cu] : 0.0f;
             }
             int ic = cn[0];
 #pragma unroll
-            for (int u = 1; u < 8; u++) ic = (cu == u) ? cn[u] : ic;
-            yv = (double)yd[o0 + min(c0 + 8 * pw + cu, n - 1)];
+            for (int u = 1; u < BCPW; u++) ic = (cu == u) ? cn[u] : ic;
+            yv = (double)yd[o0 + min(c0 + BCPW * pw + cu, n - 1)];
             lsv = ls[ic];
             omv = Omega[ic];
             l1v = cellc[3 * (size_t)ic];
@@ -202,24 +207,24 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
         auto build = [&](int c0, int nb) {
             double *X = Xs[nb];
 #pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int ci = 8 * pw + u;
+            for (int u = 0; u < BCPW; u++) {
+                const int ci = BCPW * pw + u;
 #pragma unroll
                 for (int e = 0; e < KPL; e++) {
                     const int col = lane + 64 * e;
                     if (col < M) X[ci * BLD + pg + col] = kv[u][e];
                 }
             }
-            const int ci = 8 * pw + cu;
+            const int ci = BCPW * pw + cu;
             const bool ok = c0 + ci < n;
             if (part == 0) X[ci * BLD] = ok ? 1.0 : 0.0;
 #pragma unroll
             for (int r = 0; r < BCV; r++) {
-                const int col = 1 + part + 8 * r;
+                const int col = 1 + part + BLPC * r;
                 if (col < pg) X[ci * BLD + col] = (double)cvv[r];
             }
             // covariates beyond the prefetched 8 BCV (wider designs), read here
-            for (int col = 1 + part + 8 * BCV; col < pg; col += 8)
+            for (int col = 1 + part + BLPC * BCV; col < pg; col += BLPC)
                 X[ci * BLD + col] = ok ? (double)cg[(size_t)(col - 1) * n + min(c0 + ci, n - 1)] : 0.0;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -227,18 +232,17 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
             double eta = 0.0;
             if (mode == 1) {                      // 4 loads of each stream in flight, same order
                 int k = part;
-                for (; k + 24 < P; k += 32) {
+                for (; k + 3 * BLPC < P; k += 4 * BLPC) {
                     double xv[4], bv[4];
 #pragma unroll
-                    for (int u = 0; u < 4; u++) { xv[u] = X[ci * BLD + k + 8 * u]; bv[u] = bs[k + 8 * u]; }
+                    for (int u = 0; u < 4; u++) { xv[u] = X[ci * BLD + k + BLPC * u]; bv[u] = bs[k + BLPC * u]; }
 #pragma unroll
                     for (int u = 0; u < 4; u++) eta += xv[u] * bv[u];
                 }
-                for (; k < P; k += 8) eta += X[ci * BLD + k] * bs[k];
+                for (; k < P; k += BLPC) eta += X[ci * BLD + k] * bs[k];
             }
-            eta += __shfl_xor(eta, 1, 64);
-            eta += __shfl_xor(eta, 2, 64);
-            eta += __shfl_xor(eta, 4, 64);
+#pragma unroll
+            for (int o = 1; o < BLPC; o <<= 1) eta += __shfl_xor(eta, o, 64);
             // W, nu, the weight and the likelihood term of the cell, the
             // transcendentals spread over the cell's lanes in two rounds
             // (the operations and their order are those of one lane)
@@ -253,8 +257,8 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
                 double r1 = 0.0;
                 if (part == 0) r1 = exp(eta);
                 else if (part == 1 && y > 0.0) r1 = log(y);
-                const double eeta = __shfl(r1, lane & ~7, 64);
-                const double ly = __shfl(r1, (lane & ~7) + 1, 64);
+                const double eeta = __shfl(r1, cb0, 64);
+                const double ly = __shfl(r1, cb0 + 1, 64);
                 const double logW = eta + l1 + c;
                 const double Wi = eeta * lsv * e2;
                 // round 2: exp(log y - logW) (part 0), sqrt(W / Omega) (part 1), log(W) (part 2)
@@ -262,26 +266,26 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
                 if (part == 0 && y > 0.0) r2 = exp(ly - logW);
                 else if (part == 1) r2 = sqrt(Wi / omv);
                 else if (part == 2 && y > 0.0) r2 = log(Wi);
-                const double ex = __shfl(r2, lane & ~7, 64);
-                s = __shfl(r2, (lane & ~7) + 1, 64);
-                const double lw = __shfl(r2, (lane & ~7) + 2, 64);
+                const double ex = __shfl(r2, cb0, 64);
+                s = __shfl(r2, cb0 + 1, 64);
+                const double lw = __shfl(r2, cb0 + 2, 64);
                 nu = (y > 0.0) ? eta + ex - 1.0 : eta - 1.0;
                 // ggplvm.cu:703-724
                 if (part == 0 && ok) lks += (y > 0.0 ? (lw - c) * y : 0.0) - Wi;
             }
             if (!ok) s = 0.0;
-            s = __shfl(s, lane & ~7, 64);
-            nu = __shfl(nu, lane & ~7, 64);
+            s = __shfl(s, cb0, 64);
+            nu = __shfl(nu, cb0, 64);
             {                                     // scale the row: loads first, then stores
                 int col = part;
-                for (; col + 24 < P; col += 32) {
+                for (; col + 3 * BLPC < P; col += 4 * BLPC) {
                     double v[4];
 #pragma unroll
-                    for (int u = 0; u < 4; u++) v[u] = X[ci * BLD + col + 8 * u];
+                    for (int u = 0; u < 4; u++) v[u] = X[ci * BLD + col + BLPC * u];
 #pragma unroll
-                    for (int u = 0; u < 4; u++) X[ci * BLD + col + 8 * u] = v[u] * s;
+                    for (int u = 0; u < 4; u++) X[ci * BLD + col + BLPC * u] = v[u] * s;
                 }
-                for (; col < P; col += 8) X[ci * BLD + col] *= s;
+                for (; col < P; col += BLPC) X[ci * BLD + col] *= s;
             }
             if (part == 0) X[ci * BLD + P] = s * nu;
         };
