@@ -211,6 +211,83 @@ def main_fit(args):
     print(json.dumps(out), flush=True)
 
 
+def main_fitcheck(args):
+    """The mixed fit's schedule (gp_fit's, ggplvm.cu:1146-1330: IRLS phase to
+    1e-7, quasi-likelihood start, L-BFGS with the reference line search) driven
+    step by step through the C ABI; at every accepted L-BFGS iterate the ELBO
+    and the gradient are evaluated again in FP64 at the same parameters and
+    the relative errors of the mixed values are recorded.  The mixed state is
+    restored after each check, so the trajectory is the mixed fit's own."""
+    from gaspacho2_amd import lib, synth
+    t0 = time.perf_counter()
+    ds = synth.make_dataset(args.N, args.G, args.Q, args.M, seed=0)
+    log(f"data: N={ds.N} G={ds.J} ({time.perf_counter() - t0:.1f} s)")
+    T0 = lib.init_ta_with_per(ds.M, ds.Q)
+    g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0, T0,
+                      precision=lib.PREC_MIXED)
+    scratch = 49999                              # an objective slot the schedule never reaches
+    lk = {}
+    jtrmax = min(200, args.fit_cap) if args.fit_cap > 0 else 200   # first outer iteration
+    g.init_fit()
+    for jtr in range(jtrmax):
+        lk[jtr] = g.irls_step(jtr)
+        if jtr > 0 and (jtr == jtrmax - 1 or abs(lk[jtr - 1] - lk[jtr]) / abs(lk[jtr]) < 1e-7):
+            break
+    lk[0] = g.ql_start()
+    rows = []
+
+    def check(it, slot):
+        g.grad()
+        gm = g.get("grad").copy()
+        em = lk[slot]
+        g.set_precision(lib.PREC_FP64)
+        e64 = g.elbo(scratch)
+        g.grad()
+        g64 = g.get("grad").copy()
+        g.set_precision(lib.PREC_MIXED)
+        g.elbo(slot)                             # the mixed state back (same bits as before)
+        gerr = float(np.max(np.abs(gm - g64)) / np.max(np.abs(g64)))
+        rows.append({"iter": it, "elbo_mixed": em, "elbo_fp64": e64,
+                     "elbo_rel_err": abs(em - e64) / abs(e64), "grad_rel_err": gerr})
+        log(f"iterate {it}: ELBO mixed {em:.10e} fp64 {e64:.10e} rel {abs(em - e64) / abs(e64):.2e}, "
+            f"gradient rel (max entry) {gerr:.2e}")
+
+    check(0, 0)
+    t1 = time.perf_counter()
+    for jtr in range(jtrmax):
+        g.grad()
+        g.lbfgs_step(jtr, 0)
+        sr = 1.0
+        for ktr in range(1, 8):
+            lk[jtr + 1] = g.elbo(jtr + 1)
+            v = lk[jtr + 1]
+            if v > lk[jtr] or ktr == 7:
+                break
+            if not np.isfinite(v):
+                sr *= 0.01
+            elif v < lk[jtr]:
+                sr *= 0.1
+            g.downdate(jtr, sr, 0)
+        check(jtr + 1, jtr + 1)
+        if jtr > 0 and abs(lk[jtr + 1] - lk[jtr]) < abs(lk[jtr + 1]) * 1e-8:
+            break
+    dt = time.perf_counter() - t1
+    g.close()
+    ee = [r["elbo_rel_err"] for r in rows]
+    ge = [r["grad_rel_err"] for r in rows]
+    out = {"metric": "max relative ELBO error of the mixed fit vs FP64 at every L-BFGS iterate",
+           "value": max(ee), "unit": "relative", "n_gpus": 1, "steps": len(rows) - 1, "warmup": 0,
+           "ms_per_step": dt / max(len(rows) - 1, 1) * 1e3, "higher_is_better": False,
+           "scaling": "strong", "vs_baseline": None, "dtype": "fp16x3/f32 MFMA + f64 vs f64",
+           "data": "synthetic (seed 0), PCA initial latent",
+           "config": {"workload": f"mixed fit with FP64 checks, N={ds.N} cells, G={ds.J} genes, "
+                                  f"Q={ds.Q}, M={ds.M}, L-BFGS cap {args.fit_cap}"},
+           "elbo_rel_err_max": max(ee), "elbo_rel_err_median": float(np.median(ee)),
+           "grad_rel_err_max": max(ge), "grad_rel_err_median": float(np.median(ge)),
+           "iterates": rows}
+    print(json.dumps(out), flush=True)
+
+
 def _make_ds(args_tuple):
     from gaspacho2_amd import synth
     N, G, Q, M, seed, model_seed = args_tuple
@@ -303,8 +380,10 @@ def main():
     ap.add_argument("--enc-epochs", type=int, default=30, help="warmstart workload: encoder epochs")
     ap.add_argument("--enc-target", default="residual,aligned",
                     help="warmstart workload: encoder targets, comma-separated (first = headline)")
-    ap.add_argument("--workload", choices=["elbo", "batched", "fit", "warmstart"], default="elbo",
-                    help="elbo: the ELBO+gradient unit (default); batched: uneven per-gene GLM fits")
+    ap.add_argument("--workload", choices=["elbo", "batched", "fit", "fitcheck", "warmstart"],
+                    default="elbo",
+                    help="elbo: the ELBO+gradient unit (default); batched: uneven per-gene GLM fits; "
+                         "fitcheck: mixed-vs-FP64 ELBO and gradient errors at every L-BFGS iterate")
     ap.add_argument("--nmin", type=int, default=500)
     ap.add_argument("--nmax", type=int, default=50000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -327,6 +406,9 @@ def main():
         return
     if args.workload == "fit":
         main_fit(args)
+        return
+    if args.workload == "fitcheck":
+        main_fitcheck(args)
         return
     if args.workload == "warmstart":
         main_warmstart(args)
