@@ -177,9 +177,11 @@ def main_fit(args):
     log(f"data: N={ds.N} G={ds.J} ({time.perf_counter() - t0:.1f} s)")
     T0 = lib.init_ta_with_per(ds.M, ds.Q)
     res = {}
-    for prec in ("f64", "mixed"):
+    precs = ("f64", "mixed", "f64w") if args.fit_f64w else ("f64", "mixed")
+    pmap = {"f64": lib.PREC_FP64, "mixed": lib.PREC_MIXED, "f64w": lib.PREC_FP64_WHITENED}
+    for prec in precs:
         g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0, T0,
-                          precision=lib.PREC_MIXED if prec == "mixed" else lib.PREC_FP64)
+                          precision=pmap[prec])
         if prec == "mixed" and not args.no_tune:
             g.set_option("autotune", 1)
         t1 = time.perf_counter()
@@ -187,11 +189,28 @@ def main_fit(args):
         dt = time.perf_counter() - t1
         o, sec, kind = g.fit_trace()
         res[prec] = {"seconds": dt, "final": obj, "trace": (o, sec, kind), **_trace_summary(o, sec, kind)}
+        res[prec]["X"] = g.get("X")[:ds.N * ds.Q].reshape(ds.Q, ds.N).T.copy()
         log(f"{prec} fit: {dt:.2f} s, final objective {obj:.10e}, "
             f"{res[prec]['lbfgs_steps']} L-BFGS steps ({res[prec]['lbfgs_s']:.2f} s), "
             f"{res[prec]['irls_steps']} IRLS steps ({res[prec]['irls_s']:.2f} s)")
         g.close()
     f64, mx = res["f64"], res["mixed"]
+
+    # latent posterior means after orthogonal Procrustes alignment onto the
+    # FP64 fit's
+    def xdiff(a):
+        U, _, Vt = np.linalg.svd(a.T @ f64["X"])
+        xa = a @ (U @ Vt)
+        n = np.linalg.norm(f64["X"])
+        return float(np.linalg.norm(a - f64["X"]) / n), float(np.linalg.norm(xa - f64["X"]) / n)
+    x_rel_raw, x_rel = xdiff(mx["X"])
+    log(f"latent means, mixed vs FP64 fit: {x_rel_raw:.2e} relative, {x_rel:.2e} after Procrustes")
+    xw = None
+    if "f64w" in res:
+        xw = xdiff(res["f64w"]["X"])
+        log(f"latent means, FP64 whitened-basis vs FP64 fit: {xw[0]:.2e} relative, {xw[1]:.2e} after "
+            f"Procrustes; final objectives differ by "
+            f"{(res['f64w']['final'] - f64['final']) / abs(f64['final']):.2e}")
     from gaspacho2_amd.warmstart import steps_to_reach
     n_hit, t_hit = steps_to_reach(mx["trace"], f64["final"])
     gap = (mx["final"] - f64["final"]) / abs(f64["final"])
@@ -202,9 +221,14 @@ def main_fit(args):
            "data": "synthetic (seed 0), PCA initial latent",
            "config": {"workload": f"sparse GP-LVM fit, N={ds.N} cells, G={ds.J} genes, Q={ds.Q}, M={ds.M}, "
                                   f"{args.fit_itr} outer iteration(s), L-BFGS cap {args.fit_cap}"},
-           "mixed": {k: v for k, v in mx.items() if k != "trace"},
-           "fp64_plain": {k: v for k, v in f64.items() if k != "trace"},
+           "mixed": {k: v for k, v in mx.items() if k not in ("trace", "X")},
+           "fp64_plain": {k: v for k, v in f64.items() if k not in ("trace", "X")},
            "final_rel_gap_mixed_vs_fp64": gap,
+           "latent_rel_diff_mixed_vs_fp64": {"raw": x_rel_raw, "procrustes": x_rel},
+           "fp64_whitened_vs_fp64": (None if xw is None else
+                                     {"latent_raw": xw[0], "latent_procrustes": xw[1],
+                                      "final_rel_gap": (res["f64w"]["final"] - f64["final"]) / abs(f64["final"]),
+                                      "seconds": res["f64w"]["seconds"]}),
            "mixed_reaches_fp64_final": {"seconds": t_hit, "lbfgs_steps": n_hit,
                                         "speedup_vs_fp64_fit": (f64["seconds"] / t_hit) if t_hit else None},
            "speedup_same_schedule": f64["seconds"] / mx["seconds"]}
@@ -376,6 +400,8 @@ def main():
     ap.add_argument("--fit-itr", type=int, default=1, help="fit workload: outer iterations")
     ap.add_argument("--verbose", action="store_true", help="fit workload: print every objective")
     ap.add_argument("--fit-cap", type=int, default=100, help="fit workload: L-BFGS steps per outer iteration")
+    ap.add_argument("--fit-f64w", action="store_true",
+                    help="fit workload: also fit in FP64 on the whitened basis (rounding-only divergence)")
     ap.add_argument("--train-seeds", type=int, default=8, help="warmstart workload: training fits")
     ap.add_argument("--enc-epochs", type=int, default=30, help="warmstart workload: encoder epochs")
     ap.add_argument("--enc-target", default="residual,aligned",
