@@ -521,7 +521,8 @@ def main():
     traffic, tsrc = measured_traffic(dom + "_kernel", main_p, (ds.N, ds.J, ds.Q, ds.M))
     if main_p == "mixed":
         kname = {"gram": "k_gram_sym4" if g.get_option("gram_waves") == 4 else "k_gram_sym",
-                 "gradcontract": {0: "k_gc_mixed", 1: "k_gc_rows", 2: "k_gc_rows2"}[g.get_option("gc_form")]}[dom]
+                 "gradcontract": {0: "k_gc_mixed", 1: "k_gc_rows", 2: "k_gc_rows2",
+                                  3: "k_gc_rows4"}[g.get_option("gc_form")]}[dom]
     else:
         kname = {"gram": "k_gram_mfma_f64", "gradcontract": "k_gc_mfma_f64"}[dom]
     roofline = {"kernel": f"{kname}<{(P + 15) // 16}> ({dom})", "bound": "mfma", "achieved": achieved / 1e12,
@@ -529,6 +530,9 @@ def main():
                 "traffic": traffic, "traffic_source": tsrc,
                 "flops_per_launch": gram_flops(ds.N, ds.J, P) if dom == "gram" else gradcontract_flops(ds.N, ds.J, P),
                 "kernel_ms": kern[dom + "_kernel"],
+                # HBM rate of the same kernel: measured bytes per launch over its time
+                "hbm_gbs": (traffic / (kern[dom + "_kernel"] * 1e-3) / 1e9) if traffic else None,
+                "hbm_frac": (traffic / (kern[dom + "_kernel"] * 1e-3) / 8e12) if traffic else None,
                 "others": {k: {"achieved_tflops": v / 1e12, "frac": v / peaks[k], "avg_ms": kern[k + "_kernel"]}
                            for k, v in cands.items()}}
     ms_step = r["t"] / args.steps * 1e3
