@@ -10,14 +10,15 @@
 //
 // Persistent launch: k_bf_pass runs a fixed grid of workgroups that pull genes
 // from a queue (largest subsets first).  For one gene, a workgroup streams its
-// cells in chunks of 32: it gathers the design rows (Knm rows from a row-major
-// copy, 1 KB each, plus the gene's covariates), computes eta = z.beta, W, nu
-// and the likelihood terms, scales the rows to x_i = sqrt(W_i/Omega_i)
-// [z_i | nu_i] in LDS, and accumulates X^T X with v_mfma_f64_16x16x4f64 --
-// Psi_g - Delta, b_g = Z^T D nu and nu^T D nu come out of one symmetric Gram.
-// The upper 16x16 tiles of the (P_g+1)-wide Gram are dealt round robin to the
-// four waves.  Then a batched Cholesky (fixed padded size PB, identity
-// padding) and k_bf_beta (solve, damped update, phi) finish the step.
+// cells in double-buffered chunks of 32: producer waves gather the design rows
+// (Knm rows from a row-major copy, 1 KB each, plus the gene's covariates),
+// compute eta = z.beta, W, nu and the likelihood terms and scale the rows to
+// x_i = sqrt(W_i/Omega_i) [z_i | nu_i] in LDS, while four MFMA waves
+// accumulate X^T X of the previous chunk with v_mfma_f64_16x16x4f64 --
+// Psi_g - Delta, b_g = Z^T D nu and nu^T D nu come out of one symmetric Gram
+// (its upper 16x16 tiles dealt round robin to the MFMA waves).  Then a batched
+// Cholesky (fixed padded size PB, identity padding) and k_bf_beta (solve,
+// damped update, phi) finish the step.
 #include "gp_internal.h"
 
 #include <algorithm>
