@@ -16,9 +16,13 @@ namespace {
 
 constexpr int CT = 256;   // threads per workgroup
 constexpr int NB = 16;    // panel width
-constexpr int KC = 64;    // k-chunk of previous columns staged in LDS
+// k-chunk of previous columns staged in LDS: 48 keeps a P = 257 workgroup at
+// 39 KB of LDS, four per CU (64 made it 41 KB, three per CU)
+constexpr int KC = 48;
 constexpr int MAXN = 640; // largest n handled (panel rows in LDS)
+constexpr int PLD = 8;    // panel-load batch per thread
 static_assert(KC % NB == 0, "k-chunks hold whole panels");
+static_assert(NB * KC % CT == 0, "the k-chunk stage splits evenly over the threads");
 
 __global__ void __launch_bounds__(CT) k_chol(double *__restrict__ Aall, int n, size_t stride,
                                              int *__restrict__ info) {
@@ -34,10 +38,20 @@ __global__ void __launch_bounds__(CT) k_chol(double *__restrict__ Aall, int n, s
     for (int kb = 0; kb < n; kb += NB) {
         const int nbk = min(NB, n - kb);
         const int rows = n - kb;
-        // 1. load the panel's lower part
-        for (int t = tid; t < rows * nbk; t += CT) {
-            const int r = t % rows, cc = t / rows;
-            pan[r * NB + cc] = (r >= cc) ? a[(kb + r) + (size_t)(kb + cc) * n] : 0.0;
+        // 1. load the panel's lower part, PLD loads in flight per thread
+        for (int t0 = tid; t0 < rows * nbk; t0 += CT * PLD) {
+            double v[PLD];
+#pragma unroll
+            for (int u = 0; u < PLD; u++) {
+                const int t = t0 + u * CT;
+                const int r = t % rows, cc = t / rows;
+                v[u] = (t < rows * nbk && r >= cc) ? a[(kb + r) + (size_t)(kb + cc) * n] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < PLD; u++) {
+                const int t = t0 + u * CT;
+                if (t < rows * nbk) pan[(t % rows) * NB + t / rows] = v[u];
+            }
         }
         __syncthreads();
         // 2. subtract the contribution of the columns already factored
@@ -49,9 +63,20 @@ __global__ void __launch_bounds__(CT) k_chol(double *__restrict__ Aall, int n, s
             for (int k0 = 0; k0 < kb; k0 += KC) {
                 const int kn = min(KC, kb - k0);
                 __syncthreads();
-                for (int t = tid; t < nbk * kn; t += CT) {
-                    const int cc = t / kn, kk = t % kn;
-                    lc[cc * KC + kk] = a[(kb + cc) + (size_t)(k0 + kk) * n];
+                {
+                    // nbk * kn <= NB * KC = 4 * CT: every load in flight at once
+                    double v[NB * KC / CT];
+#pragma unroll
+                    for (int u = 0; u < NB * KC / CT; u++) {
+                        const int t = tid + u * CT;
+                        const int cc = t / kn, kk = t % kn;
+                        v[u] = (t < nbk * kn) ? a[(kb + cc) + (size_t)(k0 + kk) * n] : 0.0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < NB * KC / CT; u++) {
+                        const int t = tid + u * CT;
+                        if (t < nbk * kn) lc[(t / kn) * KC + t % kn] = v[u];
+                    }
                 }
                 __syncthreads();
                 if (r < rows) {
