@@ -310,7 +310,8 @@ __global__ void __launch_bounds__(CT) k_trinv_mfma(const double *__restrict__ La
 }
 
 // S_j = V_j^T V_j (full symmetric) from the lower-triangular V_j: 64x64 tiles
-// of the upper triangle, 32-row chunks of V staged in LDS, 4x4 per thread.
+// of the upper triangle, 32-row chunks of V staged in LDS (the next chunk
+// prefetched into registers), 4x4 per thread.
 // Entry (a, b), a <= b, sums rows k >= b only (V[k, b] = 0 above the diagonal).
 __global__ void __launch_bounds__(CT) k_vtv(const double *__restrict__ Vall, int n, size_t stride,
                                             int ntile, double *__restrict__ Sall) {
@@ -328,17 +329,33 @@ __global__ void __launch_bounds__(CT) k_vtv(const double *__restrict__ Vall, int
     for (int r = 0; r < 4; r++)
 #pragma unroll
         for (int q = 0; q < 4; q++) acc[r][q] = 0.0;
-    for (int k0 = b0; k0 < n; k0 += 32) {
-        __syncthreads();
-        const int row = tid % 32, cg = tid / 32;
+    // the next chunk's 16 loads per thread are issued before the current
+    // chunk's products, so their latency hides behind them
+    // thread t stages column t % 64 of both tiles, rows 8 (t / 64) .. +7 of
+    // the chunk: eight loads off one address per tile
+    const int col = tid % 64, rg = 8 * (tid / 64);
+    const int ga = a0 + col, gb = b0 + col;
+    const double *va = V + (size_t)min(ga, n - 1) * n;
+    const double *vb = V + (size_t)min(gb, n - 1) * n;
+    double pa[8], pb[8];
+    auto fetch = [&](int k0) {
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-            const int col = cg * 8 + u, k = k0 + row;
-            const int ga = a0 + col, gb = b0 + col;
-            As[row][col] = (k < n && ga < n && k >= ga) ? V[k + (size_t)ga * n] : 0.0;
-            Bs[row][col] = (k < n && gb < n && k >= gb) ? V[k + (size_t)gb * n] : 0.0;
+            const int k = k0 + rg + u;
+            pa[u] = (k < n && ga < n && k >= ga) ? va[k] : 0.0;
+            pb[u] = (k < n && gb < n && k >= gb) ? vb[k] : 0.0;
+        }
+    };
+    fetch(b0);
+    for (int k0 = b0; k0 < n; k0 += 32) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            As[rg + u][col] = pa[u];
+            Bs[rg + u][col] = pb[u];
         }
         __syncthreads();
+        if (k0 + 32 < n) fetch(k0 + 32);
 #pragma unroll 8
         for (int kk = 0; kk < 32; kk++) {
             double av[4], bv[4];
