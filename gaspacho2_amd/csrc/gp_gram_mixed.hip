@@ -47,9 +47,21 @@ __global__ void __launch_bounds__(256) k_split_z(int Npad, int P, const double *
     __shared__ float scale;
     const int a = blockIdx.x;
     const double *za = Z + (size_t)a * Npad;
+    // each pass keeps SZU loads per thread in flight: with one workgroup per
+    // column there is only about one wave per SIMD to hide their latency
+    constexpr int SZU = 8;
     double mx = 0.0;
-    if (a < P)
-        for (int i = threadIdx.x; i < Npad; i += 256) mx = fmax(mx, fabs(za[i]));
+    if (a < P) {
+        int i = threadIdx.x;
+        for (; i + (SZU - 1) * 256 < Npad; i += SZU * 256) {
+            double v[SZU];
+#pragma unroll
+            for (int u = 0; u < SZU; u++) v[u] = za[i + u * 256];
+#pragma unroll
+            for (int u = 0; u < SZU; u++) mx = fmax(mx, fabs(v[u]));
+        }
+        for (; i < Npad; i += 256) mx = fmax(mx, fabs(za[i]));
+    }
     for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
     __syncthreads();
@@ -65,7 +77,23 @@ __global__ void __launch_bounds__(256) k_split_z(int Npad, int P, const double *
     const float sc = scale;
     _Float16 *h = zh + (size_t)a * Npad, *l = zl + (size_t)a * Npad;
     float *f = z32 + (size_t)a * Npad;
-    for (int i = threadIdx.x; i < Npad; i += 256) {
+    int i = threadIdx.x;
+    if (a < P) {
+        for (; i + (SZU - 1) * 256 < Npad; i += SZU * 256) {
+            double zv[SZU];
+#pragma unroll
+            for (int u = 0; u < SZU; u++) zv[u] = za[i + u * 256];
+#pragma unroll
+            for (int u = 0; u < SZU; u++) {
+                const float v = (float)(zv[u] * (double)sc);
+                const _Float16 hv = (_Float16)v;
+                h[i + u * 256] = hv;
+                l[i + u * 256] = (_Float16)(v - (float)hv);
+                f[i + u * 256] = v;
+            }
+        }
+    }
+    for (; i < Npad; i += 256) {
         const float v = (a < P) ? (float)(za[i] * (double)sc) : 0.0f;
         const _Float16 hv = (_Float16)v;
         h[i] = hv;
@@ -82,8 +110,25 @@ __global__ void __launch_bounds__(256) k_wprep(int N, int Npad, const double *__
     const int j = blockIdx.x;
     const double *Wj = W + (size_t)j * Npad;
     float *wj = w32 + (size_t)j * Npad;
+    // WPU cells per thread loaded before any is used (2 WPU loads in flight)
+    constexpr int WPU = 4;
     float mx = 0.0f;
-    for (int i = threadIdx.x; i < Npad; i += 256) {
+    int i = threadIdx.x;
+    for (; i + (WPU - 1) * 256 < N; i += WPU * 256) {
+        double w[WPU], om[WPU];
+#pragma unroll
+        for (int u = 0; u < WPU; u++) {
+            w[u] = Wj[i + u * 256];
+            om[u] = Omega[i + u * 256];
+        }
+#pragma unroll
+        for (int u = 0; u < WPU; u++) {
+            const float v = (float)(w[u] / om[u]);
+            wj[i + u * 256] = v;
+            mx = fmaxf(mx, fabsf(v));
+        }
+    }
+    for (; i < Npad; i += 256) {
         const float v = (i < N) ? (float)(Wj[i] / Omega[i]) : 0.0f;
         wj[i] = v;
         mx = fmaxf(mx, fabsf(v));

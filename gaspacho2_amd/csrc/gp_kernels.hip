@@ -205,14 +205,34 @@ __global__ void k_nu_sparse(int Npad, const int *Yp, const int *Yi, const double
 }
 
 // R = (W/Omega) nu, gq_j = sum nu^2 W / Omega, gs_j = sum log(Omega / W).
-__global__ void __launch_bounds__(NT) k_r_sums(int N, int Npad, const double *W,
-        const double *Nu, const double *Omega, double *R, double *gq, double *gs) {
+// A thread loads RSU of its cells before using any of them (3 RSU loads in
+// flight instead of 3); its sums still run in cell order.
+constexpr int RSU = 4;
+__global__ void __launch_bounds__(NT) k_r_sums(int N, int Npad, const double *__restrict__ W,
+        const double *__restrict__ Nu, const double *__restrict__ Omega, double *__restrict__ R,
+        double *__restrict__ gq, double *__restrict__ gs) {
     __shared__ double sh[4];
     const int j = blockIdx.x;
     const double *Wj = W + (size_t)j * Npad, *nj = Nu + (size_t)j * Npad;
     double *Rj = R + (size_t)j * Npad;
     double q = 0.0, s = 0.0;
-    for (int i = threadIdx.x; i < Npad; i += NT) {
+    int i = threadIdx.x;
+    for (; i + (RSU - 1) * NT < N; i += RSU * NT) {
+        double w[RSU], nu[RSU], om[RSU];
+#pragma unroll
+        for (int u = 0; u < RSU; u++) {
+            w[u] = Wj[i + u * NT];
+            nu[u] = nj[i + u * NT];
+            om[u] = Omega[i + u * NT];
+        }
+#pragma unroll
+        for (int u = 0; u < RSU; u++) {
+            Rj[i + u * NT] = w[u] / om[u] * nu[u];
+            q += nu[u] * nu[u] * w[u] / om[u];
+            s += log(om[u] / w[u]);
+        }
+    }
+    for (; i < Npad; i += NT) {
         if (i < N) {
             const double w = Wj[i], nu = nj[i], om = Omega[i];
             Rj[i] = w / om * nu;
@@ -437,10 +457,14 @@ __global__ void k_grad_delta_sum(int p, const double *part, double *gd) {
 // E_i = sum_j tn^2 / W / phi / 2, TN_ij = tn_ij / phi_j.  The genes are cut
 // into TN_SPLITS ranges (blockIdx.y) for occupancy; Epart holds the partial
 // sums of E, added in order by k_tn_sum.
+// A thread loads W, Nu, ZB and phi of TNU genes before using any of them
+// (4 TNU loads in flight instead of 4); the sums still run in gene order.
 constexpr int TN_SPLITS = GP_TN_SPLITS;
-__global__ void k_tn(int N, int Npad, int J, const double *W, const double *Nu,
-                     const double *ZB, const double *Omega, const double *phi,
-                     double *TN, double *Epart) {
+constexpr int TNU = 8;
+__global__ void k_tn(int N, int Npad, int J, const double *__restrict__ W,
+                     const double *__restrict__ Nu, const double *__restrict__ ZB,
+                     const double *__restrict__ Omega, const double *__restrict__ phi,
+                     double *__restrict__ TN, double *__restrict__ Epart) {
     const int i = blockIdx.x * NT + threadIdx.x;
     const int j0 = (int)((long long)J * blockIdx.y / TN_SPLITS);
     const int j1 = (int)((long long)J * (blockIdx.y + 1) / TN_SPLITS);
@@ -451,7 +475,26 @@ __global__ void k_tn(int N, int Npad, int J, const double *W, const double *Nu,
     }
     const double om = Omega[i];
     double e = 0.0;
-    for (int j = j0; j < j1; j++) {
+    int j = j0;
+    for (; j + TNU <= j1; j += TNU) {
+        double w[TNU], nu[TNU], zb[TNU], ph[TNU];
+#pragma unroll
+        for (int u = 0; u < TNU; u++) {
+            const size_t a = i + (size_t)(j + u) * Npad;
+            w[u] = W[a];
+            nu[u] = Nu[a];
+            zb[u] = ZB[a];
+            ph[u] = phi[j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < TNU; u++) {
+            const double o = w[u] / om;
+            const double tn = o * nu[u] - o * zb[u];
+            e += tn * tn / w[u] / ph[u] / 2.0;
+            TN[i + (size_t)(j + u) * Npad] = tn / ph[u];
+        }
+    }
+    for (; j < j1; j++) {
         const size_t a = i + (size_t)j * Npad;
         const double w = W[a], o = w / om;
         const double tn = o * Nu[a] - o * ZB[a];
