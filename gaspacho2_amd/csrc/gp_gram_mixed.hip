@@ -871,8 +871,18 @@ __global__ void __launch_bounds__(256) k_gram_reduce_sym(int P, int T16, int nse
     const int j0 = blockIdx.y * RSG, j1 = min(J, j0 + RSG);
     for (int j = j0; j < j1; j++) {
         if (j > j0) __syncthreads();              // the previous gene's tile reads are done
+        // the segments' partials in order, GRU loads in flight at a time
+        constexpr int GRU = 8;
         double s = 0.0;
-        for (int g = 0; g < nseg; g++) s += (double)part[(((size_t)g * J + j) * NT + t) * 256 + tid];
+        int g = 0;
+        for (; g + GRU <= nseg; g += GRU) {
+            float pv[GRU];
+#pragma unroll
+            for (int u = 0; u < GRU; u++) pv[u] = part[(((size_t)(g + u) * J + j) * NT + t) * 256 + tid];
+#pragma unroll
+            for (int u = 0; u < GRU; u++) s += (double)pv[u];
+        }
+        for (; g < nseg; g++) s += (double)part[(((size_t)g * J + j) * NT + t) * 256 + tid];
         double v = 0.0;
         if (row < P && col < P) v = s * (wun[j] * zun[row] * zun[col]) + Delta[row + (size_t)col * P];
         tile[rr][cc] = v;
