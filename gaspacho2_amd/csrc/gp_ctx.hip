@@ -69,6 +69,10 @@ static void free_all(gp_ctx *c) {
         if (t->b) (void)hipEventDestroy(t->b);
     }
     if (c->blas) rocblas_destroy_handle(c->blas);
+    if (c->blas2) rocblas_destroy_handle(c->blas2);
+    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -114,6 +118,13 @@ extern "C" int gp_create(gp_ctx **out, int N, int J, int M, int Q, int p, const 
     if (e != hipSuccess) { gp_set_error("hipStreamCreate failed"); delete c; return (int)e; }
     if (rocblas_create_handle(&c->blas) != rocblas_status_success) { gp_set_error("rocblas_create_handle failed"); free_all(c); delete c; return 950; }
     rocblas_set_stream(c->blas, c->stream);
+    if ((e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming)) != hipSuccess) {
+        gp_set_error(std::string("side stream: ") + hipGetErrorString(e)); free_all(c); delete c; return (int)e;
+    }
+    if (rocblas_create_handle(&c->blas2) != rocblas_status_success) { gp_set_error("rocblas_create_handle failed"); free_all(c); delete c; return 950; }
+    rocblas_set_stream(c->blas2, c->stream2);
     ALLOC(c->Yp, (size_t)J + 1);
     ALLOC(c->Yi, (size_t)L + 1);
     ALLOC(c->Yx, (size_t)L + 1);
@@ -544,6 +555,38 @@ extern "C" int gp_elbo(gp_ctx *c, int slot, double *obj) {
     return bad ? -bad : 0;
 }
 
+// For its scope, the context's stream and rocBLAS handle are the side ones,
+// so the gpk_* launchers enqueue there.
+struct SideStream {
+    gp_ctx *c;
+    explicit SideStream(gp_ctx *cc) : c(cc) { swap_in(); }
+    ~SideStream() { swap_in(); }
+    void swap_in() {
+        std::swap(c->stream, c->stream2);
+        std::swap(c->blas, c->blas2);
+    }
+};
+
+// The gradient has two independent chains once tB is known: tBeta in the
+// reference basis plus the per-cell tn pieces (ZB, TN, E, C1a), which need
+// only tB, and the inverses, delta gradient and contraction.  The first runs
+// on the side stream, forked here, beside the latency-bound inverse and
+// contraction set-up kernels that leave most of the chip idle (run beside the
+// MFMA-bound contraction kernel instead, it slowed that kernel by more than
+// it saved).  It writes tBeta, ZB, R, Epart, Ecell and C1a, none of which the
+// main chain reads; the cell gradient joins it.
+static int fork_tn_chain(gp_ctx *c) {
+    GP_HIP(hipEventRecord(c->ev_fork, c->stream));
+    GP_HIP(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+    {
+        SideStream side(c);
+        if (c->psi_whitened) GP_TRY(gpk_unwhiten_beta(c));
+        GP_TRY(gpk_tn(c));
+    }
+    GP_HIP(hipEventRecord(c->ev_join, c->stream2));
+    return 0;
+}
+
 extern "C" int gp_grad(gp_ctx *c) {
     if (!c) { gp_set_error("NULL context"); return 1; }
     gp_timer_begin(c, c->t_grad);
@@ -561,6 +604,7 @@ extern "C" int gp_grad(gp_ctx *c) {
         GP_TRY(gpk_r_and_sums(c, w ? c->Zw : c->Z));
         GP_TRY(gpk_cholsl_batched(c, c->Psi, c->P, (size_t)c->P * c->P, c->bvec, tB, c->J));
     }
+    GP_TRY(fork_tn_chain(c));
     // solvePsi (explicit inverses, needed by the delta gradient, U and C2)
     GP_TRY(gpk_chol_inverse_batched(c, c->Psi, c->P, c->J, (size_t)c->P * c->P, c->Sinv));
     GP_TRY(gpk_grad_delta(c, w));
@@ -570,8 +614,8 @@ extern "C" int gp_grad(gp_ctx *c) {
     else
         GP_TRY(gpk_gradcontract(c, w ? c->Zw : c->Z, w ? c->Uw : c->U));
     gp_timer_end(c, c->t_gc);
-    if (w) GP_TRY(gpk_unwhiten_grad(c));
-    GP_TRY(gpk_tn(c));
+    if (w) GP_TRY(gpk_unwhiten_u(c));
+    GP_HIP(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     GP_TRY(gpk_cell_grad(c));
     GP_TRY(gpk_c2(c, w));
     GP_TRY(gpk_grad_tau_rho(c));

@@ -27,6 +27,12 @@ struct gp_ctx {
     int dev = 0;
     hipStream_t stream = nullptr;
     rocblas_handle blas = nullptr;
+    // side stream (default priority) and its rocBLAS handle: the gradient's
+    // per-cell tn pieces run there while the inverse and the contraction run
+    // on the main stream; fork and join through the two events
+    hipStream_t stream2 = nullptr;
+    rocblas_handle blas2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
     int N = 0, J = 0, M = 0, Q = 0, P = 0, p = 0, g0 = 0, Npad = 0;
     long long L = 0;
@@ -182,7 +188,8 @@ int gpk_lkhd_genes_with(gp_ctx *c, const double *tb, const double *Dchol);  // t
 int gpk_collect(gp_ctx *c, int Nterm, int slot, double *obj);
 int gpk_grad_delta(gp_ctx *c, int whitened);
 int gpk_whiten(gp_ctx *c);                    // Linv, Zw from the Kmm factor
-int gpk_unwhiten_grad(gp_ctx *c);             // tBeta, U from tBw, Uw
+int gpk_unwhiten_beta(gp_ctx *c);             // tBeta from tBw
+int gpk_unwhiten_u(gp_ctx *c);                // U from Uw
 int gpk_tn(gp_ctx *c);
 int gpk_cell_grad(gp_ctx *c);
 int gpk_c2(gp_ctx *c, int whitened);

@@ -1182,13 +1182,18 @@ int gpk_whiten(gp_ctx *c) {
 
 // Back to the reference basis: tBeta = T^-T tBw and U = Uw T^-1 with
 // T = blockdiag(I, L):  GP rows of tBeta = L^-T tBw_GP, GP columns of U = Uw_GP L^-1.
-int gpk_unwhiten_grad(gp_ctx *c) {
+int gpk_unwhiten_beta(gp_ctx *c) {
     const double one = 1.0, zero = 0.0;
     GP_HIP(hipMemcpy2DAsync(c->tBeta, c->P * sizeof(double), c->tBw, c->P * sizeof(double),
                             c->g0 * sizeof(double), c->J, hipMemcpyDeviceToDevice, c->stream));
     GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_transpose, rocblas_operation_none, c->M, c->J,
                           c->M, &one, c->Linv, c->M, c->tBw + c->g0, c->P, &zero,
                           c->tBeta + c->g0, c->P));
+    return 0;
+}
+
+int gpk_unwhiten_u(gp_ctx *c) {
+    const double one = 1.0, zero = 0.0;
     GP_HIP(hipMemcpyAsync(c->U, c->Uw, (size_t)c->Npad * c->g0 * sizeof(double),
                           hipMemcpyDeviceToDevice, c->stream));
     GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, c->Npad, c->M,

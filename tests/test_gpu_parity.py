@@ -401,7 +401,11 @@ def test_contraction_row_block_form_matches_16_cell_form(shape):
 def test_autotune_picks_valid_settings_and_keeps_results():
     """gp_autotune picks gram_splits / gc_groups among its candidates and the
     mixed ELBO and gradient after tuning stay within the mixed-path error of
-    the FP64 values."""
+    the FP64 values.  The pick depends on timings, so every Gram split count it
+    can choose is also checked: on this dataset the gradient error (all in the
+    inducing-point block) is 8.3e-5, 1.26e-4 and 8.5e-5 of max |grad| at 8, 16
+    and 32 splits -- where the FP32 partial sums are cut changes their
+    rounding -- and the contraction settings move it by < 1e-7."""
     from gaspacho2_amd import lib, synth
     ds = synth.make_dataset(3000, 40, 5, 64, seed=4)
     T0 = lib.init_ta_with_per(ds.M, ds.Q)
@@ -425,7 +429,13 @@ def test_autotune_picks_valid_settings_and_keeps_results():
     em = g.elbo(1)
     g.grad()
     assert abs(em - e64) <= 1e-7 * abs(e64)
-    assert rel(g.get("grad"), g64) < 1e-4
+    assert rel(g.get("grad"), g64) < 2e-4
+    for splits in (8, 16, 32):
+        g.set_option("gram_splits", splits)
+        em = g.elbo(1)
+        g.grad()
+        assert abs(em - e64) <= 1e-7 * abs(e64), splits
+        assert rel(g.get("grad"), g64) < 2e-4, splits
 
 
 @pytest.mark.parametrize("precision", [0, 1])
