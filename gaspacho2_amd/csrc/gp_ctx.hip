@@ -72,6 +72,8 @@ static void free_all(gp_ctx *c) {
     if (c->blas2) rocblas_destroy_handle(c->blas2);
     if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
+    if (c->ev_efork) (void)hipEventDestroy(c->ev_efork);
+    if (c->ev_ejoin) (void)hipEventDestroy(c->ev_ejoin);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -120,7 +122,9 @@ extern "C" int gp_create(gp_ctx **out, int N, int J, int M, int Q, int p, const 
     rocblas_set_stream(c->blas, c->stream);
     if ((e = hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming)) != hipSuccess) {
+        (e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&c->ev_efork, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&c->ev_ejoin, hipEventDisableTiming)) != hipSuccess) {
         gp_set_error(std::string("side stream: ") + hipGetErrorString(e)); free_all(c); delete c; return (int)e;
     }
     if (rocblas_create_handle(&c->blas2) != rocblas_status_success) { gp_set_error("rocblas_create_handle failed"); free_all(c); delete c; return 950; }
@@ -426,9 +430,10 @@ static int read_info(gp_ctx *c, int *bad) {
 static int clear_info(gp_ctx *c) { return gpk_fill_int(c, c->info, 1, 0); }
 
 // Kernel matrices: getKernelMats, cholKmm, explicit Kmm^{-1}, getKinvKmn.
-static int kernel_mats(gp_ctx *c) {
+static int kernel_mats(gp_ctx *c, int (*after_kuf)(gp_ctx *) = nullptr) {
     gp_timer_begin(c, c->t_kuf);
     GP_TRY(gpk_kuf(c));
+    if (after_kuf) GP_TRY(after_kuf(c));
     c->zsplit_valid = 0;
     c->zw_valid = 0;
     GP_TRY(gpk_chol_batched(c, c->Kchol, c->M, 1, 0, c->info + 1));
@@ -440,11 +445,23 @@ static int kernel_mats(gp_ctx *c) {
     return 0;
 }
 
+// For its scope, the context's stream and rocBLAS handle are the side ones,
+// so the gpk_* launchers enqueue there.
+struct SideStream {
+    gp_ctx *c;
+    explicit SideStream(gp_ctx *cc) : c(cc) { swap_in(); }
+    ~SideStream() { swap_in(); }
+    void swap_in() {
+        std::swap(c->stream, c->stream2);
+        std::swap(c->blas, c->blas2);
+    }
+};
+
 // Weighted Grams + batched Cholesky.  mixed: FP16 hi/lo MFMA Gram in the
 // whitened basis [covariates | Knm L^-T] with the diagonal prior DeltaW, whose
 // smallest eigenvalue is min(1/delta) -- well conditioned for the FP32-accumulated
 // Gram; otherwise the FP64 Gram in the reference basis.
-static int gram_chol(gp_ctx *c, int precision) {
+static int gram_chol(gp_ctx *c, int precision, bool weights_ready = false) {
     gp_timer_begin(c, c->t_gram);
     if (precision != GP_PREC_FP64) {
         if (!c->zw_valid) {
@@ -453,7 +470,7 @@ static int gram_chol(gp_ctx *c, int precision) {
             c->zsplit_valid = 0;
         }
         if (precision == GP_PREC_MIXED)
-            GP_TRY(gpk_gram_mixed(c, c->W, c->Psi, c->J, c->Zw, c->DeltaW));
+            GP_TRY(gpk_gram_mixed(c, c->W, c->Psi, c->J, c->Zw, c->DeltaW, weights_ready));
         else
             GP_TRY(gpk_gram_src(c, c->W, c->Psi, c->J, c->Zw, c->DeltaW));
         c->psi_whitened = 1;
@@ -537,16 +554,51 @@ extern "C" int gp_ql_start(gp_ctx *c, double *obj) {
     return 0;
 }
 
+// The ELBO's side chain (see gp_elbo): Oinvt, and for the mixed Gram the
+// FP32 weights, on the side stream from the point this is called.
+static int fork_weights_impl(gp_ctx *c, bool mixed) {
+    GP_HIP(hipEventRecord(c->ev_efork, c->stream));
+    GP_HIP(hipStreamWaitEvent(c->stream2, c->ev_efork, 0));
+    c->efork_pending = 1;
+    {
+        SideStream side(c);
+        GP_TRY(gpk_oinvt(c));
+        if (mixed) GP_TRY(gpk_wprep(c, c->W));
+    }
+    GP_HIP(hipEventRecord(c->ev_ejoin, c->stream2));
+    return 0;
+}
+static int fork_weights(gp_ctx *c) { return fork_weights_impl(c, false); }
+static int fork_weights_mixed(gp_ctx *c) { return fork_weights_impl(c, true); }
+
 extern "C" int gp_elbo(gp_ctx *c, int slot, double *obj) {
     if (!c) { gp_set_error("NULL context"); return 1; }
     c->lkhd_fresh = 0;   // set again by the likelihood evaluation below
     if (slot < 0 || slot >= GP_LKHD_SLOTS) { gp_set_error("slot out of range"); return 1; }
     gp_timer_begin(c, c->t_elbo);
     GP_TRY(clear_info(c));
-    GP_TRY(gpk_oinvt(c));
-    GP_TRY(kernel_mats(c));
-    GP_TRY(gpk_expand_delta(c));
-    GP_TRY(gram_chol(c, c->precision));
+    {
+        // Oinvt and (mixed) the FP32 weights W/Omega depend on neither the
+        // kernel matrices nor the basis: forked to the side stream after the
+        // HBM-bound Kuf, they run beside the single-matrix Kmm factor and its
+        // small follow-ups, which leave the chip idle, and are joined before
+        // the Gram; the join also runs on an early error return, so no side
+        // work outlives the call unjoined
+        const bool mixed = c->precision == GP_PREC_MIXED;
+        struct Join {
+            gp_ctx *c;
+            ~Join() {
+                if (c->efork_pending) (void)hipStreamWaitEvent(c->stream, c->ev_ejoin, 0);
+                c->efork_pending = 0;
+            }
+        };
+        {
+            Join join{c};
+            GP_TRY(kernel_mats(c, mixed ? fork_weights_mixed : fork_weights));
+            GP_TRY(gpk_expand_delta(c));
+        }
+        GP_TRY(gram_chol(c, c->precision, mixed));
+    }
     GP_TRY(lkhd_genes(c));
     GP_TRY(gpk_collect(c, c->N, slot, obj));
     gp_timer_end(c, c->t_elbo);
@@ -555,17 +607,6 @@ extern "C" int gp_elbo(gp_ctx *c, int slot, double *obj) {
     return bad ? -bad : 0;
 }
 
-// For its scope, the context's stream and rocBLAS handle are the side ones,
-// so the gpk_* launchers enqueue there.
-struct SideStream {
-    gp_ctx *c;
-    explicit SideStream(gp_ctx *cc) : c(cc) { swap_in(); }
-    ~SideStream() { swap_in(); }
-    void swap_in() {
-        std::swap(c->stream, c->stream2);
-        std::swap(c->blas, c->blas2);
-    }
-};
 
 // The gradient has two independent chains once tB is known: tBeta in the
 // reference basis plus the per-cell tn pieces (ZB, TN, E, C1a), which need

@@ -968,7 +968,7 @@ static int launch_gram_sym(gp_ctx *c, int seg, int nseg, int batch) {
 }
 
 int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const double *Zsrc,
-                   const double *Delta) {
+                   const double *Delta, bool weights_ready) {
     const int T16 = (c->P + 15) / 16, P16 = 16 * T16;
     const int nbk = (P16 + BLK - 1) / BLK;
     const int nbp = nbk * (nbk + 1) / 2;
@@ -980,8 +980,10 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
     int seg = (c->Npad + nseg - 1) / nseg;
     seg = (seg + GCH - 1) / GCH * GCH;
     nseg = (c->Npad + seg - 1) / seg;
-    k_wprep<<<batch, 256, 0, c->stream>>>(c->N, c->Npad, W, c->Omega, c->w32, c->wsc, c->wun);
-    GP_HIP(hipGetLastError());
+    if (!weights_ready) {
+        k_wprep<<<batch, 256, 0, c->stream>>>(c->N, c->Npad, W, c->Omega, c->w32, c->wsc, c->wun);
+        GP_HIP(hipGetLastError());
+    }
     if (T16 <= MAX_SYM_T16) {
         int r = 0;
         gp_timer_begin(c, c->t_gramk);

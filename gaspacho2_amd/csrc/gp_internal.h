@@ -33,6 +33,9 @@ struct gp_ctx {
     hipStream_t stream2 = nullptr;
     rocblas_handle blas2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // the ELBO's fork: Oinvt and the FP32 weights beside the kernel matrices
+    hipEvent_t ev_efork = nullptr, ev_ejoin = nullptr;
+    int efork_pending = 0;   // the ELBO's side chain is forked and not yet joined
 
     int N = 0, J = 0, M = 0, Q = 0, P = 0, p = 0, g0 = 0, Npad = 0;
     long long L = 0;
@@ -230,8 +233,9 @@ int gpk_gram_src(gp_ctx *c, const double *W, double *Psi, int batch, const doubl
 
 // ---------------- gp_gram_mixed.hip ----------------
 int gpk_split_z(gp_ctx *c, const double *Zsrc);
+// weights_ready: w32 / wsc / wun already hold W/Omega for these genes (gpk_wprep)
 int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const double *Zsrc,
-                   const double *Delta);
+                   const double *Delta, bool weights_ready = false);
 // w32 = W/Omega in FP32 and the per-gene power-of-two scales (all genes)
 int gpk_wprep(gp_ctx *c, const double *W);
 
