@@ -168,8 +168,12 @@ double orc_fit(orc_state *s, int itrmax, int jtr_cap, int verbose);
  * Dim 0 equispaced on (0, 2*pi); other dims N(0,1) from MT19937 seeded as GSL
  * seeds seed 0 (-> 4357) and the GSL polar Box-Muller gaussian.
  * The MT19937 core is pinned by the published KAT (10000th output of the
- * 5489-seeded generator = 4123659995); the GSL uniform/gaussian wrapping is
- * parity unpinned (GSL is not in this image). */
+ * 5489-seeded generator = 4123659995); GSL's seed-0 stream and its uniform
+ * (output / 2^32) by the GSL manual's two example outputs (first value
+ * 4293858116; Poisson mu = 3 draws 2 5 5 2 1 0 3 4 1 1); the polar Gaussian by
+ * a normality test.  Parity unpinned: which coordinate of the accepted polar
+ * pair is returned (the second, as GSL's source does; GSL is not in this
+ * image, and the two choices have the same distribution). */
 void orc_init_ta_with_per(int M, int Q, double *T);
 uint32_t orc_mt19937_kat(uint32_t seed, int n);   /* n-th output */
 

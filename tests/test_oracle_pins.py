@@ -305,6 +305,48 @@ def test_mt19937_published_kat():
     assert O.load().orc_mt19937_kat(5489, 10000) == 4123659995
 
 
+def test_gsl_mt19937_seed0_published_first_value():
+    """GSL reference manual, random number generator example: the default
+    generator (mt19937) at the default seed 0 prints "first value = 4293858116".
+    initTaWithPer (util.c:73-87) seeds with 0, which GSL maps to 4357; the oracle
+    seeds its stream with 4357."""
+    assert O.load().orc_mt19937_kat(4357, 1) == 4293858116
+
+
+def test_gsl_uniform_published_poisson_example():
+    """GSL reference manual, random number distribution example: gsl_ran_poisson
+    with mu = 3 on the default mt19937 generator at seed 0 prints
+    2 5 5 2 1 0 3 4 1 1.  For mu < 10 GSL multiplies uniforms until the product
+    falls to exp(-mu); replaying that on the oracle's stream with the oracle's
+    uniform (output / 2^32) must give the published draws, which pins the
+    uniform conversion the polar Gaussian of initTaWithPer consumes."""
+    lib = O.load()
+    stream = iter(lib.orc_mt19937_kat(4357, n) for n in range(1, 200))
+    emu, draws = np.exp(-3.0), []
+    for _ in range(10):
+        prod, k = 1.0, 0
+        while True:
+            prod *= next(stream) / 4294967296.0
+            k += 1
+            if not prod > emu:
+                break
+        draws.append(k - 1)
+    assert draws == [2, 5, 5, 2, 1, 0, 3, 4, 1, 1]
+
+
+def test_init_ta_with_per_gaussian_distribution():
+    """The N(0,1) inducing-point coordinates (gsl_ran_gaussian, util.c:82) follow
+    the standard normal: a Kolmogorov-Smirnov test over 60,000 draws and a lag-1
+    correlation near zero.  A dropped -2, log or square root in the polar
+    transform moves the KS statistic far past the bound."""
+    from scipy import stats
+    T = O.init_ta_with_per(20000, 4).reshape(20000, 4, order="F")
+    g = T[:, 1:].ravel(order="F")
+    ks = stats.kstest(g, "norm")
+    assert ks.pvalue > 1e-3, ks
+    assert abs(np.corrcoef(g[:-1], g[1:])[0, 1]) < 0.02
+
+
 def test_init_ta_with_per_layout():
     T = O.init_ta_with_per(7, 3).reshape(7, 3, order="F")
     assert np.allclose(T[:, 0], np.arange(1, 8) / 8 * 2 * 3.14159265358979)
