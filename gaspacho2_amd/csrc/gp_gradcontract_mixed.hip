@@ -37,33 +37,73 @@ constexpr int LDK = 40;      // LDS column stride in halves (32 + 8: conflict-fr
 constexpr int MAXT16 = 17;   // column tiles handled (P16 <= 272)
 constexpr int GCW = 8;       // waves per workgroup (16 cells each)
 
-// per column k of Zw: exponent zc_k with 2^zc_k >= max_i |Zw_ik| (0 beyond P)
-__global__ void __launch_bounds__(256) k_zexp(int Npad, int P, const double *__restrict__ Zw,
-                                              int *__restrict__ zexp) {
-    __shared__ double sh[4];
+// per column k of Zw: exponent zc_k with 2^zc_k >= max_i |Zw_ik| (0 beyond P).
+// One 1024-thread block per column, eight loads in flight per thread: a column
+// is 8 * Npad bytes read once, so the kernel is HBM-bound only with ~64 KB in
+// flight per CU.
+constexpr int ZX_T = 1024, ZX_U = 8;
+__global__ void __launch_bounds__(ZX_T) k_zexp(int Npad, int P, const double *__restrict__ Zw,
+                                               int *__restrict__ zexp) {
+    __shared__ double sh[ZX_T / 64];
     const int k = blockIdx.x;
     double mx = 0.0;
-    if (k < P)
-        for (int i = threadIdx.x; i < Npad; i += 256) mx = fmax(mx, fabs(Zw[i + (size_t)k * Npad]));
+    if (k < P) {
+        const double *col = Zw + (size_t)k * Npad;
+        for (int i0 = threadIdx.x; i0 < Npad; i0 += ZX_T * ZX_U) {
+            double v[ZX_U];
+#pragma unroll
+            for (int u = 0; u < ZX_U; ++u) {
+                const int i = i0 + u * ZX_T;
+                v[u] = i < Npad ? col[i] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < ZX_U; ++u) mx = fmax(mx, fabs(v[u]));
+        }
+    }
     for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
     __syncthreads();
     if (threadIdx.x == 0) {
-        const double m = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+        double m = 0.0;
+        for (int w = 0; w < ZX_T / 64; ++w) m = fmax(m, sh[w]);
         int e = 0;
         if (m > 0.0 && isfinite(m)) (void)frexp(m, &e);
         zexp[k] = e;
     }
 }
 
-// per gene: exponent s_j with 2^s_j >= max_{k,c} |Sw_j[k,c]| 2^zc_k
+// per gene: exponent s_j with 2^s_j >= max_{k,c} |Sw_j[k,c]| 2^zc_k.
+// The flat index t = k + c P walks the P x P block in steps of 256; its row k
+// advances by 256 mod P with one conditional subtraction (no division per
+// element), and eight loads are in flight per thread.  A maximum does not
+// depend on the order of its terms, so the exponent is the same whatever the
+// traversal.
+constexpr int SX_U = 8;
 __global__ void __launch_bounds__(256) k_smax(int P, const double *__restrict__ S,
                                               const int *__restrict__ zexp, int *__restrict__ sexp) {
     __shared__ double sh[4];
-    const double *Sj = S + (size_t)blockIdx.x * P * P;
+    const int PP = P * P;
+    const double *Sj = S + (size_t)blockIdx.x * PP;
+    const int step = 256 % P, stepU = (256 * SX_U) % P;
+    int k0 = (int)threadIdx.x % P;
     double mx = 0.0;
-    for (size_t t = threadIdx.x; t < (size_t)P * P; t += 256)
-        mx = fmax(mx, ldexp(fabs(Sj[t]), zexp[t % P]));
+    for (int t0 = threadIdx.x; t0 < PP; t0 += 256 * SX_U) {
+        double v[SX_U];
+        int kk[SX_U];
+        int k = k0;
+#pragma unroll
+        for (int u = 0; u < SX_U; ++u) {
+            const int t = t0 + u * 256;
+            v[u] = t < PP ? Sj[t] : 0.0;
+            kk[u] = k;
+            k += step;
+            if (k >= P) k -= P;
+        }
+#pragma unroll
+        for (int u = 0; u < SX_U; ++u) mx = fmax(mx, ldexp(fabs(v[u]), zexp[kk[u]]));
+        k0 += stepU;
+        if (k0 >= P) k0 -= P;
+    }
     for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_down(mx, off, 64));
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
     __syncthreads();
@@ -77,8 +117,8 @@ __global__ void __launch_bounds__(256) k_smax(int P, const double *__restrict__ 
 
 // Gene groups of the contraction, on the device (no host round trip, so the
 // unit can be captured in a graph): kappa_j = e_j + s_j with 2^e_j = wun[j];
-// genes stably sorted by kappa (counting sort by one thread: J keys in a
-// small integer range), cut into consecutive groups of gsize; group g gets
+// genes stably sorted by kappa (a counting sort over a small integer range:
+// LDS histogram, scan by one thread, block-parallel stable scatter), cut into consecutive groups of gsize; group g gets
 // C_g = 26 - max kappa (its last member), gene j the operand exponents
 // aexp_j = 12 - e_j and bexp_j = e_j - 12 + C_g, and gun[g] = 2^-C_g.
 __global__ void __launch_bounds__(256) k_gc_plan(int J, int gsize, int ngroup,
@@ -87,20 +127,38 @@ __global__ void __launch_bounds__(256) k_gc_plan(int J, int gsize, int ngroup,
         int *__restrict__ kappa) {
     constexpr int NB = 1024, KOFF = 512;     // key range [-512, 511]
     __shared__ int cnt[NB];
+    __shared__ int chunk[256];
     for (int b = threadIdx.x; b < NB; b += 256) cnt[b] = 0;
+    __syncthreads();
     for (int j = threadIdx.x; j < J; j += 256) {
         int e = 0;
         (void)frexp(wun[j], &e);             // wun = 2^e_j exactly -> frexp gives e_j + 1
-        kappa[j] = min(max((e - 1) + sexp[j], -KOFF), KOFF - 1);
+        const int key = min(max((e - 1) + sexp[j], -KOFF), KOFF - 1);
+        kappa[j] = key;
+        atomicAdd(&cnt[key + KOFF], 1);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int j = 0; j < J; j++) cnt[kappa[j] + KOFF]++;
+    if (threadIdx.x == 0) {                  // exclusive scan of the bucket sizes, in LDS
         int run = 0;
         for (int b = 0; b < NB; b++) { const int t = cnt[b]; cnt[b] = run; run += t; }
-        for (int j = 0; j < J; j++) perm[cnt[kappa[j] + KOFF]++] = j;
     }
     __syncthreads();
+    // stable scatter, 256 genes at a time: a gene's slot is its bucket's running
+    // offset plus the number of earlier genes of the same chunk in that bucket
+    for (int base = 0; base < J; base += 256) {
+        const int j = base + (int)threadIdx.x;
+        const int key = j < J ? kappa[j] : KOFF;   // KOFF lies outside the key range
+        chunk[threadIdx.x] = key;
+        __syncthreads();
+        if (j < J) {
+            int r = 0;
+            for (int t = 0; t < (int)threadIdx.x; ++t) r += chunk[t] == key;
+            perm[cnt[key + KOFF] + r] = j;
+        }
+        __syncthreads();
+        if (j < J) atomicAdd(&cnt[key + KOFF], 1);
+        __syncthreads();
+    }
     for (int t = threadIdx.x; t < J; t += 256) {
         const int g = t / gsize;
         const int last = min(J, (g + 1) * gsize) - 1;
@@ -810,7 +868,7 @@ int gpk_gradcontract_mixed(gp_ctx *c) {
     }
     // per-gene exponents: e_j from the weight prep (wun = 2^e_j), s_j from max |Sw_j|
     GP_TRY(gpk_wprep(c, c->W));
-    k_zexp<<<KP, 256, 0, c->stream>>>(c->Npad, c->P, c->Zw, c->gc_zexp);
+    k_zexp<<<KP, ZX_T, 0, c->stream>>>(c->Npad, c->P, c->Zw, c->gc_zexp);
     GP_HIP(hipGetLastError());
     k_smax<<<J, 256, 0, c->stream>>>(c->P, c->Sinv, c->gc_zexp, c->gc_sexp);
     GP_HIP(hipGetLastError());
