@@ -328,6 +328,7 @@ void orc_destroy(orc_state *s) {
     free(s->Psi); free(s->Beta); free(s->tBeta); free(s->lkhds); free(s->grad);
     free(s->grad0); free(s->ss); free(s->SS); free(s->XX); free(s->gradDeltaTmp);
     free(s->gradLogRhoTmp); free(s->C1); free(s->C2); free(s->lkhd);
+    free(s->tr_obj); free(s->tr_sr); free(s->tr_info);
     free(s);
 }
 
@@ -960,15 +961,41 @@ void orc_grad(orc_state *s) {                   /* ggplvm.cu:1215-1240 */
     orc_grad_log_rho(s);
 }
 
+static void tr_push(orc_state *s, int kind, int itr, int jtr, int ktr, double obj, double sr) {
+    if (s->tr_len == s->tr_cap) {
+        s->tr_cap = s->tr_cap ? 2 * s->tr_cap : 1024;
+        s->tr_obj = (double *)realloc(s->tr_obj, sizeof(double) * (size_t)s->tr_cap);
+        s->tr_sr = (double *)realloc(s->tr_sr, sizeof(double) * (size_t)s->tr_cap);
+        s->tr_info = (int *)realloc(s->tr_info, sizeof(int) * 4 * (size_t)s->tr_cap);
+    }
+    int *e = s->tr_info + 4 * (size_t)s->tr_len;
+    e[0] = kind; e[1] = itr; e[2] = jtr; e[3] = ktr;
+    s->tr_obj[s->tr_len] = obj;
+    s->tr_sr[s->tr_len] = sr;
+    s->tr_len++;
+}
+
+int orc_fit_trace(const orc_state *s, double *obj, double *sr, int *info, int cap) {
+    int n = s->tr_len < cap ? s->tr_len : cap;
+    for (int k = 0; k < n; k++) {
+        if (obj) obj[k] = s->tr_obj[k];
+        if (sr) sr[k] = s->tr_sr[k];
+        if (info) memcpy(info + 4 * (size_t)k, s->tr_info + 4 * (size_t)k, 4 * sizeof(int));
+    }
+    return s->tr_len;
+}
+
 double orc_fit(orc_state *s, int itrmax, int jtr_cap, int verbose) {   /* ggplvm.cu:1146-1330 */
     int rhoflag = 0, JTRMAX;
     double last = 0.0;
+    s->tr_len = 0;
     orc_init_fit(s);
     for (int itr = 0; itr < itrmax; itr++) {
         JTRMAX = (itr == 0 ? 200 : 100);
         if (jtr_cap > 0 && JTRMAX > jtr_cap) JTRMAX = jtr_cap;
         for (int jtr = 0; jtr < JTRMAX; jtr++) {
             double lk = orc_irls_step(s, itr, jtr);
+            tr_push(s, 0, itr, jtr, 0, lk, 0.0);
             if (verbose) printf("lkhd[%d,%d]=%lf\n", itr, jtr, lk);
             if (jtr > 0) {
                 if (isnan(s->lkhd[jtr])) return NAN;
@@ -976,6 +1003,7 @@ double orc_fit(orc_state *s, int itrmax, int jtr_cap, int verbose) {   /* ggplvm
             }
         }
         last = orc_ql_start(s, itr);
+        tr_push(s, 1, itr, 0, 0, last, 0.0);
         if (verbose) printf("lkhd[%d,0]=%lf\n", itr, last);
         if (itr > 20) rhoflag = 1;
         if (itr == 30) JTRMAX = 10000;
@@ -984,7 +1012,8 @@ double orc_fit(orc_state *s, int itrmax, int jtr_cap, int verbose) {   /* ggplvm
             orc_grad(s);
             orc_lbfgs(s, jtr, rhoflag);
             double sr = 1.0;
-            for (int ktr = 1; ktr < 8; ktr++) {
+            int ktr;
+            for (ktr = 1; ktr < 8; ktr++) {
                 double lk = orc_elbo(s, itr, jtr + 1);
                 if (verbose) printf("lkhd[%d,%d]=%lf\n", itr, jtr + 1, lk);
                 if (s->lkhd[jtr + 1] > s->lkhd[jtr] || ktr == 7) break;
@@ -993,6 +1022,7 @@ double orc_fit(orc_state *s, int itrmax, int jtr_cap, int verbose) {   /* ggplvm
                 orc_downdate(s, jtr, sr, rhoflag);
             }
             last = s->lkhd[jtr + 1];
+            tr_push(s, 2, itr, jtr, ktr, last, sr);
             if (jtr > 0 && fabs(s->lkhd[jtr + 1] - s->lkhd[jtr]) < fabs(s->lkhd[jtr + 1]) * 1e-8) break;
         }
         orc_expand_delta_k(s);

@@ -102,6 +102,12 @@ typedef struct orc_state {
     double *lkhd;               /* objective history, 50000 like lkhd_d */
     double wt, igap1, igb;      /* ggplvm.cu:20-22 */
     int nthreads;
+    /* trace of the last orc_fit: one entry per IRLS step (kind 0), quasi-
+     * likelihood start (1) and accepted L-BFGS step (2) with its objective,
+     * outer iteration, step index, line-search trials and final sr */
+    double *tr_obj, *tr_sr;
+    int *tr_info;               /* 4 ints per entry: kind, itr, jtr, ktr */
+    int tr_len, tr_cap;
 } orc_state;
 
 /* Allocates the state and runs initParams (init.cu:27-48).
@@ -163,6 +169,8 @@ void orc_grad(orc_state *s);
  * jtr_cap > 0 caps both inner loops (0 = reference schedule).  Returns the
  * last objective. */
 double orc_fit(orc_state *s, int itrmax, int jtr_cap, int verbose);
+/* copies min(len, cap) trace entries of the last orc_fit; returns len */
+int orc_fit_trace(const orc_state *s, double *obj, double *sr, int *info, int cap);
 
 /* ---- initial inducing points (util.c:73-87) ----
  * Dim 0 equispaced on (0, 2*pi); other dims N(0,1) from MT19937 seeded as GSL

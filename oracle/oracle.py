@@ -52,6 +52,7 @@ def load() -> C.CDLL:
         "orc_beta_from_tilde_beta": (None, [vp]),
         "orc_expand_delta_k": (None, [vp]),
         "orc_fit": (C.c_double, [vp, C.c_int, C.c_int, C.c_int]),
+        "orc_fit_trace": (C.c_int, [vp, dp, dp, ip, C.c_int]),
         "orc_init_ta_with_per": (None, [C.c_int, C.c_int, dp]),
         "orc_mt19937_kat": (C.c_uint32, [C.c_uint32, C.c_int]),
         # individual kernels, for pin tests
@@ -175,3 +176,13 @@ class OracleState:
 
     def fit(self, itrmax: int, jtr_cap: int = 0, verbose: bool = False) -> float:
         return self._lib.orc_fit(self._h, int(itrmax), int(jtr_cap), int(bool(verbose)))
+
+    def fit_trace(self):
+        """Trace of the last fit: objectives, final line-search sr, and an
+        (entries x 4) int array of (kind, itr, jtr, ktr); kind 0 = IRLS step,
+        1 = quasi-likelihood start, 2 = accepted L-BFGS step."""
+        n = self._lib.orc_fit_trace(self._h, None, None, None, 0)
+        obj, sr = np.zeros(n), np.zeros(n)
+        info = np.zeros(4 * n, dtype=np.int32)
+        self._lib.orc_fit_trace(self._h, _dp(obj), _dp(sr), _ip(info), n)
+        return obj, sr, info.reshape(n, 4)

@@ -299,6 +299,151 @@ def test_lbfgs_two_loop_equals_dense_bfgs():
         assert rel(z, -np.clip(H @ g, -1, 1)) < 1e-10
 
 
+def _theta(o, rhoflag):
+    """The L-BFGS coordinates of ggplvm.cu:865-934: [log Omega | log delta | X | T | log rho_0]."""
+    N, p, Q, M = o.N, o.p, o.Q, o.M
+    parts = [np.log(o.get("Omega")[:N]), np.log(o.get("delta")[:p]), o.get("X")[:N * Q],
+             o.get("T")[:M * Q]]
+    if rhoflag:
+        parts.append(np.log(o.get("rho")[:1]))
+    return np.concatenate(parts)
+
+
+def _dense_bfgs_direction(pairs, g):
+    """H g for the inverse-BFGS matrix built from the last 10 (s, y) pairs,
+    H0 = (s'y / y'y) I of the newest pair."""
+    use = pairs[-10:]
+    s_l, y_l = use[-1]
+    H = (s_l @ y_l) / (y_l @ y_l) * np.eye(g.size)
+    for s, y in use:
+        r = 1.0 / (y @ s)
+        V = np.eye(g.size) - r * np.outer(y, s)
+        H = V.T @ H @ V + r * np.outer(s, s)
+    return H @ g
+
+
+@pytest.mark.parametrize("rhoflag", [0, 1])
+def test_lbfgs_update_and_downdate_follow_dense_bfgs_on_a_quadratic(tiny_ds, rhoflag):
+    """orc_lbfgs + orc_downdate (ggplvm.cu:818-934, util_cuda.cu:187-232,
+    546-555) driven by the exact gradient of a concave quadratic in the
+    optimiser's coordinates reproduce dense BFGS ascent on the displacements
+    actually taken: first step clip(g/1e5, +-1), then clip(H g, +-1) with H from
+    the last 10 pairs (14 steps: the ring wraps), Downdate = the point at
+    fraction sr of the step from the backup (sr = 0.1, 0.1 * 0.01, 0.01),
+    the history slot holding sr times the full step.  rho_1..rho_{Q-1} are
+    rescaled to mean 10 with their ratios kept; rho_0 moves only with
+    rhoflag = 1; the backup halves hold the values before the step."""
+    o = make_oracle(tiny_ds)
+    N, p, Q, M = o.N, o.p, o.Q, o.M
+    rng = np.random.default_rng(11)
+    o.set("Omega", np.concatenate([rng.uniform(0.5, 2.0, N), np.ones(N)]))
+    o.set("delta", np.concatenate([rng.uniform(0.05, 2.0, p), np.ones(p)]))
+    rho0 = np.array([5.0] + list(np.linspace(2.0, 6.0, Q - 1)))
+    o.set("rho", np.concatenate([rho0, np.zeros(Q)]))
+    G = o.view("grad").size
+    tot = N + p + N * Q + M * Q + rhoflag
+    th = _theta(o, rhoflag)
+    assert th.size == tot
+    Qm, _ = np.linalg.qr(rng.standard_normal((tot, tot)))
+    A = (Qm * np.logspace(0, np.log10(40.0), tot)) @ Qm.T
+    th_star = th + 0.4 * rng.standard_normal(tot)
+
+    def grad(t):
+        g = np.zeros(G)
+        g[:tot] = -A @ (t - th_star)
+        return g
+
+    downdates = {5: [0.1], 8: [0.1, 0.01], 12: [0.01]}
+    pairs, g_prev, s_prev = [], None, None
+    for itr in range(14):
+        g = grad(th)[:tot]
+        o.set("grad", grad(th))
+        before = {k: o.get(k) for k in ("Omega", "delta", "X", "T")}
+        if itr == 0:
+            step = np.clip(g / 1e5, -1, 1)
+        else:
+            pairs.append((s_prev, g - g_prev))
+            step = np.clip(_dense_bfgs_direction(pairs, g), -1, 1)
+        o.lbfgs_step(itr, rhoflag)
+        assert rel(_theta(o, rhoflag), th + step) < 1e-12, itr
+        assert rel(o.get("ss")[:tot], -step) < 1e-12, itr
+        for k, n in (("Omega", N), ("delta", p), ("X", N * Q), ("T", M * Q)):
+            assert np.array_equal(o.get(k)[n:], before[k][:n]), (itr, k)
+        sr = 1.0
+        for f in downdates.get(itr, []):
+            sr *= f
+            o.downdate(itr, sr, rhoflag)
+            assert rel(_theta(o, rhoflag), th + sr * step) < 1e-12, (itr, sr)
+            for k, n in (("Omega", N), ("delta", p), ("X", N * Q), ("T", M * Q)):
+                assert np.array_equal(o.get(k)[n:], before[k][:n]), (itr, k)
+        slot = o.get("SS")[(itr % 10) * tot:(itr % 10 + 1) * tot]
+        assert rel(slot, -sr * step) < 1e-12, itr
+        s_prev = -sr * step
+        g_prev = g
+        th = th + sr * step
+        rho = o.get("rho")[:Q]
+        assert abs(np.mean(rho[1:]) - 10.0) < 1e-12
+        assert rel(rho[1:] / rho[1], rho0[1:] / rho0[1]) < 1e-13
+        if not rhoflag:
+            assert abs(rho[0] - 5.0) < 1e-13
+    if rhoflag:
+        assert abs(o.get("rho")[0] - 5.0) > 1e-3
+
+
+def test_lbfgs_clamps_omega_and_clips_the_first_step(tiny_ds):
+    """Omega is clamped to [1e-3, 1e3] after the update and after Downdate
+    (ggplvm.cu:829-830, 902-903); the first step is clip(-g/1e5, +-1)
+    component by component (ggplvm.cu:872-880)."""
+    o = make_oracle(tiny_ds)
+    N = o.N
+    Om = np.ones(2 * N)
+    Om[:3] = [900.0, 0.002, 1.0]
+    o.set("Omega", Om)
+    g = np.zeros(o.view("grad").size)
+    g[:3] = [2e5, -2e5, 3e4]
+    g[3] = -5e6
+    o.set("grad", g)
+    o.lbfgs_step(0, 0)
+    ss = o.get("ss")
+    assert np.array_equal(ss[:4], [-1.0, 1.0, -0.3, 1.0])
+    Omn = o.get("Omega")
+    assert Omn[0] == 1000.0 and Omn[1] == 0.001
+    assert abs(Omn[2] - np.exp(0.3)) < 1e-14
+    assert abs(Omn[3] - np.exp(-1.0)) < 1e-15
+    assert np.array_equal(Omn[N:N + 3], [900.0, 0.002, 1.0])
+    o.downdate(0, 0.5, 0)
+    Omd = o.get("Omega")
+    assert Omd[0] == 1000.0                               # 900 e^0.5 = 1484 -> clamped
+    assert abs(Omd[1] - 0.002 * np.exp(-0.5)) < 1e-18     # inside the bounds
+    assert abs(Omd[2] - np.exp(0.15)) < 1e-14
+
+
+def test_fit_schedule_follows_the_reference_rules(tiny_ds):
+    """orc_fit's trace obeys the schedule of ggplvm.cu:1146-1330 over 31 outer
+    iterations with no cap: JTRMAX 200 at outer iteration 0, 100 after, and
+    10000 for the L-BFGS phase of iteration 30 (1152, 1213); an IRLS phase
+    stops at the first step with relative change < 1e-7 or at JTRMAX
+    (1180-1185); one quasi-likelihood start follows; each L-BFGS step is
+    accepted when its objective rises or at the 7th trial, after one Downdate
+    per failed trial with sr multiplied by 0.1 (lower) or 0.01 (non-finite)
+    (1280-1289); the phase stops at |change| < 1e-8 |objective| or at JTRMAX
+    (1292); rho_0 is optimised only after outer iteration 20 (1212).  The run
+    reaches every rule: natural IRLS and L-BFGS stops, both JTRMAX caps, the
+    10000-step phase and Downdates."""
+    from conftest import check_fit_schedule
+    o = make_oracle(tiny_ds)
+    o.fit(21, 0)
+    assert abs(o.get("rho")[0] - 5.0) < 1e-13
+    o = make_oracle(tiny_ds)
+    last = o.fit(31, 0)
+    assert abs(o.get("rho")[0] - 5.0) > 1e-6
+    obj, sr, info = o.fit_trace()
+    seen = check_fit_schedule(obj, info, sr, 31, 0)
+    assert last == obj[-1]
+    assert seen["irls_converged"] > 0 and seen["lbfgs_converged"] > 0
+    assert seen["lbfgs_cap"] > 0 and seen["downdates"] > 0 and seen["long_phase_steps"] > 100
+
+
 # ------------------------------------------------------------------ util.c
 def test_mt19937_published_kat():
     # 10000th output of MT19937 seeded with 5489 (C++ std::mt19937 requirement)
