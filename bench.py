@@ -41,17 +41,49 @@ def gradcontract_flops(N, J, P):
     return 2.0 * J * N * P * P
 
 
-def cpu_baseline(ds, genes=(4, 8), threads=None):
-    """FP64 oracle, one ELBO+grad unit on the first g genes (all cells)."""
+def host_cpu():
+    """Host CPU model, logical CPUs of the machine, CPUs this process may run
+    on, and the OpenMP thread setting of the environment."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return {"model": model, "nproc": os.cpu_count(), "affinity": aff,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_threads():
+    """Every CPU this process is allowed to use, capped by OMP_NUM_THREADS when
+    the environment sets it (the host's CPU share for this GPU)."""
+    h = host_cpu()
+    n = h["affinity"]
+    if h["omp_num_threads"]:
+        try:
+            n = min(n, int(h["omp_num_threads"]))
+        except ValueError:
+            pass
+    return max(1, n)
+
+
+def cpu_baseline(ds, genes=(2, 4, 8), threads=None, ard_only=True):
+    """FP64 oracle: one ELBO+grad unit on all cells and the first g genes for
+    each g in genes; the unit time is fitted as a + b*g by least squares and
+    extrapolated to all genes."""
     from oracle.oracle import OracleState
     from gaspacho2_amd import lib
-    threads = threads or min(16, os.cpu_count() or 1)
+    threads = threads or cpu_threads()
     T0 = lib.init_ta_with_per(ds.M, ds.Q)
     times = []
     for g in genes:
         L = int(ds.Yp[g])
         o = OracleState(ds.N, g, ds.M, ds.Q, ds.nh, ds.Yp[: g + 1], ds.Yi[:L], ds.Yx[:L],
-                        ds.Zcov, ds.ls, ds.X0, T0, nthreads=threads)
+                        ds.Zcov, ds.ls, ds.X0, T0, ard_only=ard_only, nthreads=threads)
         o.init_fit()
         o.irls_step(0)
         o.ql_start()
@@ -61,14 +93,21 @@ def cpu_baseline(ds, genes=(4, 8), threads=None):
         times.append(time.perf_counter() - t0)
         o.close()
         log(f"cpu oracle: {g} genes, one ELBO+grad unit {times[-1]:.2f} s ({threads} threads)")
-    (g1, g2), (t1, t2) = genes, times
-    slope = max((t2 - t1) / (g2 - g1), 0.0)
-    t_full = t2 + slope * (ds.J - g2)
+    gs, ts = np.asarray(genes, float), np.asarray(times)
+    b, a = np.polyfit(gs, ts, 1)
+    resid = ts - (a + b * gs)
+    t_full = a + b * ds.J
+    host = host_cpu()
     return {"value": 1.0 / t_full, "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle (C restatement of the reference kernels, OpenMP), one "
-                      f"ELBO+grad unit at N={ds.N}, M={ds.M}, Q={ds.Q} on {genes[0]} and "
-                      f"{genes[1]} genes ({t1:.1f} s, {t2:.1f} s), linear extrapolation "
-                      f"to G={ds.J}: {t_full:.0f} s per unit"}
+            "host": host,
+            "fit": {"genes": list(genes), "seconds": [round(t, 3) for t in times],
+                    "intercept_s": a, "per_gene_s": b,
+                    "max_abs_residual_s": float(np.max(np.abs(resid))),
+                    "seconds_per_unit_at_G": t_full},
+            "sample": f"FP64 oracle (C restatement of the reference kernels, OpenMP, {threads} threads "
+                      f"on {host['model']}), one ELBO+grad unit at N={ds.N}, M={ds.M}, Q={ds.Q} on "
+                      f"{', '.join(str(g) for g in genes)} genes ({', '.join(f'{t:.1f}' for t in times)} s), "
+                      f"least-squares line extrapolated to G={ds.J}: {t_full:.0f} s per unit"}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_mixed_N100K_G2K_M256.txt")
@@ -118,7 +157,8 @@ def main_batched(args):
     npos = int(bd.off[-1])
     log(f"batched data: N={ds.N} G={args.G} positions={npos} ({time.perf_counter() - t0:.1f} s)")
     T0 = lib.init_ta_with_per(ds.M, ds.Q)
-    g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0, T0)
+    g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0, T0,
+                      ard_only=args.kernel == "ard")
     g.init_fit()
     bf = lib.BatchedFit(g, bd.off, bd.cell, bd.ncov, bd.cov, bd.yoff, bd.ypos, bd.yval)
     del bd.cov
@@ -181,7 +221,7 @@ def main_fit(args):
     pmap = {"f64": lib.PREC_FP64, "mixed": lib.PREC_MIXED, "f64w": lib.PREC_FP64_WHITENED}
     for prec in precs:
         g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0, T0,
-                          precision=pmap[prec])
+                          ard_only=args.kernel == "ard", precision=pmap[prec])
         if prec == "mixed" and not args.no_tune:
             g.set_option("autotune", 1)
         t1 = time.perf_counter()
@@ -248,7 +288,7 @@ def main_fitcheck(args):
     log(f"data: N={ds.N} G={ds.J} ({time.perf_counter() - t0:.1f} s)")
     T0 = lib.init_ta_with_per(ds.M, ds.Q)
     g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0, T0,
-                      precision=lib.PREC_MIXED)
+                      ard_only=args.kernel == "ard", precision=lib.PREC_MIXED)
     scratch = 49999                              # an objective slot the schedule never reaches
     lk = {}
     jtrmax = min(200, args.fit_cap) if args.fit_cap > 0 else 200   # first outer iteration
@@ -338,12 +378,14 @@ def main_warmstart(args):
     T0 = lib.init_ta_with_per(args.M, args.Q)
     train = []
     for s in range(S):
-        X, obj, dt, _ = warmstart.fit_from(data[s], data[s].X0, T0, args.fit_itr, args.fit_cap)
+        X, obj, dt, _ = warmstart.fit_from(data[s], data[s].X0, T0, args.fit_itr, args.fit_cap,
+                                              ard_only=args.kernel == "ard")
         train.append((data[s], X))
         log(f"seed {s}: PCA-started fit {dt:.1f} s, objective {obj:.10e}, "
             f"rms(X - X_pca) {np.sqrt(np.mean((X - data[s].X0) ** 2)):.3f}")
     ds = data[S]
-    Xp, obj_p, dt_p, tr_p = warmstart.fit_from(ds, ds.X0, T0, args.fit_itr, args.fit_cap)
+    Xp, obj_p, dt_p, tr_p = warmstart.fit_from(ds, ds.X0, T0, args.fit_itr, args.fit_cap,
+                                                ard_only=args.kernel == "ard")
     n_pca = int(np.sum(tr_p[2] == 2))
     first = lambda tr: float(tr[0][np.argmax(tr[2] == 1)])
     log(f"fresh seed, PCA start: {obj_p:.10e} after {n_pca} L-BFGS steps ({dt_p:.1f} s)")
@@ -356,7 +398,8 @@ def main_warmstart(args):
         enc.close()
         log(f"{target}: encoder {S * args.N} cells, {args.enc_epochs} epochs, {t_train:.1f} s, "
             f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
-        _, obj_e, dt_e, tr_e = warmstart.fit_from(ds, X0e, T0, args.fit_itr, args.fit_cap)
+        _, obj_e, dt_e, tr_e = warmstart.fit_from(ds, X0e, T0, args.fit_itr, args.fit_cap,
+                                            ard_only=args.kernel == "ard")
         n_hit, t_hit = warmstart.steps_to_reach(tr_e, obj_p)
         log(f"{target}: warm-started fit {obj_e:.10e} ({dt_e:.1f} s), reaches the PCA-started final after "
             f"{n_hit} of {n_pca} L-BFGS steps")
@@ -394,6 +437,10 @@ def main():
     ap.add_argument("--Q", type=int, default=10)
     ap.add_argument("--M", type=int, default=None, help="inducing points (256; batched: 128)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel", choices=["ard", "per-ard"], default="ard",
+                    help="ard: SE-ARD on every latent dimension (kernel_funcs.cu:221-255, the "
+                         "kernel the workloads name); per-ard: periodic dimension 0 + SE-ARD "
+                         "(kernel_funcs.cu:61-97, the reference driver's default)")
     ap.add_argument("--no-tune", action="store_true", help="skip the mixed-path autotuner")
     ap.add_argument("--only", action="store_true",
                     help="run only --precision (no FP64 comparison run)")
@@ -447,7 +494,7 @@ def main():
         f"({time.perf_counter() - t0:.1f} s)")
     T0 = lib.init_ta_with_per(ds.M, ds.Q)
     g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls,
-                      ds.X0, T0, precision=lib.PREC_FP64)
+                      ds.X0, T0, ard_only=args.kernel == "ard", precision=lib.PREC_FP64)
     g.init_fit()
     for jtr in range(2):
         g.irls_step(jtr)
@@ -550,7 +597,10 @@ def main():
         "dtype": "f64" if main_p == "f64" else "f16x2-mfma/f32-acc/f64",
         "data": "synthetic (Poisson counts from the SE-ARD GP recipe, seed 0; PCA-initialised X)",
         "config": {"workload": f"sparse GP-LVM ELBO+gradient, N={ds.N} cells, G={ds.J} genes, "
-                               f"Q={ds.Q}, M={ds.M}, P={P}, no covariates",
+                               f"Q={ds.Q}, M={ds.M}, P={P}, no covariates, "
+                               + ("SE-ARD kernel on every latent dimension" if args.kernel == "ard"
+                                  else "periodic dim 0 + SE-ARD kernel"),
+                   "kernel": args.kernel,
                    "N": ds.N, "G": ds.J, "Q": ds.Q, "M": ds.M, "P": P,
                    "precision": main_p, "nnz": ds.L},
         "elbo": r["elbo"],
@@ -572,7 +622,7 @@ def main():
                                 "grad_rel_err": float(np.max(np.abs(gd)) / np.max(np.abs(f["grad"])))}
     g.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(ds)
+        out["cpu_baseline"] = cpu_baseline(ds, ard_only=args.kernel == "ard")
     print(json.dumps(out), flush=True)
 
 

@@ -63,7 +63,8 @@ static void free_all(gp_ctx *c) {
                     c->gc_gun, c->gc_part, c->gc_zexp, c->skw, c->gc_kappa};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
-    gp_timer *ts[] = {&c->t_gram, &c->t_chol, &c->t_gc, &c->t_kuf, &c->t_elbo, &c->t_grad};
+    gp_timer *ts[] = {&c->t_gram, &c->t_chol, &c->t_gc, &c->t_kuf, &c->t_elbo, &c->t_grad,
+                      &c->t_gramk, &c->t_gck};
     for (gp_timer *t : ts) {
         if (t->a) (void)hipEventDestroy(t->a);
         if (t->b) (void)hipEventDestroy(t->b);
@@ -683,8 +684,9 @@ static void drop_graph(gp_ctx *c) {
 extern "C" int gp_eval_graph(gp_ctx *c, int slot, double *obj) {
     if (!c) { gp_set_error("NULL context"); return 1; }
     if (slot < 0 || slot >= GP_LKHD_SLOTS) { gp_set_error("slot out of range"); return 1; }
-    const int key[6] = {c->precision, slot, c->gram_splits, c->gc_groups, c->gram_waves, c->gc_form};
-    if (!c->graph_exec || !std::equal(key, key + 6, c->graph_key)) {
+    const int key[GP_GRAPH_KEY] = {c->precision, slot, c->gram_splits, c->gc_groups, c->gram_waves,
+                                   c->gc_form, c->gc_mixed};
+    if (!c->graph_exec || !std::equal(key, key + GP_GRAPH_KEY, c->graph_key)) {
         drop_graph(c);
         double tmp = 0.0;
         int r = gp_elbo(c, slot, &tmp);
@@ -704,7 +706,7 @@ extern "C" int gp_eval_graph(gp_ctx *c, int slot, double *obj) {
         }
         c->graph = g;
         GP_HIP(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
-        std::copy(key, key + 6, c->graph_key);
+        std::copy(key, key + GP_GRAPH_KEY, c->graph_key);
     }
     GP_HIP(hipGraphLaunch(c->graph_exec, c->stream));
     c->lkhd_fresh = 1;   // the replayed unit ran the likelihood at the current parameters
@@ -748,10 +750,14 @@ extern "C" int gp_fit(gp_ctx *c, int itrmax, int jtr_cap, int verbose, gp_outer_
     c->fit_obj.clear();
     c->fit_sec.clear();
     c->fit_kind.clear();
-    auto mark = [&](int kind, double v) {
+    c->fit_step.clear();
+    c->fit_sr.clear();
+    auto mark = [&](int kind, double v, int itr, int jtr, int ktr, double sr) {
         c->fit_obj.push_back(v);
         c->fit_sec.push_back(std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
         c->fit_kind.push_back(kind);
+        c->fit_step.insert(c->fit_step.end(), {itr, jtr, ktr});
+        c->fit_sr.push_back(sr);
     };
     int r = gp_init_fit(c);
     if (r > 0) return r;
@@ -763,7 +769,7 @@ extern "C" int gp_fit(gp_ctx *c, int itrmax, int jtr_cap, int verbose, gp_outer_
         for (int jtr = 0; jtr < JTRMAX; jtr++) {
             r = gp_irls_step(c, jtr, &lk[jtr]);
             if (r > 0) return r;
-            mark(0, lk[jtr]);
+            mark(0, lk[jtr], itr, jtr, 0, 0.0);
             if (verbose) printf("lkhd[%d,%d]=%lf\n", itr, jtr, lk[jtr]);
             if (jtr > 0) {
                 if (std::isnan(lk[jtr])) { gp_set_error("GLM objective is NaN"); if (obj) *obj = NAN; return 2; }
@@ -772,7 +778,7 @@ extern "C" int gp_fit(gp_ctx *c, int itrmax, int jtr_cap, int verbose, gp_outer_
         }
         r = gp_ql_start(c, &lk[0]);
         if (r > 0) return r;
-        mark(1, lk[0]);
+        mark(1, lk[0], itr, 0, 0, 0.0);
         last = lk[0];
         if (verbose) printf("lkhd[%d,0]=%lf\n", itr, lk[0]);
         if (itr == 0 && c->autotune && c->precision == GP_PREC_MIXED) {
@@ -786,7 +792,8 @@ extern "C" int gp_fit(gp_ctx *c, int itrmax, int jtr_cap, int verbose, gp_outer_
             if ((r = gp_grad(c)) > 0) return r;
             if ((r = gp_lbfgs_step(c, jtr, rhoflag)) > 0) return r;
             double sr = 1.0;
-            for (int ktr = 1; ktr < 8; ktr++) {
+            int ktr;
+            for (ktr = 1; ktr < 8; ktr++) {
                 r = gp_elbo(c, jtr + 1, &lk[jtr + 1]);
                 if (r > 0) return r;
                 if (verbose) printf("lkhd[%d,%d]=%lf\n", itr, jtr + 1, lk[jtr + 1]);
@@ -796,13 +803,24 @@ extern "C" int gp_fit(gp_ctx *c, int itrmax, int jtr_cap, int verbose, gp_outer_
                 if ((r = gp_downdate(c, jtr, sr, rhoflag)) > 0) return r;
             }
             last = lk[jtr + 1];
-            mark(2, last);
+            mark(2, last, itr, jtr, ktr, sr);
             if (jtr > 0 && std::fabs(lk[jtr + 1] - lk[jtr]) < std::fabs(lk[jtr + 1]) * 1e-8) break;
         }
         if (on_outer) on_outer(c, itr, last, user);
         if ((r = gp_end_outer(c)) > 0) return r;
     }
     if (obj) *obj = last;
+    return 0;
+}
+
+extern "C" int gp_fit_trace_steps(gp_ctx *c, int *step, double *sr, int cap, int *len) {
+    if (!c || !len) { gp_set_error("NULL argument"); return 1; }
+    const int n = (int)c->fit_obj.size();
+    *len = n;
+    for (int k = 0; k < std::min(n, cap); k++) {
+        if (step) std::copy(c->fit_step.begin() + 3 * k, c->fit_step.begin() + 3 * k + 3, step + 3 * k);
+        if (sr) sr[k] = c->fit_sr[k];
+    }
     return 0;
 }
 

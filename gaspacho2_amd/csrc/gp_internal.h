@@ -13,6 +13,7 @@
 
 #define GP_HIST 10            // L-BFGS history length (ggplvm.cu:872-880)
 #define GP_LKHD_SLOTS 50000   // objective history length (ggplvm.cu:1025)
+#define GP_GRAPH_KEY 7        // settings a recorded ELBO + gradient graph depends on
 
 struct gp_timer {
     hipEvent_t a = nullptr, b = nullptr;
@@ -142,10 +143,13 @@ struct gp_ctx {
     // recorded ELBO + gradient unit
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    int graph_key[6] = {-1, -1, -1, -1, -1, -1};   // precision, slot, gram_splits, gc_groups, gram_waves, gc_form
+    // precision, slot, gram_splits, gc_groups, gram_waves, gc_form, gc_mixed
+    int graph_key[GP_GRAPH_KEY] = {-1, -1, -1, -1, -1, -1, -1};
     // objective and elapsed seconds after every accepted step of the last gp_fit
     std::vector<double> fit_obj, fit_sec;
     std::vector<int> fit_kind;           // 0 IRLS step, 1 quasi-likelihood start, 2 L-BFGS step
+    std::vector<int> fit_step;           // per entry: outer iteration, inner step, line-search trials
+    std::vector<double> fit_sr;          // per entry: the line search's final step fraction
     std::string err;
 };
 
@@ -246,6 +250,16 @@ int gpk_gradcontract_mixed(gp_ctx *c);
 // ---------------- gp_gradcontract.hip ----------------
 // U = sum_j diag(W_j/Omega) Zsrc Sinv_j  (Npad x P)
 int gpk_gradcontract(gp_ctx *c, const double *Zsrc, double *Uout);
+
+// ---------------- gp_score.hip ----------------
+// doubles of partial sums per chunk of tests in flight, and the widest design
+size_t gpk_score_part_doubles(int P);
+int gpk_score_max_columns();
+int gpk_score_clear(gp_ctx *c, int *err);      // *err = 0 (a kernel, not a memset)
+// score statistics of T dosage tests, every pointer on the device; part holds
+// fly * gpk_score_part_doubles(P) doubles; *err set to 1 on an index out of range
+int gpk_score_run_dosage(gp_ctx *c, int S, const unsigned char *G, int T, const int *gene,
+                         const int *snp, double *stat, double *part, int fly, int *err);
 
 // ---------------- gp_batched.hip ----------------
 struct gp_bfit;

@@ -1,25 +1,47 @@
 // gp_score.hip -- score statistics of gene x genotype tests (getScoreStats,
 // ggplvm_de.cu:491-532) from the current fit state:
-//   b    = Z^T D_j g,  D_j = diag(W_j / Omega)
-//   gVg  = g^T D_j g - b^T Psi_j^{-1} b = g^T D_j g - |L_j^{-1} b|^2   (Psi_j = L_j L_j^T)
-//   ss   = g^T Omega^{-1} (y_j - W_j)
-//   stat = ss / sqrt(phi_j gVg)
-// The tests are processed in chunks: one kernel forms D_j g for every test of
-// the chunk (plus the scalar sums g^T D_j g, g^T D_j 1 and g^T Omega^{-1} y_j
-// over the gene's non-zeros), one split-K GEMM forms all b = Z^T (D_j g) at
-// once, and one kernel per test does the forward substitution with the
-// gene's factor.  Works in the basis of the current factor (Z or the
+//   b    = Z^T D_j h,  D_j = diag(W_j / Omega)
+//   hVh  = h^T D_j h - b^T Psi_j^{-1} b = h^T D_j h - |L_j^{-1} b|^2   (Psi_j = L_j L_j^T)
+//   ss   = h^T Omega^{-1} (y_j - W_j)
+//   stat = ss / sqrt(phi_j hVh),   p = erfc(|stat| / sqrt 2)  (two-sided normal)
+// with h the genotype column g_l, or, given a per-cell factor d, the
+// interaction column h = g_l * d - mean_i(g_l * d) that ggplvm_de.cu:989-994
+// builds on the host (G_h[i + l N] *= d_h[i]; scale() centres it).
+//
+// Tests are processed in chunks of TT: k_score_acc walks the cells in 64-cell
+// tiles (tiles dealt round robin to NBK workgroups per chunk), stages h_i D_j
+// for the chunk's tests in LDS and accumulates, per model-matrix column k (one
+// thread per column), the partial b_k of every test of the chunk, plus the
+// partial h^T D h and h^T D 1; k_score_fin sums the NBK partials in a fixed
+// order, adds the count term over the gene's non-zeros, and does the forward
+// substitution with the gene's factor.  Several chunks run side by side when
+// the partial buffer holds them.  The cell weights, dosages and model-matrix
+// columns are read once per chunk; no library routine is called, so the same
+// launcher serves the entry point that allocates nothing (its partials live in
+// a static device array).  Works in the basis of the current factor (Z, or the
 // whitened Zw: b^T Psi^{-1} b does not depend on the basis).
 #include "gp_internal.h"
+#include "../../include/gaspacho2.h"
 
 #include <algorithm>
 
 namespace {
 
-constexpr int ST = 256;
+constexpr int ST = 256;       // threads per workgroup
+constexpr int TT = 16;        // tests per chunk
+constexpr int NBK = 128;      // cell-tile workgroups per chunk
+constexpr int CT = 64;        // cells per staged tile
+constexpr int PMAX = 1024;    // widest design handled (model-matrix columns)
 
-__device__ inline double block_sum(double v, double *sh) {
+// partial layout per chunk in flight: NBK x TT x (P + 2) doubles
+inline size_t chunk_doubles(int P) { return (size_t)NBK * TT * (P + 2); }
+
+__device__ inline double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+__device__ inline double block_sum(double v, double *sh) {
+    v = wave_sum(v);
     __syncthreads();
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
@@ -29,59 +51,155 @@ __device__ inline double block_sum(double v, double *sh) {
 __device__ inline double gval(const double *G, size_t i) { return G[i]; }
 __device__ inline double gval(const unsigned char *G, size_t i) { return (double)G[i]; }
 
-// One block per test t of the chunk: Gw[:, t] = D_j g (zero on padded rows),
-// and the scalars a[t] = g^T D_j g, s[t] = g^T Omega^{-1} (y_j - W_j).
+// cmean[l] = mean_i G[i, l] d[i]  (the centring of ggplvm_de.cu:994, util.c:100-110)
 template <typename GT>
-__global__ void __launch_bounds__(ST) k_score_prep(int N, int Npad, int t0, int tc,
-        const GT *__restrict__ G, const int *__restrict__ gene, const int *__restrict__ snp,
-        const double *__restrict__ W, const double *__restrict__ Omega,
-        const int *__restrict__ Yp, const int *__restrict__ Yi, const double *__restrict__ Yx,
-        double *__restrict__ Gw, double *__restrict__ a, double *__restrict__ s) {
+__global__ void __launch_bounds__(ST) k_score_colmean(int N, const GT *__restrict__ G,
+        const double *__restrict__ d, double *__restrict__ cmean) {
     __shared__ double sh[4];
-    const int t = blockIdx.x;
-    if (t >= tc) return;
-    const int j = gene[t0 + t];
-    const GT *g = G + (size_t)snp[t0 + t] * N;
-    const double *Wj = W + (size_t)j * Npad;
-    double *gw = Gw + (size_t)t * Npad;
-    double sa = 0.0, sw = 0.0;
-    for (int i = threadIdx.x; i < Npad; i += ST) {
-        double v = 0.0;
-        if (i < N) {
-            const double gi = gval(g, i);
-            const double o = Wj[i] / Omega[i];
-            v = o * gi;
-            sa += gi * v;
-            sw += v;
+    const GT *g = G + (size_t)blockIdx.x * N;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < N; i += ST) s += gval(g, i) * d[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) cmean[blockIdx.x] = s / (double)N;
+}
+
+__device__ inline bool test_ok(int j, int l, int J, int S) {
+    return j >= 0 && j < J && l >= 0 && l < S;
+}
+
+// grid (NBK, chunks in flight); chunk c of this launch covers tests
+// t0 + c*TT .. t0 + c*TT + TT - 1 (those < T).
+template <typename GT>
+__global__ void __launch_bounds__(ST) k_score_acc(int N, int Npad, int P, int J, int S, int T,
+        int t0, const GT *__restrict__ G, const double *__restrict__ d,
+        const double *__restrict__ cmean, const int *__restrict__ gene,
+        const int *__restrict__ snp, const double *__restrict__ W,
+        const double *__restrict__ Omega, const double *__restrict__ Zsrc,
+        double *__restrict__ part, int *__restrict__ err) {
+    __shared__ double v[CT][TT];          // h_i * o_ij for the tile's cells
+    __shared__ int sj[TT], sl[TT];
+    __shared__ double sc[TT];
+    const int tid = threadIdx.x;
+    const int tb = t0 + blockIdx.y * TT;
+    if (tb >= T) return;
+    const int nt = min(TT, T - tb);
+    if (tid < TT) {
+        int j = -1, l = -1;
+        double c = 0.0;
+        if (tid < nt) {
+            j = gene[tb + tid];
+            l = snp[tb + tid];
+            if (!test_ok(j, l, J, S)) {
+                if (blockIdx.x == 0) err[0] = 1;
+                j = -1;
+                l = -1;
+            } else if (d) {
+                c = cmean[l];
+            }
         }
-        gw[i] = v;
+        sj[tid] = j;
+        sl[tid] = l;
+        sc[tid] = c;
     }
-    double sy = 0.0;
-    for (int l = Yp[j] + threadIdx.x; l < Yp[j + 1]; l += ST) {
-        const int i = Yi[l];
-        sy += gval(g, i) * Yx[l] / Omega[i];
-    }
-    sa = block_sum(sa, sh);
-    sw = block_sum(sw, sh);
-    sy = block_sum(sy, sh);
-    if (threadIdx.x == 0) {
-        a[t] = sa;
-        s[t] = sy - sw;
+    __syncthreads();
+    double *pb = part + ((size_t)blockIdx.y * NBK + blockIdx.x) * TT * (P + 2);
+    // staging map: thread -> cell tid % 64 of the tile, tests w, w+4, w+8, w+12 (w = wave)
+    const int ci = tid & (CT - 1), w = tid >> 6;
+    const int ntile = Npad / CT;
+    for (int k0 = 0; k0 < P; k0 += ST) {
+        const int k = k0 + tid;
+        double acc[TT];
+#pragma unroll
+        for (int t = 0; t < TT; t++) acc[t] = 0.0;
+        double sa[4] = {0.0, 0.0, 0.0, 0.0}, sw[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int tile = blockIdx.x; tile < ntile; tile += NBK) {
+            const int i = tile * CT + ci;
+            __syncthreads();
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int t = w + 4 * u;
+                double val = 0.0;
+                if (i < N && sj[t] >= 0) {
+                    double h = gval(G + (size_t)sl[t] * N, i);
+                    if (d) h = h * d[i] - sc[t];
+                    const double o = W[i + (size_t)sj[t] * Npad] / Omega[i];
+                    val = h * o;
+                    sa[u] += h * val;
+                    sw[u] += val;
+                }
+                v[ci][t] = val;
+            }
+            __syncthreads();
+            if (k < P) {
+                const double *zc = Zsrc + (size_t)k * Npad + (size_t)tile * CT;
+#pragma unroll 4
+                for (int ii = 0; ii < CT; ii++) {
+                    const double z = zc[ii];
+#pragma unroll
+                    for (int t = 0; t < TT; t++) acc[t] += z * v[ii][t];
+                }
+            }
+        }
+        if (k < P) {
+#pragma unroll
+            for (int t = 0; t < TT; t++) pb[(size_t)t * (P + 2) + k] = acc[t];
+        }
+        if (k0 == 0) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const double a = wave_sum(sa[u]), s = wave_sum(sw[u]);
+                if ((tid & 63) == 0) {
+                    const int t = w + 4 * u;
+                    pb[(size_t)t * (P + 2) + P] = a;
+                    pb[(size_t)t * (P + 2) + P + 1] = s;
+                }
+            }
+        }
     }
 }
 
-// One block per test: |L_j^{-1} b|^2 by forward substitution, then the statistic.
-__global__ void __launch_bounds__(ST) k_score_finish(int P, int t0, int tc,
-        const int *__restrict__ gene, const double *__restrict__ Psi,
-        const double *__restrict__ phi, const double *__restrict__ B,
-        const double *__restrict__ a, const double *__restrict__ s, double *__restrict__ stat) {
-    extern __shared__ double xs[];
+// One workgroup per test of the launch: sum the NBK partials, add the count
+// term over the gene's non-zeros, forward substitution, statistic, p-value.
+template <typename GT>
+__global__ void __launch_bounds__(ST) k_score_fin(int N, int P, int T, int t0,
+        const GT *__restrict__ G, const double *__restrict__ d, const double *__restrict__ cmean,
+        const int *__restrict__ gene, const int *__restrict__ snp, int J, int S,
+        const int *__restrict__ Yp, const int *__restrict__ Yi, const double *__restrict__ Yx,
+        const double *__restrict__ Omega, const double *__restrict__ Psi,
+        const double *__restrict__ phi, const double *__restrict__ part,
+        double *__restrict__ stat, double *__restrict__ pval) {
+    extern __shared__ double xs[];        // P + 2
     __shared__ double sh[4];
-    const int t = blockIdx.x, tid = threadIdx.x;
-    if (t >= tc) return;
-    const int j = gene[t0 + t];
+    const int tl = blockIdx.x, tid = threadIdx.x;
+    const int t = t0 + tl;
+    if (t >= T) return;
+    const int j = gene[t], l = snp[t];
+    if (!test_ok(j, l, J, S)) {
+        if (tid == 0) {
+            stat[t] = __builtin_nan("");
+            if (pval) pval[t] = __builtin_nan("");
+        }
+        return;
+    }
+    const int c = tl / TT, tt = tl % TT;
+    const double *pc = part + (size_t)c * NBK * TT * (P + 2) + (size_t)tt * (P + 2);
+    for (int k = tid; k < P + 2; k += ST) {
+        double s = 0.0;
+        for (int b = 0; b < NBK; b++) s += pc[(size_t)b * TT * (P + 2) + k];
+        xs[k] = s;
+    }
+    const GT *g = G + (size_t)l * N;
+    const double cm = d ? cmean[l] : 0.0;
+    double sy = 0.0;
+    for (int q = Yp[j] + tid; q < Yp[j + 1]; q += ST) {
+        const int i = Yi[q];
+        double h = gval(g, i);
+        if (d) h = h * d[i] - cm;
+        sy += h * Yx[q] / Omega[i];
+    }
+    sy = block_sum(sy, sh);                // also orders the xs writes before the reads below
+    const double a = xs[P], sw = xs[P + 1];
     const double *L = Psi + (size_t)j * P * P;
-    for (int k = tid; k < P; k += ST) xs[k] = B[(size_t)t * P + k];
     for (int i = 0; i < P; i++) {
         __syncthreads();
         const double xi = xs[i] / L[i + (size_t)i * P];
@@ -93,13 +211,48 @@ __global__ void __launch_bounds__(ST) k_score_finish(int P, int t0, int tc,
     double q = 0.0;
     for (int k = tid; k < P; k += ST) q += xs[k] * xs[k];
     q = block_sum(q, sh);
-    if (tid == 0) stat[t0 + t] = s[t] / sqrt(phi[j] * (a[t] - q));
+    if (tid == 0) {
+        const double st = (sy - sw) / sqrt(phi[j] * (a - q));
+        stat[t] = st;
+        if (pval) pval[t] = erfc(fabs(st) * 0.70710678118654752440);
+    }
 }
 
+// Every chunk of TT tests: accumulation and finish, `fly` chunks per launch
+// (part holds fly * chunk_doubles(P) doubles).  All pointers are device
+// pointers; d / cmean / pval may be null.
 template <typename GT>
-int score_tests(gp_ctx *c, int S, const GT *G_host, int T, const int *gene_host,
-                const int *snp_host, double *stat_host) {
+int score_run(gp_ctx *c, int S, const GT *G, const double *d, const double *cmean, int T,
+              const int *gene, const int *snp, double *stat, double *pval, double *part, int fly,
+              int *err) {
+    const int P = c->P;
+    const double *Zsrc = c->psi_whitened ? c->Zw : c->Z;
+    const int per = fly * TT;
+    for (int t0 = 0; t0 < T; t0 += per) {
+        const int n = std::min(per, T - t0);
+        const int nch = (n + TT - 1) / TT;
+        k_score_acc<GT><<<dim3(NBK, nch), ST, 0, c->stream>>>(c->N, c->Npad, P, c->J, S, T, t0, G, d,
+                                                             cmean, gene, snp, c->W, c->Omega, Zsrc,
+                                                             part, err);
+        GP_HIP(hipGetLastError());
+        k_score_fin<GT><<<n, ST, (size_t)(P + 2) * sizeof(double), c->stream>>>(
+            c->N, P, T, t0, G, d, cmean, gene, snp, c->J, S, c->Yp, c->Yi, c->Yx, c->Omega, c->Psi,
+            c->phi, part, stat, pval);
+        GP_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+__global__ void k_score_clear(int *err) { err[0] = 0; }
+
+// Host-array entry points: device copies of the inputs and a partial buffer
+// for up to 16 chunks in flight, freed before returning.
+template <typename GT>
+int score_tests_host(gp_ctx *c, int S, const GT *G_host, const double *d_host, int T,
+                     const int *gene_host, const int *snp_host, double *stat_host,
+                     double *pval_host) {
     if (S <= 0 || T <= 0) { gp_set_error("score tests: S and T must be positive"); return 1; }
+    if (c->P > PMAX) { gp_set_error("score tests: more than 1024 model-matrix columns"); return 1; }
     for (int t = 0; t < T; t++) {
         if (gene_host[t] < 0 || gene_host[t] >= c->J || snp_host[t] < 0 || snp_host[t] >= S) {
             gp_set_error("score tests: gene or genotype index out of range at test " +
@@ -107,19 +260,15 @@ int score_tests(gp_ctx *c, int S, const GT *G_host, int T, const int *gene_host,
             return 1;
         }
     }
-    const int N = c->N, Npad = c->Npad, P = c->P;
-    const double *Zsrc = c->psi_whitened ? c->Zw : c->Z;
-    // chunk: Gw (Npad x tc) within ~2 GiB
-    int tc = (int)std::max<size_t>(1, ((size_t)1 << 28) / (size_t)Npad);
-    tc = std::min(tc, T);
+    const int N = c->N;
+    const int fly = std::max(1, std::min(16, (T + TT - 1) / TT));
     GT *dG = nullptr;
-    int *dgene = nullptr, *dsnp = nullptr;
-    double *Gw = nullptr, *a = nullptr, *s = nullptr, *B = nullptr, *dstat = nullptr;
-    int rc = 0;
+    int *dgene = nullptr, *dsnp = nullptr, *derr = nullptr;
+    double *dd = nullptr, *dcm = nullptr, *part = nullptr, *dstat = nullptr, *dp = nullptr;
     auto cleanup = [&]() {
         (void)hipStreamSynchronize(c->stream);
-        for (void *p : {(void *)dG, (void *)dgene, (void *)dsnp, (void *)Gw, (void *)a, (void *)s,
-                        (void *)B, (void *)dstat})
+        for (void *p : {(void *)dG, (void *)dgene, (void *)dsnp, (void *)derr, (void *)dd,
+                        (void *)dcm, (void *)part, (void *)dstat, (void *)dp})
             if (p) (void)hipFree(p);
     };
 #define SC_HIP(call)                                                                   \
@@ -134,27 +283,29 @@ int score_tests(gp_ctx *c, int S, const GT *G_host, int T, const int *gene_host,
     SC_HIP(hipMalloc((void **)&dG, (size_t)N * S * sizeof(GT)));
     SC_HIP(hipMalloc((void **)&dgene, (size_t)T * sizeof(int)));
     SC_HIP(hipMalloc((void **)&dsnp, (size_t)T * sizeof(int)));
-    SC_HIP(hipMalloc((void **)&Gw, (size_t)Npad * tc * sizeof(double)));
-    SC_HIP(hipMalloc((void **)&a, (size_t)tc * sizeof(double)));
-    SC_HIP(hipMalloc((void **)&s, (size_t)tc * sizeof(double)));
-    SC_HIP(hipMalloc((void **)&B, (size_t)P * tc * sizeof(double)));
+    SC_HIP(hipMalloc((void **)&derr, sizeof(int)));
+    SC_HIP(hipMalloc((void **)&part, (size_t)fly * chunk_doubles(c->P) * sizeof(double)));
     SC_HIP(hipMalloc((void **)&dstat, (size_t)T * sizeof(double)));
+    if (pval_host) SC_HIP(hipMalloc((void **)&dp, (size_t)T * sizeof(double)));
+    if (d_host) {
+        SC_HIP(hipMalloc((void **)&dd, (size_t)N * sizeof(double)));
+        SC_HIP(hipMalloc((void **)&dcm, (size_t)S * sizeof(double)));
+        SC_HIP(hipMemcpyAsync(dd, d_host, (size_t)N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    }
     SC_HIP(hipMemcpyAsync(dG, G_host, (size_t)N * S * sizeof(GT), hipMemcpyHostToDevice, c->stream));
     SC_HIP(hipMemcpyAsync(dgene, gene_host, (size_t)T * sizeof(int), hipMemcpyHostToDevice, c->stream));
     SC_HIP(hipMemcpyAsync(dsnp, snp_host, (size_t)T * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    for (int t0 = 0; t0 < T; t0 += tc) {
-        const int n = std::min(tc, T - t0);
-        k_score_prep<GT><<<n, ST, 0, c->stream>>>(N, Npad, t0, n, dG, dgene, dsnp, c->W, c->Omega,
-                                                  c->Yp, c->Yi, c->Yx, Gw, a, s);
-        SC_HIP(hipGetLastError());
-        rc = gpk_gemm_tn_splitk(c, P, n, Npad, Zsrc, Npad, Gw, Npad, B, P);
-        if (rc > 0) { cleanup(); return rc; }
-        k_score_finish<<<n, ST, (size_t)P * sizeof(double), c->stream>>>(P, t0, n, dgene, c->Psi,
-                                                                          c->phi, B, a, s, dstat);
+    k_score_clear<<<1, 1, 0, c->stream>>>(derr);
+    SC_HIP(hipGetLastError());
+    if (d_host) {
+        k_score_colmean<GT><<<S, ST, 0, c->stream>>>(N, dG, dd, dcm);
         SC_HIP(hipGetLastError());
     }
-    SC_HIP(hipMemcpyAsync(stat_host, dstat, (size_t)T * sizeof(double), hipMemcpyDeviceToHost,
-                          c->stream));
+    const int rc = score_run<GT>(c, S, dG, dd, dcm, T, dgene, dsnp, dstat, dp, part, fly, derr);
+    if (rc > 0) { cleanup(); return rc; }
+    SC_HIP(hipMemcpyAsync(stat_host, dstat, (size_t)T * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (pval_host)
+        SC_HIP(hipMemcpyAsync(pval_host, dp, (size_t)T * sizeof(double), hipMemcpyDeviceToHost, c->stream));
 #undef SC_HIP
     cleanup();
     return 0;
@@ -162,14 +313,34 @@ int score_tests(gp_ctx *c, int S, const GT *G_host, int T, const int *gene_host,
 
 }  // namespace
 
+size_t gpk_score_part_doubles(int P) { return chunk_doubles(P); }
+int gpk_score_max_columns() { return PMAX; }
+int gpk_score_clear(gp_ctx *c, int *err) {
+    k_score_clear<<<1, 1, 0, c->stream>>>(err);
+    GP_HIP(hipGetLastError());
+    return 0;
+}
+int gpk_score_run_dosage(gp_ctx *c, int S, const unsigned char *G, int T, const int *gene,
+                         const int *snp, double *stat, double *part, int fly, int *err) {
+    return score_run<unsigned char>(c, S, G, nullptr, nullptr, T, gene, snp, stat, nullptr, part,
+                                    fly, err);
+}
+
 extern "C" int gp_score_tests(gp_ctx *c, int S, const double *G, int T, const int *gene,
                               const int *snp, double *stat) {
     if (!c || !G || !gene || !snp || !stat) { gp_set_error("NULL argument"); return 1; }
-    return score_tests<double>(c, S, G, T, gene, snp, stat);
+    return score_tests_host<double>(c, S, G, nullptr, T, gene, snp, stat, nullptr);
 }
 
 extern "C" int gp_score_tests_dosage(gp_ctx *c, int S, const unsigned char *dosage, int T,
                                      const int *gene, const int *snp, double *stat) {
     if (!c || !dosage || !gene || !snp || !stat) { gp_set_error("NULL argument"); return 1; }
-    return score_tests<unsigned char>(c, S, dosage, T, gene, snp, stat);
+    return score_tests_host<unsigned char>(c, S, dosage, nullptr, T, gene, snp, stat, nullptr);
+}
+
+extern "C" int gp_score_tests_interaction(gp_ctx *c, int S, const unsigned char *dosage,
+                                          const double *factor, int T, const int *gene,
+                                          const int *snp, double *stat, double *pval) {
+    if (!c || !dosage || !gene || !snp || !stat) { gp_set_error("NULL argument"); return 1; }
+    return score_tests_host<unsigned char>(c, S, dosage, factor, T, gene, snp, stat, pval);
 }

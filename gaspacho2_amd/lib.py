@@ -60,6 +60,8 @@ def load() -> C.CDLL:
         "gp_bfit_get": (C.c_int, [vp, C.c_char_p, dp, C.c_size_t]),
         "gp_bfit_dims": (C.c_int, [vp, ip, ip]),
         "gp_score_tests_dosage": (C.c_int, [vp, C.c_int, C.POINTER(C.c_ubyte), C.c_int, ip, ip, dp]),
+        "gp_score_tests_interaction": (C.c_int, [vp, C.c_int, C.POINTER(C.c_ubyte), dp, C.c_int, ip,
+                                                 ip, dp, dp]),
         "gp_pca_init": (C.c_int, [C.c_int, C.c_int, ip, ip, dp, dp, C.c_int, dp, C.c_int, dp, dp]),
         "gp_enc_create": (C.c_int, [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_longlong, C.c_ulonglong]),
@@ -85,6 +87,7 @@ def load() -> C.CDLL:
         "gp_end_outer": (C.c_int, [vp]),
         "gp_fit": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, vp, vp, dp]),
         "gp_fit_trace": (C.c_int, [vp, dp, dp, ip, C.c_int, ip]),
+        "gp_fit_trace_steps": (C.c_int, [vp, ip, dp, C.c_int, ip]),
         "gp_timing_enable": (C.c_int, [vp, C.c_int]),
         "gp_timing_reset": (C.c_int, [vp]),
         "gp_timing_get": (C.c_int, [vp, C.c_char_p, dp, ip]),
@@ -269,6 +272,17 @@ class GPContext:
         self._check(self._lib.gp_fit_trace(self._h, _dp(o), _dp(t), _ip(k), n.value, C.byref(n)))
         return o, t, k
 
+    def fit_trace_steps(self):
+        """(entries x 4) int array (kind, itr, jtr, ktr) and the final
+        line-search sr of every entry of the last fit (the oracle's
+        fit_trace layout)."""
+        o, _, k = self.fit_trace()
+        n = C.c_int()
+        step, sr = np.zeros(3 * len(o), np.int32), np.zeros(len(o))
+        self._check(self._lib.gp_fit_trace_steps(self._h, _ip(step), _dp(sr), len(o), C.byref(n)))
+        info = np.concatenate([k[:, None], step.reshape(-1, 3)], axis=1)
+        return o, sr, info
+
     # -- score tests (ggplvm_de.cu:491-532)
     def score_tests(self, G, gene, snp) -> np.ndarray:
         """Statistics of tests (gene[t], snp[t]); G is N x S (float: doubles,
@@ -290,6 +304,23 @@ class GPContext:
                                           _ip(snp), _dp(out))
         self._check(st)
         return out
+
+    def score_tests_interaction(self, dosage, factor, gene, snp):
+        """(statistics, two-sided p-values) of tests (gene[t], snp[t]) on the
+        interaction columns g_l * factor - mean(g_l * factor) (factor None:
+        the dosage columns themselves); dosage is N x S uint8."""
+        D = np.asfortranarray(np.asarray(dosage, dtype=np.uint8).reshape(self.N, -1, order="F"))
+        flat = D.ravel(order="F")
+        gene = np.ascontiguousarray(gene, dtype=np.int32)
+        snp = np.ascontiguousarray(snp, dtype=np.int32)
+        stat, pval = np.zeros(len(gene)), np.zeros(len(gene))
+        f = None if factor is None else np.ascontiguousarray(factor, dtype=np.float64)
+        st = self._lib.gp_score_tests_interaction(self._h, D.shape[1],
+                                                  flat.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                                  None if f is None else _dp(f), len(gene), _ip(gene),
+                                                  _ip(snp), _dp(stat), _dp(pval))
+        self._check(st)
+        return stat, pval
 
     # -- timing
     def timing(self, on: bool = True) -> None:

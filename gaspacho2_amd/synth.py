@@ -17,6 +17,8 @@ import struct
 from dataclasses import dataclass, field
 from typing import Optional
 
+import os
+
 import numpy as np
 
 
@@ -203,11 +205,43 @@ class BatchedData:
     yval: np.ndarray
 
 
+def _batched_block(ds, Kxz, sizes, ncov, g0, g1, seed):
+    """Genes g0..g1-1 of make_batched, from their own stream."""
+    rng = np.random.default_rng([seed, 104729, g0])
+    N, Mg = ds.N, Kxz.shape[1]
+    A = rng.normal(0.0, 0.5, (Mg, g1 - g0))
+    F = Kxz @ A                                  # latent part of eta, N x block
+    out = []
+    for g in range(g0, g1):
+        n, pc = int(sizes[g]), int(ncov[g]) - 1
+        if 2 * n < N:     # sorted sample without replacement, no full permutation
+            c = np.unique(rng.integers(0, N, int(n * 1.3) + 16))
+            while c.size < n:
+                c = np.unique(np.concatenate([c, rng.integers(0, N, n)]))
+            c = np.sort(rng.choice(c, n, replace=False))
+        else:
+            c = np.sort(rng.choice(N, n, replace=False))
+        c = c.astype(np.int32)
+        C = rng.standard_normal((pc, n), dtype=np.float32)      # column-major n x pc
+        gam = rng.normal(0.0, 0.2, pc)
+        b = rng.normal(-1.0, 1.0)
+        eta = F[c, g - g0] + gam @ C + b
+        y = rng.poisson(ds.ls[c] * np.exp(eta))
+        nz = np.flatnonzero(y)
+        out.append((c, C.ravel(), nz.astype(np.int32), y[nz].astype(np.float64)))
+    return out
+
+
 def make_batched(ds: Dataset, G: int, nmin: int, nmax: int, pmin: int = 2, pmax: int = 16,
-                 seed: int = 0, block_cells: int = 1 << 22) -> BatchedData:
+                 seed: int = 0, gene_block: int = 256, threads: Optional[int] = None) -> BatchedData:
     """Subset sizes log-uniform in [nmin, nmax], p_g ~ U{pmin..pmax}; counts
     y ~ Poisson(s_i exp(f_g(x_i) + c_i . gamma_g + b_g)) with f_g from the
-    generating SE-ARD model of ds (its true latent X)."""
+    generating SE-ARD model of ds (its true latent X).  Blocks of gene_block
+    genes draw their subsets, covariates, coefficients and counts from their
+    own streams (so the data do not depend on the thread count) and the
+    latent part of eta of a block is one product Kxz (N x 64) by its
+    coefficients."""
+    from concurrent.futures import ThreadPoolExecutor
     rng = np.random.default_rng([seed, 104729])
     N, Q = ds.N, ds.Q
     X = ds.X_true if ds.X_true is not None else np.asarray(ds.X0)
@@ -220,30 +254,17 @@ def make_batched(ds: Dataset, G: int, nmin: int, nmax: int, pmin: int = 2, pmax:
     ncov = rng.integers(pmin, pmax + 1, G).astype(np.int32)
     off = np.zeros(G + 1, np.int64)
     off[1:] = np.cumsum(sizes)
-    cells, covs, yoff, ypos, yval = [], [], [0], [], []
-    for g in range(G):
-        n = int(sizes[g])
-        if 2 * n < N:     # sorted sample without replacement, no full permutation
-            c = np.unique(rng.integers(0, N, int(n * 1.3) + 16))
-            while c.size < n:
-                c = np.unique(np.concatenate([c, rng.integers(0, N, n)]))
-            c = np.sort(rng.choice(c, n, replace=False))
-        else:
-            c = np.sort(rng.choice(N, n, replace=False))
-        c = c.astype(np.int32)
-        C = rng.standard_normal((n, int(ncov[g]) - 1), dtype=np.float32)
-        a = rng.normal(0.0, 0.5, Mg)
-        gam = rng.normal(0.0, 0.2, int(ncov[g]) - 1)
-        b = rng.normal(-1.0, 1.0)
-        eta = Kxz[c] @ a + C.astype(np.float64) @ gam + b
-        y = rng.poisson(ds.ls[c] * np.exp(eta))
-        nz = np.flatnonzero(y)
-        cells.append(c)
-        covs.append(np.asfortranarray(C).ravel(order="F"))
-        ypos.append(nz.astype(np.int32))
-        yval.append(y[nz].astype(np.float64))
-        yoff.append(yoff[-1] + nz.size)
-    return BatchedData(off=off.astype(np.int32), cell=np.concatenate(cells), ncov=ncov,
-                       cov=np.concatenate(covs) if covs else np.zeros(0, np.float32),
-                       yoff=np.array(yoff, np.int32), ypos=np.concatenate(ypos),
-                       yval=np.concatenate(yval))
+    blocks = [(g0, min(G, g0 + gene_block)) for g0 in range(0, G, gene_block)]
+    nt = threads or min(16, os.cpu_count() or 1)
+    # single-threaded BLAS inside each worker: a block's product then does
+    # not depend on how many threads the library would have split it over
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(limits=1, user_api="blas"), ThreadPoolExecutor(max_workers=nt) as ex:
+        parts = list(ex.map(lambda b: _batched_block(ds, Kxz, sizes, ncov, b[0], b[1], seed), blocks))
+    genes = [x for part in parts for x in part]
+    yoff = np.zeros(G + 1, np.int64)
+    yoff[1:] = np.cumsum([x[2].size for x in genes])
+    return BatchedData(off=off.astype(np.int32), cell=np.concatenate([x[0] for x in genes]), ncov=ncov,
+                       cov=np.concatenate([x[1] for x in genes]) if genes else np.zeros(0, np.float32),
+                       yoff=yoff.astype(np.int32), ypos=np.concatenate([x[2] for x in genes]),
+                       yval=np.concatenate([x[3] for x in genes]))

@@ -33,11 +33,11 @@ def fitted_latent(ctx: lib.GPContext, N: int, Q: int) -> np.ndarray:
 
 
 def fit_from(ds, X0: np.ndarray, T0: np.ndarray, itrmax: int, jtr_cap: int,
-             precision: int = lib.PREC_MIXED):
+             precision: int = lib.PREC_MIXED, ard_only: bool = False):
     """gp_fit from the starting latent X0; returns (X, objective, seconds,
     trace) with trace = (objectives, seconds, kinds) of gp_fit_trace."""
     g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls,
-                      np.asfortranarray(X0), T0, precision=precision)
+                      np.asfortranarray(X0), T0, ard_only=ard_only, precision=precision)
     try:
         t = time.perf_counter()
         obj = g.fit(itrmax, jtr_cap)

@@ -176,6 +176,15 @@ int gp_score_tests(gp_ctx *ctx, int S, const double *G, int T, const int *gene,
                    const int *snp, double *stat);
 int gp_score_tests_dosage(gp_ctx *ctx, int S, const unsigned char *dosage, int T,
                           const int *gene, const int *snp, double *stat);
+/* Interaction tests as the differential-expression driver builds them
+ * (ggplvm_de.cu:989-994, with util.c:100-110 scale): the test column of SNP l
+ * is h = g_l * factor - mean_i(g_l * factor) for a per-cell factor[N] (NULL:
+ * h = g_l), then the statistic above with h in place of g
+ * (ggplvm_de.cu:491-532, 1206-1210); pval[t] (may be NULL) = the two-sided
+ * normal p-value erfc(|stat| / sqrt 2).  dosage: N x S bytes (0/1/2). */
+int gp_score_tests_interaction(gp_ctx *ctx, int S, const unsigned char *dosage,
+                               const double *factor, int T, const int *gene, const int *snp,
+                               double *stat, double *pval);
 
 /* ---- Batched per-gene GLM / Laplace fits on cell subsets ----
  * No counterpart in the reference, which fits every gene on all N cells
@@ -279,6 +288,11 @@ int gp_fit(gp_ctx *ctx, int itrmax, int jtr_cap, int verbose,
  * host seconds since gp_fit began.  Copies min(*len, cap) entries; any output
  * pointer may be NULL; *len = entries recorded. */
 int gp_fit_trace(gp_ctx *ctx, double *obj, double *seconds, int *kind, int cap, int *len);
+/* The same entries' positions in the schedule: step[3k..3k+2] = outer
+ * iteration itr, inner step jtr and (L-BFGS entries) the line-search trials
+ * ktr of ggplvm.cu:1257-1290; sr[k] = the step fraction the line search
+ * ended on (1 when the first trial was accepted).  Either pointer may be NULL. */
+int gp_fit_trace_steps(gp_ctx *ctx, int *step, double *sr, int cap, int *len);
 
 /* util.c:73-87 initTaWithPer: initial inducing points T (M x Q,
  * column-major).  Dim 0 equispaced on (0, 2*pi); dims 1..Q-1 N(0,1) from

@@ -61,8 +61,13 @@ def test_irls_and_quasi_likelihood_start(small_ds, ard):
     _cmp(o, g, "lkhds")
 
 
-def test_elbo_and_gradient(small_ds):
-    o, g = _pair(small_ds)
+@pytest.mark.parametrize("ard", [False, True])
+def test_elbo_and_gradient(small_ds, ard):
+    """ELBO + gradient unit vs the oracle with the periodic-dim-0 kernel
+    (kernel_funcs.cu:61-97) and with SE-ARD on every dimension
+    (kernel_funcs.cu:221-255): the cell gradient, the C1 T product, and the
+    tau / rho gradients take their non-periodic branches for dim 0."""
+    o, g = _pair(small_ds, ard_only=ard)
     o.init_fit()
     g.init_fit()
     for jtr in range(3):
@@ -520,6 +525,40 @@ def test_score_tests_match_oracle(small_ds):
     assert np.array_equal(sg, sd)
 
 
+@pytest.mark.parametrize("ard", [False, True])
+def test_interaction_score_tests_and_p_values_match_oracle(small_ds, ard):
+    """Interaction tests (ggplvm_de.cu:989-994): the test column g * d centred
+    over cells, built here on the host as the reference does and given to the
+    oracle's getScoreStats, against the GPU's on-device construction from the
+    dosage bytes and the per-cell factor; two-sided normal p-values against
+    scipy's normal survival function; an absent factor gives the plain
+    dosage statistics."""
+    import scipy.stats
+    o, g = _pair(small_ds, ard_only=ard)
+    o.init_fit()
+    g.init_fit()
+    for jtr in range(3):
+        o.irls_step(jtr)
+        g.irls_step(jtr)
+    N, J = small_ds.N, small_ds.J
+    rng = np.random.default_rng(12)
+    S = 9
+    D = rng.integers(0, 3, (N, S)).astype(np.uint8)
+    d = rng.uniform(0.0, 3.0, N)
+    Gi = D.astype(np.float64) * d[:, None]
+    Gi -= Gi.mean(axis=0)
+    gene = np.repeat(np.arange(J), S).astype(np.int32)
+    snp = np.tile(np.arange(S), J).astype(np.int32)
+    so = o.score_stats(Gi, gene, snp)
+    sg, pg = g.score_tests_interaction(D, d, gene, snp)
+    print(f"interaction score statistics: max rel err {rel(sg, so):.2e}")
+    assert rel(sg, so) < 1e-9
+    pref = 2.0 * scipy.stats.norm.sf(np.abs(sg))
+    assert np.max(np.abs(pg - pref) / pref) < 1e-12
+    s0, _ = g.score_tests_interaction(D, None, gene, snp)
+    assert np.array_equal(s0, g.score_tests(D, gene, snp))
+
+
 def test_score_tests_mixed_basis_close_to_oracle():
     """Score statistics from the mixed path's whitened factors (after gp_elbo)
     against the oracle's reference-basis factors at the same W."""
@@ -638,3 +677,65 @@ def test_pca_init_matches_oracle(case, tmp_path):
     assert p.returncode == 0, p.stderr
     Xt = np.frombuffer(open(tmp_path / "Xinit.bin", "rb").read(), dtype="<f8").reshape(Q, N).T
     assert np.array_equal(Xt, Xg)
+
+
+@pytest.mark.parametrize("M", [288, 512, 600])
+def test_wide_design_fp64_matches_oracle(M):
+    """P = M + 1 beyond the 272-column tiles of the fast kernels: P16 = 304
+    and 528 (FP64 MFMA Gram and contraction past their instantiated widths)
+    and 608 (the generic FP64 Gram past its LDS bound), SE-ARD, Q = 20.  IRLS
+    objectives, factors, ELBO and gradient against the oracle."""
+    from gaspacho2_amd import synth
+    N, J, Q = (500, 3, 20) if M == 600 else (600, 4, 20)
+    ds = synth.make_dataset(N, J, Q, M, seed=5, pca=False)
+    o, g = _pair(ds, ard_only=True)
+    o.init_fit()
+    g.init_fit()
+    for jtr in range(2):
+        a, b = o.irls_step(jtr), g.irls_step(jtr)
+        assert abs(a - b) <= TOL_OBJ * abs(a), (jtr, a, b)
+    P = ds.P
+    assert rel(_psi_lower(g.get("Psi"), J, P), _psi_lower(o.get("Psi"), J, P)) < TOL
+    a, b = o.ql_start(), g.ql_start()
+    assert abs(a - b) <= TOL_OBJ * abs(a)
+    o.grad()
+    g.grad()
+    assert rel(g.get("grad"), o.get("grad")) < TOL
+    _cmp(o, g, "C2")
+
+
+@pytest.mark.parametrize("M", [288, 512])
+def test_wide_design_mixed_close_to_fp64(M):
+    """The mixed path at P16 = 304 and 528 against the FP64 whitened path of
+    the same state (pinned to the oracle by the whitened-basis test): Gram
+    entries within 5e-7 of each gene's largest, ELBO within 1e-7, gradient
+    within 1e-4 of its largest entry."""
+    from gaspacho2_amd import lib, synth
+    ds = synth.make_dataset(3000, 24, 20, M, seed=9, pca=False)
+    T0 = lib.init_ta_with_per(ds.M, ds.Q)
+    g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0,
+                      T0, ard_only=True, precision=lib.PREC_FP64)
+    g.init_fit()
+    for jtr in range(2):
+        g.irls_step(jtr)
+    g.ql_start()
+    J, P = ds.J, ds.P
+    iu = np.triu_indices(P, 1)
+    g.set_precision(lib.PREC_FP64_WHITENED)
+    e64 = g.elbo(1)
+    Pw = g.get("Psi")[:J * P * P].reshape(J, P, P)[:, iu[1], iu[0]]
+    g.grad()
+    g64 = g.get("grad")
+    g.set_precision(lib.PREC_MIXED)
+    em = g.elbo(1)
+    Pm = g.get("Psi")[:J * P * P].reshape(J, P, P)[:, iu[1], iu[0]]
+    g.grad()
+    gm = g.get("grad")
+    err = np.max(np.abs(Pm - Pw), axis=1) / np.max(np.abs(Pw), axis=1)
+    ge = rel(gm, g64)
+    print(f"M={M} mixed: Gram max rel err {err.max():.2e}, ELBO {abs(em - e64) / abs(e64):.2e}, "
+          f"gradient {ge:.2e}")
+    assert err.max() < 5e-7
+    assert abs(em - e64) <= 1e-7 * abs(e64)
+    assert ge < 1e-4
+    g.close()
