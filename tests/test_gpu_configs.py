@@ -26,7 +26,8 @@ def _blocks(N, p, Q, M):
 def test_n100k_m256_gene_sample_matches_oracle():
     """N=100,000 cells, Q=10, M=256 (P=257), SE-ARD, 8 genes: two IRLS steps,
     the quasi-likelihood start and one gradient on the FP64 path against the
-    oracle (objectives 1e-11, arrays 1e-9), then the mixed ELBO within 1e-6
+    oracle (objectives 1e-11, arrays 1e-9; the tau / rho gradient blocks and
+    C2, sums over all 100,000 cells, 2e-8), then the mixed ELBO within 1e-6
     and the mixed gradient within 1e-4 of the oracle's."""
     from gaspacho2_amd import lib, synth
     ds = synth.make_dataset(100000, 8, 10, 256, seed=0, pca=False)
@@ -53,7 +54,7 @@ def test_n100k_m256_gene_sample_matches_oracle():
         # gT and grho sum C1-weighted terms over 100,000 cells: the two
         # summation orders part by ~6e-9 of the largest entry at this N
         assert r < (2e-8 if k in ("gT", "grho") else 1e-9), k
-    assert rel(g.get("C2"), o.get("C2")) < 1e-9
+    assert rel(g.get("C2"), o.get("C2")) < 2e-8            # also a sum over the 100,000 cells
     eo = o.elbo(1)
     g.set_precision(lib.PREC_MIXED)
     em = g.elbo(1)
