@@ -23,7 +23,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 HIP_SOURCES = ["gp_ctx.hip", "gp_kernels.hip", "gp_chol.hip", "gp_gram.hip",
                "gp_gram_mixed.hip", "gp_gradcontract.hip", "gp_gradcontract_mixed.hip",
-               "gp_score.hip", "gp_batched.hip", "gp_encoder.hip", "gp_pca.hip"]
+               "gp_score.hip", "gp_score_noalloc.hip", "gp_batched.hip", "gp_encoder.hip",
+               "gp_pca.hip"]
 # per-source extra flags: the Gram kernel keeps its MFMA accumulators in
 # VGPRs (no per-chunk AGPR<->VGPR copies, two waves per SIMD fit)
 EXTRA_FLAGS = {"gp_gram.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}

@@ -62,6 +62,7 @@ def load() -> C.CDLL:
         "gp_score_tests_dosage": (C.c_int, [vp, C.c_int, C.POINTER(C.c_ubyte), C.c_int, ip, ip, dp]),
         "gp_score_tests_interaction": (C.c_int, [vp, C.c_int, C.POINTER(C.c_ubyte), dp, C.c_int, ip,
                                                  ip, dp, dp]),
+        "gp_score_tests_dosage_noalloc": (C.c_int, [vp, C.c_int, vp, C.c_int, vp, vp, vp]),
         "gp_pca_init": (C.c_int, [C.c_int, C.c_int, ip, ip, dp, dp, C.c_int, dp, C.c_int, dp, dp]),
         "gp_enc_create": (C.c_int, [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_longlong, C.c_ulonglong]),
@@ -321,6 +322,15 @@ class GPContext:
                                                   _ip(snp), _dp(stat), _dp(pval))
         self._check(st)
         return stat, pval
+
+    def score_tests_device(self, S: int, dosage_ptr: int, T: int, gene_ptr: int, snp_ptr: int,
+                           stat_ptr: int) -> None:
+        """gp_score_tests_dosage_noalloc: every argument is a device address
+        (N x S dosage bytes, T gene and SNP ints, T statistics); nothing is
+        allocated."""
+        self._check(self._lib.gp_score_tests_dosage_noalloc(self._h, int(S), C.c_void_p(dosage_ptr),
+                                                            int(T), C.c_void_p(gene_ptr),
+                                                            C.c_void_p(snp_ptr), C.c_void_p(stat_ptr)))
 
     # -- timing
     def timing(self, on: bool = True) -> None:

@@ -185,6 +185,18 @@ int gp_score_tests_dosage(gp_ctx *ctx, int S, const unsigned char *dosage, int T
 int gp_score_tests_interaction(gp_ctx *ctx, int S, const unsigned char *dosage,
                                const double *factor, int T, const int *gene, const int *snp,
                                double *stat, double *pval);
+/* No counterpart in the reference.  gp_score_tests_dosage for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc): dosage_dev (N x S bytes), gene_dev and
+ * snp_dev (T ints) and stat_dev (T doubles) are the caller's device buffers,
+ * complete before the call; the partial sums live in a static 32 MiB device
+ * array that each GPU holds until the process ends, so nothing is allocated.
+ * Returns after stat_dev is written.  Designs wider than 1024 model-matrix
+ * columns are rejected (status 1).  An out-of-range gene or SNP index sets
+ * that test's statistic to NaN and returns status 1.  Calls that overlap on
+ * one GPU share the array and race. */
+int gp_score_tests_dosage_noalloc(gp_ctx *ctx, int S, const unsigned char *dosage_dev, int T,
+                                  const int *gene_dev, const int *snp_dev, double *stat_dev);
 
 /* ---- Batched per-gene GLM / Laplace fits on cell subsets ----
  * No counterpart in the reference, which fits every gene on all N cells
