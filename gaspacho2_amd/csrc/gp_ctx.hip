@@ -60,7 +60,8 @@ static void free_all(gp_ctx *c) {
                     c->lkhd, c->info, c->Zh, c->Zl, c->z32, c->gram_part, c->pwork, c->w32, c->wsc,
                     c->wun, c->zun, c->Linv, c->Zw, c->DeltaW, c->DeltaWChol, c->Uw, c->tBw,
                     c->Vtmp, c->Sh, c->Sl, c->gc_perm, c->gc_aexp, c->gc_sexp, c->gc_bexp,
-                    c->gc_gun, c->gc_part, c->gc_zexp, c->skw, c->gc_kappa};
+                    c->gc_gun, c->gc_part, c->gc_zexp, c->skw, c->gc_kappa, c->gc_t0, c->gc_s0,
+                    c->gc_y};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     gp_timer *ts[] = {&c->t_gram, &c->t_chol, &c->t_gc, &c->t_kuf, &c->t_elbo, &c->t_grad,
@@ -260,6 +261,14 @@ extern "C" int gp_set_option(gp_ctx *c, const char *name, int value) {
         c->gc_form = value;
     } else if (n == "gc_mixed") {
         c->gc_mixed = value ? 1 : 0;
+    } else if (n == "gc_peel") {
+        c->gc_peel = value ? 1 : 0;
+    } else if (n == "gc_wave_rows") {
+        if (value != 8 && value != 4) { gp_set_error("gc_wave_rows must be 8 or 4"); return 1; }
+        c->gc_wave_rows = value;
+    } else if (n == "gc_prefetch") {
+        if (value != 0 && value != 1 && value != 3 && value != 7) { gp_set_error("gc_prefetch must be 0, 1, 3 or 7"); return 1; }
+        c->gc_prefetch = value;
     } else if (n == "irls_mixed") {
         c->irls_mixed = value ? 1 : 0;
     } else if (n == "autotune") {
@@ -277,6 +286,9 @@ extern "C" int gp_get_option(gp_ctx *c, const char *name, int *value) {
     if (n == "gram_splits") *value = c->gram_splits;
     else if (n == "gc_groups") *value = c->gc_groups;
     else if (n == "gc_mixed") *value = c->gc_mixed;
+    else if (n == "gc_peel") *value = c->gc_peel;
+    else if (n == "gc_wave_rows") *value = c->gc_wave_rows;
+    else if (n == "gc_prefetch") *value = c->gc_prefetch;
     else if (n == "gram_waves") *value = c->gram_waves;
     else if (n == "gc_form") *value = c->gc_form;
     else if (n == "irls_mixed") *value = c->irls_mixed;
@@ -685,7 +697,8 @@ extern "C" int gp_eval_graph(gp_ctx *c, int slot, double *obj) {
     if (!c) { gp_set_error("NULL context"); return 1; }
     if (slot < 0 || slot >= GP_LKHD_SLOTS) { gp_set_error("slot out of range"); return 1; }
     const int key[GP_GRAPH_KEY] = {c->precision, slot, c->gram_splits, c->gc_groups, c->gram_waves,
-                                   c->gc_form, c->gc_mixed};
+                                   c->gc_form, c->gc_mixed, c->gc_peel, c->gc_wave_rows,
+                                   c->gc_prefetch};
     if (!c->graph_exec || !std::equal(key, key + GP_GRAPH_KEY, c->graph_key)) {
         drop_graph(c);
         double tmp = 0.0;

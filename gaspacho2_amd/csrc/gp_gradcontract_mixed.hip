@@ -842,6 +842,347 @@ static int launch_gc_mixed(gp_ctx *c, int gsize, int ngroup) {
     return 0;
 }
 
+// ---- intercept-peeled factored row-block form (default for a design whose
+// only column before the GP block is the intercept).  With zw_i = [1 | g_i]
+// (g_i the M GP columns) and Sw_j = [s00 s0G; sG0 SGG]:
+//   Uw_i[G] = sum_j o_ij (SGG g_i + sG0),   Uw_i[0] = sum_j o_ij (s00 + s0G . g_i).
+// The MFMAs of the factored form run over the GP block only (K = M, M / 16
+// column tiles: an even count, so no odd tile, no 16-column tile and no 32-k
+// step spent on the single intercept column); the intercept's k term sG0
+// (scaled like the product) joins each gene's FP32 product in the update
+// U += o (acc + t0).  The intercept output column is one FP32 GEMM
+// Y = w32 S0^T over the genes (w32 = W / Omega, S0[j, :] = [s0G | s00]) and
+// a per-cell dot Uw_i[0] = g_i . Y_i[0:M] + Y_i[M] in FP64.
+//
+// Operands of gene j: SGG hi/lo as [col < M][k < M] (k contiguous), scaled as
+// the full form's B (2^(bexp_j + zc_{1+k})); t0[j][c] = Sw_j[0, 1+c] 2^bexp_j.
+__global__ void k_ssplit_gp(int P, int M, const double *__restrict__ S, const int *__restrict__ bexp,
+                            const int *__restrict__ zexp, int J,
+                            _Float16 *__restrict__ sh, _Float16 *__restrict__ sl,
+                            float *__restrict__ t0, float *__restrict__ s0) {
+    const int j = blockIdx.y;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;   // over M * M
+    const double *Sj = S + (size_t)j * P * P;
+    if (idx < (size_t)M * M) {
+        const int col = (int)(idx / M), k = (int)(idx % M);
+        const float v = (float)ldexp(Sj[(1 + k) + (size_t)(1 + col) * P], bexp[j] + zexp[1 + k]);
+        const _Float16 h = (_Float16)v;
+        const size_t o = (size_t)j * M * M + idx;
+        sh[o] = h;
+        sl[o] = (_Float16)(v - (float)h);
+    }
+    if (idx < (size_t)M) {
+        const double r0 = Sj[(size_t)(1 + idx) * P];              // Sw_j[0, 1+c]
+        t0[(size_t)j * M + idx] = (float)ldexp(r0, bexp[j]);
+        s0[j + idx * (size_t)J] = (float)r0;
+    }
+    if (idx == 0) s0[j + (size_t)M * J] = (float)Sj[0];
+}
+
+// Wave tiling: the 8 row tiles (128 cells) x TG column tiles of a workgroup
+// are cut into blocks of RG row tiles x CG = 16 / RG column tiles, one per
+// wave (TG / 2 waves).  RG = 8 is the factored form's shape (each wave reads
+// every A row fragment, 2 column tiles of B); RG = 4 halves the A fragment
+// reads from LDS per MFMA and doubles the B fragment loads from L2 (measured
+// 50 % slower at M = 256: the B stream from L2 is the limit).  B fragments
+// are loaded PD k-steps ahead through a ring of PD + 1 register sets (PD + 1
+// divides the k-steps of a gene, so the ring lines up at every gene); the
+// last PD k-steps of a gene load the first ones of the next.
+template <int TG, int RG, int PD>
+__global__ void __launch_bounds__(TG / 2 * 64) k_gc_gp(int N, int Npad, int gsize, int J,
+        const double *__restrict__ Zg, const float *__restrict__ w32,
+        const int *__restrict__ perm, const int *__restrict__ aexp, const int *__restrict__ zexp,
+        const _Float16 *__restrict__ shi, const _Float16 *__restrict__ slo,
+        const float *__restrict__ t0, float *__restrict__ part) {
+    constexpr int M = 16 * TG;
+    constexpr int CG = 16 / RG;
+    constexpr int NWC = TG / CG;
+    constexpr int NW = TG / 2;
+    static_assert(RG * CG == 16 && TG % CG == 0 && NWC * (8 / RG) == NW, "wave tiling");
+    constexpr int KS = M / 32;
+    constexpr int LDA = M + 8;
+    constexpr int NITEM = GRC * KS * 4;
+    constexpr int IPT = (NITEM + NW * 64 - 1) / (NW * 64);
+    extern __shared__ __attribute__((aligned(16))) _Float16 As[];   // [2 hl][GRC][LDA]
+    __shared__ float os[2][GRC];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int g = blockIdx.y;
+    const int j0 = g * gsize, j1 = min(J, j0 + gsize);
+    const int cb = blockIdx.x * GRC;
+    const int rt0 = (w / NWC) * RG, ct0 = (w % NWC) * CG;     // first row / column tile of the wave
+    _Float16 *Ah = As, *Al = As + (size_t)GRC * LDA;
+    // A = g (the GP block of zw, scaled by 2^-zc), split hi/lo, once for the
+    // group; every load of an item is issued before any is used (a row past
+    // N reads row N - 1 and is zeroed)
+#pragma unroll
+    for (int r = 0; r < IPT; r++) {
+        const int it = min(tid + NW * 64 * r, NITEM - 1);
+        const int cl = it % GRC, k0 = 8 * (it / GRC);
+        const int cell = cb + cl, cellc = min(cell, N - 1);
+        double zv[8];
+        int ze[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            zv[e] = Zg[cellc + (size_t)(k0 + e) * Npad];
+            ze[e] = zexp[k0 + e];
+        }
+        h8 hi, lo;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            float v = (cell < N) ? (float)ldexp(zv[e], -ze[e]) : 0.0f;
+            asm volatile("" : "+v"(v));
+            const _Float16 h = (_Float16)v;
+            hi[e] = h;
+            lo[e] = (_Float16)(v - (float)h);
+        }
+        *(h8 *)&Ah[cl * LDA + k0] = hi;
+        *(h8 *)&Al[cl * LDA + k0] = lo;
+    }
+    // cell weights: gene jj + 1's are loaded into registers at the start of
+    // gene jj and stored to LDS at its end, so no store waits on a fresh load
+    constexpr int WPT = (GRC + NW * 64 - 1) / (NW * 64);      // weights per thread (1 or 2)
+    auto wload = [&](int jj, float (&wv)[WPT]) {
+        const int j = perm[min(jj, j1 - 1)];
+        const float sc = ldexpf(1.0f, aexp[j]);
+#pragma unroll
+        for (int u = 0; u < WPT; u++) {
+            const int t = tid + u * NW * 64;
+            const int cell = min(cb + t, N - 1);
+            const float v = w32[cell + (size_t)j * Npad];
+            wv[u] = (t < GRC && cb + t < N) ? v * sc : 0.0f;
+        }
+    };
+    auto wstore = [&](const float (&wv)[WPT], int buf) {
+#pragma unroll
+        for (int u = 0; u < WPT; u++) {
+            const int t = tid + u * NW * 64;
+            if (t < GRC) os[buf][t] = wv[u];
+        }
+    };
+    float wnext[WPT];
+    if (j0 < j1) {
+        wload(j0, wnext);
+        wstore(wnext, 0);
+    }
+    __syncthreads();
+
+    f4 U[RG][CG];
+#pragma unroll
+    for (int r = 0; r < RG; r++)
+#pragma unroll
+        for (int c = 0; c < CG; c++) U[r][c] = f4{0.f, 0.f, 0.f, 0.f};
+    const int lk = (lane & 15), lq = 8 * (lane >> 4);
+    int bcol[CG];
+#pragma unroll
+    for (int c = 0; c < CG; c++) bcol[c] = (16 * (ct0 + c) + lk) * M + lq;
+    const int arow = lk * LDA + lq;
+    const int orow = 4 * (lane >> 4);
+
+    static_assert(KS % (PD + 1) == 0, "prefetch ring must line up with the gene's k-steps");
+    // A fragments are read from LDS AD (k-step, row tile) steps ahead into a
+    // ring of AD + 1 register pairs, so each MFMA group finds its operands
+    // loaded instead of waiting on the LDS latency; A is the same for every
+    // gene of the group, so the ring wraps across genes.  The scheduling
+    // barriers keep the compiler from sinking the reads next to their use.
+    constexpr int AD = 3, NA = AD + 1;
+    constexpr int TS = KS * RG;                  // (k-step, row tile) steps per gene
+    static_assert(TS % NA == 0, "A ring must line up with the gene's steps");
+    h8 ar_h[NA], ar_l[NA];
+    auto aload = [&](int t, int slot) {
+        const int ks = (t / RG) % KS, r = t % RG;
+        ar_h[slot] = *(const h8 *)(Ah + 16 * (rt0 + r) * LDA + arow + 32 * ks);
+        ar_l[slot] = *(const h8 *)(Al + 16 * (rt0 + r) * LDA + arow + 32 * ks);
+    };
+#pragma unroll
+    for (int t = 0; t < AD; t++) aload(t, t);
+    h8 bh[PD + 1][CG], bl[PD + 1][CG];
+    if (j0 < j1) {
+        const int j = perm[j0];
+        const _Float16 *hj = shi + (size_t)j * M * M, *lj = slo + (size_t)j * M * M;
+#pragma unroll
+        for (int d = 0; d < PD; d++)
+#pragma unroll
+            for (int c = 0; c < CG; c++) {
+                bh[d][c] = *(const h8 *)(hj + bcol[c] + 32 * d);
+                bl[d][c] = *(const h8 *)(lj + bcol[c] + 32 * d);
+            }
+    }
+    for (int jj = j0; jj < j1; jj++) {
+        const int j = perm[jj];
+        const int jn = perm[min(jj + 1, j1 - 1)];
+        const int buf = (jj - j0) & 1;
+        wload(jj + 1, wnext);
+        const _Float16 *hj = shi + (size_t)j * M * M, *lj = slo + (size_t)j * M * M;
+        const _Float16 *hn = shi + (size_t)jn * M * M, *ln = slo + (size_t)jn * M * M;
+        float tv[CG];
+#pragma unroll
+        for (int c = 0; c < CG; c++) tv[c] = t0[(size_t)j * M + 16 * (ct0 + c) + lk];
+        f4 acc[RG][CG];
+#pragma unroll
+        for (int r = 0; r < RG; r++)
+#pragma unroll
+            for (int c = 0; c < CG; c++) acc[r][c] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < KS; ks++) {
+            // PD k-steps ahead: this gene's, or the next gene's first ones
+            constexpr int NB = PD + 1;
+            const int sl = (ks + PD) % NB, s0 = ks % NB;
+            const int sa = ks + PD;
+            const _Float16 *ph = sa < KS ? hj + 32 * sa : hn + 32 * (sa - KS);
+            const _Float16 *pl = sa < KS ? lj + 32 * sa : ln + 32 * (sa - KS);
+#pragma unroll
+            for (int c = 0; c < CG; c++) {
+                bh[sl][c] = *(const h8 *)(ph + bcol[c]);
+                bl[sl][c] = *(const h8 *)(pl + bcol[c]);
+            }
+#pragma unroll
+            for (int r = 0; r < RG; r++) {
+                const int t = ks * RG + r;
+                aload(t + AD, (t + AD) % NA);
+                __builtin_amdgcn_sched_barrier(0);
+                const h8 ah = ar_h[t % NA], al = ar_l[t % NA];
+#pragma unroll
+                for (int c = 0; c < CG; c++) {
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[s0][c], acc[r][c], 0, 0, 0);
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[s0][c], acc[r][c], 0, 0, 0);
+                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[s0][c], acc[r][c], 0, 0, 0);
+                }
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        // U += o (rows) * (C + intercept term)
+#pragma unroll
+        for (int r = 0; r < RG; r++) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const float o = os[buf][16 * (rt0 + r) + orow + q];
+#pragma unroll
+                for (int c = 0; c < CG; c++) U[r][c][q] += o * (acc[r][c][q] + tv[c]);
+            }
+        }
+        wstore(wnext, buf ^ 1);
+        __syncthreads();                          // os[buf ^ 1] written; os[buf] free
+    }
+    float *pg = part + (size_t)g * M * Npad;
+#pragma unroll
+    for (int r = 0; r < RG; r++) {
+        const int row0 = cb + 16 * (rt0 + r) + orow;
+#pragma unroll
+        for (int c = 0; c < CG; c++) {
+            const int col = 16 * (ct0 + c) + lk;
+#pragma unroll
+            for (int q = 0; q < 4; q++) pg[(size_t)col * Npad + row0 + q] = U[r][c][q];
+        }
+    }
+}
+
+// Uw[:, 1 + c] = sum_g part[g][c][:] 2^-C_g (c < M)
+__global__ void k_gc_reduce_gp(int Npad, int M, int ngroup, const float *__restrict__ part,
+                               const double *__restrict__ gun, double *__restrict__ Uw) {
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (size_t)Npad * M) return;
+    double s = 0.0;
+    for (int g = 0; g < ngroup; g++) s += (double)part[(size_t)g * M * Npad + idx] * gun[g];
+    Uw[(size_t)Npad + idx] = s;
+}
+
+// Uw[i, 0] = sum_{k<M} Zw[i, 1+k] Y[i, k] + Y[i, M]
+__global__ void k_gc_u0(int Npad, int M, const double *__restrict__ Zw, const float *__restrict__ Y,
+                        double *__restrict__ Uw) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= Npad) return;
+    double s = (double)Y[i + (size_t)M * Npad];
+    for (int k = 0; k < M; k++) s += Zw[i + (size_t)(1 + k) * Npad] * (double)Y[i + (size_t)k * Npad];
+    Uw[i] = s;
+}
+
+template <int TG, int RG, int PD>
+static int launch_gc_gp(gp_ctx *c, int gsize, int ngroup) {
+    constexpr int M = 16 * TG;
+    const size_t shm = (size_t)2 * GRC * (M + 8) * sizeof(_Float16);
+    if (shm > 65536 && !c->capturing)
+        GP_HIP(hipFuncSetAttribute((const void *)k_gc_gp<TG, RG, PD>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    dim3 g(c->Npad / GRC, ngroup);
+    k_gc_gp<TG, RG, PD><<<g, TG / 2 * 64, shm, c->stream>>>(c->N, c->Npad, gsize, c->J, c->Zw + c->Npad,
+                                                           c->w32, c->gc_perm, c->gc_aexp, c->gc_zexp + 1,
+                                                           (const _Float16 *)c->Sh, (const _Float16 *)c->Sl,
+                                                           c->gc_t0, c->gc_part);
+    GP_HIP(hipGetLastError());
+    return 0;
+}
+
+// prefetch depth: the deepest of 1, 3 and 7 whose ring divides the k-steps
+// (M / 32) and does not exceed the option gc_prefetch
+template <int TG, int RG>
+static int launch_gc_gp_pd(gp_ctx *c, int gsize, int ngroup) {
+    constexpr int KS = TG / 2;
+    if constexpr (KS % 8 == 0) {
+        if (c->gc_prefetch >= 7) return launch_gc_gp<TG, RG, 7>(c, gsize, ngroup);
+    }
+    if constexpr (KS % 4 == 0) {
+        if (c->gc_prefetch >= 3) return launch_gc_gp<TG, RG, 3>(c, gsize, ngroup);
+    }
+    if constexpr (KS % 2 == 0) {
+        if (c->gc_prefetch >= 1) return launch_gc_gp<TG, RG, 1>(c, gsize, ngroup);
+    }
+    return launch_gc_gp<TG, RG, 0>(c, gsize, ngroup);
+}
+
+template <int TG>
+static int launch_gc_gp_rg(gp_ctx *c, int gsize, int ngroup) {
+    if constexpr (TG % 4 == 0) {
+        if (c->gc_wave_rows == 4) return launch_gc_gp_pd<TG, 4>(c, gsize, ngroup);
+    }
+    return launch_gc_gp_pd<TG, 8>(c, gsize, ngroup);
+}
+
+// the peeled form applies: the intercept is the only column before the GP
+// block, and M is a multiple of 32 with at most 16 column tiles
+static bool gc_peel_ok(const gp_ctx *c) {
+    return c->gc_peel && c->gc_form == 2 && c->g0 == 1 && c->M % 32 == 0 && c->M / 16 <= 16 &&
+           c->Npad % GRC == 0;
+}
+
+static int gradcontract_gp(gp_ctx *c, int gsize, int ngroup) {
+    const int M = c->M, J = c->J, TG = M / 16;
+    if (!c->gc_t0) {
+        GP_HIP(hipMalloc((void **)&c->gc_t0, (size_t)J * M * sizeof(float)));
+        GP_HIP(hipMalloc((void **)&c->gc_s0, (size_t)J * (M + 1) * sizeof(float)));
+        GP_HIP(hipMalloc((void **)&c->gc_y, (size_t)c->Npad * (M + 1) * sizeof(float)));
+    }
+    dim3 gs((unsigned)(((size_t)M * M + 255) / 256), J);
+    k_ssplit_gp<<<gs, 256, 0, c->stream>>>(c->P, M, c->Sinv, c->gc_bexp, c->gc_zexp, J,
+                                           (_Float16 *)c->Sh, (_Float16 *)c->Sl, c->gc_t0, c->gc_s0);
+    GP_HIP(hipGetLastError());
+    // intercept output column: Y = w32 (Npad x J) . S0 (J x (M+1)), FP32
+    const float one = 1.0f, zero = 0.0f;
+    GP_BLAS(rocblas_sgemm(c->blas, rocblas_operation_none, rocblas_operation_none, c->Npad, M + 1, J,
+                          &one, c->w32, c->Npad, c->gc_s0, J, &zero, c->gc_y, c->Npad));
+    int r = 0;
+    gp_timer_begin(c, c->t_gck);
+    switch (TG) {
+    case 2: r = launch_gc_gp_rg<2>(c, gsize, ngroup); break;
+    case 4: r = launch_gc_gp_rg<4>(c, gsize, ngroup); break;
+    case 6: r = launch_gc_gp_rg<6>(c, gsize, ngroup); break;
+    case 8: r = launch_gc_gp_rg<8>(c, gsize, ngroup); break;
+    case 10: r = launch_gc_gp_rg<10>(c, gsize, ngroup); break;
+    case 12: r = launch_gc_gp_rg<12>(c, gsize, ngroup); break;
+    case 14: r = launch_gc_gp_rg<14>(c, gsize, ngroup); break;
+    default: r = launch_gc_gp_rg<16>(c, gsize, ngroup); break;
+    }
+    if (r) return r;
+    gp_timer_end(c, c->t_gck);
+    const size_t n = (size_t)c->Npad * M;
+    k_gc_reduce_gp<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(c->Npad, M, ngroup, c->gc_part,
+                                                                      c->gc_gun, c->Uw);
+    GP_HIP(hipGetLastError());
+    k_gc_u0<<<(unsigned)((c->Npad + 255) / 256), 256, 0, c->stream>>>(c->Npad, M, c->Zw, c->gc_y, c->Uw);
+    GP_HIP(hipGetLastError());
+    return 0;
+}
+
 int gpk_gradcontract_mixed(gp_ctx *c) {
     const int T16 = (c->P + 15) / 16, P16 = 16 * T16;
     if (T16 > MAXT16 || c->Npad % 128 != 0) return gpk_gradcontract(c, c->Zw, c->Uw);
@@ -875,6 +1216,7 @@ int gpk_gradcontract_mixed(gp_ctx *c) {
     k_gc_plan<<<1, 256, 0, c->stream>>>(J, gsize, ngroup, c->wun, c->gc_sexp, c->gc_perm,
                                         c->gc_aexp, c->gc_bexp, c->gc_gun, c->gc_kappa);
     GP_HIP(hipGetLastError());
+    if (gc_peel_ok(c)) return gradcontract_gp(c, gsize, ngroup);
     dim3 gs((unsigned)(((size_t)P16 * KP + 255) / 256), J);
     k_ssplit<<<gs, 256, 0, c->stream>>>(c->P, P16, c->Sinv, c->gc_bexp, c->gc_zexp,
                                         (_Float16 *)c->Sh, (_Float16 *)c->Sl);

@@ -13,7 +13,7 @@
 
 #define GP_HIST 10            // L-BFGS history length (ggplvm.cu:872-880)
 #define GP_LKHD_SLOTS 50000   // objective history length (ggplvm.cu:1025)
-#define GP_GRAPH_KEY 7        // settings a recorded ELBO + gradient graph depends on
+#define GP_GRAPH_KEY 10       // settings a recorded ELBO + gradient graph depends on
 
 struct gp_timer {
     hipEvent_t a = nullptr, b = nullptr;
@@ -128,6 +128,12 @@ struct gp_ctx {
     int gc_groups = 16;        // gene groups of the mixed contraction
     int gc_mixed = 1;          // mixed path: contraction on FP16 MFMA (0: FP64)
     int gc_form = 2;           // mixed contraction: 0 = 16-cell waves, 1 = 128-cell row blocks, 2 = factored row blocks
+    int gc_peel = 1;           // factored form on the GP block only when the intercept is the sole covariate column
+    int gc_wave_rows = 8;      // peeled form: row tiles per wave (8 or 4; column tiles 16 / that)
+    int gc_prefetch = 3;       // peeled form: k-steps of B fragments loaded ahead (0, 1, 3 or 7)
+    float *gc_t0 = nullptr;    // J x M  the intercept's k term of each gene (peeled form)
+    float *gc_s0 = nullptr;    // J x (M+1)  [Sw_j[0, 1:] | Sw_j[0, 0]] (intercept output column)
+    float *gc_y = nullptr;     // Npad x (M+1)  w32 . gc_s0
     int irls_mixed = 1;        // mixed path: IRLS steps on the whitened FP16 hi/lo Gram (0: FP64)
     int autotune = 0;          // gp_fit runs gp_autotune before its first quasi-likelihood phase
     // 1 while R, bvec, Oinvt and pwork = Psi^-1 bvec hold the likelihood
@@ -143,8 +149,9 @@ struct gp_ctx {
     // recorded ELBO + gradient unit
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
-    // precision, slot, gram_splits, gc_groups, gram_waves, gc_form, gc_mixed
-    int graph_key[GP_GRAPH_KEY] = {-1, -1, -1, -1, -1, -1, -1};
+    // precision, slot, gram_splits, gc_groups, gram_waves, gc_form, gc_mixed, gc_peel,
+    // gc_wave_rows, gc_prefetch
+    int graph_key[GP_GRAPH_KEY] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
     // objective and elapsed seconds after every accepted step of the last gp_fit
     std::vector<double> fit_obj, fit_sec;
     std::vector<int> fit_kind;           // 0 IRLS step, 1 quasi-likelihood start, 2 L-BFGS step

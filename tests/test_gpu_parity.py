@@ -386,6 +386,7 @@ def test_contraction_row_block_form_matches_16_cell_form(shape):
     g.irls_step(0)
     g.ql_start()
     g.set_precision(lib.PREC_MIXED)
+    g.set_option("gc_peel", 0)              # form 2 on the whole design, as forms 0, 1, 3
     out = {}
     for form in (0, 1, 2, 3):
         g.set_option("gc_form", form)
@@ -400,6 +401,62 @@ def test_contraction_row_block_form_matches_16_cell_form(shape):
     assert ru < 1e-6 and rg < 1e-5
     assert np.array_equal(out[3][2], out[2][2])
     assert np.array_equal(out[3][0], out[2][0])
+    g.close()
+
+
+@pytest.mark.parametrize("shape", [(2000, 24, 5, 32), (3000, 24, 10, 256), (3000, 16, 4, 64),
+                                   (2500, 20, 6, 160)])
+def test_contraction_intercept_peeled_form(shape):
+    """The intercept-peeled contraction (MFMAs over the GP block only, the
+    intercept's k term in each gene's update, the intercept output column
+    from one FP32 GEMM and a row dot) against the factored form on the whole
+    design and against the FP64 contraction of the same inverses: Uw as close
+    to FP64 as the whole-design form is (measured 3e-7 at M = 256), the
+    gradient within 1e-5 of the factored form's and within 1e-4 of the FP64
+    one's; M = 32, 256, 64 and 160 (2, 16, 4 and 10 GP column tiles).  Its
+    wave tilings and prefetch depths feed every accumulator the same MFMA
+    sequence: same bits."""
+    from gaspacho2_amd import lib, synth
+    N, G, Q, M = shape
+    ds = synth.make_dataset(N, G, Q, M, seed=8)
+    T0 = lib.init_ta_with_per(ds.M, ds.Q)
+    g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0, T0,
+                      ard_only=True, precision=lib.PREC_FP64)
+    g.init_fit()
+    g.irls_step(0)
+    g.ql_start()
+    g.set_precision(lib.PREC_MIXED)
+    g.elbo(1)
+    out = {}
+    for name, opts in (("peel", {"gc_peel": 1, "gc_mixed": 1}), ("full", {"gc_peel": 0, "gc_mixed": 1}),
+                       ("f64", {"gc_mixed": 0})):
+        for k, v in opts.items():
+            g.set_option(k, v)
+        g.grad()
+        out[name] = (g.get("Uw").copy(), g.get("grad").copy())
+    ru, rg = rel(out["peel"][0], out["full"][0]), rel(out["peel"][1], out["full"][1])
+    ru64, rg64 = rel(out["peel"][0], out["f64"][0]), rel(out["peel"][1], out["f64"][1])
+    rf64 = rel(out["full"][0], out["f64"][0])
+    Np = g.N
+    u64 = out["f64"][0].reshape(-1, Np)
+    col0 = float(np.max(np.abs(out["peel"][0].reshape(-1, Np)[0] - u64[0])) / np.max(np.abs(u64)))
+    colg = float(np.max(np.abs(out["peel"][0].reshape(-1, Np)[1:] - u64[1:])) / np.max(np.abs(u64)))
+    print(f"M={M} peeled vs full: Uw {ru:.2e}, grad {rg:.2e}; vs FP64: Uw {ru64:.2e} (intercept column "
+          f"{col0:.2e}, GP columns {colg:.2e}), grad {rg64:.2e}; full vs FP64: Uw {rf64:.2e}")
+    # about as close to FP64 as the factored form on the whole design (both
+    # are at the FP32-accumulation level: 2e-7 to 2e-6 here), and within
+    # 5e-6 of FP64 everywhere
+    assert ru64 < 5e-6 and ru64 <= 3.0 * rf64 + 1e-7
+    assert ru < 5e-6
+    assert rg < 1e-5 and rg64 < 1e-4
+    # the wave tilings and prefetch depths of the peeled kernel give the same bits
+    g.set_option("gc_mixed", 1)
+    g.set_option("gc_peel", 1)
+    for opts in ({"gc_prefetch": 0}, {"gc_prefetch": 1}, {"gc_wave_rows": 4, "gc_prefetch": 3}):
+        for k, v in opts.items():
+            g.set_option(k, v)
+        g.grad()
+        assert np.array_equal(g.get("Uw"), out["peel"][0]), opts
     g.close()
 
 
@@ -469,13 +526,14 @@ def test_gradient_reuses_likelihood_solve(precision):
     g.close()
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_graph_replay_matches_eager(precision):
+@pytest.mark.parametrize("precision,M,nnum", [(0, 48, 1), (1, 48, 1), (1, 64, 0)])
+def test_graph_replay_matches_eager(precision, M, nnum):
     """gp_eval_graph (the ELBO + gradient unit recorded as a HIP graph) gives
     the eager results, replay after replay, with eager work and parameter
-    changes between the replays."""
+    changes between the replays; M = 64 with no covariates takes the
+    intercept-peeled contraction and its FP32 GEMM inside the graph."""
     from gaspacho2_amd import lib, synth
-    ds = synth.make_dataset(1500, 30, 4, 48, seed=6, n_numeric=1)
+    ds = synth.make_dataset(1500, 30, 4, M, seed=6, n_numeric=nnum)
     T0 = lib.init_ta_with_per(ds.M, ds.Q)
     g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0,
                       T0, precision=lib.PREC_FP64)
