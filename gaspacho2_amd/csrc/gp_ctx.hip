@@ -61,7 +61,7 @@ static void free_all(gp_ctx *c) {
                     c->wun, c->zun, c->Linv, c->Zw, c->DeltaW, c->DeltaWChol, c->Uw, c->tBw,
                     c->Vtmp, c->Sh, c->Sl, c->gc_perm, c->gc_aexp, c->gc_sexp, c->gc_bexp,
                     c->gc_gun, c->gc_part, c->gc_zexp, c->skw, c->gc_kappa, c->gc_t0, c->gc_s0,
-                    c->gc_y};
+                    c->gc_y, c->gram_row0};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     gp_timer *ts[] = {&c->t_gram, &c->t_chol, &c->t_gc, &c->t_kuf, &c->t_elbo, &c->t_grad,
@@ -263,6 +263,8 @@ extern "C" int gp_set_option(gp_ctx *c, const char *name, int value) {
         c->gc_mixed = value ? 1 : 0;
     } else if (n == "gc_peel") {
         c->gc_peel = value ? 1 : 0;
+    } else if (n == "gram_peel") {
+        c->gram_peel = value ? 1 : 0;
     } else if (n == "gc_wave_rows") {
         if (value != 8 && value != 4) { gp_set_error("gc_wave_rows must be 8 or 4"); return 1; }
         c->gc_wave_rows = value;
@@ -287,6 +289,7 @@ extern "C" int gp_get_option(gp_ctx *c, const char *name, int *value) {
     else if (n == "gc_groups") *value = c->gc_groups;
     else if (n == "gc_mixed") *value = c->gc_mixed;
     else if (n == "gc_peel") *value = c->gc_peel;
+    else if (n == "gram_peel") *value = c->gram_peel;
     else if (n == "gc_wave_rows") *value = c->gc_wave_rows;
     else if (n == "gc_prefetch") *value = c->gc_prefetch;
     else if (n == "gram_waves") *value = c->gram_waves;
@@ -698,7 +701,7 @@ extern "C" int gp_eval_graph(gp_ctx *c, int slot, double *obj) {
     if (slot < 0 || slot >= GP_LKHD_SLOTS) { gp_set_error("slot out of range"); return 1; }
     const int key[GP_GRAPH_KEY] = {c->precision, slot, c->gram_splits, c->gc_groups, c->gram_waves,
                                    c->gc_form, c->gc_mixed, c->gc_peel, c->gc_wave_rows,
-                                   c->gc_prefetch};
+                                   c->gc_prefetch, c->gram_peel};
     if (!c->graph_exec || !std::equal(key, key + GP_GRAPH_KEY, c->graph_key)) {
         drop_graph(c);
         double tmp = 0.0;

@@ -849,12 +849,15 @@ __global__ void __launch_bounds__(GW4 * 64) k_gram_sym4(int Npad, int seg, int n
 }
 
 constexpr int RSG = 4;       // genes per block of the symmetric reduction
+constexpr int GRK_SEG = 64;  // cell segments of the intercept row's FP32 products
 
 // Psi_j from the symmetric-form partials: one 256-thread block per (tile, RSG genes);
 // sums the segments in FP64, rescales by 2^e_j 2^-s_r 2^-s_c, adds the prior
 // and writes the tile and its mirror (upper-triangle values only, so Psi is
 // exactly symmetric).
-__global__ void __launch_bounds__(256) k_gram_reduce_sym(int P, int T16, int nseg, int J,
+// off: the tiles cover columns off .. off + 16 T16 - 1 of Psi (1 when the
+// intercept row and column come from k_gram_row0)
+__global__ void __launch_bounds__(256) k_gram_reduce_sym(int P, int T16, int nseg, int J, int off,
         const float *__restrict__ part, const double *__restrict__ wun,
         const double *__restrict__ zun, const double *__restrict__ Delta, double *__restrict__ Psi) {
     __shared__ double tile[16][17];
@@ -866,7 +869,7 @@ __global__ void __launch_bounds__(256) k_gram_reduce_sym(int P, int T16, int nse
     // thread tid = 4*lane + r  <->  tile element (4(lane/16) + r, lane%16)
     const int lane = tid >> 2, r = tid & 3;
     const int rr = 4 * (lane >> 4) + r, cc = lane & 15;
-    const int row = 16 * ta + rr, col = 16 * tb + cc;
+    const int row = off + 16 * ta + rr, col = off + 16 * tb + cc;
     const int i1 = tid & 15, i2 = tid >> 4;
     const int j0 = blockIdx.y * RSG, j1 = min(J, j0 + RSG);
     for (int j = j0; j < j1; j++) {
@@ -889,14 +892,32 @@ __global__ void __launch_bounds__(256) k_gram_reduce_sym(int P, int T16, int nse
         __syncthreads();
         double *Pj = Psi + (size_t)j * P * P;
         {   // Psi[16ta + i1, 16tb + i2]   (coalesced over i1)
-            const int R = 16 * ta + i1, C = 16 * tb + i2;
+            const int R = off + 16 * ta + i1, C = off + 16 * tb + i2;
             if (R < P && C < P && (ta != tb || i1 <= i2)) Pj[R + (size_t)C * P] = tile[i1][i2];
         }
         {   // mirror: Psi[16tb + i1, 16ta + i2] = tile[i2][i1]
-            const int R = 16 * tb + i1, C = 16 * ta + i2;
+            const int R = off + 16 * tb + i1, C = off + 16 * ta + i2;
             if (R < P && C < P && (ta != tb || i2 <= i1)) Pj[R + (size_t)C * P] = tile[i2][i1];
         }
     }
+}
+
+// The intercept row and column of Psi_j when the symmetric Gram covers only
+// the GP block: Psi_j[0, c] = sum_i o_ij zw_ic + Delta[0, c], from the
+// per-segment FP32 products part[s][c + j P] = sum_{i in s} z32_ic w32_ij
+// (z32 = zw 2^s_c) summed in FP64 and unscaled by zun[c] = 2^-s_c.
+__global__ void k_gram_row0(int P, int J, int nsk, const float *__restrict__ part,
+                            const double *__restrict__ zun, const double *__restrict__ Delta,
+                            double *__restrict__ Psi) {
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= (size_t)P * J) return;
+    const int c = (int)(idx % P), j = (int)(idx / P);
+    double s = 0.0;
+    for (int g = 0; g < nsk; g++) s += (double)part[(size_t)g * P * J + idx];
+    const double v = s * zun[c] + Delta[(size_t)c * P];
+    double *Pj = Psi + (size_t)j * P * P;
+    Pj[(size_t)c * P] = v;
+    Pj[c] = v;
 }
 
 }  // namespace
@@ -945,7 +966,7 @@ int gpk_wprep(gp_ctx *c, const double *W) {
 }
 
 template <int T16>
-static int launch_gram_sym(gp_ctx *c, int seg, int nseg, int batch) {
+static int launch_gram_sym(gp_ctx *c, int seg, int nseg, int batch, const float *zsrc) {
     constexpr int P16 = 16 * T16;
     static_assert(plan_fits(T16), "block plan does not fit the waves");
     if (c->gram_waves == 4) {
@@ -954,14 +975,14 @@ static int launch_gram_sym(gp_ctx *c, int seg, int nseg, int batch) {
             GP_HIP(hipFuncSetAttribute((const void *)k_gram_sym4<T16>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         k_gram_sym4<T16><<<(unsigned)(batch * nseg), GW4 * 64, shm, c->stream>>>(
-            c->Npad, seg, nseg, batch, c->z32, c->w32, c->wsc, c->gram_part);
+            c->Npad, seg, nseg, batch, zsrc, c->w32, c->wsc, c->gram_part);
     } else {
         const size_t shm = (size_t)2 * 2 * P16 * GLDX * sizeof(_Float16);
         if (shm > 65536 && !c->capturing)
             GP_HIP(hipFuncSetAttribute((const void *)k_gram_sym<T16>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         k_gram_sym<T16><<<(unsigned)(batch * nseg), GW * 64, shm, c->stream>>>(
-            c->Npad, seg, nseg, batch, c->z32, c->w32, c->wsc, c->gram_part);
+            c->Npad, seg, nseg, batch, zsrc, c->w32, c->wsc, c->gram_part);
     }
     GP_HIP(hipGetLastError());
     return 0;
@@ -984,32 +1005,87 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
         k_wprep<<<batch, 256, 0, c->stream>>>(c->N, c->Npad, W, c->Omega, c->w32, c->wsc, c->wun);
         GP_HIP(hipGetLastError());
     }
+    // intercept-peeled form: the symmetric Gram over the GP block only (M / 16
+    // tiles: no 16-column tile for the single intercept column), the
+    // intercept row and column from split-K FP32 products Zw^T (W / Omega)
+    const int TG = c->M / 16;
+    const bool peel = c->gram_peel && c->g0 == 1 && c->M % 16 == 0 && TG <= MAX_SYM_T16 &&
+                      TG >= 1 && c->Npad % GRK_SEG == 0;
+    if (peel) {
+        int r = 0;
+        gp_timer_begin(c, c->t_gramk);
+        const float *zg = c->z32 + c->Npad;
+        switch (TG) {
+        case 1: r = launch_gram_sym<1>(c, seg, nseg, batch, zg); break;
+        case 2: r = launch_gram_sym<2>(c, seg, nseg, batch, zg); break;
+        case 3: r = launch_gram_sym<3>(c, seg, nseg, batch, zg); break;
+        case 4: r = launch_gram_sym<4>(c, seg, nseg, batch, zg); break;
+        case 5: r = launch_gram_sym<5>(c, seg, nseg, batch, zg); break;
+        case 6: r = launch_gram_sym<6>(c, seg, nseg, batch, zg); break;
+        case 7: r = launch_gram_sym<7>(c, seg, nseg, batch, zg); break;
+        case 8: r = launch_gram_sym<8>(c, seg, nseg, batch, zg); break;
+        case 9: r = launch_gram_sym<9>(c, seg, nseg, batch, zg); break;
+        case 10: r = launch_gram_sym<10>(c, seg, nseg, batch, zg); break;
+        case 11: r = launch_gram_sym<11>(c, seg, nseg, batch, zg); break;
+        case 12: r = launch_gram_sym<12>(c, seg, nseg, batch, zg); break;
+        case 13: r = launch_gram_sym<13>(c, seg, nseg, batch, zg); break;
+        case 14: r = launch_gram_sym<14>(c, seg, nseg, batch, zg); break;
+        case 15: r = launch_gram_sym<15>(c, seg, nseg, batch, zg); break;
+        case 16: r = launch_gram_sym<16>(c, seg, nseg, batch, zg); break;
+        default: r = launch_gram_sym<17>(c, seg, nseg, batch, zg); break;
+        }
+        if (r) return r;
+        gp_timer_end(c, c->t_gramk);
+        dim3 gr((unsigned)(TG * (TG + 1) / 2), (unsigned)((batch + RSG - 1) / RSG));
+        k_gram_reduce_sym<<<gr, 256, 0, c->stream>>>(c->P, TG, nseg, batch, 1, c->gram_part, c->wun,
+                                                      c->zun, Delta, Psi);
+        GP_HIP(hipGetLastError());
+        // intercept row: GRK_SEG cell segments of z32^T w32 (FP32), summed in FP64
+        const size_t rp = (size_t)GRK_SEG * c->P * batch;
+        if (c->gram_row0_len < rp) {
+            if (c->gram_row0) (void)hipFree(c->gram_row0);
+            c->gram_row0 = nullptr;
+            GP_HIP(hipMalloc((void **)&c->gram_row0, rp * sizeof(float)));
+            c->gram_row0_len = rp;
+        }
+        const int kseg = c->Npad / GRK_SEG;
+        const float one = 1.0f, zero = 0.0f;
+        GP_BLAS(rocblas_sgemm_strided_batched(c->blas, rocblas_operation_transpose, rocblas_operation_none,
+                                              c->P, batch, kseg, &one, c->z32, c->Npad, kseg, c->w32,
+                                              c->Npad, kseg, &zero, c->gram_row0, c->P,
+                                              (rocblas_stride)c->P * batch, GRK_SEG));
+        const size_t n0 = (size_t)c->P * batch;
+        k_gram_row0<<<(unsigned)((n0 + 255) / 256), 256, 0, c->stream>>>(c->P, batch, GRK_SEG,
+                                                                        c->gram_row0, c->zun, Delta, Psi);
+        GP_HIP(hipGetLastError());
+        return 0;
+    }
     if (T16 <= MAX_SYM_T16) {
         int r = 0;
         gp_timer_begin(c, c->t_gramk);
         switch (T16) {
-        case 1: r = launch_gram_sym<1>(c, seg, nseg, batch); break;
-        case 2: r = launch_gram_sym<2>(c, seg, nseg, batch); break;
-        case 3: r = launch_gram_sym<3>(c, seg, nseg, batch); break;
-        case 4: r = launch_gram_sym<4>(c, seg, nseg, batch); break;
-        case 5: r = launch_gram_sym<5>(c, seg, nseg, batch); break;
-        case 6: r = launch_gram_sym<6>(c, seg, nseg, batch); break;
-        case 7: r = launch_gram_sym<7>(c, seg, nseg, batch); break;
-        case 8: r = launch_gram_sym<8>(c, seg, nseg, batch); break;
-        case 9: r = launch_gram_sym<9>(c, seg, nseg, batch); break;
-        case 10: r = launch_gram_sym<10>(c, seg, nseg, batch); break;
-        case 11: r = launch_gram_sym<11>(c, seg, nseg, batch); break;
-        case 12: r = launch_gram_sym<12>(c, seg, nseg, batch); break;
-        case 13: r = launch_gram_sym<13>(c, seg, nseg, batch); break;
-        case 14: r = launch_gram_sym<14>(c, seg, nseg, batch); break;
-        case 15: r = launch_gram_sym<15>(c, seg, nseg, batch); break;
-        case 16: r = launch_gram_sym<16>(c, seg, nseg, batch); break;
-        default: r = launch_gram_sym<17>(c, seg, nseg, batch); break;
+        case 1: r = launch_gram_sym<1>(c, seg, nseg, batch, c->z32); break;
+        case 2: r = launch_gram_sym<2>(c, seg, nseg, batch, c->z32); break;
+        case 3: r = launch_gram_sym<3>(c, seg, nseg, batch, c->z32); break;
+        case 4: r = launch_gram_sym<4>(c, seg, nseg, batch, c->z32); break;
+        case 5: r = launch_gram_sym<5>(c, seg, nseg, batch, c->z32); break;
+        case 6: r = launch_gram_sym<6>(c, seg, nseg, batch, c->z32); break;
+        case 7: r = launch_gram_sym<7>(c, seg, nseg, batch, c->z32); break;
+        case 8: r = launch_gram_sym<8>(c, seg, nseg, batch, c->z32); break;
+        case 9: r = launch_gram_sym<9>(c, seg, nseg, batch, c->z32); break;
+        case 10: r = launch_gram_sym<10>(c, seg, nseg, batch, c->z32); break;
+        case 11: r = launch_gram_sym<11>(c, seg, nseg, batch, c->z32); break;
+        case 12: r = launch_gram_sym<12>(c, seg, nseg, batch, c->z32); break;
+        case 13: r = launch_gram_sym<13>(c, seg, nseg, batch, c->z32); break;
+        case 14: r = launch_gram_sym<14>(c, seg, nseg, batch, c->z32); break;
+        case 15: r = launch_gram_sym<15>(c, seg, nseg, batch, c->z32); break;
+        case 16: r = launch_gram_sym<16>(c, seg, nseg, batch, c->z32); break;
+        default: r = launch_gram_sym<17>(c, seg, nseg, batch, c->z32); break;
         }
         if (r) return r;
         gp_timer_end(c, c->t_gramk);
         dim3 gr((unsigned)(T16 * (T16 + 1) / 2), (unsigned)((batch + RSG - 1) / RSG));
-        k_gram_reduce_sym<<<gr, 256, 0, c->stream>>>(c->P, T16, nseg, batch, c->gram_part, c->wun,
+        k_gram_reduce_sym<<<gr, 256, 0, c->stream>>>(c->P, T16, nseg, batch, 0, c->gram_part, c->wun,
                                                       c->zun, Delta, Psi);
         GP_HIP(hipGetLastError());
         return 0;

@@ -13,7 +13,7 @@
 
 #define GP_HIST 10            // L-BFGS history length (ggplvm.cu:872-880)
 #define GP_LKHD_SLOTS 50000   // objective history length (ggplvm.cu:1025)
-#define GP_GRAPH_KEY 10       // settings a recorded ELBO + gradient graph depends on
+#define GP_GRAPH_KEY 11       // settings a recorded ELBO + gradient graph depends on
 
 struct gp_timer {
     hipEvent_t a = nullptr, b = nullptr;
@@ -115,6 +115,9 @@ struct gp_ctx {
     size_t gram_part_bytes = 0;
     int gram_splits = 8;       // cell segments of the mixed Gram
     int gram_waves = 8;        // waves per workgroup of the symmetric mixed Gram (8 or 4)
+    int gram_peel = 1;         // symmetric Gram on the GP block only when the intercept is the sole covariate column
+    float *gram_row0 = nullptr;  // per-segment FP32 intercept-row products (peeled Gram)
+    size_t gram_row0_len = 0;
     int zsplit_valid = 0;
     // mixed gradient contraction: per-gene FP16 hi/lo of Sw_j, gene groups
     void *Sh = nullptr, *Sl = nullptr;  // J x P16 x 288 halves, [col][k]
@@ -150,8 +153,8 @@ struct gp_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     // precision, slot, gram_splits, gc_groups, gram_waves, gc_form, gc_mixed, gc_peel,
-    // gc_wave_rows, gc_prefetch
-    int graph_key[GP_GRAPH_KEY] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+    // gc_wave_rows, gc_prefetch, gram_peel
+    int graph_key[GP_GRAPH_KEY] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
     // objective and elapsed seconds after every accepted step of the last gp_fit
     std::vector<double> fit_obj, fit_sec;
     std::vector<int> fit_kind;           // 0 IRLS step, 1 quasi-likelihood start, 2 L-BFGS step

@@ -460,6 +460,49 @@ def test_contraction_intercept_peeled_form(shape):
     g.close()
 
 
+@pytest.mark.parametrize("shape", [(2000, 24, 5, 32), (3000, 24, 10, 256), (3000, 16, 4, 64),
+                                   (2500, 20, 6, 160)])
+def test_gram_intercept_peeled_form(shape):
+    """The intercept-peeled mixed Gram (symmetric FP16 hi/lo Gram over the GP
+    block only, the intercept row and column from split-K FP32 products
+    Zw^T (W / Omega) summed in FP64) against the Gram over the whole design
+    and against the FP64 Gram of the same whitened basis: entries within
+    5e-7 of each gene's largest (as the whole-design form), the ELBO within
+    1e-7 of FP64; the eight- and four-wave forms of the peeled Gram give the
+    same bits."""
+    from gaspacho2_amd import lib, synth
+    N, G, Q, M = shape
+    ds = synth.make_dataset(N, G, Q, M, seed=11)
+    T0 = lib.init_ta_with_per(ds.M, ds.Q)
+    g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0, T0,
+                      ard_only=True, precision=lib.PREC_FP64)
+    g.init_fit()
+    g.irls_step(0)
+    g.ql_start()
+    J, P = ds.J, ds.P
+    iu = np.triu_indices(P)
+    out = {}
+    for name, prec, opts in (("f64", lib.PREC_FP64_WHITENED, {}),
+                             ("full", lib.PREC_MIXED, {"gram_peel": 0}),
+                             ("peel", lib.PREC_MIXED, {"gram_peel": 1}),
+                             ("peel4", lib.PREC_MIXED, {"gram_peel": 1, "gram_waves": 4})):
+        g.set_precision(prec)
+        for k, v in opts.items():
+            g.set_option(k, v)
+        e = g.elbo(1)
+        out[name] = (e, g.get("Psi")[:J * P * P].reshape(J, P, P)[:, iu[1], iu[0]].copy())
+    ref = out["f64"][1]
+    scale = np.max(np.abs(ref), axis=1)
+    errs = {k: float(np.max(np.max(np.abs(out[k][1] - ref), axis=1) / scale)) for k in ("full", "peel")}
+    e64 = out["f64"][0]
+    print(f"M={M} Gram rel err vs FP64: {errs}; ELBO rel err peeled {abs(out['peel'][0] - e64) / abs(e64):.2e}, "
+          f"whole {abs(out['full'][0] - e64) / abs(e64):.2e}")
+    assert errs["peel"] < 5e-7 and errs["full"] < 5e-7
+    assert abs(out["peel"][0] - e64) <= 1e-7 * abs(e64)
+    assert np.array_equal(out["peel4"][1], out["peel"][1]) and out["peel4"][0] == out["peel"][0]
+    g.close()
+
+
 def test_autotune_picks_valid_settings_and_keeps_results():
     """gp_autotune picks gram_splits / gc_groups among its candidates and the
     mixed ELBO and gradient after tuning stay within the mixed-path error of
