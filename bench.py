@@ -358,6 +358,108 @@ def _make_ds(args_tuple):
     return seed, synth.make_dataset(N, G, Q, M, seed=seed, model_seed=model_seed)
 
 
+def hip_mem_info():
+    """(free, total) device bytes from the HIP runtime the library uses."""
+    import ctypes as C
+    rt = C.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+    f, t = C.c_size_t(), C.c_size_t()
+    rt.hipMemGetInfo(C.byref(f), C.byref(t))
+    return f.value, t.value
+
+
+def main_score(args):
+    """Genetic-mapping score tests along the latent space: N cells from D
+    donors (uniform assignment), G genes, Q latent dims, M inducing points;
+    each gene has its own S cis SNPs with donor dosages 0/1/2 drawn under
+    Hardy-Weinberg equilibrium (MAF ~ U(0.05, 0.5)); every gene x cis-SNP
+    pair is tested with the interaction column dosage x X[:, 0] (the first
+    latent coordinate), centred, as ggplvm_de.cu:989-994 builds it.  The
+    fit state is the mixed-precision IRLS phase (K steps) at the latent the
+    counts were drawn from; the statistics and p-values of a gene sample are
+    checked against an FP64 context on those genes alone (genes are
+    independent in the IRLS phase)."""
+    from gaspacho2_amd import lib, synth
+    t0 = time.perf_counter()
+    N, G, Q, M = args.N, args.G, args.Q, args.M
+    last = [time.perf_counter()]
+
+    def progress(done, total):
+        if time.perf_counter() - last[0] > 20 or done == total:
+            log(f"data: {done} of {total} genes ({time.perf_counter() - t0:.0f} s)")
+            last[0] = time.perf_counter()
+
+    ds = synth.make_dataset(N, G, Q, M, seed=0, gen_M=64, pca=False, block=128, progress=progress)
+    if ds.L >= 2 ** 31:
+        raise SystemExit(f"{ds.L} non-zero counts do not fit the 32-bit SparseMat offsets")
+    log(f"data: N={N} G={G} L={ds.L} zeros={ds.zero_fraction():.3f} ({time.perf_counter() - t0:.1f} s)")
+    rng = np.random.default_rng(11)
+    D, S = args.donors, args.snps_per_gene
+    donor = rng.integers(0, D, N).astype(np.int32)
+    maf = rng.uniform(0.05, 0.5, G * S)
+    dos = rng.binomial(2, np.broadcast_to(maf, (D, G * S))).astype(np.uint8)   # D x (G S)
+    gene = np.repeat(np.arange(G), S).astype(np.int32)
+    snp = np.arange(G * S).astype(np.int32)
+    factor = np.ascontiguousarray(np.asarray(ds.X0)[:, 0])
+    T0 = lib.init_ta_with_per(M, Q)
+    f0, tot = hip_mem_info()
+    t1 = time.perf_counter()
+    g = lib.GPContext(N, G, M, Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0, T0,
+                      ard_only=True, precision=lib.PREC_MIXED)
+    g.set_option("gram_splits", args.gram_splits)     # FP32 Gram partials: segments x genes x blocks
+    g.init_fit()
+    irls = []
+    for jtr in range(args.irls):
+        t2 = time.perf_counter()
+        irls.append(g.irls_step(jtr))
+        log(f"mixed IRLS step {jtr}: {time.perf_counter() - t2:.2f} s, objective {irls[-1]:.10e}")
+    f1, _ = hip_mem_info()
+    log(f"context + {args.irls} IRLS steps: {time.perf_counter() - t1:.1f} s; device memory in use "
+        f"{(f0 - f1) / 2**30:.1f} GiB of {tot / 2**30:.0f}")
+    g.score_tests_donor(donor, dos, factor, gene[:64], snp[:64])      # warm-up
+    t2 = time.perf_counter()
+    stat, pval = g.score_tests_donor(donor, dos, factor, gene, snp)
+    dt = time.perf_counter() - t2
+    log(f"{len(gene)} interaction score tests: {dt:.2f} s ({len(gene) / dt:.0f} tests/s); "
+        f"p < 0.05 in {np.mean(pval < 0.05):.3f} of them; finite {np.all(np.isfinite(stat))}")
+    g.close()
+    # FP64 on a gene sample (same cells, latent, inducing points; its own IRLS)
+    pick = np.sort(rng.choice(G, args.check_genes, replace=False))
+    sel = np.concatenate([np.arange(ds.Yp[j], ds.Yp[j + 1]) for j in pick])
+    Yp = np.zeros(len(pick) + 1, np.int64)
+    Yp[1:] = np.cumsum(np.diff(ds.Yp)[pick])
+    h = lib.GPContext(N, len(pick), M, Q, ds.nh, Yp.astype(np.int32), ds.Yi[sel], ds.Yx[sel], ds.Zcov,
+                      ds.ls, ds.X0, T0, ard_only=True, precision=lib.PREC_FP64)
+    h.init_fit()
+    for jtr in range(args.irls):
+        h.irls_step(jtr)
+    gl = np.repeat(np.arange(len(pick)), S).astype(np.int32)
+    sl = (pick[:, None] * S + np.arange(S)[None, :]).ravel().astype(np.int32)
+    s64, p64 = h.score_tests_donor(donor, dos, factor, gl, sl)
+    h.close()
+    sm = stat[sl]
+    pm = pval[sl]
+    ea = float(np.max(np.abs(sm - s64)))
+    es = float(np.max(np.abs(sm - s64) / np.maximum(np.abs(s64), 1e-300)))
+    ep = float(np.max(np.abs(np.log(pm) - np.log(p64))))
+    log(f"FP64 check on {len(pick)} genes ({len(sl)} tests): statistic max abs err {ea:.2e} "
+        f"(max rel {es:.2e}), p-value max |log ratio| {ep:.2e}")
+    P = ds.P
+    flops = 2.0 * len(gene) * N * P                      # b = Z^T D h per test
+    out = {"metric": "gene x cis-SNP interaction score tests/s (genotype x latent coordinate)",
+           "value": len(gene) / dt, "unit": "tests/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+           "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+           "dtype": "f64 (statistics) on mixed-precision IRLS factors",
+           "data": "synthetic (SE-ARD GP counts, HWE dosages per donor, seed 0/11)",
+           "config": {"workload": f"score tests N={N} cells from {D} donors, G={G} genes x {S} cis SNPs, "
+                                  f"Q={Q}, M={M}, P={P}, SE-ARD, {args.irls} mixed IRLS steps",
+                      "N": N, "G": G, "Q": Q, "M": M, "donors": D, "tests": len(gene)},
+           "irls_objectives": irls, "p_below_0.05": float(np.mean(pval < 0.05)),
+           "fp64_check": {"genes": int(len(pick)), "tests": int(len(sl)), "stat_max_abs_err": ea,
+                          "stat_max_rel_err": es, "pval_max_abs_log_ratio": ep},
+           "device_GiB_in_use": (f0 - f1) / 2 ** 30, "gflops_b": flops / dt / 1e9}
+    print(json.dumps(out), flush=True)
+
+
 def main_warmstart(args):
     """Amortization-encoder warm start (gaspacho2_amd.warmstart).  Fits
     (mixed precision, PCA initial latent) on seeds 0..S-1 that share one set
@@ -453,11 +555,16 @@ def main():
     ap.add_argument("--enc-epochs", type=int, default=30, help="warmstart workload: encoder epochs")
     ap.add_argument("--enc-target", default="residual,aligned",
                     help="warmstart workload: encoder targets, comma-separated (first = headline)")
-    ap.add_argument("--workload", choices=["elbo", "batched", "fit", "fitcheck", "warmstart"],
+    ap.add_argument("--workload", choices=["elbo", "batched", "fit", "fitcheck", "warmstart", "score"],
                     default="elbo",
                     help="elbo: the ELBO+gradient unit (default); batched: uneven per-gene GLM fits; "
                          "fitcheck: mixed-vs-FP64 ELBO and gradient errors at every L-BFGS iterate")
     ap.add_argument("--nmin", type=int, default=500)
+    ap.add_argument("--donors", type=int, default=200, help="score workload: donors")
+    ap.add_argument("--snps-per-gene", type=int, default=10, help="score workload: cis SNPs per gene")
+    ap.add_argument("--irls", type=int, default=6, help="score workload: IRLS steps before the tests")
+    ap.add_argument("--gram-splits", type=int, default=2, help="score workload: mixed Gram cell segments")
+    ap.add_argument("--check-genes", type=int, default=16, help="score workload: genes checked in FP64")
     ap.add_argument("--nmax", type=int, default=50000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="time a plain device copy of MIB MiB (HBM counter calibration) and exit")
@@ -471,6 +578,14 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     batched = args.workload == "batched"
+    if args.workload == "score":
+        args.N = args.N or 1000000
+        args.G = args.G or 5000
+        args.M = args.M or 512
+        if args.Q == 10:
+            args.Q = 20
+        main_score(args)
+        return
     args.N = args.N or (200000 if batched else 100000)
     args.G = args.G or (20000 if batched else 2000)
     args.M = args.M or (128 if batched else 256)

@@ -48,6 +48,24 @@ template <class T> static int dmalloc(T **p, size_t n) {
     return 0;
 }
 
+// Buffers only the gradient uses (the explicit inverses, the contraction U /
+// Uw, the C1 pieces, per-cell sums): allocated, zero-filled, on the first
+// call that needs them, so that a context used for IRLS and score tests
+// alone (the N = 1M, G = 5K, M = 512 genetic-mapping workload) does not
+// hold them.
+static int ensure_grad_buffers(gp_ctx *c) {
+    if (c->Sinv) return 0;
+    const size_t N = c->N, J = c->J, M = c->M, P = c->P, Np = c->Npad;
+    int r = 0;
+    if ((r = dmalloc(&c->Sinv, J * P * P)) || (r = dmalloc(&c->C1, Np * M)) ||
+        (r = dmalloc(&c->U, Np * P)) || (r = dmalloc(&c->Uw, Np * P)) ||
+        (r = dmalloc(&c->C1a, Np * M)) || (r = dmalloc(&c->AO, Np * M)) ||
+        (r = dmalloc(&c->Ecell, N)) || (r = dmalloc(&c->Epart, (size_t)GP_TN_SPLITS * Np)) ||
+        (r = dmalloc(&c->ci1, N)))
+        return r;
+    return 0;
+}
+
 static void drop_graph(gp_ctx *c);
 static void free_all(gp_ctx *c) {
     drop_graph(c);
@@ -156,13 +174,11 @@ extern "C" int gp_create(gp_ctx **out, int N, int J, int M, int Q, int p, const 
     ALLOC(c->Zw, Np * P);
     ALLOC(c->DeltaW, P * P);
     ALLOC(c->DeltaWChol, P * P);
-    ALLOC(c->Uw, Np * P);
     ALLOC(c->tBw, P * J);
     ALLOC(c->W, Np * J);
     ALLOC(c->Oinvt, (size_t)N);
     ALLOC(c->Nu, Np * J);
     ALLOC(c->Psi, (size_t)J * P * P);
-    ALLOC(c->Sinv, (size_t)J * P * P);
     ALLOC(c->Beta, P * J);
     ALLOC(c->tBeta, P * J);
     ALLOC(c->bvec, P * J);
@@ -179,19 +195,12 @@ extern "C" int gp_create(gp_ctx **out, int N, int J, int M, int Q, int p, const 
     ALLOC(c->lbq, c->tot_max);
     ALLOC(c->gradDeltaTmp, P * J);
     ALLOC(c->gradLogRhoTmp, (size_t)M * Q);
-    ALLOC(c->C1, Np * M);
     ALLOC(c->C2, (size_t)M * M);
-    ALLOC(c->U, Np * P);
     ALLOC(c->ZB, Np * J);
     ALLOC(c->R, Np * J);
-    ALLOC(c->C1a, Np * M);
-    ALLOC(c->AO, Np * M);
     ALLOC(c->G2, (size_t)M * M);
     ALLOC(c->B2, (size_t)M * M);
     ALLOC(c->S2, (size_t)M * M);
-    ALLOC(c->Ecell, (size_t)N);
-    ALLOC(c->Epart, (size_t)GP_TN_SPLITS * Np);
-    ALLOC(c->ci1, (size_t)N);
     ALLOC(c->red, 4096);
     ALLOC(c->scal, 64);
     ALLOC(c->lkhd, GP_LKHD_SLOTS);
@@ -358,6 +367,9 @@ extern "C" size_t gp_buffer_len(gp_ctx *c, const char *name) {
 
 extern "C" int gp_get_buffer(gp_ctx *c, const char *name, double *host, size_t n) {
     if (!c || !name || !host) { gp_set_error("NULL argument"); return 1; }
+    if (!std::strcmp(name, "Psi") || !std::strcmp(name, "C1") || !std::strcmp(name, "U") ||
+        !std::strcmp(name, "Uw"))
+        GP_TRY(ensure_grad_buffers(c));
     BufDesc d = describe(c, name);
     if (!d.dev) { gp_set_error(std::string("unknown buffer ") + name); return 1; }
     if (n != d.len) { gp_set_error(std::string("length mismatch for ") + name); return 1; }
@@ -402,7 +414,11 @@ extern "C" int gp_get_buffer(gp_ctx *c, const char *name, double *host, size_t n
 
 extern "C" int gp_set_buffer(gp_ctx *c, const char *name, const double *host, size_t n) {
     if (!c || !name || !host) { gp_set_error("NULL argument"); return 1; }
+    if (!std::strcmp(name, "Psi") || !std::strcmp(name, "C1") || !std::strcmp(name, "U") ||
+        !std::strcmp(name, "Uw"))
+        GP_TRY(ensure_grad_buffers(c));
     c->lkhd_fresh = 0;
+    c->w32_valid = 0;
     BufDesc d = describe(c, name);
     if (!d.dev) { gp_set_error(std::string("unknown buffer ") + name); return 1; }
     if (!d.writable) { gp_set_error(std::string("buffer is read-only: ") + name); return 1; }
@@ -504,6 +520,7 @@ static int gram_chol(gp_ctx *c, int precision, bool weights_ready = false) {
 extern "C" int gp_init_fit(gp_ctx *c) {
     if (!c) { gp_set_error("NULL context"); return 1; }
     c->lkhd_fresh = 0;
+    c->w32_valid = 0;
     GP_TRY(clear_info(c));
     GP_TRY(kernel_mats(c));
     GP_TRY(gpk_expand_delta(c));
@@ -521,6 +538,7 @@ extern "C" int gp_init_fit(gp_ctx *c) {
 extern "C" int gp_irls_step(gp_ctx *c, int jtr, double *obj) {
     if (!c) { gp_set_error("NULL context"); return 1; }
     c->lkhd_fresh = 0;
+    c->w32_valid = 0;
     if (jtr < 0 || jtr >= GP_LKHD_SLOTS) { gp_set_error("jtr out of range"); return 1; }
     GP_TRY(clear_info(c));
     GP_TRY(gpk_w_nu(c));
@@ -646,6 +664,7 @@ static int fork_tn_chain(gp_ctx *c) {
 
 extern "C" int gp_grad(gp_ctx *c) {
     if (!c) { gp_set_error("NULL context"); return 1; }
+    if (!c->capturing) GP_TRY(ensure_grad_buffers(c));
     gp_timer_begin(c, c->t_grad);
     GP_TRY(gpk_expand_delta(c));
     const bool w = c->psi_whitened;
@@ -736,6 +755,7 @@ extern "C" int gp_eval_graph(gp_ctx *c, int slot, double *obj) {
 extern "C" int gp_lbfgs_step(gp_ctx *c, int jtr, int rhoflag) {
     if (!c) { gp_set_error("NULL context"); return 1; }
     c->lkhd_fresh = 0;
+    c->w32_valid = 0;
     if (jtr < 0) { gp_set_error("jtr < 0"); return 1; }
     GP_TRY(gpk_lbfgs(c, jtr, rhoflag ? 1 : 0));
     return 0;
@@ -744,6 +764,7 @@ extern "C" int gp_lbfgs_step(gp_ctx *c, int jtr, int rhoflag) {
 extern "C" int gp_downdate(gp_ctx *c, int jtr, double sr, int rhoflag) {
     if (!c) { gp_set_error("NULL context"); return 1; }
     c->lkhd_fresh = 0;
+    c->w32_valid = 0;
     GP_TRY(gpk_downdate(c, jtr, sr, rhoflag ? 1 : 0));
     return 0;
 }
@@ -882,6 +903,7 @@ extern "C" int gp_autotune(gp_ctx *c, int verbose) {
     if (!c) { gp_set_error("NULL context"); return 1; }
     c->lkhd_fresh = 0;
     if (c->precision != GP_PREC_MIXED) { gp_set_error("gp_autotune needs GP_PREC_MIXED"); return 1; }
+    GP_TRY(ensure_grad_buffers(c));
     static const int splits[] = {8, 16, 32};
     static const int groups[] = {16, 32, 64};
     static const int waves[] = {8, 4};

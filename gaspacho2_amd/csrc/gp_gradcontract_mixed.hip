@@ -1207,8 +1207,9 @@ int gpk_gradcontract_mixed(gp_ctx *c) {
         c->gc_part_len = (size_t)ngroup * P16 * c->Npad;
         GP_HIP(hipMalloc((void **)&c->gc_part, c->gc_part_len * sizeof(float)));
     }
-    // per-gene exponents: e_j from the weight prep (wun = 2^e_j), s_j from max |Sw_j|
-    GP_TRY(gpk_wprep(c, c->W));
+    // per-gene exponents: e_j from the weight prep (wun = 2^e_j), s_j from max |Sw_j|;
+    // the ELBO's weight prep still holds when W and Omega have not changed since
+    if (!c->w32_valid) GP_TRY(gpk_wprep(c, c->W));
     k_zexp<<<KP, ZX_T, 0, c->stream>>>(c->Npad, c->P, c->Zw, c->gc_zexp);
     GP_HIP(hipGetLastError());
     k_smax<<<J, 256, 0, c->stream>>>(c->P, c->Sinv, c->gc_zexp, c->gc_sexp);

@@ -962,6 +962,7 @@ int gpk_wprep(gp_ctx *c, const double *W) {
     GP_TRY(ensure_mixed(c, c->gram_splits));
     k_wprep<<<c->J, 256, 0, c->stream>>>(c->N, c->Npad, W, c->Omega, c->w32, c->wsc, c->wun);
     GP_HIP(hipGetLastError());
+    c->w32_valid = (W == c->W);
     return 0;
 }
 
@@ -1004,6 +1005,7 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
     if (!weights_ready) {
         k_wprep<<<batch, 256, 0, c->stream>>>(c->N, c->Npad, W, c->Omega, c->w32, c->wsc, c->wun);
         GP_HIP(hipGetLastError());
+        c->w32_valid = (W == c->W && batch == c->J);
     }
     // intercept-peeled form: the symmetric Gram over the GP block only (M / 16
     // tiles: no 16-column tile for the single intercept column), the

@@ -108,6 +108,7 @@ struct gp_ctx {
     void *Zh = nullptr, *Zl = nullptr;  // FP16 hi/lo of Z (Npad x P16)
     float *z32 = nullptr;      // the same column-scaled model matrix in FP32
     float *w32 = nullptr;      // Npad x J  W/Omega in FP32
+    int w32_valid = 0;         // w32 / wsc / wun hold the current W / Omega (cleared by every call that may change them)
     float *wsc = nullptr;      // J  2^-e_j weight scale
     double *wun = nullptr;     // J  2^e_j
     double *zun = nullptr;     // P16  2^-s_a column unscale of the FP16 model matrix

@@ -51,14 +51,22 @@ __device__ inline double block_sum(double v, double *sh) {
 __device__ inline double gval(const double *G, size_t i) { return G[i]; }
 __device__ inline double gval(const unsigned char *G, size_t i) { return (double)G[i]; }
 
+// Genotype of SNP l at cell i: column l of G (N rows), or with a donor index
+// the row of the cell's donor in column l of G (ND rows): cells share their
+// donor's genotype
+template <typename GT>
+__device__ inline double gat(const GT *G, const int *donor, int ld, int l, int i) {
+    return gval(G + (size_t)l * ld, donor ? donor[i] : i);
+}
+
 // cmean[l] = mean_i G[i, l] d[i]  (the centring of ggplvm_de.cu:994, util.c:100-110)
 template <typename GT>
 __global__ void __launch_bounds__(ST) k_score_colmean(int N, const GT *__restrict__ G,
-        const double *__restrict__ d, double *__restrict__ cmean) {
+        const int *__restrict__ donor, int ld, const double *__restrict__ d,
+        double *__restrict__ cmean) {
     __shared__ double sh[4];
-    const GT *g = G + (size_t)blockIdx.x * N;
     double s = 0.0;
-    for (int i = threadIdx.x; i < N; i += ST) s += gval(g, i) * d[i];
+    for (int i = threadIdx.x; i < N; i += ST) s += gat(G, donor, ld, blockIdx.x, i) * d[i];
     s = block_sum(s, sh);
     if (threadIdx.x == 0) cmean[blockIdx.x] = s / (double)N;
 }
@@ -71,7 +79,8 @@ __device__ inline bool test_ok(int j, int l, int J, int S) {
 // t0 + c*TT .. t0 + c*TT + TT - 1 (those < T).
 template <typename GT>
 __global__ void __launch_bounds__(ST) k_score_acc(int N, int Npad, int P, int J, int S, int T,
-        int t0, const GT *__restrict__ G, const double *__restrict__ d,
+        int t0, const GT *__restrict__ G, const int *__restrict__ donor, int ld,
+        const double *__restrict__ d,
         const double *__restrict__ cmean, const int *__restrict__ gene,
         const int *__restrict__ snp, const double *__restrict__ W,
         const double *__restrict__ Omega, const double *__restrict__ Zsrc,
@@ -120,7 +129,7 @@ __global__ void __launch_bounds__(ST) k_score_acc(int N, int Npad, int P, int J,
                 const int t = w + 4 * u;
                 double val = 0.0;
                 if (i < N && sj[t] >= 0) {
-                    double h = gval(G + (size_t)sl[t] * N, i);
+                    double h = gat(G, donor, ld, sl[t], i);
                     if (d) h = h * d[i] - sc[t];
                     const double o = W[i + (size_t)sj[t] * Npad] / Omega[i];
                     val = h * o;
@@ -162,7 +171,8 @@ __global__ void __launch_bounds__(ST) k_score_acc(int N, int Npad, int P, int J,
 // term over the gene's non-zeros, forward substitution, statistic, p-value.
 template <typename GT>
 __global__ void __launch_bounds__(ST) k_score_fin(int N, int P, int T, int t0,
-        const GT *__restrict__ G, const double *__restrict__ d, const double *__restrict__ cmean,
+        const GT *__restrict__ G, const int *__restrict__ donor, int ld,
+        const double *__restrict__ d, const double *__restrict__ cmean,
         const int *__restrict__ gene, const int *__restrict__ snp, int J, int S,
         const int *__restrict__ Yp, const int *__restrict__ Yi, const double *__restrict__ Yx,
         const double *__restrict__ Omega, const double *__restrict__ Psi,
@@ -188,12 +198,11 @@ __global__ void __launch_bounds__(ST) k_score_fin(int N, int P, int T, int t0,
         for (int b = 0; b < NBK; b++) s += pc[(size_t)b * TT * (P + 2) + k];
         xs[k] = s;
     }
-    const GT *g = G + (size_t)l * N;
     const double cm = d ? cmean[l] : 0.0;
     double sy = 0.0;
     for (int q = Yp[j] + tid; q < Yp[j + 1]; q += ST) {
         const int i = Yi[q];
-        double h = gval(g, i);
+        double h = gat(G, donor, ld, l, i);
         if (d) h = h * d[i] - cm;
         sy += h * Yx[q] / Omega[i];
     }
@@ -222,21 +231,21 @@ __global__ void __launch_bounds__(ST) k_score_fin(int N, int P, int T, int t0,
 // (part holds fly * chunk_doubles(P) doubles).  All pointers are device
 // pointers; d / cmean / pval may be null.
 template <typename GT>
-int score_run(gp_ctx *c, int S, const GT *G, const double *d, const double *cmean, int T,
-              const int *gene, const int *snp, double *stat, double *pval, double *part, int fly,
-              int *err) {
+int score_run(gp_ctx *c, int S, const GT *G, const int *donor, int ld, const double *d,
+              const double *cmean, int T, const int *gene, const int *snp, double *stat,
+              double *pval, double *part, int fly, int *err) {
     const int P = c->P;
     const double *Zsrc = c->psi_whitened ? c->Zw : c->Z;
     const int per = fly * TT;
     for (int t0 = 0; t0 < T; t0 += per) {
         const int n = std::min(per, T - t0);
         const int nch = (n + TT - 1) / TT;
-        k_score_acc<GT><<<dim3(NBK, nch), ST, 0, c->stream>>>(c->N, c->Npad, P, c->J, S, T, t0, G, d,
-                                                             cmean, gene, snp, c->W, c->Omega, Zsrc,
-                                                             part, err);
+        k_score_acc<GT><<<dim3(NBK, nch), ST, 0, c->stream>>>(c->N, c->Npad, P, c->J, S, T, t0, G, donor,
+                                                             ld, d, cmean, gene, snp, c->W, c->Omega,
+                                                             Zsrc, part, err);
         GP_HIP(hipGetLastError());
         k_score_fin<GT><<<n, ST, (size_t)(P + 2) * sizeof(double), c->stream>>>(
-            c->N, P, T, t0, G, d, cmean, gene, snp, c->J, S, c->Yp, c->Yi, c->Yx, c->Omega, c->Psi,
+            c->N, P, T, t0, G, donor, ld, d, cmean, gene, snp, c->J, S, c->Yp, c->Yi, c->Yx, c->Omega, c->Psi,
             c->phi, part, stat, pval);
         GP_HIP(hipGetLastError());
     }
@@ -248,10 +257,18 @@ __global__ void k_score_clear(int *err) { err[0] = 0; }
 // Host-array entry points: device copies of the inputs and a partial buffer
 // for up to 16 chunks in flight, freed before returning.
 template <typename GT>
-int score_tests_host(gp_ctx *c, int S, const GT *G_host, const double *d_host, int T,
-                     const int *gene_host, const int *snp_host, double *stat_host,
-                     double *pval_host) {
+int score_tests_host(gp_ctx *c, int S, const GT *G_host, int ND, const int *donor_host,
+                     const double *d_host, int T, const int *gene_host, const int *snp_host,
+                     double *stat_host, double *pval_host) {
     if (S <= 0 || T <= 0) { gp_set_error("score tests: S and T must be positive"); return 1; }
+    if (donor_host) {
+        if (ND <= 0) { gp_set_error("score tests: the donor count must be positive"); return 1; }
+        for (int i = 0; i < c->N; i++)
+            if (donor_host[i] < 0 || donor_host[i] >= ND) {
+                gp_set_error("score tests: donor index out of range at cell " + std::to_string(i));
+                return 1;
+            }
+    }
     if (c->P > PMAX) { gp_set_error("score tests: more than 1024 model-matrix columns"); return 1; }
     for (int t = 0; t < T; t++) {
         if (gene_host[t] < 0 || gene_host[t] >= c->J || snp_host[t] < 0 || snp_host[t] >= S) {
@@ -261,14 +278,15 @@ int score_tests_host(gp_ctx *c, int S, const GT *G_host, const double *d_host, i
         }
     }
     const int N = c->N;
+    const int ld = donor_host ? ND : N;          // rows of G
     const int fly = std::max(1, std::min(16, (T + TT - 1) / TT));
     GT *dG = nullptr;
-    int *dgene = nullptr, *dsnp = nullptr, *derr = nullptr;
+    int *dgene = nullptr, *dsnp = nullptr, *derr = nullptr, *ddon = nullptr;
     double *dd = nullptr, *dcm = nullptr, *part = nullptr, *dstat = nullptr, *dp = nullptr;
     auto cleanup = [&]() {
         (void)hipStreamSynchronize(c->stream);
         for (void *p : {(void *)dG, (void *)dgene, (void *)dsnp, (void *)derr, (void *)dd,
-                        (void *)dcm, (void *)part, (void *)dstat, (void *)dp})
+                        (void *)dcm, (void *)part, (void *)dstat, (void *)dp, (void *)ddon})
             if (p) (void)hipFree(p);
     };
 #define SC_HIP(call)                                                                   \
@@ -280,7 +298,11 @@ int score_tests_host(gp_ctx *c, int S, const GT *G_host, const double *d_host, i
             return (int)e_;                                                            \
         }                                                                              \
     } while (0)
-    SC_HIP(hipMalloc((void **)&dG, (size_t)N * S * sizeof(GT)));
+    SC_HIP(hipMalloc((void **)&dG, (size_t)ld * S * sizeof(GT)));
+    if (donor_host) {
+        SC_HIP(hipMalloc((void **)&ddon, (size_t)N * sizeof(int)));
+        SC_HIP(hipMemcpyAsync(ddon, donor_host, (size_t)N * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    }
     SC_HIP(hipMalloc((void **)&dgene, (size_t)T * sizeof(int)));
     SC_HIP(hipMalloc((void **)&dsnp, (size_t)T * sizeof(int)));
     SC_HIP(hipMalloc((void **)&derr, sizeof(int)));
@@ -292,16 +314,16 @@ int score_tests_host(gp_ctx *c, int S, const GT *G_host, const double *d_host, i
         SC_HIP(hipMalloc((void **)&dcm, (size_t)S * sizeof(double)));
         SC_HIP(hipMemcpyAsync(dd, d_host, (size_t)N * sizeof(double), hipMemcpyHostToDevice, c->stream));
     }
-    SC_HIP(hipMemcpyAsync(dG, G_host, (size_t)N * S * sizeof(GT), hipMemcpyHostToDevice, c->stream));
+    SC_HIP(hipMemcpyAsync(dG, G_host, (size_t)ld * S * sizeof(GT), hipMemcpyHostToDevice, c->stream));
     SC_HIP(hipMemcpyAsync(dgene, gene_host, (size_t)T * sizeof(int), hipMemcpyHostToDevice, c->stream));
     SC_HIP(hipMemcpyAsync(dsnp, snp_host, (size_t)T * sizeof(int), hipMemcpyHostToDevice, c->stream));
     k_score_clear<<<1, 1, 0, c->stream>>>(derr);
     SC_HIP(hipGetLastError());
     if (d_host) {
-        k_score_colmean<GT><<<S, ST, 0, c->stream>>>(N, dG, dd, dcm);
+        k_score_colmean<GT><<<S, ST, 0, c->stream>>>(N, dG, ddon, ld, dd, dcm);
         SC_HIP(hipGetLastError());
     }
-    const int rc = score_run<GT>(c, S, dG, dd, dcm, T, dgene, dsnp, dstat, dp, part, fly, derr);
+    const int rc = score_run<GT>(c, S, dG, ddon, ld, dd, dcm, T, dgene, dsnp, dstat, dp, part, fly, derr);
     if (rc > 0) { cleanup(); return rc; }
     SC_HIP(hipMemcpyAsync(stat_host, dstat, (size_t)T * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     if (pval_host)
@@ -322,25 +344,32 @@ int gpk_score_clear(gp_ctx *c, int *err) {
 }
 int gpk_score_run_dosage(gp_ctx *c, int S, const unsigned char *G, int T, const int *gene,
                          const int *snp, double *stat, double *part, int fly, int *err) {
-    return score_run<unsigned char>(c, S, G, nullptr, nullptr, T, gene, snp, stat, nullptr, part,
-                                    fly, err);
+    return score_run<unsigned char>(c, S, G, nullptr, c->N, nullptr, nullptr, T, gene, snp, stat,
+                                    nullptr, part, fly, err);
 }
 
 extern "C" int gp_score_tests(gp_ctx *c, int S, const double *G, int T, const int *gene,
                               const int *snp, double *stat) {
     if (!c || !G || !gene || !snp || !stat) { gp_set_error("NULL argument"); return 1; }
-    return score_tests_host<double>(c, S, G, nullptr, T, gene, snp, stat, nullptr);
+    return score_tests_host<double>(c, S, G, 0, nullptr, nullptr, T, gene, snp, stat, nullptr);
 }
 
 extern "C" int gp_score_tests_dosage(gp_ctx *c, int S, const unsigned char *dosage, int T,
                                      const int *gene, const int *snp, double *stat) {
     if (!c || !dosage || !gene || !snp || !stat) { gp_set_error("NULL argument"); return 1; }
-    return score_tests_host<unsigned char>(c, S, dosage, nullptr, T, gene, snp, stat, nullptr);
+    return score_tests_host<unsigned char>(c, S, dosage, 0, nullptr, nullptr, T, gene, snp, stat, nullptr);
 }
 
 extern "C" int gp_score_tests_interaction(gp_ctx *c, int S, const unsigned char *dosage,
                                           const double *factor, int T, const int *gene,
                                           const int *snp, double *stat, double *pval) {
     if (!c || !dosage || !gene || !snp || !stat) { gp_set_error("NULL argument"); return 1; }
-    return score_tests_host<unsigned char>(c, S, dosage, factor, T, gene, snp, stat, pval);
+    return score_tests_host<unsigned char>(c, S, dosage, 0, nullptr, factor, T, gene, snp, stat, pval);
+}
+
+extern "C" int gp_score_tests_donor(gp_ctx *c, int ndonor, const int *donor, int S,
+                                    const unsigned char *dosage, const double *factor, int T,
+                                    const int *gene, const int *snp, double *stat, double *pval) {
+    if (!c || !donor || !dosage || !gene || !snp || !stat) { gp_set_error("NULL argument"); return 1; }
+    return score_tests_host<unsigned char>(c, S, dosage, ndonor, donor, factor, T, gene, snp, stat, pval);
 }

@@ -63,6 +63,8 @@ def load() -> C.CDLL:
         "gp_score_tests_interaction": (C.c_int, [vp, C.c_int, C.POINTER(C.c_ubyte), dp, C.c_int, ip,
                                                  ip, dp, dp]),
         "gp_score_tests_dosage_noalloc": (C.c_int, [vp, C.c_int, vp, C.c_int, vp, vp, vp]),
+        "gp_score_tests_donor": (C.c_int, [vp, C.c_int, ip, C.c_int, C.POINTER(C.c_ubyte), dp, C.c_int,
+                                           ip, ip, dp, dp]),
         "gp_pca_init": (C.c_int, [C.c_int, C.c_int, ip, ip, dp, dp, C.c_int, dp, C.c_int, dp, dp]),
         "gp_enc_create": (C.c_int, [C.POINTER(vp), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                     C.c_longlong, C.c_ulonglong]),
@@ -320,6 +322,25 @@ class GPContext:
                                                   flat.ctypes.data_as(C.POINTER(C.c_ubyte)),
                                                   None if f is None else _dp(f), len(gene), _ip(gene),
                                                   _ip(snp), _dp(stat), _dp(pval))
+        self._check(st)
+        return stat, pval
+
+    def score_tests_donor(self, donor, dosage, factor, gene, snp):
+        """(statistics, two-sided p-values) with genotypes per donor: cell i
+        has donor[i]'s genotypes, dosage is ndonor x S uint8; factor None
+        or a per-cell interaction factor (see score_tests_interaction)."""
+        donor = np.ascontiguousarray(donor, dtype=np.int32)
+        D = np.asfortranarray(np.asarray(dosage, dtype=np.uint8))
+        nd = D.shape[0]
+        flat = D.ravel(order="F")
+        gene = np.ascontiguousarray(gene, dtype=np.int32)
+        snp = np.ascontiguousarray(snp, dtype=np.int32)
+        stat, pval = np.zeros(len(gene)), np.zeros(len(gene))
+        f = None if factor is None else np.ascontiguousarray(factor, dtype=np.float64)
+        st = self._lib.gp_score_tests_donor(self._h, nd, _ip(donor), D.shape[1],
+                                            flat.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                            None if f is None else _dp(f), len(gene), _ip(gene),
+                                            _ip(snp), _dp(stat), _dp(pval))
         self._check(st)
         return stat, pval
 

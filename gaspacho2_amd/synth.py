@@ -61,13 +61,14 @@ def _se_ard(X: np.ndarray, Z: np.ndarray, rho: np.ndarray) -> np.ndarray:
 
 def make_dataset(N: int, G: int, Q: int, M: int, seed: int = 0, n_numeric: int = 0,
                  n_levels: int = 0, gen_M: Optional[int] = None, pca: bool = True,
-                 block: int = 256, model_seed: Optional[int] = None) -> Dataset:
+                 block: int = 256, model_seed: Optional[int] = None, progress=None) -> Dataset:
     """Counts from the generative model above; optional covariates
     (n_numeric N(0,1) columns and one categorical variable with n_levels).
 
     model_seed: draw the per-gene functions (generating inducing points,
     lengthscales, a_g, b_g) from their own stream, so several data seeds share
-    one set of genes (new cells, size factors and counts per seed)."""
+    one set of genes (new cells, size factors and counts per seed).
+    progress, if given, is called with (genes done, G) after each gene block."""
     rng = np.random.default_rng(seed)
     Mg = gen_M or M
     X = rng.standard_normal((N, Q))
@@ -119,6 +120,8 @@ def make_dataset(N: int, G: int, Q: int, M: int, seed: int = 0, n_numeric: int =
             Yi_parts.append(nz.astype(np.int32))
             Yx_parts.append(y[nz, k].astype(np.float64))
             Yp[g0 + k + 1] = Yp[g0 + k] + nz.size
+        if progress is not None:
+            progress(g1, G)
     Yi = np.concatenate(Yi_parts) if Yi_parts else np.zeros(0, np.int32)
     Yx = np.concatenate(Yx_parts) if Yx_parts else np.zeros(0)
     ncol = Zcov.shape[1]

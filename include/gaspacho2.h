@@ -185,6 +185,15 @@ int gp_score_tests_dosage(gp_ctx *ctx, int S, const unsigned char *dosage, int T
 int gp_score_tests_interaction(gp_ctx *ctx, int S, const unsigned char *dosage,
                                const double *factor, int T, const int *gene, const int *snp,
                                double *stat, double *pval);
+/* The interaction tests above with genotypes given per donor: cell i carries
+ * the genotypes of donor donor[i] (0 <= donor[i] < ndonor), so dosage is
+ * ndonor x S bytes and g_l(i) = dosage[donor[i] + l * ndonor]; otherwise as
+ * gp_score_tests_interaction (factor may be NULL, pval may be NULL).  The
+ * reference expands the genotype matrix to N rows on the host
+ * (ggplvm_de.cu:976-996); the per-cell lookup gives the same column. */
+int gp_score_tests_donor(gp_ctx *ctx, int ndonor, const int *donor, int S,
+                         const unsigned char *dosage, const double *factor, int T,
+                         const int *gene, const int *snp, double *stat, double *pval);
 /* No counterpart in the reference.  gp_score_tests_dosage for callers that
  * cannot allocate device memory (code under HIP graph capture, serving loops,
  * harnesses that forbid hipMalloc): dosage_dev (N x S bytes), gene_dev and

@@ -660,6 +660,35 @@ def test_interaction_score_tests_and_p_values_match_oracle(small_ds, ard):
     assert np.array_equal(s0, g.score_tests(D, gene, snp))
 
 
+def test_donor_genotype_score_tests_match_expanded_genotypes(small_ds):
+    """Genotypes given per donor (cell i carries donor[i]'s dosages) give the
+    statistics and p-values of the same genotypes expanded to one row per
+    cell (the reference's host-side layout) to the last bit, plain and
+    with an interaction factor; an out-of-range donor index is an error."""
+    from gaspacho2_amd import lib
+    _, g = _pair(small_ds)
+    g.init_fit()
+    for jtr in range(3):
+        g.irls_step(jtr)
+    N, J = small_ds.N, small_ds.J
+    rng = np.random.default_rng(13)
+    ND, S = 7, 5
+    donor = rng.integers(0, ND, N).astype(np.int32)
+    Dd = rng.integers(0, 3, (ND, S)).astype(np.uint8)
+    Dc = Dd[donor]
+    gene = np.repeat(np.arange(J), S).astype(np.int32)
+    snp = np.tile(np.arange(S), J).astype(np.int32)
+    d = rng.normal(0.0, 1.0, N)
+    for f in (None, d):
+        s1, p1 = g.score_tests_interaction(Dc, f, gene, snp)
+        s2, p2 = g.score_tests_donor(donor, Dd, f, gene, snp)
+        assert np.array_equal(s1, s2) and np.array_equal(p1, p2)
+    bad = donor.copy()
+    bad[3] = ND
+    with pytest.raises(lib.GPError):
+        g.score_tests_donor(bad, Dd, None, gene, snp)
+
+
 def test_score_tests_mixed_basis_close_to_oracle():
     """Score statistics from the mixed path's whitened factors (after gp_elbo)
     against the oracle's reference-basis factors at the same W."""
