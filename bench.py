@@ -681,13 +681,24 @@ def main():
     dom = max(("gram", "gradcontract"), key=lambda k: kern[k + "_kernel"])
     achieved = cands[dom]
     traffic, tsrc = measured_traffic(dom + "_kernel", main_p, (ds.N, ds.J, ds.Q, ds.M))
+    # the intercept-peeled forms apply when the intercept is the only column
+    # before the GP block and M fills whole 16- (Gram) or 32-column (contraction) tiles
+    g0 = int(ds.nh[-2])
+    peel_gram = main_p == "mixed" and g0 == 1 and ds.M % 16 == 0 and g.get_option("gram_peel") == 1
+    peel_gc = (main_p == "mixed" and g0 == 1 and ds.M % 32 == 0 and ds.M <= 256 and
+               g.get_option("gc_form") == 2 and g.get_option("gc_peel") == 1)
     if main_p == "mixed":
-        kname = {"gram": "k_gram_sym4" if g.get_option("gram_waves") == 4 else "k_gram_sym",
-                 "gradcontract": {0: "k_gc_mixed", 1: "k_gc_rows", 2: "k_gc_rows2",
-                                  3: "k_gc_rows4"}[g.get_option("gc_form")]}[dom]
+        gname = "k_gram_sym4" if g.get_option("gram_waves") == 4 else "k_gram_sym"
+        gname += f"<{ds.M // 16 if peel_gram else (P + 15) // 16}>"
+        if peel_gc:
+            cname = f"k_gc_gp<{ds.M // 16},{g.get_option('gc_wave_rows')},{g.get_option('gc_prefetch')}>"
+        else:
+            cname = {0: "k_gc_mixed", 1: "k_gc_rows", 2: "k_gc_rows2",
+                     3: "k_gc_rows4"}[g.get_option("gc_form")] + f"<{(P + 15) // 16}>"
+        kname = {"gram": gname, "gradcontract": cname}[dom]
     else:
-        kname = {"gram": "k_gram_mfma_f64", "gradcontract": "k_gc_mfma_f64"}[dom]
-    roofline = {"kernel": f"{kname}<{(P + 15) // 16}> ({dom})", "bound": "mfma", "achieved": achieved / 1e12,
+        kname = {"gram": "k_gram_mfma_f64", "gradcontract": "k_gc_mfma_f64"}[dom] + f"<{(P + 15) // 16}>"
+    roofline = {"kernel": f"{kname} ({dom})", "bound": "mfma", "achieved": achieved / 1e12,
                 "peak": peaks[dom] / 1e12, "unit": "TFLOP/s", "frac": achieved / peaks[dom],
                 "traffic": traffic, "traffic_source": tsrc,
                 "flops_per_launch": gram_flops(ds.N, ds.J, P) if dom == "gram" else gradcontract_flops(ds.N, ds.J, P),
