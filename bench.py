@@ -205,6 +205,46 @@ def _trace_summary(o, sec, kind):
             "lbfgs_s": spent[2], "lbfgs_steps": counts[2]}
 
 
+def _run_fit(args, ds, prec):
+    """One gp_fit of ds in precision prec ("f64", "mixed", "f64w") from its
+    PCA latent: wall time, final objective, trace and fitted latent means."""
+    from gaspacho2_amd import lib
+    pmap = {"f64": lib.PREC_FP64, "mixed": lib.PREC_MIXED, "f64w": lib.PREC_FP64_WHITENED}
+    T0 = lib.init_ta_with_per(ds.M, ds.Q)
+    g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0, T0,
+                      ard_only=args.kernel == "ard", precision=pmap[prec])
+    if prec == "mixed" and not args.no_tune:
+        g.set_option("autotune", 1)
+    log(f"{prec} fit: {args.fit_itr} outer iteration(s), L-BFGS cap {args.fit_cap or 'none'}")
+    t1 = time.perf_counter()
+    obj = g.fit(args.fit_itr, args.fit_cap, verbose=args.verbose)
+    dt = time.perf_counter() - t1
+    o, sec, kind = g.fit_trace()
+    _, _, info = g.fit_trace_steps()
+    r = {"seconds": dt, "final": obj, "trace": (o, sec, kind), "info": info, **_trace_summary(o, sec, kind)}
+    r["X"] = g.get("X")[:ds.N * ds.Q].reshape(ds.Q, ds.N).T.copy()
+    g.close()
+    return r
+
+
+def _phases(o, info):
+    """Per outer iteration: IRLS and L-BFGS steps, and whether each phase
+    ended on its convergence rule (1e-7 IRLS, 1e-8 L-BFGS relative change of
+    the objective, ggplvm.cu:1183, 1292) rather than on its step limit."""
+    out = []
+    for itr in np.unique(info[:, 1]):
+        row = {"itr": int(itr)}
+        for kind, name, tol in ((0, "irls", 1e-7), (2, "lbfgs", 1e-8)):
+            v = o[(info[:, 1] == itr) & (info[:, 0] == kind)]
+            if kind == 2:     # the L-BFGS phase starts from the quasi-likelihood objective
+                v = np.concatenate([o[(info[:, 1] == itr) & (info[:, 0] == 1)], v])
+            d = abs(v[-1] - v[-2]) / abs(v[-1]) if len(v) > 1 else float("nan")
+            row[name] = {"steps": int(len(v) - (kind == 2)), "last_rel_change": float(d),
+                         "converged": bool(d < tol), "end_objective": float(v[-1])}
+        out.append(row)
+    return out
+
+
 def main_fit(args):
     """End-to-end fit (gp_fit: init, IRLS phase, quasi-likelihood start,
     L-BFGS phase with its line search, per outer iteration) from the PCA
@@ -215,25 +255,36 @@ def main_fit(args):
     t0 = time.perf_counter()
     ds = synth.make_dataset(args.N, args.G, args.Q, args.M, seed=0)
     log(f"data: N={ds.N} G={ds.J} ({time.perf_counter() - t0:.1f} s)")
-    T0 = lib.init_ta_with_per(ds.M, ds.Q)
     res = {}
-    precs = ("f64", "mixed", "f64w") if args.fit_f64w else ("f64", "mixed")
-    pmap = {"f64": lib.PREC_FP64, "mixed": lib.PREC_MIXED, "f64w": lib.PREC_FP64_WHITENED}
+    precs = args.fit_precs.split(",")
+    if args.fit_f64w and "f64w" not in precs:
+        precs.append("f64w")
+    tag = f"N{ds.N}_G{ds.J}_M{ds.M}_itr{args.fit_itr}_cap{args.fit_cap}"
     for prec in precs:
-        g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0, T0,
-                          ard_only=args.kernel == "ard", precision=pmap[prec])
-        if prec == "mixed" and not args.no_tune:
-            g.set_option("autotune", 1)
-        t1 = time.perf_counter()
-        obj = g.fit(args.fit_itr, args.fit_cap, verbose=args.verbose)
-        dt = time.perf_counter() - t1
-        o, sec, kind = g.fit_trace()
-        res[prec] = {"seconds": dt, "final": obj, "trace": (o, sec, kind), **_trace_summary(o, sec, kind)}
-        res[prec]["X"] = g.get("X")[:ds.N * ds.Q].reshape(ds.Q, ds.N).T.copy()
-        log(f"{prec} fit: {dt:.2f} s, final objective {obj:.10e}, "
-            f"{res[prec]['lbfgs_steps']} L-BFGS steps ({res[prec]['lbfgs_s']:.2f} s), "
-            f"{res[prec]['irls_steps']} IRLS steps ({res[prec]['irls_s']:.2f} s)")
-        g.close()
+        path = os.path.join(args.fit_load, f"fit_{prec}_{tag}.npz") if args.fit_load else None
+        if path and os.path.exists(path):
+            # a fit of the same data and schedule saved by an earlier run (--fit-save)
+            z = np.load(path)
+            res[prec] = {"seconds": float(z["seconds"]), "final": float(z["final"]),
+                         "trace": (z["obj"], z["sec"], z["kind"]), "info": z["info"], "X": z["X"],
+                         **_trace_summary(z["obj"], z["sec"], z["kind"])}
+            log(f"{prec} fit loaded from {path}")
+        else:
+            res[prec] = _run_fit(args, ds, prec)
+            if args.fit_save:
+                os.makedirs(args.fit_save, exist_ok=True)
+                r = res[prec]
+                np.savez(os.path.join(args.fit_save, f"fit_{prec}_{tag}.npz"), seconds=r["seconds"],
+                         final=r["final"], obj=r["trace"][0], sec=r["trace"][1], kind=r["trace"][2],
+                         info=r["info"], X=r["X"])
+        r = res[prec]
+        r["phases"] = _phases(r["trace"][0], r["info"])
+        log(f"{prec} fit: {r['seconds']:.2f} s, final objective {r['final']:.10e}, "
+            f"{r['lbfgs_steps']} L-BFGS steps ({r['lbfgs_s']:.2f} s), "
+            f"{r['irls_steps']} IRLS steps ({r['irls_s']:.2f} s); phases {r['phases']}")
+    if "f64" not in res or "mixed" not in res:
+        log("the comparison needs the f64 and mixed fits (run them, or load them with --fit-load)")
+        return
     f64, mx = res["f64"], res["mixed"]
 
     # latent posterior means after orthogonal Procrustes alignment onto the
@@ -254,6 +305,13 @@ def main_fit(args):
     from gaspacho2_amd.warmstart import steps_to_reach
     n_hit, t_hit = steps_to_reach(mx["trace"], f64["final"])
     gap = (mx["final"] - f64["final"]) / abs(f64["final"])
+
+    def phase_gaps(r):
+        # relative gap of each outer iteration's last L-BFGS objective to the FP64 fit's
+        return [(a["lbfgs"]["end_objective"] - b["lbfgs"]["end_objective"]) / abs(b["lbfgs"]["end_objective"])
+                for a, b in zip(r["phases"], f64["phases"])]
+    log(f"end-of-phase objective gaps to the FP64 fit: mixed {phase_gaps(mx)}"
+        + (f", FP64 whitened {phase_gaps(res['f64w'])}" if "f64w" in res else ""))
     out = {"metric": "end-to-end fit seconds at N=100K,G=2K,M=256: mixed vs FP64 plain",
            "value": mx["seconds"], "unit": "s", "n_gpus": 1, "steps": mx["lbfgs_steps"], "warmup": 0,
            "ms_per_step": mx["seconds"] / max(mx["lbfgs_steps"], 1) * 1e3, "higher_is_better": False,
@@ -261,14 +319,15 @@ def main_fit(args):
            "data": "synthetic (seed 0), PCA initial latent",
            "config": {"workload": f"sparse GP-LVM fit, N={ds.N} cells, G={ds.J} genes, Q={ds.Q}, M={ds.M}, "
                                   f"{args.fit_itr} outer iteration(s), L-BFGS cap {args.fit_cap}"},
-           "mixed": {k: v for k, v in mx.items() if k not in ("trace", "X")},
-           "fp64_plain": {k: v for k, v in f64.items() if k not in ("trace", "X")},
-           "final_rel_gap_mixed_vs_fp64": gap,
+           "mixed": {k: v for k, v in mx.items() if k not in ("trace", "X", "info")},
+           "fp64_plain": {k: v for k, v in f64.items() if k not in ("trace", "X", "info")},
+           "final_rel_gap_mixed_vs_fp64": gap, "phase_end_gaps_mixed_vs_fp64": phase_gaps(mx),
            "latent_rel_diff_mixed_vs_fp64": {"raw": x_rel_raw, "procrustes": x_rel},
            "fp64_whitened_vs_fp64": (None if xw is None else
                                      {"latent_raw": xw[0], "latent_procrustes": xw[1],
                                       "final_rel_gap": (res["f64w"]["final"] - f64["final"]) / abs(f64["final"]),
-                                      "seconds": res["f64w"]["seconds"]}),
+                                      "seconds": res["f64w"]["seconds"], "phases": res["f64w"]["phases"],
+                                      "phase_end_gaps": phase_gaps(res["f64w"])}),
            "mixed_reaches_fp64_final": {"seconds": t_hit, "lbfgs_steps": n_hit,
                                         "speedup_vs_fp64_fit": (f64["seconds"] / t_hit) if t_hit else None},
            "speedup_same_schedule": f64["seconds"] / mx["seconds"]}
@@ -479,8 +538,10 @@ def main_warmstart(args):
     log(f"data: {S + 1} seeds of N={args.N} G={args.G} ({time.perf_counter() - t0:.1f} s)")
     T0 = lib.init_ta_with_per(args.M, args.Q)
     train = []
+    t_itr = args.fit_itr if args.train_itr is None else args.train_itr
+    t_cap = args.fit_cap if args.train_cap is None else args.train_cap
     for s in range(S):
-        X, obj, dt, _ = warmstart.fit_from(data[s], data[s].X0, T0, args.fit_itr, args.fit_cap,
+        X, obj, dt, _ = warmstart.fit_from(data[s], data[s].X0, T0, t_itr, t_cap,
                                               ard_only=args.kernel == "ard")
         train.append((data[s], X))
         log(f"seed {s}: PCA-started fit {dt:.1f} s, objective {obj:.10e}, "
@@ -500,8 +561,10 @@ def main_warmstart(args):
         enc.close()
         log(f"{target}: encoder {S * args.N} cells, {args.enc_epochs} epochs, {t_train:.1f} s, "
             f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
-        _, obj_e, dt_e, tr_e = warmstart.fit_from(ds, X0e, T0, args.fit_itr, args.fit_cap,
-                                            ard_only=args.kernel == "ard")
+        Xe, obj_e, dt_e, tr_e = warmstart.fit_from(ds, X0e, T0, args.fit_itr, args.fit_cap,
+                                                   ard_only=args.kernel == "ard")
+        U, _, Vt = np.linalg.svd(Xe.T @ Xp)
+        x_proc = float(np.linalg.norm(Xe @ (U @ Vt) - Xp) / np.linalg.norm(Xp))
         n_hit, t_hit = warmstart.steps_to_reach(tr_e, obj_p)
         log(f"{target}: warm-started fit {obj_e:.10e} ({dt_e:.1f} s), reaches the PCA-started final after "
             f"{n_hit} of {n_pca} L-BFGS steps")
@@ -509,6 +572,7 @@ def main_warmstart(args):
             "steps_saved": (n_pca - n_hit) if n_hit else None,
             "final_objective": obj_e, "rel_gap_vs_pca_start": (obj_e - obj_p) / abs(obj_p),
             "objective_after_ql_start": first(tr_e), "fit_seconds": dt_e,
+            "lbfgs_steps": int(np.sum(tr_e[2] == 2)), "latent_procrustes_vs_pca_start": x_proc,
             "reaches_pca_final": {"lbfgs_steps": n_hit, "seconds": t_hit},
             "encoder_train_seconds": t_train, "encoder_epoch_loss": [float(losses[0]), float(losses[-1])],
             "warm_start_rms_to_pca_fit": float(np.sqrt(np.mean((X0e - Xp) ** 2)))}
@@ -520,10 +584,11 @@ def main_warmstart(args):
            "dtype": "bf16/f32 MFMA encoder, fp16x3/f32 + f64 fit", "data": f"synthetic, seeds 0..{S}",
            "config": {"workload": f"encoder warm start: {S} training seeds x N={args.N}, G={args.G}, "
                                   f"Q={args.Q}, M={args.M}; MLP {args.G}-512-256-{2 * args.Q}; "
-                                  f"{args.fit_itr} outer iteration(s), L-BFGS cap {args.fit_cap}",
+                                  f"{args.fit_itr} outer iteration(s), L-BFGS cap {args.fit_cap} "
+                                  f"(training fits: {t_itr}, {t_cap})",
                       "target": targets[0]},
            "pca_start": {"final_objective": obj_p, "objective_after_ql_start": first(tr_p), "fit_seconds": dt_p,
-                         "pca_rms_to_pca_fit": float(np.sqrt(np.mean((ds.X0 - Xp) ** 2)))},
+                         "lbfgs_steps": n_pca, "pca_rms_to_pca_fit": float(np.sqrt(np.mean((ds.X0 - Xp) ** 2)))},
            "variants": variants}
     print(json.dumps(out), flush=True)
 
@@ -549,9 +614,18 @@ def main():
     ap.add_argument("--fit-itr", type=int, default=1, help="fit workload: outer iterations")
     ap.add_argument("--verbose", action="store_true", help="fit workload: print every objective")
     ap.add_argument("--fit-cap", type=int, default=100, help="fit workload: L-BFGS steps per outer iteration")
+    ap.add_argument("--fit-precs", default="f64,mixed",
+                    help="fit workload: precisions to fit (f64, mixed, f64w), comma separated")
+    ap.add_argument("--fit-save", default="", metavar="DIR", help="fit workload: save each fit to DIR")
+    ap.add_argument("--fit-load", default="", metavar="DIR",
+                    help="fit workload: take fits of the same data and schedule saved in DIR")
     ap.add_argument("--fit-f64w", action="store_true",
                     help="fit workload: also fit in FP64 on the whitened basis (rounding-only divergence)")
     ap.add_argument("--train-seeds", type=int, default=8, help="warmstart workload: training fits")
+    ap.add_argument("--train-itr", type=int, default=None,
+                    help="warmstart workload: outer iterations of the training fits (default --fit-itr)")
+    ap.add_argument("--train-cap", type=int, default=None,
+                    help="warmstart workload: L-BFGS cap of the training fits (default --fit-cap)")
     ap.add_argument("--enc-epochs", type=int, default=30, help="warmstart workload: encoder epochs")
     ap.add_argument("--enc-target", default="residual,aligned",
                     help="warmstart workload: encoder targets, comma-separated (first = headline)")

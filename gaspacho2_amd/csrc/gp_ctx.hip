@@ -277,6 +277,9 @@ extern "C" int gp_set_option(gp_ctx *c, const char *name, int value) {
     } else if (n == "gc_wave_rows") {
         if (value != 8 && value != 4) { gp_set_error("gc_wave_rows must be 8 or 4"); return 1; }
         c->gc_wave_rows = value;
+    } else if (n == "gc_waves") {
+        if (value != 8 && value != 4) { gp_set_error("gc_waves must be 8 or 4"); return 1; }
+        c->gc_waves = value;
     } else if (n == "gc_prefetch") {
         if (value != 0 && value != 1 && value != 3 && value != 7) { gp_set_error("gc_prefetch must be 0, 1, 3 or 7"); return 1; }
         c->gc_prefetch = value;
@@ -301,6 +304,7 @@ extern "C" int gp_get_option(gp_ctx *c, const char *name, int *value) {
     else if (n == "gram_peel") *value = c->gram_peel;
     else if (n == "gc_wave_rows") *value = c->gc_wave_rows;
     else if (n == "gc_prefetch") *value = c->gc_prefetch;
+    else if (n == "gc_waves") *value = c->gc_waves;
     else if (n == "gram_waves") *value = c->gram_waves;
     else if (n == "gc_form") *value = c->gc_form;
     else if (n == "irls_mixed") *value = c->irls_mixed;
@@ -720,7 +724,7 @@ extern "C" int gp_eval_graph(gp_ctx *c, int slot, double *obj) {
     if (slot < 0 || slot >= GP_LKHD_SLOTS) { gp_set_error("slot out of range"); return 1; }
     const int key[GP_GRAPH_KEY] = {c->precision, slot, c->gram_splits, c->gc_groups, c->gram_waves,
                                    c->gc_form, c->gc_mixed, c->gc_peel, c->gc_wave_rows,
-                                   c->gc_prefetch, c->gram_peel};
+                                   c->gc_prefetch, c->gram_peel, c->gc_waves};
     if (!c->graph_exec || !std::equal(key, key + GP_GRAPH_KEY, c->graph_key)) {
         drop_graph(c);
         double tmp = 0.0;
@@ -807,7 +811,7 @@ extern "C" int gp_fit(gp_ctx *c, int itrmax, int jtr_cap, int verbose, gp_outer_
             r = gp_irls_step(c, jtr, &lk[jtr]);
             if (r > 0) return r;
             mark(0, lk[jtr], itr, jtr, 0, 0.0);
-            if (verbose) printf("lkhd[%d,%d]=%lf\n", itr, jtr, lk[jtr]);
+            if (verbose) { printf("lkhd[%d,%d]=%lf\n", itr, jtr, lk[jtr]); fflush(stdout); }
             if (jtr > 0) {
                 if (std::isnan(lk[jtr])) { gp_set_error("GLM objective is NaN"); if (obj) *obj = NAN; return 2; }
                 if (jtr == JTRMAX - 1 || std::fabs(lk[jtr - 1] - lk[jtr]) / std::fabs(lk[jtr]) < 1e-7) break;
@@ -817,7 +821,7 @@ extern "C" int gp_fit(gp_ctx *c, int itrmax, int jtr_cap, int verbose, gp_outer_
         if (r > 0) return r;
         mark(1, lk[0], itr, 0, 0, 0.0);
         last = lk[0];
-        if (verbose) printf("lkhd[%d,0]=%lf\n", itr, lk[0]);
+        if (verbose) { printf("lkhd[%d,0]=%lf\n", itr, lk[0]); fflush(stdout); }
         if (itr == 0 && c->autotune && c->precision == GP_PREC_MIXED) {
             r = gp_autotune(c, verbose);
             if (r > 0) return r;
@@ -833,7 +837,7 @@ extern "C" int gp_fit(gp_ctx *c, int itrmax, int jtr_cap, int verbose, gp_outer_
             for (ktr = 1; ktr < 8; ktr++) {
                 r = gp_elbo(c, jtr + 1, &lk[jtr + 1]);
                 if (r > 0) return r;
-                if (verbose) printf("lkhd[%d,%d]=%lf\n", itr, jtr + 1, lk[jtr + 1]);
+                if (verbose) { printf("lkhd[%d,%d]=%lf\n", itr, jtr + 1, lk[jtr + 1]); fflush(stdout); }
                 if (lk[jtr + 1] > lk[jtr] || ktr == 7) break;
                 else if (std::isnan(lk[jtr + 1]) || std::isinf(lk[jtr + 1])) sr *= 0.01;
                 else if (lk[jtr + 1] < lk[jtr]) sr *= 0.1;

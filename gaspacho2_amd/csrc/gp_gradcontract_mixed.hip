@@ -880,25 +880,32 @@ __global__ void k_ssplit_gp(int P, int M, const double *__restrict__ S, const in
 }
 
 // Wave tiling: the 8 row tiles (128 cells) x TG column tiles of a workgroup
-// are cut into blocks of RG row tiles x CG = 16 / RG column tiles, one per
-// wave (TG / 2 waves).  RG = 8 is the factored form's shape (each wave reads
-// every A row fragment, 2 column tiles of B); RG = 4 halves the A fragment
-// reads from LDS per MFMA and doubles the B fragment loads from L2 (measured
-// 50 % slower at M = 256: the B stream from L2 is the limit).  B fragments
-// are loaded PD k-steps ahead through a ring of PD + 1 register sets (PD + 1
-// divides the k-steps of a gene, so the ring lines up at every gene); the
-// last PD k-steps of a gene load the first ones of the next.
-template <int TG, int RG, int PD>
-__global__ void __launch_bounds__(TG / 2 * 64) k_gc_gp(int N, int Npad, int gsize, int J,
+// are cut into blocks of RG row tiles x CG column tiles, one per wave
+// ((8 / RG) (TG / CG) waves).  RG = 8, CG = 2 is the factored form's shape
+// (8 waves at M = 256, two per SIMD: each wave reads every A row fragment
+// and 2 column tiles of B); RG = 4, CG = 4 halves the A fragment reads from
+// LDS per MFMA and doubles the B fragment loads from L2 (measured 50 %
+// slower at M = 256: the B stream from L2 is the limit); RG = 8, CG = 4
+// halves the A reads with the B loads unchanged, on 4 waves of 512
+// registers (one per SIMD, the accumulators of the gene and of the running
+// sum together take 256).  B fragments are loaded PD k-steps ahead through
+// a ring of PD + 1 register sets (PD + 1 divides the k-steps of a gene, so
+// the ring lines up at every gene); the last PD k-steps of a gene load the
+// first ones of the next.
+template <int TG, int RG, int CG>
+constexpr int gc_gp_waves() { return (8 / RG) * (TG / CG); }
+
+template <int TG, int RG, int CG, int PD>
+__global__ void __launch_bounds__((gc_gp_waves<TG, RG, CG>() * 64))
+k_gc_gp(int N, int Npad, int gsize, int J,
         const double *__restrict__ Zg, const float *__restrict__ w32,
         const int *__restrict__ perm, const int *__restrict__ aexp, const int *__restrict__ zexp,
         const _Float16 *__restrict__ shi, const _Float16 *__restrict__ slo,
         const float *__restrict__ t0, float *__restrict__ part) {
     constexpr int M = 16 * TG;
-    constexpr int CG = 16 / RG;
     constexpr int NWC = TG / CG;
-    constexpr int NW = TG / 2;
-    static_assert(RG * CG == 16 && TG % CG == 0 && NWC * (8 / RG) == NW, "wave tiling");
+    constexpr int NW = gc_gp_waves<TG, RG, CG>();
+    static_assert(8 % RG == 0 && TG % CG == 0 && NW >= 1, "wave tiling");
     constexpr int KS = M / 32;
     constexpr int LDA = M + 8;
     constexpr int NITEM = GRC * KS * 4;
@@ -1097,15 +1104,15 @@ __global__ void k_gc_u0(int Npad, int M, const double *__restrict__ Zw, const fl
     Uw[i] = s;
 }
 
-template <int TG, int RG, int PD>
+template <int TG, int RG, int CG, int PD>
 static int launch_gc_gp(gp_ctx *c, int gsize, int ngroup) {
     constexpr int M = 16 * TG;
     const size_t shm = (size_t)2 * GRC * (M + 8) * sizeof(_Float16);
     if (shm > 65536 && !c->capturing)
-        GP_HIP(hipFuncSetAttribute((const void *)k_gc_gp<TG, RG, PD>,
+        GP_HIP(hipFuncSetAttribute((const void *)k_gc_gp<TG, RG, CG, PD>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     dim3 g(c->Npad / GRC, ngroup);
-    k_gc_gp<TG, RG, PD><<<g, TG / 2 * 64, shm, c->stream>>>(c->N, c->Npad, gsize, c->J, c->Zw + c->Npad,
+    k_gc_gp<TG, RG, CG, PD><<<g, gc_gp_waves<TG, RG, CG>() * 64, shm, c->stream>>>(c->N, c->Npad, gsize, c->J, c->Zw + c->Npad,
                                                            c->w32, c->gc_perm, c->gc_aexp, c->gc_zexp + 1,
                                                            (const _Float16 *)c->Sh, (const _Float16 *)c->Sl,
                                                            c->gc_t0, c->gc_part);
@@ -1114,28 +1121,32 @@ static int launch_gc_gp(gp_ctx *c, int gsize, int ngroup) {
 }
 
 // prefetch depth: the deepest of 1, 3 and 7 whose ring divides the k-steps
-// (M / 32) and does not exceed the option gc_prefetch
-template <int TG, int RG>
+// (M / 32) and does not exceed the option gc_prefetch (at most 1 for the
+// 4-wave shape)
+template <int TG, int RG, int CG>
 static int launch_gc_gp_pd(gp_ctx *c, int gsize, int ngroup) {
     constexpr int KS = TG / 2;
-    if constexpr (KS % 8 == 0) {
-        if (c->gc_prefetch >= 7) return launch_gc_gp<TG, RG, 7>(c, gsize, ngroup);
+    if constexpr (KS % 8 == 0 && RG * CG <= 16) {
+        if (c->gc_prefetch >= 7) return launch_gc_gp<TG, RG, CG, 7>(c, gsize, ngroup);
     }
-    if constexpr (KS % 4 == 0) {
-        if (c->gc_prefetch >= 3) return launch_gc_gp<TG, RG, 3>(c, gsize, ngroup);
+    if constexpr (KS % 4 == 0 && RG * CG <= 16) {     // 4 B sets of 32 tiles spill
+        if (c->gc_prefetch >= 3) return launch_gc_gp<TG, RG, CG, 3>(c, gsize, ngroup);
     }
     if constexpr (KS % 2 == 0) {
-        if (c->gc_prefetch >= 1) return launch_gc_gp<TG, RG, 1>(c, gsize, ngroup);
+        if (c->gc_prefetch >= 1) return launch_gc_gp<TG, RG, CG, 1>(c, gsize, ngroup);
     }
-    return launch_gc_gp<TG, RG, 0>(c, gsize, ngroup);
+    return launch_gc_gp<TG, RG, CG, 0>(c, gsize, ngroup);
 }
 
+// wave shape: gc_waves 4 (RG = 8, CG = 4; M a multiple of 64), else
+// gc_wave_rows 4 (RG = 4, CG = 4) or the default RG = 8, CG = 2
 template <int TG>
 static int launch_gc_gp_rg(gp_ctx *c, int gsize, int ngroup) {
     if constexpr (TG % 4 == 0) {
-        if (c->gc_wave_rows == 4) return launch_gc_gp_pd<TG, 4>(c, gsize, ngroup);
+        if (c->gc_waves == 4) return launch_gc_gp_pd<TG, 8, 4>(c, gsize, ngroup);
+        if (c->gc_wave_rows == 4) return launch_gc_gp_pd<TG, 4, 4>(c, gsize, ngroup);
     }
-    return launch_gc_gp_pd<TG, 8>(c, gsize, ngroup);
+    return launch_gc_gp_pd<TG, 8, 2>(c, gsize, ngroup);
 }
 
 // the peeled form applies: the intercept is the only column before the GP

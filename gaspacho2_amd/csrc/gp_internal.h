@@ -13,7 +13,7 @@
 
 #define GP_HIST 10            // L-BFGS history length (ggplvm.cu:872-880)
 #define GP_LKHD_SLOTS 50000   // objective history length (ggplvm.cu:1025)
-#define GP_GRAPH_KEY 11       // settings a recorded ELBO + gradient graph depends on
+#define GP_GRAPH_KEY 12       // settings a recorded ELBO + gradient graph depends on
 
 struct gp_timer {
     hipEvent_t a = nullptr, b = nullptr;
@@ -134,6 +134,7 @@ struct gp_ctx {
     int gc_form = 2;           // mixed contraction: 0 = 16-cell waves, 1 = 128-cell row blocks, 2 = factored row blocks
     int gc_peel = 1;           // factored form on the GP block only when the intercept is the sole covariate column
     int gc_wave_rows = 8;      // peeled form: row tiles per wave (8 or 4; column tiles 16 / that)
+    int gc_waves = 8;          // peeled form: 8 waves (2 column tiles each) or 4 (4 column tiles, 8 row tiles)
     int gc_prefetch = 3;       // peeled form: k-steps of B fragments loaded ahead (0, 1, 3 or 7)
     float *gc_t0 = nullptr;    // J x M  the intercept's k term of each gene (peeled form)
     float *gc_s0 = nullptr;    // J x (M+1)  [Sw_j[0, 1:] | Sw_j[0, 0]] (intercept output column)
@@ -154,8 +155,8 @@ struct gp_ctx {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     // precision, slot, gram_splits, gc_groups, gram_waves, gc_form, gc_mixed, gc_peel,
-    // gc_wave_rows, gc_prefetch, gram_peel
-    int graph_key[GP_GRAPH_KEY] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+    // gc_wave_rows, gc_prefetch, gram_peel, gc_waves
+    int graph_key[GP_GRAPH_KEY] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
     // objective and elapsed seconds after every accepted step of the last gp_fit
     std::vector<double> fit_obj, fit_sec;
     std::vector<int> fit_kind;           // 0 IRLS step, 1 quasi-likelihood start, 2 L-BFGS step

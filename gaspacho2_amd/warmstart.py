@@ -49,13 +49,18 @@ def fit_from(ds, X0: np.ndarray, T0: np.ndarray, itrmax: int, jtr_cap: int,
 
 def steps_to_reach(trace, target: float, rtol: float = 1e-6) -> Tuple[int, float]:
     """(L-BFGS steps, seconds) after which a fit trace first reaches the
-    objective target within rtol, or (None, None)."""
+    objective target within rtol, or (None, None).  Only the last outer
+    iteration's L-BFGS phase is searched: every outer iteration refits the
+    GLM before its quasi-likelihood start, so objectives of different outer
+    iterations are not on one scale (the steps count from the fit's start)."""
     obj, sec, kind = trace
+    kind = np.asarray(kind)
+    last = int(np.flatnonzero(kind == 1)[-1]) if np.any(kind == 1) else 0
     n = 0
-    for v, t, k in zip(obj, sec, kind):
+    for i, (v, t, k) in enumerate(zip(obj, sec, kind)):
         if k == 2:
             n += 1
-            if v >= target - rtol * abs(target):
+            if i > last and v >= target - rtol * abs(target):
                 return n, float(t)
     return None, None
 

@@ -452,7 +452,8 @@ def test_contraction_intercept_peeled_form(shape):
     # the wave tilings and prefetch depths of the peeled kernel give the same bits
     g.set_option("gc_mixed", 1)
     g.set_option("gc_peel", 1)
-    for opts in ({"gc_prefetch": 0}, {"gc_prefetch": 1}, {"gc_wave_rows": 4, "gc_prefetch": 3}):
+    for opts in ({"gc_prefetch": 0}, {"gc_prefetch": 1}, {"gc_wave_rows": 4, "gc_prefetch": 3},
+                 {"gc_wave_rows": 8, "gc_waves": 4, "gc_prefetch": 1}, {"gc_prefetch": 0}):
         for k, v in opts.items():
             g.set_option(k, v)
         g.grad()
