@@ -637,7 +637,7 @@ def main():
     ap.add_argument("--donors", type=int, default=200, help="score workload: donors")
     ap.add_argument("--snps-per-gene", type=int, default=10, help="score workload: cis SNPs per gene")
     ap.add_argument("--irls", type=int, default=6, help="score workload: IRLS steps before the tests")
-    ap.add_argument("--gram-splits", type=int, default=2, help="score workload: mixed Gram cell segments")
+    ap.add_argument("--gram-splits", type=int, default=8, help="score workload: mixed Gram cell segments")
     ap.add_argument("--check-genes", type=int, default=16, help="score workload: genes checked in FP64")
     ap.add_argument("--nmax", type=int, default=50000)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -765,7 +765,10 @@ def main():
         gname = "k_gram_sym4" if g.get_option("gram_waves") == 4 else "k_gram_sym"
         gname += f"<{ds.M // 16 if peel_gram else (P + 15) // 16}>"
         if peel_gc:
-            cname = f"k_gc_gp<{ds.M // 16},{g.get_option('gc_wave_rows')},{g.get_option('gc_prefetch')}>"
+            rg = 8 if g.get_option("gc_waves") == 4 else g.get_option("gc_wave_rows")
+            cg = 4 if g.get_option("gc_waves") == 4 else 16 // rg
+            pd = min(g.get_option("gc_prefetch"), 1 if rg * cg > 16 else 7)
+            cname = f"k_gc_gp<{ds.M // 16},{rg},{cg},{pd}>"
         else:
             cname = {0: "k_gc_mixed", 1: "k_gc_rows", 2: "k_gc_rows2",
                      3: "k_gc_rows4"}[g.get_option("gc_form")] + f"<{(P + 15) // 16}>"

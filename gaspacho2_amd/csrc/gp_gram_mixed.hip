@@ -296,15 +296,17 @@ __global__ void __launch_bounds__(256) k_gram_mixed(int Npad, int P16, int T16, 
         }
 }
 
-// Psi_j[r,c] = DeltaInv[r,c] + 2^e_j 2^-s_r 2^-s_c sum_s part[s][j][blk][r',c'] for r <= c,
-// written to both triangles.
-__global__ void k_gram_reduce(int P, int nbk, int nseg, int J, const float *__restrict__ part,
+// Psi_j[R,C] = DeltaInv[R,C] + 2^e_j 2^-s_R 2^-s_C sum_s part[s][j][blk][r',c'] for r <= c,
+// written to both triangles; the partials cover a Pg x Pg block of the
+// design starting at column off (R = off + r, C = off + c): the whole design
+// (Pg = P, off = 0) or its GP block (Pg = M, off = 1).
+__global__ void k_gram_reduce(int P, int Pg, int off, int nbk, int nseg, int J, const float *__restrict__ part,
                               const double *__restrict__ wun, const double *__restrict__ zun,
                               const double *__restrict__ DeltaInv, double *__restrict__ Psi) {
     const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
     const int j = blockIdx.y;
-    if (idx >= (size_t)P * P) return;
-    const int r = (int)(idx % P), c = (int)(idx / P);
+    if (idx >= (size_t)Pg * Pg) return;
+    const int r = (int)(idx % Pg), c = (int)(idx / Pg);
     if (r > c) return;
     const int bi = r / BLK, bj = c / BLK;
     const int bp = bi * nbk - bi * (bi - 1) / 2 + (bj - bi);
@@ -313,10 +315,11 @@ __global__ void k_gram_reduce(int P, int nbk, int nseg, int J, const float *__re
     double s = 0.0;
     for (int g = 0; g < nseg; g++)
         s += (double)part[(((size_t)g * J + j) * nbp + bp) * (size_t)(BLK * BLK) + inblk];
-    const double v = s * (wun[j] * zun[r] * zun[c]) + DeltaInv[r + (size_t)c * P];
+    const int R = off + r, C = off + c;
+    const double v = s * (wun[j] * zun[R] * zun[C]) + DeltaInv[R + (size_t)C * P];
     double *Pj = Psi + (size_t)j * P * P;
-    Pj[r + (size_t)c * P] = v;
-    Pj[c + (size_t)r * P] = v;
+    Pj[R + (size_t)C * P] = v;
+    Pj[C + (size_t)R * P] = v;
 }
 
 
@@ -989,6 +992,29 @@ static int launch_gram_sym(gp_ctx *c, int seg, int nseg, int batch, const float 
     return 0;
 }
 
+// The intercept row and column of the peeled forms: GRK_SEG cell segments
+// of z32^T w32 (FP32 products over Npad / GRK_SEG cells each), summed in FP64
+static int gram_intercept_row(gp_ctx *c, int batch, const double *Delta, double *Psi) {
+    const size_t rp = (size_t)GRK_SEG * c->P * batch;
+    if (c->gram_row0_len < rp) {
+        if (c->gram_row0) (void)hipFree(c->gram_row0);
+        c->gram_row0 = nullptr;
+        GP_HIP(hipMalloc((void **)&c->gram_row0, rp * sizeof(float)));
+        c->gram_row0_len = rp;
+    }
+    const int kseg = c->Npad / GRK_SEG;
+    const float one = 1.0f, zero = 0.0f;
+    GP_BLAS(rocblas_sgemm_strided_batched(c->blas, rocblas_operation_transpose, rocblas_operation_none,
+                                          c->P, batch, kseg, &one, c->z32, c->Npad, kseg, c->w32,
+                                          c->Npad, kseg, &zero, c->gram_row0, c->P,
+                                          (rocblas_stride)c->P * batch, GRK_SEG));
+    const size_t n0 = (size_t)c->P * batch;
+    k_gram_row0<<<(unsigned)((n0 + 255) / 256), 256, 0, c->stream>>>(c->P, batch, GRK_SEG,
+                                                                    c->gram_row0, c->zun, Delta, Psi);
+    GP_HIP(hipGetLastError());
+    return 0;
+}
+
 int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const double *Zsrc,
                    const double *Delta, bool weights_ready) {
     const int T16 = (c->P + 15) / 16, P16 = 16 * T16;
@@ -1042,25 +1068,27 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
         k_gram_reduce_sym<<<gr, 256, 0, c->stream>>>(c->P, TG, nseg, batch, 1, c->gram_part, c->wun,
                                                       c->zun, Delta, Psi);
         GP_HIP(hipGetLastError());
-        // intercept row: GRK_SEG cell segments of z32^T w32 (FP32), summed in FP64
-        const size_t rp = (size_t)GRK_SEG * c->P * batch;
-        if (c->gram_row0_len < rp) {
-            if (c->gram_row0) (void)hipFree(c->gram_row0);
-            c->gram_row0 = nullptr;
-            GP_HIP(hipMalloc((void **)&c->gram_row0, rp * sizeof(float)));
-            c->gram_row0_len = rp;
-        }
-        const int kseg = c->Npad / GRK_SEG;
-        const float one = 1.0f, zero = 0.0f;
-        GP_BLAS(rocblas_sgemm_strided_batched(c->blas, rocblas_operation_transpose, rocblas_operation_none,
-                                              c->P, batch, kseg, &one, c->z32, c->Npad, kseg, c->w32,
-                                              c->Npad, kseg, &zero, c->gram_row0, c->P,
-                                              (rocblas_stride)c->P * batch, GRK_SEG));
-        const size_t n0 = (size_t)c->P * batch;
-        k_gram_row0<<<(unsigned)((n0 + 255) / 256), 256, 0, c->stream>>>(c->P, batch, GRK_SEG,
-                                                                        c->gram_row0, c->zun, Delta, Psi);
+        return gram_intercept_row(c, batch, Delta, Psi);
+    }
+    // the same peeling for designs wider than the symmetric kernel's tiles
+    // (M = 512): the tiled general kernel over the GP block of the split
+    // model matrix (columns 1..M of Zh / Zl), its reduction written at offset 1
+    if (c->gram_peel && c->g0 == 1 && c->M % 16 == 0 && TG > MAX_SYM_T16 && c->Npad % GRK_SEG == 0) {
+        const int nbk_g = (c->M + BLK - 1) / BLK;
+        const int nbp_g = nbk_g * (nbk_g + 1) / 2;
+        gp_timer_begin(c, c->t_gramk);
+        dim3 g(nbp_g, batch, nseg);
+        k_gram_mixed<<<g, 256, 0, c->stream>>>(c->Npad, c->M, TG, nbk_g, seg,
+                                                (const _Float16 *)c->Zh + c->Npad,
+                                                (const _Float16 *)c->Zl + c->Npad, c->w32, c->wsc,
+                                                c->gram_part);
         GP_HIP(hipGetLastError());
-        return 0;
+        gp_timer_end(c, c->t_gramk);
+        dim3 gr((unsigned)(((size_t)c->M * c->M + 255) / 256), batch);
+        k_gram_reduce<<<gr, 256, 0, c->stream>>>(c->P, c->M, 1, nbk_g, nseg, batch, c->gram_part, c->wun,
+                                                  c->zun, Delta, Psi);
+        GP_HIP(hipGetLastError());
+        return gram_intercept_row(c, batch, Delta, Psi);
     }
     if (T16 <= MAX_SYM_T16) {
         int r = 0;
@@ -1097,7 +1125,7 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
                                             (const _Float16 *)c->Zl, c->w32, c->wsc, c->gram_part);
     GP_HIP(hipGetLastError());
     dim3 gr((unsigned)(((size_t)c->P * c->P + 255) / 256), batch);
-    k_gram_reduce<<<gr, 256, 0, c->stream>>>(c->P, nbk, nseg, batch, c->gram_part, c->wun,
+    k_gram_reduce<<<gr, 256, 0, c->stream>>>(c->P, c->P, 0, nbk, nseg, batch, c->gram_part, c->wun,
                                               c->zun, Delta, Psi);
     GP_HIP(hipGetLastError());
     return 0;

@@ -462,7 +462,7 @@ def test_contraction_intercept_peeled_form(shape):
 
 
 @pytest.mark.parametrize("shape", [(2000, 24, 5, 32), (3000, 24, 10, 256), (3000, 16, 4, 64),
-                                   (2500, 20, 6, 160)])
+                                   (2500, 20, 6, 160), (2500, 12, 6, 320)])
 def test_gram_intercept_peeled_form(shape):
     """The intercept-peeled mixed Gram (symmetric FP16 hi/lo Gram over the GP
     block only, the intercept row and column from split-K FP32 products
@@ -470,7 +470,9 @@ def test_gram_intercept_peeled_form(shape):
     and against the FP64 Gram of the same whitened basis: entries within
     5e-7 of each gene's largest (as the whole-design form), the ELBO within
     1e-7 of FP64; the eight- and four-wave forms of the peeled Gram give the
-    same bits."""
+    same bits.  M = 320 (21 GP tiles) is wider than the symmetric kernel's
+    tiles: both forms run the tiled general kernel there, the peeled one over
+    the GP block."""
     from gaspacho2_amd import lib, synth
     N, G, Q, M = shape
     ds = synth.make_dataset(N, G, Q, M, seed=11)
