@@ -32,6 +32,9 @@ namespace {
 
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef float f2 __attribute__((ext_vector_type(2)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+typedef unsigned u4 __attribute__((ext_vector_type(4)));
 
 constexpr int BLK = 128;     // output block edge
 constexpr int CH = 32;       // cells per chunk (one MFMA k-step)
@@ -149,6 +152,23 @@ __global__ void __launch_bounds__(256) k_wprep(int N, int Npad, const double *__
 // to itself the compiler folds the producing multiply into the conversions
 // (v_fma_mix), and that form loses accuracy in the lo part (measured: 3e-5
 // instead of 2e-7 relative error in the gradient contraction).
+// The same split two values at a time: each pair converts with one packed
+// round-to-nearest-even conversion (the bits of converting one at a time),
+// and hi goes back to FP32 from the packed halves, so no value is converted
+// to FP16 twice.  Results as 16-bit patterns, two per word.
+__device__ __forceinline__ void split8_pk(float (&v)[8], u4 &hi, u4 &lo) {
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        float a = v[2 * e], b = v[2 * e + 1];
+        asm volatile("" : "+v"(a), "+v"(b));
+        const unsigned hu = __builtin_bit_cast(unsigned, __builtin_convertvector((f2){a, b}, h2));
+        const float ha = (float)__builtin_bit_cast(_Float16, (unsigned short)(hu & 0xffffu));
+        const float hb = (float)__builtin_bit_cast(_Float16, (unsigned short)(hu >> 16));
+        hi[e] = hu;
+        lo[e] = __builtin_bit_cast(unsigned, __builtin_convertvector((f2){a - ha, b - hb}, h2));
+    }
+}
+
 __device__ inline void split8(const float *p, h8 &hi, h8 &lo) {
 #pragma unroll
     for (int e = 0; e < 8; e++) {
@@ -528,17 +548,13 @@ __device__ __forceinline__ void gram_wave(const GramArgs &g, _Float16 *Xs, float
             const int col = it / QC, q = it % QC;
             const f4 s0 = *(const f4 *)&wsl[wb][8 * q];
             const f4 s1 = *(const f4 *)&wsl[wb][8 * q + 4];
-            h8 hi, lo;
+            float v[8];
 #pragma unroll
-            for (int e = 0; e < 8; e++) {
-                float v = (e < 4 ? pz[r][0][e] * s0[e] : pz[r][1][e - 4] * s1[e - 4]);
-                asm volatile("" : "+v"(v));
-                const _Float16 h = (_Float16)v;
-                hi[e] = h;
-                lo[e] = (_Float16)(v - (float)h);
-            }
-            *(h8 *)&Xs[((buf * 2 + 0) * P16 + col) * GLDX + 8 * q] = hi;
-            *(h8 *)&Xs[((buf * 2 + 1) * P16 + col) * GLDX + 8 * q] = lo;
+            for (int e = 0; e < 8; e++) v[e] = (e < 4 ? pz[r][0][e] * s0[e] : pz[r][1][e - 4] * s1[e - 4]);
+            u4 hi, lo;
+            split8_pk(v, hi, lo);
+            *(u4 *)&Xs[((buf * 2 + 0) * P16 + col) * GLDX + 8 * q] = hi;
+            *(u4 *)&Xs[((buf * 2 + 1) * P16 + col) * GLDX + 8 * q] = lo;
         }
     };
     auto sqw = [&](int ch) { return sqrtf(g.wj[g.c0 + ch * GCH + tid] * g.sc); };   // tid < GCH
@@ -762,17 +778,13 @@ __device__ __forceinline__ void gram_wave4(const GramArgs &g, _Float16 *Xs, floa
             const int col = it >> 2, q = it & 3;
             const f4 s0 = *(const f4 *)&wsl[wb][8 * q];
             const f4 s1 = *(const f4 *)&wsl[wb][8 * q + 4];
-            h8 hi, lo;
+            float v[8];
 #pragma unroll
-            for (int e = 0; e < 8; e++) {
-                float v = (e < 4 ? pz[r][0][e] * s0[e] : pz[r][1][e - 4] * s1[e - 4]);
-                asm volatile("" : "+v"(v));
-                const _Float16 h = (_Float16)v;
-                hi[e] = h;
-                lo[e] = (_Float16)(v - (float)h);
-            }
-            *(h8 *)&Xs[((buf * 2 + 0) * P16 + col) * LDX + 8 * q] = hi;
-            *(h8 *)&Xs[((buf * 2 + 1) * P16 + col) * LDX + 8 * q] = lo;
+            for (int e = 0; e < 8; e++) v[e] = (e < 4 ? pz[r][0][e] * s0[e] : pz[r][1][e - 4] * s1[e - 4]);
+            u4 hi, lo;
+            split8_pk(v, hi, lo);
+            *(u4 *)&Xs[((buf * 2 + 0) * P16 + col) * LDX + 8 * q] = hi;
+            *(u4 *)&Xs[((buf * 2 + 1) * P16 + col) * LDX + 8 * q] = lo;
         }
     };
     auto sqw = [&](int ch) { return sqrtf(g.wj[g.c0 + ch * CH + (tid & (CH - 1))] * g.sc); };
