@@ -263,7 +263,7 @@ extern "C" int gp_set_option(gp_ctx *c, const char *name, int value) {
         if (value < 1 || value > 4096) { gp_set_error("gc_groups must be in [1, 4096]"); return 1; }
         c->gc_groups = value;
     } else if (n == "gram_waves") {
-        if (value != 4 && value != 8) { gp_set_error("gram_waves must be 4 or 8"); return 1; }
+        if (value != 4 && value != 8 && value != 12) { gp_set_error("gram_waves must be 4, 8 or 12"); return 1; }
         c->gram_waves = value;
     } else if (n == "gc_form") {
         if (value < 0 || value > 3) { gp_set_error("gc_form must be 0, 1, 2 or 3"); return 1; }
@@ -910,7 +910,7 @@ extern "C" int gp_autotune(gp_ctx *c, int verbose) {
     GP_TRY(ensure_grad_buffers(c));
     static const int splits[] = {8, 16, 32};
     static const int groups[] = {16, 32, 64};
-    static const int waves[] = {8, 4};
+    static const int waves[] = {8, 12, 4};
     static const int forms[] = {2, 1, 0};
     GP_TRY(gpk_expand_delta(c));
     if (!c->zw_valid) {

@@ -863,6 +863,214 @@ __global__ void __launch_bounds__(GW4 * 64) k_gram_sym4(int Npad, int seg, int n
     }
 }
 
+// ---- producer / consumer form (option gram_waves 12; 16 GP column tiles,
+// M = 256): 12 waves, three per SIMD.  Waves 8-11 (one per SIMD) only stage:
+// they hold the next chunk's z rows in registers, scale, split and store
+// them to the LDS buffer the consumers are not reading, then load the chunk
+// after.  Waves 0-7 (two per SIMD) only run MFMAs out of the other buffer,
+// so the staging arithmetic and its global loads overlap the matrix work
+// instead of alternating with it; one barrier per 64-cell chunk hands the
+// buffers over.  The 136 upper tiles are cut into units (whole 4x4 blocks,
+// diagonal blocks, and 4x2 / 2x4 halves of off-diagonal blocks) so that
+// every SIMD's two consumers carry 34 tiles; a wave loads each tile's
+// fragments it touches once per k-step (60 fragment pairs per k-step).
+// Every accumulator sees the MFMA sequence and operands of k_gram_sym, so
+// the two forms give the same bits.
+constexpr int PC_CW = 8;     // consumer waves
+constexpr int PC_PW = 4;     // producer waves
+constexpr int PC_T16 = 16;   // the column tiles this form is planned for
+
+struct PcUnit {
+    int ra, nr, cb, nc;      // row tiles ra .. ra + nr - 1 x column tiles cb .. cb + nc - 1 (nr = 0: none)
+};
+struct PcWave {
+    PcUnit u0, u1;
+};
+
+// wave w pairs with wave w + 4 on SIMD w mod 4: 18 + 16 = 34 tiles each
+constexpr PcWave kPcPlan[PC_CW] = {
+    {{0, 4, 0, 4}, {0, 4, 4, 2}},      // block (0,0); (0,1) columns 4-5
+    {{4, 4, 4, 4}, {4, 4, 8, 2}},      // block (1,1); (1,2) columns 8-9
+    {{8, 4, 8, 4}, {8, 4, 12, 2}},     // block (2,2); (2,3) columns 12-13
+    {{12, 4, 12, 4}, {0, 2, 12, 4}},   // block (3,3); (0,3) rows 0-1
+    {{0, 4, 8, 4}, {0, 0, 0, 0}},      // block (0,2)
+    {{4, 4, 12, 4}, {0, 0, 0, 0}},     // block (1,3)
+    {{0, 4, 6, 2}, {4, 4, 10, 2}},     // (0,1) columns 6-7; (1,2) columns 10-11
+    {{8, 4, 14, 2}, {2, 2, 12, 4}},    // (2,3) columns 14-15; (0,3) rows 2-3
+};
+
+constexpr unsigned pc_mask(const PcUnit &u) {
+    unsigned m = 0;
+    for (int t = 0; t < u.nr; t++) m |= 1u << (u.ra + t);
+    for (int t = 0; t < u.nc; t++) m |= 1u << (u.cb + t);
+    return u.nr ? m : 0u;
+}
+
+// one k-step of unit U from the wave's fragments (fh / fl indexed by tile)
+template <int RA, int NR, int CB, int NC>
+__device__ __forceinline__ void pc_unit(f4 (&acc)[4][4], const h8 (&fh)[PC_T16], const h8 (&fl)[PC_T16]) {
+    if constexpr (NR > 0) {
+        if constexpr (RA == CB) {          // diagonal block: upper triangle
+#pragma unroll
+            for (int x = 0; x < NR; x++)
+#pragma unroll
+                for (int y = x; y < NC; y++) mfma3(acc[x][y], fh[RA + x], fl[RA + x], fh[CB + y], fl[CB + y]);
+        } else {
+#pragma unroll
+            for (int y = 0; y < NC; y++)
+#pragma unroll
+                for (int x = 0; x < NR; x++) mfma3(acc[x][y], fh[RA + x], fl[RA + x], fh[CB + y], fl[CB + y]);
+        }
+    }
+}
+
+template <int RA, int NR, int CB, int NC>
+__device__ __forceinline__ void pc_unit_store(const f4 (&acc)[4][4], float *pg, int lane) {
+    if constexpr (NR > 0) {
+#pragma unroll
+        for (int x = 0; x < NR; x++)
+#pragma unroll
+            for (int y = (RA == CB ? x : 0); y < NC; y++) {
+                const int ta = RA + x, tb = CB + y;
+                const int t = ta * PC_T16 - ta * (ta - 1) / 2 + (tb - ta);
+                *(f4 *)(pg + (size_t)t * 256 + 4 * lane) = acc[x][y];
+            }
+    }
+}
+
+template <int W>
+__device__ __forceinline__ void pc_consumer(const GramArgs &g, const _Float16 *Xs) {
+    constexpr int P16 = 16 * PC_T16;
+    constexpr PcUnit U0 = kPcPlan[W].u0, U1 = kPcPlan[W].u1;
+    constexpr unsigned MASK = pc_mask(U0) | pc_mask(U1);
+    const int lane = threadIdx.x & 63;
+    const int lrow = lane & 15, lk = 8 * (lane >> 4);
+    f4 acc0[4][4], acc1[4][4];
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) {
+            acc0[x][y] = f4{0.f, 0.f, 0.f, 0.f};
+            acc1[x][y] = f4{0.f, 0.f, 0.f, 0.f};
+        }
+    __syncthreads();                              // chunk 0 staged
+    for (int ch = 0; ch < g.nch; ch++) {
+        const int buf = ch & 1;
+        const _Float16 *xh = Xs + (size_t)(buf * 2 + 0) * P16 * GLDX;
+        const _Float16 *xl = Xs + (size_t)(buf * 2 + 1) * P16 * GLDX;
+#pragma unroll
+        for (int s = 0; s < GKC; s++) {
+            h8 fh[PC_T16], fl[PC_T16];
+#pragma unroll
+            for (int t = 0; t < PC_T16; t++) {
+                if ((MASK >> t) & 1u) {              // folded: the loop is unrolled
+                    const int off = (t * 16 + lrow) * GLDX + lk + 32 * s;
+                    fh[t] = *(const h8 *)(xh + off);
+                    fl[t] = *(const h8 *)(xl + off);
+                }
+            }
+            pc_unit<U0.ra, U0.nr, U0.cb, U0.nc>(acc0, fh, fl);
+            pc_unit<U1.ra, U1.nr, U1.cb, U1.nc>(acc1, fh, fl);
+        }
+        __syncthreads();                          // this buffer read; the next one staged
+    }
+    pc_unit_store<U0.ra, U0.nr, U0.cb, U0.nc>(acc0, g.pg, lane);
+    pc_unit_store<U1.ra, U1.nr, U1.cb, U1.nc>(acc1, g.pg, lane);
+}
+
+// producer wave PW: columns 64 PW .. 64 PW + 63 of every chunk; lane l
+// handles cells 8 (l mod 8) .. + 7 of the columns 8 r + l / 8 (rounds r < 8),
+// so its eight square-root weights are the same in every round
+template <int PW>
+__device__ __forceinline__ void pc_producer(const GramArgs &g, _Float16 *Xs) {
+    constexpr int P16 = 16 * PC_T16;
+    constexpr int NRD = P16 / PC_PW / 8;         // rounds of 8 columns per wave
+    const int lane = threadIdx.x & 63;
+    const int q = lane & 7, cl = lane >> 3;
+    f4 pz[NRD][2];
+    f4 pw[2];
+    auto load = [&](int ch) {                     // chunk ch (clamped by the caller)
+        const int cell = g.c0 + ch * GCH + 8 * q;
+#pragma unroll
+        for (int r = 0; r < NRD; r++) {
+            const float *src = g.z32 + (size_t)(64 * PW + 8 * r + cl) * g.Npad + cell;
+            pz[r][0] = *(const f4 *)src;
+            pz[r][1] = *(const f4 *)(src + 4);
+        }
+        pw[0] = *(const f4 *)(g.wj + cell);
+        pw[1] = *(const f4 *)(g.wj + cell + 4);
+    };
+    auto stage = [&](int buf) {
+        float sw[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) sw[e] = sqrtf((e < 4 ? pw[0][e] : pw[1][e - 4]) * g.sc);
+#pragma unroll
+        for (int r = 0; r < NRD; r++) {
+            const int col = 64 * PW + 8 * r + cl;
+            float v[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) v[e] = (e < 4 ? pz[r][0][e] : pz[r][1][e - 4]) * sw[e];
+            u4 hi, lo;
+            split8_pk(v, hi, lo);
+            *(u4 *)&Xs[((buf * 2 + 0) * P16 + col) * GLDX + 8 * q] = hi;
+            *(u4 *)&Xs[((buf * 2 + 1) * P16 + col) * GLDX + 8 * q] = lo;
+        }
+    };
+    load(0);
+    stage(0);
+    load(min(1, g.nch - 1));
+    __syncthreads();                              // chunk 0 staged
+    for (int ch = 0; ch < g.nch; ch++) {
+        // chunk ch + 1 into the buffer the consumers are not reading (past
+        // the last chunk: a copy of it, never read), then the loads of ch + 2
+        stage((ch + 1) & 1);
+        load(min(ch + 2, g.nch - 1));
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__((PC_CW + PC_PW) * 64) k_gram_pc(int Npad, int seg, int nseg, int J,
+        const float *__restrict__ z32, const float *__restrict__ w32,
+        const float *__restrict__ wsc, float *__restrict__ part) {
+    constexpr int NT = PC_T16 * (PC_T16 + 1) / 2;
+    extern __shared__ __attribute__((aligned(16))) _Float16 Xs[];   // [2 buf][2 hl][P16][GLDX]
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int sgi, j;
+    {
+        const int id = blockIdx.x;
+        if (nseg % 8 == 0) {
+            const int xcd = id % 8, loc = id / 8, nsx = nseg / 8;
+            sgi = xcd + 8 * (loc % nsx);
+            j = loc / nsx;
+        } else {
+            sgi = id % nseg;
+            j = id / nseg;
+        }
+    }
+    GramArgs g;
+    g.Npad = Npad;
+    g.c0 = sgi * seg;
+    g.nch = (min(Npad, g.c0 + seg) - g.c0) / GCH;
+    g.sc = wsc[j];
+    g.z32 = z32;
+    g.wj = w32 + (size_t)j * Npad;
+    g.pg = part + ((size_t)sgi * J + j) * NT * 256;
+    switch (w) {
+    case 0: pc_consumer<0>(g, Xs); break;
+    case 1: pc_consumer<1>(g, Xs); break;
+    case 2: pc_consumer<2>(g, Xs); break;
+    case 3: pc_consumer<3>(g, Xs); break;
+    case 4: pc_consumer<4>(g, Xs); break;
+    case 5: pc_consumer<5>(g, Xs); break;
+    case 6: pc_consumer<6>(g, Xs); break;
+    case 7: pc_consumer<7>(g, Xs); break;
+    case 8: pc_producer<0>(g, Xs); break;
+    case 9: pc_producer<1>(g, Xs); break;
+    case 10: pc_producer<2>(g, Xs); break;
+    default: pc_producer<3>(g, Xs); break;
+    }
+}
+
 constexpr int RSG = 4;       // genes per block of the symmetric reduction
 constexpr int GRK_SEG = 64;  // cell segments of the intercept row's FP32 products
 
@@ -985,6 +1193,18 @@ template <int T16>
 static int launch_gram_sym(gp_ctx *c, int seg, int nseg, int batch, const float *zsrc) {
     constexpr int P16 = 16 * T16;
     static_assert(plan_fits(T16), "block plan does not fit the waves");
+    if constexpr (T16 == PC_T16) {
+        if (c->gram_waves == 12) {
+            const size_t shm = (size_t)2 * 2 * P16 * GLDX * sizeof(_Float16);
+            if (!c->capturing)
+                GP_HIP(hipFuncSetAttribute((const void *)k_gram_pc,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+            k_gram_pc<<<(unsigned)(batch * nseg), (PC_CW + PC_PW) * 64, shm, c->stream>>>(
+                c->Npad, seg, nseg, batch, zsrc, c->w32, c->wsc, c->gram_part);
+            GP_HIP(hipGetLastError());
+            return 0;
+        }
+    }
     if (c->gram_waves == 4) {
         const size_t shm = (size_t)2 * 2 * P16 * LDX * sizeof(_Float16);
         if (shm > 65536 && !c->capturing)   // set before any recording: not a stream operation

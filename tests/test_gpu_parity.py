@@ -469,8 +469,8 @@ def test_gram_intercept_peeled_form(shape):
     Zw^T (W / Omega) summed in FP64) against the Gram over the whole design
     and against the FP64 Gram of the same whitened basis: entries within
     5e-7 of each gene's largest (as the whole-design form), the ELBO within
-    1e-7 of FP64; the eight- and four-wave forms of the peeled Gram give the
-    same bits.  M = 320 (21 GP tiles) is wider than the symmetric kernel's
+    1e-7 of FP64; the eight-, four- and (16 GP tiles) twelve-wave
+    producer/consumer forms of the peeled Gram give the same bits.  M = 320 (21 GP tiles) is wider than the symmetric kernel's
     tiles: both forms run the tiled general kernel there, the peeled one over
     the GP block."""
     from gaspacho2_amd import lib, synth
@@ -488,7 +488,8 @@ def test_gram_intercept_peeled_form(shape):
     for name, prec, opts in (("f64", lib.PREC_FP64_WHITENED, {}),
                              ("full", lib.PREC_MIXED, {"gram_peel": 0}),
                              ("peel", lib.PREC_MIXED, {"gram_peel": 1}),
-                             ("peel4", lib.PREC_MIXED, {"gram_peel": 1, "gram_waves": 4})):
+                             ("peel4", lib.PREC_MIXED, {"gram_peel": 1, "gram_waves": 4}),
+                             ("peel12", lib.PREC_MIXED, {"gram_peel": 1, "gram_waves": 12})):
         g.set_precision(prec)
         for k, v in opts.items():
             g.set_option(k, v)
@@ -503,6 +504,7 @@ def test_gram_intercept_peeled_form(shape):
     assert errs["peel"] < 5e-7 and errs["full"] < 5e-7
     assert abs(out["peel"][0] - e64) <= 1e-7 * abs(e64)
     assert np.array_equal(out["peel4"][1], out["peel"][1]) and out["peel4"][0] == out["peel"][0]
+    assert np.array_equal(out["peel12"][1], out["peel"][1]) and out["peel12"][0] == out["peel"][0]
     g.close()
 
 
