@@ -898,7 +898,7 @@ constexpr int gc_gp_waves() { return (8 / RG) * (TG / CG); }
 template <int TG, int RG, int CG, int PD>
 __global__ void __launch_bounds__((gc_gp_waves<TG, RG, CG>() * 64))
 k_gc_gp(int N, int Npad, int gsize, int J,
-        const double *__restrict__ Zg, const float *__restrict__ w32,
+        const float *__restrict__ zg, const float *__restrict__ w32,
         const int *__restrict__ perm, const int *__restrict__ aexp, const int *__restrict__ zexp,
         const _Float16 *__restrict__ shi, const _Float16 *__restrict__ slo,
         const float *__restrict__ t0, float *__restrict__ part) {
@@ -921,23 +921,22 @@ k_gc_gp(int N, int Npad, int gsize, int J,
     _Float16 *Ah = As, *Al = As + (size_t)GRC * LDA;
     // A = g (the GP block of zw, scaled by 2^-zc), split hi/lo, once for the
     // group; every load of an item is issued before any is used (a row past
-    // N reads row N - 1 and is zeroed)
+    // N reads row N - 1 and is zeroed).  zg is the Gram's FP32 copy of the
+    // same columns, zw 2^(14 - zc) rounded once (its column scale comes from
+    // the same maximum's exponent), so zg 2^-14 is (float)(zw 2^-zc) exactly
+    // and A is read at 4 bytes per element instead of 8.
 #pragma unroll
     for (int r = 0; r < IPT; r++) {
         const int it = min(tid + NW * 64 * r, NITEM - 1);
         const int cl = it % GRC, k0 = 8 * (it / GRC);
         const int cell = cb + cl, cellc = min(cell, N - 1);
-        double zv[8];
-        int ze[8];
+        float zv[8];
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            zv[e] = Zg[cellc + (size_t)(k0 + e) * Npad];
-            ze[e] = zexp[k0 + e];
-        }
+        for (int e = 0; e < 8; e++) zv[e] = zg[cellc + (size_t)(k0 + e) * Npad];
         h8 hi, lo;
 #pragma unroll
         for (int e = 0; e < 8; e++) {
-            float v = (cell < N) ? (float)ldexp(zv[e], -ze[e]) : 0.0f;
+            float v = (cell < N) ? zv[e] * 0x1p-14f : 0.0f;
             asm volatile("" : "+v"(v));
             const _Float16 h = (_Float16)v;
             hi[e] = h;
@@ -1112,7 +1111,7 @@ static int launch_gc_gp(gp_ctx *c, int gsize, int ngroup) {
         GP_HIP(hipFuncSetAttribute((const void *)k_gc_gp<TG, RG, CG, PD>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     dim3 g(c->Npad / GRC, ngroup);
-    k_gc_gp<TG, RG, CG, PD><<<g, gc_gp_waves<TG, RG, CG>() * 64, shm, c->stream>>>(c->N, c->Npad, gsize, c->J, c->Zw + c->Npad,
+    k_gc_gp<TG, RG, CG, PD><<<g, gc_gp_waves<TG, RG, CG>() * 64, shm, c->stream>>>(c->N, c->Npad, gsize, c->J, c->z32 + c->Npad,
                                                            c->w32, c->gc_perm, c->gc_aexp, c->gc_zexp + 1,
                                                            (const _Float16 *)c->Sh, (const _Float16 *)c->Sl,
                                                            c->gc_t0, c->gc_part);
@@ -1158,6 +1157,7 @@ static bool gc_peel_ok(const gp_ctx *c) {
 
 static int gradcontract_gp(gp_ctx *c, int gsize, int ngroup) {
     const int M = c->M, J = c->J, TG = M / 16;
+    if (!c->zsplit_valid) GP_TRY(gpk_split_z(c, c->Zw));     // the FP32 copy of zw the kernel reads
     if (!c->gc_t0) {
         GP_HIP(hipMalloc((void **)&c->gc_t0, (size_t)J * M * sizeof(float)));
         GP_HIP(hipMalloc((void **)&c->gc_s0, (size_t)J * (M + 1) * sizeof(float)));
