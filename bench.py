@@ -227,6 +227,20 @@ def _run_fit(args, ds, prec):
     return r
 
 
+def _lbfgs_phases(o, kind):
+    """L-BFGS steps and last relative change of each outer iteration's phase
+    from a fit trace's objectives and kinds (a phase starts at its
+    quasi-likelihood start, kind 1)."""
+    o, kind = np.asarray(o), np.asarray(kind)
+    starts = list(np.flatnonzero(kind == 1)) + [len(kind)]
+    out = []
+    for a, b in zip(starts[:-1], starts[1:]):
+        v = o[a:b][(kind[a:b] == 1) | (kind[a:b] == 2)]
+        d = abs(v[-1] - v[-2]) / abs(v[-1]) if len(v) > 1 else float("nan")
+        out.append({"steps": int(len(v) - 1), "last_rel_change": float(d), "converged": bool(d < 1e-8)})
+    return out
+
+
 def _phases(o, info):
     """Per outer iteration: IRLS and L-BFGS steps, and whether each phase
     ended on its convergence rule (1e-7 IRLS, 1e-8 L-BFGS relative change of
@@ -552,6 +566,25 @@ def main_warmstart(args):
     n_pca = int(np.sum(tr_p[2] == 2))
     first = lambda tr: float(tr[0][np.argmax(tr[2] == 1)])
     log(f"fresh seed, PCA start: {obj_p:.10e} after {n_pca} L-BFGS steps ({dt_p:.1f} s)")
+
+    def procrustes(a, b):
+        U, _, Vt = np.linalg.svd(a.T @ b)
+        return float(np.linalg.norm(a @ (U @ Vt) - b) / np.linalg.norm(b))
+    ref = None
+    if args.fp64_ref:
+        # the FP64 fit of the fresh seed from its PCA latent, same schedule: the
+        # reference the mixed fits are measured against
+        X64, obj64, dt64, tr64 = warmstart.fit_from(ds, ds.X0, T0, args.fit_itr, args.fit_cap,
+                                                    precision=lib.PREC_FP64, ard_only=args.kernel == "ard")
+        n64, t64 = warmstart.steps_to_reach(tr_p, obj64)
+        ref = {"final_objective": obj64, "fit_seconds": dt64, "lbfgs_steps": int(np.sum(tr64[2] == 2)),
+               "lbfgs_phases": _lbfgs_phases(tr64[0], tr64[2]),
+               "pca_mixed": {"final_rel_gap": (obj_p - obj64) / abs(obj64),
+                             "latent_procrustes": procrustes(Xp, X64),
+                             "reaches_fp64_final": {"lbfgs_steps": n64, "seconds": t64,
+                                                    "speedup": (dt64 / t64) if t64 else None}}}
+        log(f"fresh seed, FP64 plain from PCA: {obj64:.10e} after {ref['lbfgs_steps']} L-BFGS steps "
+            f"({dt64:.1f} s); the PCA-started mixed fit reaches it after {n64} steps ({t64} s)")
     variants = {}
     for target in targets:
         t1 = time.perf_counter()
@@ -568,14 +601,25 @@ def main_warmstart(args):
         n_hit, t_hit = warmstart.steps_to_reach(tr_e, obj_p)
         log(f"{target}: warm-started fit {obj_e:.10e} ({dt_e:.1f} s), reaches the PCA-started final after "
             f"{n_hit} of {n_pca} L-BFGS steps")
+        vs64 = None
+        if ref is not None:
+            n6, t6 = warmstart.steps_to_reach(tr_e, ref["final_objective"])
+            vs64 = {"final_rel_gap": (obj_e - ref["final_objective"]) / abs(ref["final_objective"]),
+                    "latent_procrustes": procrustes(Xe, X64),
+                    "reaches_fp64_final": {"lbfgs_steps": n6, "seconds": t6,
+                                           "speedup": (ref["fit_seconds"] / t6) if t6 else None}}
+            log(f"{target}: reaches the FP64 fit's final after {n6} L-BFGS steps ({t6} s): "
+                f"{vs64['reaches_fp64_final']['speedup']}x less wall time than the FP64 fit")
         variants[target] = {
             "steps_saved": (n_pca - n_hit) if n_hit else None,
             "final_objective": obj_e, "rel_gap_vs_pca_start": (obj_e - obj_p) / abs(obj_p),
             "objective_after_ql_start": first(tr_e), "fit_seconds": dt_e,
             "lbfgs_steps": int(np.sum(tr_e[2] == 2)), "latent_procrustes_vs_pca_start": x_proc,
+            "lbfgs_phases": _lbfgs_phases(tr_e[0], tr_e[2]),
             "reaches_pca_final": {"lbfgs_steps": n_hit, "seconds": t_hit},
             "encoder_train_seconds": t_train, "encoder_epoch_loss": [float(losses[0]), float(losses[-1])],
-            "warm_start_rms_to_pca_fit": float(np.sqrt(np.mean((X0e - Xp) ** 2)))}
+            "warm_start_rms_to_pca_fit": float(np.sqrt(np.mean((X0e - Xp) ** 2))),
+            "vs_fp64_plain": vs64}
     head = variants[targets[0]]
     out = {"metric": "L-BFGS steps saved by the encoder warm start (fresh seed, mixed fit)",
            "value": head["steps_saved"], "unit": "steps", "n_gpus": 1,
@@ -588,8 +632,9 @@ def main_warmstart(args):
                                   f"(training fits: {t_itr}, {t_cap})",
                       "target": targets[0]},
            "pca_start": {"final_objective": obj_p, "objective_after_ql_start": first(tr_p), "fit_seconds": dt_p,
-                         "lbfgs_steps": n_pca, "pca_rms_to_pca_fit": float(np.sqrt(np.mean((ds.X0 - Xp) ** 2)))},
-           "variants": variants}
+                         "lbfgs_steps": n_pca, "lbfgs_phases": _lbfgs_phases(tr_p[0], tr_p[2]),
+                         "pca_rms_to_pca_fit": float(np.sqrt(np.mean((ds.X0 - Xp) ** 2)))},
+           "fp64_plain": ref, "variants": variants}
     print(json.dumps(out), flush=True)
 
 
@@ -626,6 +671,8 @@ def main():
                     help="warmstart workload: outer iterations of the training fits (default --fit-itr)")
     ap.add_argument("--train-cap", type=int, default=None,
                     help="warmstart workload: L-BFGS cap of the training fits (default --fit-cap)")
+    ap.add_argument("--fp64-ref", action="store_true",
+                    help="warmstart workload: also fit the fresh seed in FP64 and measure the mixed fits against it")
     ap.add_argument("--enc-epochs", type=int, default=30, help="warmstart workload: encoder epochs")
     ap.add_argument("--enc-target", default="residual,aligned",
                     help="warmstart workload: encoder targets, comma-separated (first = headline)")

@@ -535,7 +535,7 @@ def test_autotune_picks_valid_settings_and_keeps_results():
     g.set_precision(lib.PREC_MIXED)
     choice = g.autotune()
     assert choice["gram_splits"] in (8, 16, 32) and choice["gc_groups"] in (16, 32, 64)
-    assert choice["gram_waves"] in (4, 8) and choice["gc_form"] in (0, 1, 2, 3)
+    assert choice["gram_waves"] in (4, 8, 12) and choice["gc_form"] in (0, 1, 2, 3)
     em = g.elbo(1)
     g.grad()
     assert abs(em - e64) <= 1e-7 * abs(e64)
