@@ -35,10 +35,15 @@ inline unsigned nblk(size_t n, int nt = NT) { return (unsigned)((n + nt - 1) / n
 
 // ---------------------------------------------------------------- Kuf / Kmm
 // kernel_funcs.cu:61-97 (periodic dim 0 + SE-ARD) and 221-255 (SE-ARD only).
-// 256 cells x 8 inducing points per block; T rows staged in LDS.
+// 256 cells x 8 inducing points per block.  The block's rows of the latent X
+// (Q columns of 256 consecutive cells: one coalesced 2 KB read per column)
+// and the 8 inducing points' rows of T are staged in LDS, so each X value
+// is read from HBM once per block and from LDS by its cell's thread for
+// every inducing point; the arithmetic is the reference's, term by term.
 __global__ void __launch_bounds__(NT) k_kuf(int N, int Npad, int M, int Q,
         const double *__restrict__ X, const double *__restrict__ T,
         const double *__restrict__ rho, int ard, double *__restrict__ Knm) {
+    extern __shared__ double Xs[];               // [Q][NT]
     __shared__ double Ts[8][64];
     __shared__ double rh[64];
     const int i = blockIdx.x * NT + threadIdx.x;
@@ -48,8 +53,10 @@ __global__ void __launch_bounds__(NT) k_kuf(int N, int Npad, int M, int Q,
         Ts[mm][q] = (m0 + mm < M) ? T[(m0 + mm) + (size_t)q * M] : 0.0;
     }
     for (int q = threadIdx.x; q < Q; q += NT) rh[q] = rho[q];
+    for (int q = 0; q < Q; q++) Xs[q * NT + threadIdx.x] = (i < N) ? X[i + (size_t)q * N] : 0.0;
     __syncthreads();
     if (i >= Npad) return;
+    const double *xi = Xs + threadIdx.x;
     for (int mm = 0; mm < 8; mm++) {
         const int m = m0 + mm;
         if (m >= M) break;
@@ -58,12 +65,12 @@ __global__ void __launch_bounds__(NT) k_kuf(int N, int Npad, int M, int Q,
             double sum = 0.0, t;
             int k0 = 0;
             if (!ard) {
-                t = sin((X[i] - Ts[mm][0]) / 2.0);
+                t = sin((xi[0] - Ts[mm][0]) / 2.0);
                 sum += t * t / rh[0];
                 k0 = 1;
             }
             for (int k = k0; k < Q; k++) {
-                t = X[i + (size_t)k * N] - Ts[mm][k];
+                t = xi[k * NT] - Ts[mm][k];
                 sum += t * t / 2.0 / rh[k];
             }
             v = exp(-sum);
@@ -818,7 +825,10 @@ int gpk_fill(gp_ctx *c, double *p, size_t n, double v) {
 
 int gpk_kuf(gp_ctx *c) {
     dim3 g(nblk(c->Npad), (c->M + 7) / 8);
-    k_kuf<<<g, NT, 0, c->stream>>>(c->N, c->Npad, c->M, c->Q, c->X, c->T, c->rho,
+    const size_t shm = (size_t)c->Q * NT * sizeof(double);      // Q <= 64: at most 128 KB
+    if (shm > 65536 && !c->capturing)    // set before any recording (init_fit runs first)
+        GP_HIP(hipFuncSetAttribute((const void *)k_kuf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    k_kuf<<<g, NT, shm, c->stream>>>(c->N, c->Npad, c->M, c->Q, c->X, c->T, c->rho,
                                    c->ard_only, c->Z + (size_t)c->g0 * c->Npad);
     LAUNCH_CHECK();
     k_kmm<<<nblk((size_t)c->M * c->M), NT, 0, c->stream>>>(c->M, c->Q, c->T, c->rho,

@@ -44,6 +44,21 @@ def test_init_fit(small_ds):
         _cmp(o, g, name)
 
 
+@pytest.mark.parametrize("Q", [40, 64])
+def test_init_fit_wide_latent(Q):
+    """Kuf, Kmm and the basis for a latent of Q = 40 and 64 dimensions (the
+    block's X rows in LDS take 80 and 128 KB there) against the oracle."""
+    from gaspacho2_amd import synth
+    ds = synth.make_dataset(700, 6, Q, 24, seed=4, pca=False)
+    for ard in (False, True):
+        o, g = _pair(ds, ard_only=ard)
+        o.init_fit()
+        g.init_fit()
+        for name in ("Z", "K", "Kinv", "KinvKmn", "titsias"):
+            _cmp(o, g, name)
+        g.close()
+
+
 @pytest.mark.parametrize("ard", [False, True])
 def test_irls_and_quasi_likelihood_start(small_ds, ard):
     o, g = _pair(small_ds, ard_only=ard)
