@@ -574,8 +574,10 @@ def main_warmstart(args):
     if args.fp64_ref:
         # the FP64 fit of the fresh seed from its PCA latent, same schedule: the
         # reference the mixed fits are measured against
+        log("fresh seed, FP64 plain fit from PCA (every objective printed)")
         X64, obj64, dt64, tr64 = warmstart.fit_from(ds, ds.X0, T0, args.fit_itr, args.fit_cap,
-                                                    precision=lib.PREC_FP64, ard_only=args.kernel == "ard")
+                                                    precision=lib.PREC_FP64, ard_only=args.kernel == "ard",
+                                                    verbose=True)
         n64, t64 = warmstart.steps_to_reach(tr_p, obj64)
         ref = {"final_objective": obj64, "fit_seconds": dt64, "lbfgs_steps": int(np.sum(tr64[2] == 2)),
                "lbfgs_phases": _lbfgs_phases(tr64[0], tr64[2]),

@@ -33,14 +33,15 @@ def fitted_latent(ctx: lib.GPContext, N: int, Q: int) -> np.ndarray:
 
 
 def fit_from(ds, X0: np.ndarray, T0: np.ndarray, itrmax: int, jtr_cap: int,
-             precision: int = lib.PREC_MIXED, ard_only: bool = False):
+             precision: int = lib.PREC_MIXED, ard_only: bool = False, verbose: bool = False):
     """gp_fit from the starting latent X0; returns (X, objective, seconds,
-    trace) with trace = (objectives, seconds, kinds) of gp_fit_trace."""
+    trace) with trace = (objectives, seconds, kinds) of gp_fit_trace
+    (verbose: the library prints every objective as the fit runs)."""
     g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls,
                       np.asfortranarray(X0), T0, ard_only=ard_only, precision=precision)
     try:
         t = time.perf_counter()
-        obj = g.fit(itrmax, jtr_cap)
+        obj = g.fit(itrmax, jtr_cap, verbose=verbose)
         dt = time.perf_counter() - t
         return fitted_latent(g, ds.N, ds.Q), obj, dt, g.fit_trace()
     finally:
