@@ -28,9 +28,8 @@
 namespace {
 
 constexpr int ST = 256;       // threads per workgroup
-constexpr int TT = 32;        // tests per chunk (the model matrix is read once per chunk)
-constexpr int NBK = 64;       // cell-tile workgroups per chunk
-constexpr int FST = 64;       // threads of the finishing workgroup (one wave: cheap barriers)
+constexpr int TT = 16;        // tests per chunk
+constexpr int NBK = 128;      // cell-tile workgroups per chunk
 constexpr int CT = 64;        // cells per staged tile
 constexpr int PMAX = 1024;    // widest design handled (model-matrix columns)
 
@@ -113,7 +112,7 @@ __global__ void __launch_bounds__(ST) k_score_acc(int N, int Npad, int P, int J,
     }
     __syncthreads();
     double *pb = part + ((size_t)blockIdx.y * NBK + blockIdx.x) * TT * (P + 2);
-    // staging map: thread -> cell tid % 64 of the tile, tests w, w + 4, ..., w + TT - 4 (w = wave)
+    // staging map: thread -> cell tid % 64 of the tile, tests w, w+4, w+8, w+12 (w = wave)
     const int ci = tid & (CT - 1), w = tid >> 6;
     const int ntile = Npad / CT;
     for (int k0 = 0; k0 < P; k0 += ST) {
@@ -121,15 +120,12 @@ __global__ void __launch_bounds__(ST) k_score_acc(int N, int Npad, int P, int J,
         double acc[TT];
 #pragma unroll
         for (int t = 0; t < TT; t++) acc[t] = 0.0;
-        constexpr int TW = TT / 4;            // tests staged per thread
-        double sa[TW], sw[TW];
-#pragma unroll
-        for (int u = 0; u < TW; u++) sa[u] = sw[u] = 0.0;
+        double sa[4] = {0.0, 0.0, 0.0, 0.0}, sw[4] = {0.0, 0.0, 0.0, 0.0};
         for (int tile = blockIdx.x; tile < ntile; tile += NBK) {
             const int i = tile * CT + ci;
             __syncthreads();
 #pragma unroll
-            for (int u = 0; u < TW; u++) {
+            for (int u = 0; u < 4; u++) {
                 const int t = w + 4 * u;
                 double val = 0.0;
                 if (i < N && sj[t] >= 0) {
@@ -159,7 +155,7 @@ __global__ void __launch_bounds__(ST) k_score_acc(int N, int Npad, int P, int J,
         }
         if (k0 == 0) {
 #pragma unroll
-            for (int u = 0; u < TW; u++) {
+            for (int u = 0; u < 4; u++) {
                 const double a = wave_sum(sa[u]), s = wave_sum(sw[u]);
                 if ((tid & 63) == 0) {
                     const int t = w + 4 * u;
@@ -171,11 +167,10 @@ __global__ void __launch_bounds__(ST) k_score_acc(int N, int Npad, int P, int J,
     }
 }
 
-// One single-wave workgroup per test of the launch: sum the NBK partials, add
-// the count term over the gene's non-zeros, forward substitution (one row per
-// step: a single wave's barrier is cheap), statistic, p-value.
+// One workgroup per test of the launch: sum the NBK partials, add the count
+// term over the gene's non-zeros, forward substitution, statistic, p-value.
 template <typename GT>
-__global__ void __launch_bounds__(FST) k_score_fin(int N, int P, int T, int t0,
+__global__ void __launch_bounds__(ST) k_score_fin(int N, int P, int T, int t0,
         const GT *__restrict__ G, const int *__restrict__ donor, int ld,
         const double *__restrict__ d, const double *__restrict__ cmean,
         const int *__restrict__ gene, const int *__restrict__ snp, int J, int S,
@@ -184,6 +179,7 @@ __global__ void __launch_bounds__(FST) k_score_fin(int N, int P, int T, int t0,
         const double *__restrict__ phi, const double *__restrict__ part,
         double *__restrict__ stat, double *__restrict__ pval) {
     extern __shared__ double xs[];        // P + 2
+    __shared__ double sh[4];
     const int tl = blockIdx.x, tid = threadIdx.x;
     const int t = t0 + tl;
     if (t >= T) return;
@@ -197,21 +193,20 @@ __global__ void __launch_bounds__(FST) k_score_fin(int N, int P, int T, int t0,
     }
     const int c = tl / TT, tt = tl % TT;
     const double *pc = part + (size_t)c * NBK * TT * (P + 2) + (size_t)tt * (P + 2);
-    for (int k = tid; k < P + 2; k += FST) {
+    for (int k = tid; k < P + 2; k += ST) {
         double s = 0.0;
         for (int b = 0; b < NBK; b++) s += pc[(size_t)b * TT * (P + 2) + k];
         xs[k] = s;
     }
     const double cm = d ? cmean[l] : 0.0;
     double sy = 0.0;
-    for (int q = Yp[j] + tid; q < Yp[j + 1]; q += FST) {
+    for (int q = Yp[j] + tid; q < Yp[j + 1]; q += ST) {
         const int i = Yi[q];
         double h = gat(G, donor, ld, l, i);
         if (d) h = h * d[i] - cm;
         sy += h * Yx[q] / Omega[i];
     }
-    sy = wave_sum(sy);                     // valid on lane 0
-    __syncthreads();                       // the xs writes before the reads below
+    sy = block_sum(sy, sh);                // also orders the xs writes before the reads below
     const double a = xs[P], sw = xs[P + 1];
     const double *L = Psi + (size_t)j * P * P;
     for (int i = 0; i < P; i++) {
@@ -219,12 +214,12 @@ __global__ void __launch_bounds__(FST) k_score_fin(int N, int P, int T, int t0,
         const double xi = xs[i] / L[i + (size_t)i * P];
         __syncthreads();
         if (tid == 0) xs[i] = xi;
-        for (int r = i + 1 + tid; r < P; r += FST) xs[r] -= L[r + (size_t)i * P] * xi;
+        for (int r = i + 1 + tid; r < P; r += ST) xs[r] -= L[r + (size_t)i * P] * xi;
     }
     __syncthreads();
     double q = 0.0;
-    for (int k = tid; k < P; k += FST) q += xs[k] * xs[k];
-    q = wave_sum(q);
+    for (int k = tid; k < P; k += ST) q += xs[k] * xs[k];
+    q = block_sum(q, sh);
     if (tid == 0) {
         const double st = (sy - sw) / sqrt(phi[j] * (a - q));
         stat[t] = st;
@@ -249,7 +244,7 @@ int score_run(gp_ctx *c, int S, const GT *G, const int *donor, int ld, const dou
                                                              ld, d, cmean, gene, snp, c->W, c->Omega,
                                                              Zsrc, part, err);
         GP_HIP(hipGetLastError());
-        k_score_fin<GT><<<n, FST, (size_t)(P + 2) * sizeof(double), c->stream>>>(
+        k_score_fin<GT><<<n, ST, (size_t)(P + 2) * sizeof(double), c->stream>>>(
             c->N, P, T, t0, G, donor, ld, d, cmean, gene, snp, c->J, S, c->Yp, c->Yi, c->Yx, c->Omega, c->Psi,
             c->phi, part, stat, pval);
         GP_HIP(hipGetLastError());
