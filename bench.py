@@ -165,6 +165,10 @@ def main_batched(args):
     t1 = time.perf_counter()
     bf.init()
     log(f"init (unweighted fits of every gene): {time.perf_counter() - t1:.3f} s")
+    grid = None
+    if not args.no_tune:
+        grid = bf.autotune(verbose=args.verbose)
+        log(f"autotuned: {grid} persistent workgroups per CU")
     for w in range(args.warmup):
         bf.run(1, 0.0, graph=True)
     t1 = time.perf_counter()
@@ -181,7 +185,7 @@ def main_batched(args):
            "config": {"workload": f"uneven per-gene GLM/Laplace fits: N={ds.N} cells, G={args.G} genes, "
                                   f"subsets log-uniform [{args.nmin}, {args.nmax}], M={ds.M}, Q={ds.Q}, "
                                   f"p_g in 2..16", "positions": npos},
-           "objective": obj,
+           "objective": obj, "tuning": {"workgroups_per_cu": grid},
            "roofline": {"kernel": "k_bf_pass (Gram part)", "bound": "mfma", "achieved": ach / 1e12,
                         "peak": PEAK["f64"] / 1e12, "unit": "TFLOP/s", "frac": ach / PEAK["f64"],
                         "traffic": None, "flops_per_step": flops}}

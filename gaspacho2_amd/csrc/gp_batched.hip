@@ -40,6 +40,7 @@ struct gp_bfit {
     int *info = nullptr, *jtr = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
+    int grid_mult = 2;                        // persistent workgroups per CU of the pass
 };
 
 namespace {
@@ -452,7 +453,8 @@ __global__ void k_bf_fill_int(int n, int *p, int v) {
     if (i < n) p[i] = v;
 }
 
-int bf_pass(gp_bfit *b, int mode) {
+// the persistent pass alone: Psi (unfactored), bvec, sov2 and lk at the current beta
+int bf_launch_pass(gp_bfit *b, int mode) {
     gp_ctx *c = b->c;
     k_bf_fill_int<<<1, 256, 0, c->stream>>>(1, b->queue, 0);
     GP_HIP(hipGetLastError());
@@ -461,12 +463,18 @@ int bf_pass(gp_bfit *b, int mode) {
     int dev = 0, ncu = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    const int grid = std::min(b->J, 2 * ncu);
+    const int grid = std::min(b->J, b->grid_mult * ncu);
     k_bf_pass<<<grid, BTH, 0, c->stream>>>(mode, b->J, b->M, b->PB, b->order, b->queue, b->off,
                                           b->cell, b->ncov, b->covoff, b->cov, b->yd, b->Krm,
                                           c->ls, c->Omega, b->cellc, c->K, b->delta, b->Beta,
                                           b->Psi, b->bvec, b->sov2, b->lk);
     GP_HIP(hipGetLastError());
+    return 0;
+}
+
+int bf_pass(gp_bfit *b, int mode) {
+    gp_ctx *c = b->c;
+    GP_TRY(bf_launch_pass(b, mode));
     GP_TRY(gpk_chol_batched(c, b->Psi, b->PB, b->J, (size_t)b->PB * b->PB, b->info));
     k_bf_beta<<<b->J, 256, (size_t)b->PB * sizeof(double), c->stream>>>(
         mode, b->PB, b->off, b->jtr, b->Psi, b->bvec, b->sov2, b->Beta, b->phi);
@@ -634,6 +642,53 @@ extern "C" int gp_bfit_step(gp_bfit *b, double *obj) {
     GP_HIP(hipMemcpyAsync(&bad, b->info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     GP_HIP(hipStreamSynchronize(c->stream));
     return bad ? -bad : 0;
+}
+
+// Times the persistent pass (the Gram, bvec and likelihood terms at the
+// current beta; beta and phi are untouched, the stored factors are
+// overwritten by unfactored Grams until the next step) with 1, 2, 3 and 4
+// persistent workgroups per CU and keeps the fastest.  Every gene is
+// processed by one workgroup in a fixed order whatever the grid, so the
+// choice does not change any result.  A recorded step graph is dropped (it
+// holds the old grid).
+extern "C" int gp_bfit_autotune(gp_bfit *b, int verbose, int *grid_mult) {
+    if (!b) { gp_set_error("NULL handle"); return 1; }
+    gp_ctx *c = b->c;
+    if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    hipEvent_t e0, e1;
+    GP_HIP(hipEventCreate(&e0));
+    GP_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    int pick = b->grid_mult;
+    for (int m = 1; m <= 4; m++) {
+        b->grid_mult = m;
+        int r = bf_launch_pass(b, 1);             // warm-up at this grid
+        if (r) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return r; }
+        (void)hipEventRecord(e0, c->stream);
+        for (int k = 0; k < 2 && !r; k++) r = bf_launch_pass(b, 1);
+        (void)hipEventRecord(e1, c->stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (r) { (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); return r; }
+        if (verbose) std::printf("bfit autotune: %d workgroups per CU: %.3f ms per pass\n", m, ms / 2);
+        if (ms < best) { best = ms; pick = m; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->grid_mult = pick;
+    if (grid_mult) *grid_mult = pick;
+    return 0;
+}
+
+extern "C" int gp_bfit_set_grid(gp_bfit *b, int grid_mult) {
+    if (!b) { gp_set_error("NULL handle"); return 1; }
+    if (grid_mult < 1 || grid_mult > 8) { gp_set_error("grid_mult must be in [1, 8]"); return 1; }
+    if (b->gexec) { (void)hipGraphExecDestroy(b->gexec); b->gexec = nullptr; }
+    if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    b->grid_mult = grid_mult;
+    return 0;
 }
 
 // The IRLS loop with the reference's stop rule (ggplvm.cu:1156-1181: stop at

@@ -59,6 +59,8 @@ def load() -> C.CDLL:
         "gp_bfit_run": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, ip, dp]),
         "gp_bfit_get": (C.c_int, [vp, C.c_char_p, dp, C.c_size_t]),
         "gp_bfit_dims": (C.c_int, [vp, ip, ip]),
+        "gp_bfit_autotune": (C.c_int, [vp, C.c_int, ip]),
+        "gp_bfit_set_grid": (C.c_int, [vp, C.c_int]),
         "gp_score_tests_dosage": (C.c_int, [vp, C.c_int, C.POINTER(C.c_ubyte), C.c_int, ip, ip, dp]),
         "gp_score_tests_interaction": (C.c_int, [vp, C.c_int, C.POINTER(C.c_ubyte), dp, C.c_int, ip,
                                                  ip, dp, dp]),
@@ -406,6 +408,15 @@ class BatchedFit:
         o = C.c_double()
         self.ctx._check(self._lib.gp_bfit_step(self._h, C.byref(o)))
         return o.value
+
+    def autotune(self, verbose: bool = False) -> int:
+        """Persistent workgroups per CU picked by timing the pass (1-4)."""
+        m = C.c_int()
+        self.ctx._check(self._lib.gp_bfit_autotune(self._h, int(bool(verbose)), C.byref(m)))
+        return m.value
+
+    def set_grid(self, grid_mult: int) -> None:
+        self.ctx._check(self._lib.gp_bfit_set_grid(self._h, int(grid_mult)))
 
     def run(self, maxit: int, tol: float = 1e-7, graph: bool = True):
         it, o = C.c_int(), C.c_double()

@@ -233,6 +233,12 @@ int gp_bfit_step(gp_bfit *b, double *obj);
 /* Up to maxit steps, stopping when the relative objective change < tol
  * (ggplvm.cu:1156-1181); use_graph records one step as a HIP graph. */
 int gp_bfit_run(gp_bfit *b, int maxit, double tol, int use_graph, int *iters, double *obj);
+/* No counterpart (config-2 launch shape): times the persistent pass with 1-4
+ * workgroups per CU at the current beta and keeps the fastest (*grid_mult,
+ * may be NULL; results do not depend on it).  Beta and phi are kept; "Psi"
+ * holds unfactored Grams until the next step.  gp_bfit_set_grid sets it. */
+int gp_bfit_autotune(gp_bfit *b, int verbose, int *grid_mult);
+int gp_bfit_set_grid(gp_bfit *b, int grid_mult);
 /* "Beta" (J x PB), "phi" (J), "lkhd" (J), "Psi" (J x PB x PB factors). */
 int gp_bfit_get(gp_bfit *b, const char *name, double *host, size_t n);
 int gp_bfit_dims(gp_bfit *b, int *J, int *PB);

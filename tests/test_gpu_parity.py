@@ -750,6 +750,42 @@ def _batched_oracle(ds, bd, g, T0):
                        bd.yval[lo:hi], Zcov, ds.ls[c], X0, T0)
 
 
+def test_batched_grid_autotune_keeps_results():
+    """The batched pass's grid (persistent workgroups per CU) only changes how
+    genes are dealt to workgroups: 4 IRLS steps (replayed as a graph) give
+    the same beta, phi and objectives at every grid, and after
+    gp_bfit_autotune (which keeps beta and phi and drops the recorded graph)
+    the steps continue as they would have."""
+    from gaspacho2_amd import lib, synth
+    ds = synth.make_dataset(3000, 4, 4, 24, seed=12)
+    bd = synth.make_batched(ds, 60, 40, 2500, seed=3)
+    T0 = lib.init_ta_with_per(ds.M, ds.Q)
+    g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0, T0)
+    g.init_fit()
+    ref = None
+    for m in (2, 1, 3):
+        bf = lib.BatchedFit(g, bd.off, bd.cell, bd.ncov, bd.cov, bd.yoff, bd.ypos, bd.yval)
+        bf.set_grid(m)
+        bf.init()
+        _, o2 = bf.run(2, tol=0.0, graph=True)
+        if m == 3:
+            assert bf.autotune() in (1, 2, 3, 4)
+        _, o4 = bf.run(2, tol=0.0, graph=True)
+        out = (o2, o4, bf.get("Beta").copy(), bf.get("phi").copy())
+        bf.close()
+        if ref is None:
+            ref = out
+        assert out[0] == ref[0] and out[1] == ref[1], m
+        assert np.array_equal(out[2], ref[2]) and np.array_equal(out[3], ref[3]), m
+    with pytest.raises(lib.GPError):
+        bf2 = lib.BatchedFit(g, bd.off, bd.cell, bd.ncov, bd.cov, bd.yoff, bd.ypos, bd.yval)
+        try:
+            bf2.set_grid(0)
+        finally:
+            bf2.close()
+    g.close()
+
+
 @pytest.mark.parametrize("graph", [False, True])
 def test_batched_subset_glm_matches_oracle(graph):
     """Batched per-gene IRLS on uneven cell subsets (persistent launch) against
