@@ -79,7 +79,8 @@ static void free_all(gp_ctx *c) {
                     c->wun, c->zun, c->Linv, c->Zw, c->DeltaW, c->DeltaWChol, c->Uw, c->tBw,
                     c->Vtmp, c->Sh, c->Sl, c->gc_perm, c->gc_aexp, c->gc_sexp, c->gc_bexp,
                     c->gc_gun, c->gc_part, c->gc_zexp, c->skw, c->gc_kappa, c->gc_t0, c->gc_s0,
-                    c->gc_y, c->gram_row0};
+                    c->gc_y, c->gram_row0, c->irls_cc, c->irls_segptr, c->irls_sums,
+                    c->irls_segun};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     gp_timer *ts[] = {&c->t_gram, &c->t_chol, &c->t_gc, &c->t_kuf, &c->t_elbo, &c->t_grad,
@@ -285,6 +286,8 @@ extern "C" int gp_set_option(gp_ctx *c, const char *name, int value) {
         c->gc_prefetch = value;
     } else if (n == "irls_mixed") {
         c->irls_mixed = value ? 1 : 0;
+    } else if (n == "irls_fused") {
+        c->irls_fused = value ? 1 : 0;
     } else if (n == "autotune") {
         c->autotune = value ? 1 : 0;
     } else {
@@ -308,6 +311,7 @@ extern "C" int gp_get_option(gp_ctx *c, const char *name, int *value) {
     else if (n == "gram_waves") *value = c->gram_waves;
     else if (n == "gc_form") *value = c->gc_form;
     else if (n == "irls_mixed") *value = c->irls_mixed;
+    else if (n == "irls_fused") *value = c->irls_fused;
     else if (n == "autotune") *value = c->autotune;
     else { gp_set_error("unknown option " + n); return 1; }
     return 0;
@@ -371,6 +375,7 @@ extern "C" size_t gp_buffer_len(gp_ctx *c, const char *name) {
 
 extern "C" int gp_get_buffer(gp_ctx *c, const char *name, double *host, size_t n) {
     if (!c || !name || !host) { gp_set_error("NULL argument"); return 1; }
+    GP_TRY(gp_wnu_sync(c));
     if (!std::strcmp(name, "Psi") || !std::strcmp(name, "C1") || !std::strcmp(name, "U") ||
         !std::strcmp(name, "Uw"))
         GP_TRY(ensure_grad_buffers(c));
@@ -418,6 +423,7 @@ extern "C" int gp_get_buffer(gp_ctx *c, const char *name, double *host, size_t n
 
 extern "C" int gp_set_buffer(gp_ctx *c, const char *name, const double *host, size_t n) {
     if (!c || !name || !host) { gp_set_error("NULL argument"); return 1; }
+    GP_TRY(gp_wnu_sync(c));
     if (!std::strcmp(name, "Psi") || !std::strcmp(name, "C1") || !std::strcmp(name, "U") ||
         !std::strcmp(name, "Uw"))
         GP_TRY(ensure_grad_buffers(c));
@@ -451,6 +457,15 @@ extern "C" int gp_set_buffer(gp_ctx *c, const char *name, const double *host, si
         return 1;
     }
     return 0;
+}
+
+// W and Nu of a fused IRLS step, written from the eta and per-cell constants
+// it used (ZB and irls_cc are not changed before this runs: every entry point
+// that could change them, or read W / Nu, calls it first)
+int gp_wnu_sync(gp_ctx *c) {
+    if (!c->wnu_pending) return 0;
+    c->wnu_pending = 0;
+    return gpk_irls_wnu(c);
 }
 
 // ------------------------------------------------------------------ groups
@@ -523,6 +538,7 @@ static int gram_chol(gp_ctx *c, int precision, bool weights_ready = false) {
 
 extern "C" int gp_init_fit(gp_ctx *c) {
     if (!c) { gp_set_error("NULL context"); return 1; }
+    GP_TRY(gp_wnu_sync(c));
     c->lkhd_fresh = 0;
     c->w32_valid = 0;
     GP_TRY(clear_info(c));
@@ -545,6 +561,39 @@ extern "C" int gp_irls_step(gp_ctx *c, int jtr, double *obj) {
     c->w32_valid = 0;
     if (jtr < 0 || jtr >= GP_LKHD_SLOTS) { gp_set_error("jtr out of range"); return 1; }
     GP_TRY(clear_info(c));
+    // the step writes W and Nu itself (or leaves them pending again)
+    c->wnu_pending = 0;
+    if (c->precision == GP_PREC_MIXED && c->irls_mixed && c->irls_fused && gpk_gram_irls_fits(c)) {
+        // getWForBeta + getPsi in one kernel on the whitened FP16 hi/lo Gram:
+        // it builds W / Omega, nu and R = W nu / Omega from eta = Z beta,
+        // accumulates the Gram and the sums gq, gs and the GLM objective
+        // (getBetaPhi / getLkhdPois), and writes R and the FP32 weights
+        // instead of W and nu; those are written only when something reads
+        // them (gp_wnu_sync).  The rest is the whitened step below.
+        if (!c->zw_valid) {
+            GP_TRY(gpk_whiten(c));
+            c->zw_valid = 1;
+            c->zsplit_valid = 0;
+        }
+        gp_timer_begin(c, c->t_gram);
+        GP_TRY(gpk_gram_irls(c, c->Psi, c->DeltaW));
+        gp_timer_end(c, c->t_gram);
+        c->psi_whitened = 1;
+        gp_timer_begin(c, c->t_chol);
+        GP_TRY(gpk_chol_batched(c, c->Psi, c->P, c->J, (size_t)c->P * c->P, c->info));
+        gp_timer_end(c, c->t_chol);
+        // b_j = Zw^T R_j (getZtOinvjnj, util_cuda.cu:462-487)
+        GP_TRY(gpk_gemm_tn_splitk(c, c->P, c->J, c->Npad, c->Zw, c->Npad, c->R, c->Npad, c->bvec, c->P));
+        GP_TRY(gpk_cholsl_batched(c, c->Psi, c->P, (size_t)c->P * c->P, c->bvec, c->tBphi, c->J));
+        GP_TRY(gpk_beta_whiten(c, c->Beta, c->tBw));
+        GP_TRY(gpk_beta_phi_in(c, jtr, c->tBw));
+        GP_TRY(gpk_beta_unwhiten(c, c->tBw, c->Beta));
+        GP_TRY(gpk_collect(c, 0, jtr, obj));
+        c->wnu_pending = 1;
+        int bad = 0;
+        GP_TRY(read_info(c, &bad));
+        return bad ? -bad : 0;
+    }
     GP_TRY(gpk_w_nu(c));
     if (c->precision == GP_PREC_MIXED && c->irls_mixed) {
         // Same step in the whitened basis T = blockdiag(I, L^-T): Psi_w = T^T Psi T
@@ -584,6 +633,7 @@ static int lkhd_genes(gp_ctx *c) {
 
 extern "C" int gp_ql_start(gp_ctx *c, double *obj) {
     if (!c) { gp_set_error("NULL context"); return 1; }
+    GP_TRY(gp_wnu_sync(c));
     c->lkhd_fresh = 0;   // set again by the likelihood evaluation below
     GP_TRY(clear_info(c));
     GP_TRY(gpk_oinvt(c));
@@ -611,6 +661,7 @@ static int fork_weights_mixed(gp_ctx *c) { return fork_weights_impl(c, true); }
 
 extern "C" int gp_elbo(gp_ctx *c, int slot, double *obj) {
     if (!c) { gp_set_error("NULL context"); return 1; }
+    if (!c->capturing) GP_TRY(gp_wnu_sync(c));
     c->lkhd_fresh = 0;   // set again by the likelihood evaluation below
     if (slot < 0 || slot >= GP_LKHD_SLOTS) { gp_set_error("slot out of range"); return 1; }
     gp_timer_begin(c, c->t_elbo);
@@ -668,6 +719,7 @@ static int fork_tn_chain(gp_ctx *c) {
 
 extern "C" int gp_grad(gp_ctx *c) {
     if (!c) { gp_set_error("NULL context"); return 1; }
+    if (!c->capturing) GP_TRY(gp_wnu_sync(c));
     if (!c->capturing) GP_TRY(ensure_grad_buffers(c));
     gp_timer_begin(c, c->t_grad);
     GP_TRY(gpk_expand_delta(c));
@@ -721,6 +773,7 @@ static void drop_graph(gp_ctx *c) {
 // allocation -- and then records it.
 extern "C" int gp_eval_graph(gp_ctx *c, int slot, double *obj) {
     if (!c) { gp_set_error("NULL context"); return 1; }
+    GP_TRY(gp_wnu_sync(c));
     if (slot < 0 || slot >= GP_LKHD_SLOTS) { gp_set_error("slot out of range"); return 1; }
     const int key[GP_GRAPH_KEY] = {c->precision, slot, c->gram_splits, c->gc_groups, c->gram_waves,
                                    c->gc_form, c->gc_mixed, c->gc_peel, c->gc_wave_rows,
@@ -758,6 +811,7 @@ extern "C" int gp_eval_graph(gp_ctx *c, int slot, double *obj) {
 
 extern "C" int gp_lbfgs_step(gp_ctx *c, int jtr, int rhoflag) {
     if (!c) { gp_set_error("NULL context"); return 1; }
+    GP_TRY(gp_wnu_sync(c));
     c->lkhd_fresh = 0;
     c->w32_valid = 0;
     if (jtr < 0) { gp_set_error("jtr < 0"); return 1; }
@@ -767,6 +821,7 @@ extern "C" int gp_lbfgs_step(gp_ctx *c, int jtr, int rhoflag) {
 
 extern "C" int gp_downdate(gp_ctx *c, int jtr, double sr, int rhoflag) {
     if (!c) { gp_set_error("NULL context"); return 1; }
+    GP_TRY(gp_wnu_sync(c));
     c->lkhd_fresh = 0;
     c->w32_valid = 0;
     GP_TRY(gpk_downdate(c, jtr, sr, rhoflag ? 1 : 0));
@@ -775,6 +830,7 @@ extern "C" int gp_downdate(gp_ctx *c, int jtr, double sr, int rhoflag) {
 
 extern "C" int gp_end_outer(gp_ctx *c) {
     if (!c) { gp_set_error("NULL context"); return 1; }
+    GP_TRY(gp_wnu_sync(c));
     c->lkhd_fresh = 0;
     GP_TRY(gpk_expand_delta(c));
     GP_TRY(gpk_beta_from_tilde(c));
@@ -905,6 +961,7 @@ int tune_gc(gp_ctx *c) { return gpk_gradcontract_mixed(c); }
 
 extern "C" int gp_autotune(gp_ctx *c, int verbose) {
     if (!c) { gp_set_error("NULL context"); return 1; }
+    GP_TRY(gp_wnu_sync(c));
     c->lkhd_fresh = 0;
     if (c->precision != GP_PREC_MIXED) { gp_set_error("gp_autotune needs GP_PREC_MIXED"); return 1; }
     GP_TRY(ensure_grad_buffers(c));

@@ -424,10 +424,15 @@ struct GramRounds {
     int nr[GW];
 };
 
-constexpr GramRounds make_gram_rounds(int T16) {
+// VALU cycles per chunk of wave 0's IRLS work in the fused form (per cell:
+// one FP64 exp, one FP64 division, the count scatter and a dozen FP64 ops)
+constexpr int IRLS_CYC = 400;
+
+constexpr GramRounds make_gram_rounds(int T16, bool irls = false) {
     GramRounds g{};
     const GramPlan pl = make_gram_plan(T16);
     int t[GW] = {};
+    if (irls) t[0] += IRLS_CYC;
     for (int w = 0; w < GW; w++)
         for (int k = 0; k < GSL; k++)
             if (pl.a[w * GSL + k] >= 0) t[w] += 96 * gram_blk_cost(T16, pl.a[w * GSL + k], pl.b[w * GSL + k]);
@@ -498,17 +503,96 @@ struct GramArgs {
     float sc;
     const float *z32, *wj;
     float *pg;
+    // fused IRLS pass (k_gram_irls): wave 0 builds each chunk's weights from
+    // eta = z^T beta instead of reading them, together with nu, R and the sums
+    int N = 0;
+    int nz0 = 0, nz1 = 0;                 // the segment's non-zeros: Yi[nz0 .. nz1)
+    const double *eta = nullptr;          // eta_j (Npad)
+    const double *cd = nullptr, *com = nullptr, *cbl = nullptr;   // per cell: d, Omega, log ls
+    const int *Yi = nullptr;
+    const double *Yx = nullptr;
+    double *Rj = nullptr;                 // R_j = w nu (Npad)
+    float *w32o = nullptr;                // w32_j = w in FP32 (Npad)
+    double *ywl = nullptr;                // LDS: one chunk's counts by cell
+    double *sums = nullptr;               // the (segment, gene)'s sum(w nu^2), sum(d - eta), sum W, sum y(eta + log ls)
 };
 
-// the chunk loop of wave W (its blocks are compile-time constants)
-template <int T16, int W>
+// Per-cell IRLS quantities (ggplvm.cu:468-514 getWForBeta with zeta = 0, the
+// sums of getBetaPhi / getLkhd, ggplvm.cu:418-464, and getLkhdPois,
+// ggplvm.cu:703-724) in the form the fused Gram uses.  With the per-cell
+// constants d_i = log Omega_i - log ls_i - dt t_i wt and Omega_i:
+//   w = W / Omega = exp(eta - d),  W = w Omega,
+//   nu = eta + y / W - 1 (y / W = exp(log y - log W); nu = eta - 1 at zeros),
+//   R = w nu,  gq += w nu^2,  gs += log(Omega / W) = d - eta,
+//   the GLM objective sum_nz y (log W - dt t wt) - sum W = sum_nz y (eta + log ls) - sum W.
+struct IrlsLoad {
+    double eta, d, om, bl, yx;
+    int yi;
+};
+
+__device__ __forceinline__ IrlsLoad irls_load(const GramArgs &g, int ch, int nzp, int lane) {
+    IrlsLoad v;
+    const int i = g.c0 + ch * GCH + lane;
+    v.eta = g.eta[i];
+    v.d = g.cd[i];
+    v.om = g.com[i];
+    v.bl = g.cbl[i];
+    const int k = nzp + lane;
+    v.yi = 0x7fffffff;
+    v.yx = 0.0;
+    if (k < g.nz1) {
+        v.yi = g.Yi[k];
+        v.yx = g.Yx[k];
+    }
+    return v;
+}
+
+// wave 0: the chunk's counts by cell (its non-zeros are the next entries of
+// the gene's ascending cell list), then the cell's IRLS quantities; returns
+// w in FP32 and advances the list pointer
+__device__ __forceinline__ float irls_finish(const GramArgs &g, int ch, int &nzp, int lane,
+                                             const IrlsLoad &v, double (&acc)[4]) {
+    const int cc = g.c0 + ch * GCH;
+    const bool in = v.yi < cc + GCH;
+    const unsigned long long m = __ballot(in);
+    // lane L's slot may be written by another lane: volatile keeps the three
+    // accesses in program order and the read from being taken from the
+    // lane's own zeroing store; a wave's LDS operations execute in order
+    volatile double *yw = g.ywl;
+    yw[lane] = 0.0;
+    if (in && v.yi >= cc) yw[v.yi - cc] = v.yx;
+    const double y = yw[lane];
+    nzp += __popcll(m);
+    const int i = cc + lane;
+    float wf = 0.0f;
+    double r = 0.0;
+    if (i < g.N) {
+        const double w = exp(v.eta - v.d);
+        const double Wv = w * v.om;
+        const double q = y / Wv;
+        const double nu = (v.eta + (y > 0.0 ? q : 0.0)) - 1.0;
+        r = w * nu;
+        acc[0] += nu * nu * w;
+        acc[1] += v.d - v.eta;
+        acc[2] += Wv;
+        acc[3] += y * (v.eta + v.bl);
+        wf = (float)w;
+    }
+    g.Rj[i] = r;
+    g.w32o[i] = wf;
+    return wf;
+}
+
+// the chunk loop of wave W (its blocks are compile-time constants); IR: the
+// fused IRLS form, whose wave 0 builds the weights from eta
+template <int T16, int W, bool IR = false>
 __device__ __forceinline__ void gram_wave(const GramArgs &g, _Float16 *Xs, float (*wsl)[GCH]) {
     constexpr int P16 = 16 * T16;
     constexpr GramPlan pl = make_gram_plan(T16);
     constexpr int A0 = pl.a[W * GSL], B0 = pl.b[W * GSL];
     constexpr int A1 = pl.a[W * GSL + 1], B1 = pl.b[W * GSL + 1];
     constexpr int QC = 4 * GKC;                   // 8-cell groups per chunk
-    constexpr GramRounds rp = make_gram_rounds(T16);
+    constexpr GramRounds rp = make_gram_rounds(T16, IR);
     constexpr int RB = rp.r0[W], IPT = rp.nr[W];  // this wave's staging rounds
     const int tid = threadIdx.x, lane = tid & 63;
     const int lrow = lane & 15, lk = 8 * (lane >> 4);
@@ -558,11 +642,23 @@ __device__ __forceinline__ void gram_wave(const GramArgs &g, _Float16 *Xs, float
         }
     };
     auto sqw = [&](int ch) { return sqrtf(g.wj[g.c0 + ch * GCH + tid] * g.sc); };   // tid < GCH
+    // fused IRLS form (wave 0): the segment's non-zero pointer and the sums
+    int nzp = g.nz0;
+    double iacc[4] = {0.0, 0.0, 0.0, 0.0};
 
     // prologue: weights of chunks 0 and 1, X of chunk 0
     if (tid < GCH) {
-        wsl[0][tid] = sqw(0);
-        if (g.nch > 1) wsl[1][tid] = sqw(1);
+        if constexpr (IR) {
+            const IrlsLoad v0 = irls_load(g, 0, nzp, lane);
+            wsl[0][tid] = sqrtf(irls_finish(g, 0, nzp, lane, v0, iacc) * g.sc);
+            if (g.nch > 1) {
+                const IrlsLoad v1 = irls_load(g, 1, nzp, lane);
+                wsl[1][tid] = sqrtf(irls_finish(g, 1, nzp, lane, v1, iacc) * g.sc);
+            }
+        } else {
+            wsl[0][tid] = sqw(0);
+            if (g.nch > 1) wsl[1][tid] = sqw(1);
+        }
     }
     fetch(g.c0, 0, IPT);
     __syncthreads();
@@ -573,7 +669,11 @@ __device__ __forceinline__ void gram_wave(const GramArgs &g, _Float16 *Xs, float
         const int buf = ch & 1;
         const int cn = g.c0 + min(ch + 1, g.nch - 1) * GCH;
         float pw = 0.0f;
-        if constexpr (W == 0) pw = g.wj[g.c0 + min(ch + 2, g.nch - 1) * GCH + (lane & (GCH - 1))];
+        [[maybe_unused]] IrlsLoad iv;
+        if constexpr (W == 0) {
+            if constexpr (IR) iv = irls_load(g, min(ch + 2, g.nch - 1), nzp, lane);
+            else pw = g.wj[g.c0 + min(ch + 2, g.nch - 1) * GCH + (lane & (GCH - 1))];
+        }
         fetch(cn, 0, min(IPS, IPT));
         __builtin_amdgcn_sched_barrier(0);   // issue the prefetch ahead of this chunk's MFMAs
         const _Float16 *xh = Xs + (size_t)(buf * 2 + 0) * P16 * GLDX;
@@ -591,12 +691,29 @@ __device__ __forceinline__ void gram_wave(const GramArgs &g, _Float16 *Xs, float
         }
         // wave 0 writes the weights two chunks ahead; past the last chunk the
         // slot holds the last chunk's weights and is never read.  Kept after
-        // the MFMAs so that its wait does not stall them.
-        if constexpr (W == 0) wsl[(ch + 2) % 3][lane & (GCH - 1)] = sqrtf(pw * g.sc);
+        // the MFMAs so that its wait does not stall them.  The fused form
+        // builds them (and R, w32, the sums) only for chunks that exist.
+        if constexpr (W == 0) {
+            if constexpr (IR) {
+                if (ch + 2 < g.nch)
+                    wsl[(ch + 2) % 3][lane] = sqrtf(irls_finish(g, ch + 2, nzp, lane, iv, iacc) * g.sc);
+            } else {
+                wsl[(ch + 2) % 3][lane & (GCH - 1)] = sqrtf(pw * g.sc);
+            }
+        }
         __syncthreads();
     }
     if constexpr (A0 >= 0) gram_block_store<T16, A0, B0>(acc0, g.pg, lane);
     if constexpr (A1 >= 0) gram_block_store<T16, A1, B1>(acc1, g.pg, lane);
+    if constexpr (IR && W == 0) {
+        // the wave's sums over the segment, lanes added in a fixed tree order
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            double s = iacc[k];
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+            if (lane == 0) g.sums[k] = s;
+        }
+    }
 }
 
 template <int T16>
@@ -638,6 +755,193 @@ __global__ void __launch_bounds__(GW * 64) k_gram_sym(int Npad, int seg, int nse
     case 5: gram_wave<T16, 5>(g, Xs, wsl); break;
     case 6: gram_wave<T16, 6>(g, Xs, wsl); break;
     default: gram_wave<T16, 7>(g, Xs, wsl); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused IRLS step + weighted Gram (getWForBeta, ggplvm.cu:468-514, and
+// getPsi, ggplvm.cu:621-680, in one launch).  Workgroup = (gene, cell
+// segment) as in k_gram_sym.  The FP16 hi/lo operands need a power-of-two
+// weight scale 2^e >= max w over the cells they accumulate, so the workgroup
+// first takes max(eta - d) over its segment (w = exp(eta - d) is monotone in
+// it) and scales by 2^-e for that segment alone (k_gram_reduce_sym unscales
+// each segment's partial).  Then the chunk loop of the symmetric Gram runs,
+// wave 0 building each chunk's weights from eta two chunks ahead -- with the
+// chunk's counts, nu, R = w nu, the FP32 weights and the sums -- instead of
+// reading them.  W and nu are not stored: R and w32 are what the rest of the
+// step reads (b = Zw^T R, the intercept row from w32).
+template <int T16>
+__global__ void __launch_bounds__(GW * 64) k_gram_irls(int N, int Npad, int seg, int nseg, int J,
+        const float *__restrict__ z32, const double *__restrict__ eta,
+        const double *__restrict__ cellc, const int *__restrict__ Yi,
+        const double *__restrict__ Yx, const int *__restrict__ segptr, float *__restrict__ w32,
+        double *__restrict__ R, double *__restrict__ sums, double *__restrict__ segun,
+        float *__restrict__ part) {
+    extern __shared__ __attribute__((aligned(16))) _Float16 Xs[];
+    __shared__ __attribute__((aligned(16))) float wsl[3][GCH];
+    __shared__ double ywl[GCH];
+    __shared__ double red[GW];
+    const int tid = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    int sgi, j;
+    {
+        const int id = blockIdx.x;
+        if (nseg % 8 == 0) {
+            const int xcd = id % 8, loc = id / 8, nsx = nseg / 8;
+            sgi = xcd + 8 * (loc % nsx);
+            j = loc / nsx;
+        } else {
+            sgi = id % nseg;
+            j = id / nseg;
+        }
+    }
+    const int c0 = sgi * seg, c1 = min(Npad, c0 + seg);
+    const double *ej = eta + (size_t)j * Npad;
+    const double *cd = cellc;
+    // the segment's largest eta - d over its cells, eight loads in flight
+    double m = -INFINITY;
+    {
+        const int hi = min(c1, N);
+        constexpr int MU = 8;
+        int i = c0 + tid;
+        for (; i + (MU - 1) * GW * 64 < hi; i += MU * GW * 64) {
+            double e[MU], d[MU];
+#pragma unroll
+            for (int u = 0; u < MU; u++) {
+                e[u] = ej[i + u * GW * 64];
+                d[u] = cd[i + u * GW * 64];
+            }
+#pragma unroll
+            for (int u = 0; u < MU; u++) m = fmax(m, e[u] - d[u]);
+        }
+        for (; i < hi; i += GW * 64) m = fmax(m, ej[i] - cd[i]);
+    }
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+    if ((tid & 63) == 0) red[w] = m;
+    __syncthreads();
+    m = red[0];
+#pragma unroll
+    for (int k = 1; k < GW; k++) m = fmax(m, red[k]);
+    // 2^e >= the largest FP32 weight: the cell at the maximum has w = exp(m)
+    const float wmax = (m > -INFINITY) ? (float)exp(m) : 0.0f;
+    int e = 0;
+    if (wmax > 0.0f) (void)frexpf(wmax, &e);
+    if (tid == 0) segun[(size_t)sgi * J + j] = ldexp(1.0, e);
+
+    GramArgs g;
+    g.Npad = Npad;
+    g.c0 = c0;
+    g.nch = (c1 - c0) / GCH;
+    g.sc = ldexpf(1.0f, -e);
+    g.z32 = z32;
+    g.wj = w32 + (size_t)j * Npad;
+    g.pg = part + ((size_t)sgi * J + j) * (T16 * (T16 + 1) / 2) * 256;
+    g.N = N;
+    g.nz0 = segptr[(size_t)j * (nseg + 1) + sgi];
+    g.nz1 = segptr[(size_t)j * (nseg + 1) + sgi + 1];
+    g.eta = ej;
+    g.cd = cellc;
+    g.com = cellc + Npad;
+    g.cbl = cellc + 2 * (size_t)Npad;
+    g.Yi = Yi;
+    g.Yx = Yx;
+    g.Rj = R + (size_t)j * Npad;
+    g.w32o = w32 + (size_t)j * Npad;
+    g.ywl = ywl;
+    g.sums = sums + ((size_t)sgi * J + j) * 4;
+    switch (w) {
+    case 0: gram_wave<T16, 0, true>(g, Xs, wsl); break;
+    case 1: gram_wave<T16, 1, true>(g, Xs, wsl); break;
+    case 2: gram_wave<T16, 2, true>(g, Xs, wsl); break;
+    case 3: gram_wave<T16, 3, true>(g, Xs, wsl); break;
+    case 4: gram_wave<T16, 4, true>(g, Xs, wsl); break;
+    case 5: gram_wave<T16, 5, true>(g, Xs, wsl); break;
+    case 6: gram_wave<T16, 6, true>(g, Xs, wsl); break;
+    default: gram_wave<T16, 7, true>(g, Xs, wsl); break;
+    }
+}
+
+// Per-cell constants of the fused IRLS pass: d_i = log Omega_i - log ls_i -
+// dt t_i wt (dt = delta[p-1], t = the titsias term), Omega_i and log ls_i;
+// zero on the padding cells (which the pass skips).
+__global__ void k_irls_cellc(int N, int Npad, const double *__restrict__ Omega,
+                             const double *__restrict__ ls, const double *__restrict__ tits,
+                             const double *__restrict__ dtp, double wt, double *__restrict__ cellc) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= Npad) return;
+    double d = 0.0, om = 0.0, bl = 0.0;
+    if (i < N) {
+        const double x = *dtp * tits[i] * wt;
+        bl = log(ls[i]);
+        om = Omega[i];
+        d = (log(om) - bl) - x;
+    }
+    cellc[i] = d;
+    cellc[Npad + i] = om;
+    cellc[2 * (size_t)Npad + i] = bl;
+}
+
+// segptr[j (nseg + 1) + s] = the first non-zero of gene j at cell >= s seg
+// (a lower bound in the gene's ascending cell list)
+__global__ void k_irls_segptr(int J, int nseg, int seg, const int *__restrict__ Yp,
+                              const int *__restrict__ Yi, int *__restrict__ segptr) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= J * (nseg + 1)) return;
+    const int j = t / (nseg + 1), s = t % (nseg + 1);
+    const long long c = (long long)s * seg;
+    int lo = Yp[j], hi = Yp[j + 1];
+    while (lo < hi) {
+        const int mid = lo + (hi - lo) / 2;
+        if (Yi[mid] < c) lo = mid + 1;
+        else hi = mid;
+    }
+    segptr[t] = lo;
+}
+
+// gq_j, gs_j and the GLM objective of gene j from its segments' sums, added
+// in segment order
+__global__ void k_irls_sums(int J, int nseg, const double *__restrict__ sums, double *__restrict__ gq,
+                            double *__restrict__ gs, double *__restrict__ lkhds) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= J) return;
+    double q = 0.0, s = 0.0, sw = 0.0, ly = 0.0;
+    for (int g = 0; g < nseg; g++) {
+        const double *p = sums + ((size_t)g * J + j) * 4;
+        q += p[0];
+        s += p[1];
+        sw += p[2];
+        ly += p[3];
+    }
+    gq[j] = q;
+    gs[j] = s;
+    lkhds[j] = ly - sw;
+}
+
+// W and nu of the fused pass written out in the reference layout (from the
+// same eta and per-cell constants, so the values the pass used): dense part
+// here, the non-zeros' nu in k_irls_nu_nz
+__global__ void k_irls_wnu(int N, int Npad, const double *__restrict__ eta,
+                           const double *__restrict__ cellc, double *__restrict__ W,
+                           double *__restrict__ Nu) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= Npad) return;
+    const size_t a = i + (size_t)j * Npad;
+    if (i >= N) { W[a] = 0.0; Nu[a] = 0.0; return; }
+    const double e = eta[a];
+    W[a] = exp(e - cellc[i]) * cellc[Npad + i];
+    Nu[a] = (e + 0.0) - 1.0;
+}
+
+__global__ void k_irls_nu_nz(int Npad, const int *__restrict__ Yp, const int *__restrict__ Yi,
+                             const double *__restrict__ Yx, const double *__restrict__ eta,
+                             const double *__restrict__ W, double *__restrict__ Nu) {
+    const int j = blockIdx.x;
+    for (int l = Yp[j] + threadIdx.x; l < Yp[j + 1]; l += 256) {
+        const size_t a = Yi[l] + (size_t)j * Npad;
+        const double y = Yx[l];
+        const double q = y / W[a];
+        Nu[a] = (eta[a] + (y > 0.0 ? q : 0.0)) - 1.0;
     }
 }
 
@@ -1075,13 +1379,16 @@ constexpr int RSG = 4;       // genes per block of the symmetric reduction
 constexpr int GRK_SEG = 64;  // cell segments of the intercept row's FP32 products
 
 // Psi_j from the symmetric-form partials: one 256-thread block per (tile, RSG genes);
-// sums the segments in FP64, rescales by 2^e_j 2^-s_r 2^-s_c, adds the prior
-// and writes the tile and its mirror (upper-triangle values only, so Psi is
-// exactly symmetric).
+// sums the segments in FP64, each rescaled by its weight unscale 2^e
+// (wun[s wstride + j]: wstride 0 for one scale per gene, J for the fused IRLS
+// pass's per-segment scales; powers of two, so a common scale gives the bits
+// of scaling the sum), rescales by 2^-s_r 2^-s_c, adds the prior and writes
+// the tile and its mirror (upper-triangle values only, so Psi is exactly
+// symmetric).
 // off: the tiles cover columns off .. off + 16 T16 - 1 of Psi (1 when the
 // intercept row and column come from k_gram_row0)
 __global__ void __launch_bounds__(256) k_gram_reduce_sym(int P, int T16, int nseg, int J, int off,
-        const float *__restrict__ part, const double *__restrict__ wun,
+        const float *__restrict__ part, const double *__restrict__ wun, size_t wstride,
         const double *__restrict__ zun, const double *__restrict__ Delta, double *__restrict__ Psi) {
     __shared__ double tile[16][17];
     const int t = blockIdx.x, tid = threadIdx.x;
@@ -1106,11 +1413,12 @@ __global__ void __launch_bounds__(256) k_gram_reduce_sym(int P, int T16, int nse
 #pragma unroll
             for (int u = 0; u < GRU; u++) pv[u] = part[(((size_t)(g + u) * J + j) * NT + t) * 256 + tid];
 #pragma unroll
-            for (int u = 0; u < GRU; u++) s += (double)pv[u];
+            for (int u = 0; u < GRU; u++) s += (double)pv[u] * wun[(size_t)(g + u) * wstride + j];
         }
-        for (; g < nseg; g++) s += (double)part[(((size_t)g * J + j) * NT + t) * 256 + tid];
+        for (; g < nseg; g++)
+            s += (double)part[(((size_t)g * J + j) * NT + t) * 256 + tid] * wun[(size_t)g * wstride + j];
         double v = 0.0;
-        if (row < P && col < P) v = s * (wun[j] * zun[row] * zun[col]) + Delta[row + (size_t)col * P];
+        if (row < P && col < P) v = s * (zun[row] * zun[col]) + Delta[row + (size_t)col * P];
         tile[rr][cc] = v;
         __syncthreads();
         double *Pj = Psi + (size_t)j * P * P;
@@ -1297,7 +1605,7 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
         if (r) return r;
         gp_timer_end(c, c->t_gramk);
         dim3 gr((unsigned)(TG * (TG + 1) / 2), (unsigned)((batch + RSG - 1) / RSG));
-        k_gram_reduce_sym<<<gr, 256, 0, c->stream>>>(c->P, TG, nseg, batch, 1, c->gram_part, c->wun,
+        k_gram_reduce_sym<<<gr, 256, 0, c->stream>>>(c->P, TG, nseg, batch, 1, c->gram_part, c->wun, 0,
                                                       c->zun, Delta, Psi);
         GP_HIP(hipGetLastError());
         return gram_intercept_row(c, batch, Delta, Psi);
@@ -1347,7 +1655,7 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
         if (r) return r;
         gp_timer_end(c, c->t_gramk);
         dim3 gr((unsigned)(T16 * (T16 + 1) / 2), (unsigned)((batch + RSG - 1) / RSG));
-        k_gram_reduce_sym<<<gr, 256, 0, c->stream>>>(c->P, T16, nseg, batch, 0, c->gram_part, c->wun,
+        k_gram_reduce_sym<<<gr, 256, 0, c->stream>>>(c->P, T16, nseg, batch, 0, c->gram_part, c->wun, 0,
                                                       c->zun, Delta, Psi);
         GP_HIP(hipGetLastError());
         return 0;
@@ -1359,6 +1667,121 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
     dim3 gr((unsigned)(((size_t)c->P * c->P + 255) / 256), batch);
     k_gram_reduce<<<gr, 256, 0, c->stream>>>(c->P, c->P, 0, nbk, nseg, batch, c->gram_part, c->wun,
                                               c->zun, Delta, Psi);
+    GP_HIP(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Fused IRLS step on the mixed path: eta = Z beta (rocBLAS, FP64), the
+// per-cell constants, then one launch of k_gram_irls that builds the weights,
+// nu, R and the sums and accumulates the symmetric Gram; the per-gene sums,
+// the segment reduction (per-segment unscale) and, in the peeled form, the
+// intercept row from the w32 it wrote.
+
+// the symmetric eight-wave kernel covers this design: the GP block (peeled)
+// or the whole design in at most MAX_SYM_T16 column tiles
+static bool irls_peel(const gp_ctx *c) {
+    return c->gram_peel && c->g0 == 1 && c->M % 16 == 0 && c->M / 16 <= MAX_SYM_T16 &&
+           c->Npad % GRK_SEG == 0;
+}
+
+bool gpk_gram_irls_fits(const gp_ctx *c) {
+    return irls_peel(c) || (c->P + 15) / 16 <= MAX_SYM_T16;
+}
+
+template <int T16>
+static int launch_gram_irls(gp_ctx *c, int seg, int nseg, const float *zsrc) {
+    constexpr int P16 = 16 * T16;
+    const size_t shm = (size_t)2 * 2 * P16 * GLDX * sizeof(_Float16);
+    if (shm > 65536 && !c->capturing)
+        GP_HIP(hipFuncSetAttribute((const void *)k_gram_irls<T16>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    k_gram_irls<T16><<<(unsigned)(c->J * nseg), GW * 64, shm, c->stream>>>(
+        c->N, c->Npad, seg, nseg, c->J, zsrc, c->ZB, c->irls_cc, c->Yi, c->Yx, c->irls_segptr,
+        c->w32, c->R, c->irls_sums, c->irls_segun, c->gram_part);
+    GP_HIP(hipGetLastError());
+    return 0;
+}
+
+int gpk_gram_irls(gp_ctx *c, double *Psi, const double *Delta) {
+    if (!gpk_gram_irls_fits(c)) { gp_set_error("fused IRLS Gram: design too wide"); return 1; }
+    const int T16 = (c->P + 15) / 16;
+    int nseg = c->gram_splits > 0 ? c->gram_splits : 1;
+    GP_TRY(ensure_mixed(c, nseg));
+    if (!c->zsplit_valid) GP_TRY(gpk_split_z(c, c->Zw));
+    int seg = (c->Npad + nseg - 1) / nseg;
+    seg = (seg + GCH - 1) / GCH * GCH;
+    nseg = (c->Npad + seg - 1) / seg;
+    if (!c->irls_cc) {
+        GP_HIP(hipMalloc((void **)&c->irls_cc, (size_t)3 * c->Npad * sizeof(double)));
+    }
+    if (c->irls_nseg != nseg || c->irls_seg != seg) {
+        if (c->irls_segptr) (void)hipFree(c->irls_segptr);
+        if (c->irls_sums) (void)hipFree(c->irls_sums);
+        if (c->irls_segun) (void)hipFree(c->irls_segun);
+        c->irls_segptr = nullptr;
+        c->irls_sums = c->irls_segun = nullptr;
+        c->irls_nseg = c->irls_seg = 0;
+        const size_t ns = (size_t)c->J * (nseg + 1);
+        GP_HIP(hipMalloc((void **)&c->irls_segptr, ns * sizeof(int)));
+        GP_HIP(hipMalloc((void **)&c->irls_sums, (size_t)nseg * c->J * 4 * sizeof(double)));
+        GP_HIP(hipMalloc((void **)&c->irls_segun, (size_t)nseg * c->J * sizeof(double)));
+        k_irls_segptr<<<(unsigned)((ns + 255) / 256), 256, 0, c->stream>>>(c->J, nseg, seg, c->Yp,
+                                                                           c->Yi, c->irls_segptr);
+        GP_HIP(hipGetLastError());
+        c->irls_nseg = nseg;
+        c->irls_seg = seg;
+    }
+    // eta = Z beta for every cell and gene (beta in the reference basis)
+    const double one = 1.0, zero = 0.0;
+    GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, c->Npad, c->J,
+                          c->P, &one, c->Z, c->Npad, c->Beta, c->P, &zero, c->ZB, c->Npad));
+    k_irls_cellc<<<(unsigned)((c->Npad + 255) / 256), 256, 0, c->stream>>>(
+        c->N, c->Npad, c->Omega, c->ls, c->titsias, c->delta + c->p - 1, 0.1, c->irls_cc);
+    GP_HIP(hipGetLastError());
+    const bool peel = irls_peel(c);
+    const int T = peel ? c->M / 16 : T16;
+    const float *zg = peel ? c->z32 + c->Npad : c->z32;
+    int r = 0;
+    gp_timer_begin(c, c->t_gramk);
+    switch (T) {
+    case 1: r = launch_gram_irls<1>(c, seg, nseg, zg); break;
+    case 2: r = launch_gram_irls<2>(c, seg, nseg, zg); break;
+    case 3: r = launch_gram_irls<3>(c, seg, nseg, zg); break;
+    case 4: r = launch_gram_irls<4>(c, seg, nseg, zg); break;
+    case 5: r = launch_gram_irls<5>(c, seg, nseg, zg); break;
+    case 6: r = launch_gram_irls<6>(c, seg, nseg, zg); break;
+    case 7: r = launch_gram_irls<7>(c, seg, nseg, zg); break;
+    case 8: r = launch_gram_irls<8>(c, seg, nseg, zg); break;
+    case 9: r = launch_gram_irls<9>(c, seg, nseg, zg); break;
+    case 10: r = launch_gram_irls<10>(c, seg, nseg, zg); break;
+    case 11: r = launch_gram_irls<11>(c, seg, nseg, zg); break;
+    case 12: r = launch_gram_irls<12>(c, seg, nseg, zg); break;
+    case 13: r = launch_gram_irls<13>(c, seg, nseg, zg); break;
+    case 14: r = launch_gram_irls<14>(c, seg, nseg, zg); break;
+    case 15: r = launch_gram_irls<15>(c, seg, nseg, zg); break;
+    case 16: r = launch_gram_irls<16>(c, seg, nseg, zg); break;
+    default: r = launch_gram_irls<17>(c, seg, nseg, zg); break;
+    }
+    if (r) return r;
+    gp_timer_end(c, c->t_gramk);
+    k_irls_sums<<<(unsigned)((c->J + 255) / 256), 256, 0, c->stream>>>(c->J, nseg, c->irls_sums, c->gq,
+                                                                      c->gs, c->lkhds);
+    GP_HIP(hipGetLastError());
+    dim3 gr((unsigned)(T * (T + 1) / 2), (unsigned)((c->J + RSG - 1) / RSG));
+    k_gram_reduce_sym<<<gr, 256, 0, c->stream>>>(c->P, T, nseg, c->J, peel ? 1 : 0, c->gram_part,
+                                                  c->irls_segun, (size_t)c->J, c->zun, Delta, Psi);
+    GP_HIP(hipGetLastError());
+    if (peel) GP_TRY(gram_intercept_row(c, c->J, Delta, Psi));
+    return 0;
+}
+
+// W and nu of the last fused IRLS pass in the reference layout
+int gpk_irls_wnu(gp_ctx *c) {
+    dim3 g((unsigned)((c->Npad + 255) / 256), c->J);
+    k_irls_wnu<<<g, 256, 0, c->stream>>>(c->N, c->Npad, c->ZB, c->irls_cc, c->W, c->Nu);
+    GP_HIP(hipGetLastError());
+    k_irls_nu_nz<<<c->J, 256, 0, c->stream>>>(c->Npad, c->Yp, c->Yi, c->Yx, c->ZB, c->W, c->Nu);
     GP_HIP(hipGetLastError());
     return 0;
 }

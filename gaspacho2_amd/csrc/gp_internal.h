@@ -140,6 +140,17 @@ struct gp_ctx {
     float *gc_s0 = nullptr;    // J x (M+1)  [Sw_j[0, 1:] | Sw_j[0, 0]] (intercept output column)
     float *gc_y = nullptr;     // Npad x (M+1)  w32 . gc_s0
     int irls_mixed = 1;        // mixed path: IRLS steps on the whitened FP16 hi/lo Gram (0: FP64)
+    int irls_fused = 1;        // mixed IRLS steps: weights, nu, R and the sums built inside the Gram kernel
+    // fused IRLS pass: per-cell constants [d | Omega | log ls] (3 x Npad), each
+    // gene's first non-zero per cell segment, per-(segment, gene) sums and
+    // weight unscales, and the segmentation they were made for
+    double *irls_cc = nullptr;
+    int *irls_segptr = nullptr;
+    double *irls_sums = nullptr, *irls_segun = nullptr;
+    int irls_nseg = 0, irls_seg = 0;
+    // 1 after a fused IRLS step: W and Nu are not written yet; ZB and
+    // irls_cc still hold the eta and constants they follow from
+    int wnu_pending = 0;
     int autotune = 0;          // gp_fit runs gp_autotune before its first quasi-likelihood phase
     // 1 while R, bvec, Oinvt and pwork = Psi^-1 bvec hold the likelihood
     // evaluation (gp_elbo / gp_ql_start) at the current parameters and factors,
@@ -254,6 +265,16 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
                    const double *Delta, bool weights_ready = false);
 // w32 = W/Omega in FP32 and the per-gene power-of-two scales (all genes)
 int gpk_wprep(gp_ctx *c, const double *W);
+// fused IRLS step on the mixed path: eta = Z beta, then one kernel that builds
+// W / Omega, nu and R = W nu / Omega from eta and accumulates the whitened
+// Gram; writes Psi (whitened, unfactored), R, w32, gq, gs and lkhds
+bool gpk_gram_irls_fits(const gp_ctx *c);
+int gpk_gram_irls(gp_ctx *c, double *Psi, const double *Delta);
+// W and Nu of the last fused IRLS step, in the reference layout
+int gpk_irls_wnu(gp_ctx *c);
+// writes W and Nu if a fused IRLS step left them pending (every entry point
+// that may read them or change what they follow from calls it first)
+int gp_wnu_sync(gp_ctx *c);
 
 // ---------------- gp_gradcontract_mixed.hip ----------------
 // Uw = sum_j diag(W_j/Omega) Zw Sinv_j on FP16 hi/lo MFMA (whitened basis)

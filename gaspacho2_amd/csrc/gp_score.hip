@@ -261,6 +261,7 @@ int score_tests_host(gp_ctx *c, int S, const GT *G_host, int ND, const int *dono
                      const double *d_host, int T, const int *gene_host, const int *snp_host,
                      double *stat_host, double *pval_host) {
     if (S <= 0 || T <= 0) { gp_set_error("score tests: S and T must be positive"); return 1; }
+    GP_TRY(gp_wnu_sync(c));   // the tests read W
     if (donor_host) {
         if (ND <= 0) { gp_set_error("score tests: the donor count must be positive"); return 1; }
         for (int i = 0; i < c->N; i++)

@@ -57,6 +57,7 @@ extern "C" int gp_score_tests_dosage_noalloc(gp_ctx *c, int S, const unsigned ch
                                              double *stat_dev) {
     if (!c || !dosage_dev || !gene_dev || !snp_dev || !stat_dev) { gp_set_error("NULL argument"); return 1; }
     if (S <= 0 || T <= 0) { gp_set_error("score tests: S and T must be positive"); return 1; }
+    GP_TRY(gp_wnu_sync(c));   // the tests read W (a kernel; no allocation)
     const size_t per = gpk_score_part_doubles(c->P);
     if (c->P > gpk_score_max_columns() || per > SCRATCH_PART) {
         gp_set_error("score tests without allocation: " + std::to_string(c->P) +

@@ -301,6 +301,63 @@ def test_mixed_irls_steps_close_to_fp64(shape):
         g.close()
 
 
+@pytest.mark.parametrize("shape", [(300, 20, 3, 8, 2, 3), (2000, 200, 5, 32, 0, 0),
+                                   (3000, 24, 5, 256, 0, 0)])
+def test_fused_irls_step_matches_separate_kernels(shape):
+    """The fused IRLS step of a mixed context (option irls_fused, the default:
+    W / Omega, nu, R and the sums built inside the Gram kernel from
+    eta = z^T beta, ggplvm.cu:468-514 with 621-680) and the same step on
+    separate kernels (W and nu written, then the FP32 weights, the Gram, R and
+    the sums, the GLM objective), both against the FP64 steps from the same
+    start.  After the first step the fused form's W and nu, written out only
+    when read, equal the separate kernels' to rounding (1e-13: it computes W
+    as exp(eta - d) Omega).  Over six steps both mixed forms keep the
+    objectives and the quasi-likelihood start that follows within 1e-7 of
+    FP64 (at M = 256 both measure ~1.2e-8; at M = 8 and 32 they are within the
+    1e-8 of the test above) and beta and phi within 1e-5 of their largest
+    entry, and the fused objectives are no further from FP64 than twice the
+    separate form's (the fused Gram scales its FP16 operands per cell segment
+    rather than per gene, so the two round differently).  Covariates (the
+    whole-design kernel), M = 32 and M = 256 (the GP-block kernel at the main
+    workload's width); N = 300 leaves cell segments that hold only padding."""
+    from gaspacho2_amd import lib, synth
+    N, G, Q, M, nnum, nlev = shape
+    ds = synth.make_dataset(N, G, Q, M, seed=2, n_numeric=nnum, n_levels=nlev)
+    T0 = lib.init_ta_with_per(ds.M, ds.Q)
+    ctx = {}
+    for key, prec, fused in (("f64", lib.PREC_FP64, 0), ("sep", lib.PREC_MIXED, 0),
+                             ("fused", lib.PREC_MIXED, 1)):
+        g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls,
+                          ds.X0, T0, precision=prec)
+        g.set_option("irls_fused", fused)
+        g.init_fit()
+        ctx[key] = g
+    assert ctx["fused"].get_option("irls_fused") == 1
+    err = {"sep": 0.0, "fused": 0.0}
+    for jtr in range(6):
+        ref = ctx["f64"].irls_step(jtr)
+        for k in err:
+            err[k] = max(err[k], abs(ctx[k].irls_step(jtr) - ref) / abs(ref))
+        if jtr == 0:
+            rw = rel(ctx["fused"].get("W"), ctx["sep"].get("W"))
+            rn = rel(ctx["fused"].get("Nu"), ctx["sep"].get("Nu"))
+    rb = {k: rel(ctx[k].get("Beta"), ctx["f64"].get("Beta")) for k in err}
+    rp = {k: rel(ctx[k].get("phi"), ctx["f64"].get("phi")) for k in err}
+    q64 = ctx["f64"].ql_start()
+    rq = {k: abs(ctx[k].ql_start() - q64) / abs(q64) for k in err}
+    print(f"first step W {rw:.2e}, Nu {rn:.2e}; vs FP64 (fused / separate): objectives "
+          f"{err['fused']:.2e} / {err['sep']:.2e}, beta {rb['fused']:.2e} / {rb['sep']:.2e}, "
+          f"phi {rp['fused']:.2e} / {rp['sep']:.2e}, quasi-likelihood start "
+          f"{rq['fused']:.2e} / {rq['sep']:.2e}")
+    assert rw < 1e-13 and rn < 1e-13
+    for k in err:
+        assert err[k] < 1e-7 and rq[k] < 1e-7, k
+        assert rb[k] < 1e-5 and rp[k] < 1e-5, k
+    assert err["fused"] <= 2.0 * err["sep"] + 1e-9
+    for g in ctx.values():
+        g.close()
+
+
 def test_mixed_gradient_contraction_at_m256():
     """Mixed path at P = 257 (M = 256, the widest column-tile variant of both
     FP16 MFMA kernels).  The FP16 hi/lo contraction against the FP64
