@@ -644,6 +644,65 @@ def main_warmstart(args):
     print(json.dumps(out), flush=True)
 
 
+def main_irls(args):
+    """GLM-phase IRLS steps (getWForBeta, getPsi, cholPsi, getBetaPhi,
+    getLkhdPois: ggplvm.cu:1156-1181) at the ELBO workload's size: the mixed
+    step with the IRLS weight pass fused into the Gram kernel (the default),
+    the same step on separate kernels, and the FP64 step, each from init_fit
+    in its own context.  Reports per-step times, the Gram kernel's HIP-event
+    time and each mixed form's objective error against FP64 at every step."""
+    from gaspacho2_amd import lib, synth
+    ds = synth.make_dataset(args.N, args.G, args.Q, args.M, seed=0)
+    T0 = lib.init_ta_with_per(ds.M, ds.Q)
+    P = ds.P
+    nsteps = args.warmup + args.steps
+    res = {}
+    for name, prec, fused in (("f64", lib.PREC_FP64, 0), ("fused", lib.PREC_MIXED, 1),
+                              ("separate", lib.PREC_MIXED, 0)):
+        g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls,
+                          ds.X0, T0, ard_only=args.kernel == "ard", precision=prec)
+        g.set_option("irls_fused", fused)
+        g.init_fit()
+        obj, ms = [], []
+        for jtr in range(nsteps):
+            if jtr == args.warmup:
+                g.timing(True)
+                g.timing_reset()
+            t1 = time.perf_counter()
+            obj.append(g.irls_step(jtr))      # returns after the objective's read-back
+            ms.append((time.perf_counter() - t1) * 1e3)
+        g.timing(False)
+        kern = {k: g.timing_get(k)[0] for k in ("gram", "gram_kernel", "chol")}
+        g.close()
+        res[name] = {"obj": obj, "ms": ms[args.warmup:], "kern": kern}
+        log(f"[irls {name}] {np.median(ms[args.warmup:]):.2f} ms per step, Gram kernel "
+            f"{kern['gram_kernel']:.2f} ms, objective {obj[-1]:.12e}")
+    ref = np.array(res["f64"]["obj"])
+    for k in ("fused", "separate"):
+        res[k]["obj_rel_err"] = float(np.max(np.abs(np.array(res[k]["obj"]) - ref) / np.abs(ref)))
+    f = res["fused"]
+    ms_step = float(np.median(f["ms"]))
+    kt = f["kern"]["gram_kernel"]
+    fl = gram_flops(ds.N, ds.J, P)
+    T = ds.M // 16 if int(ds.nh[-2]) == 1 and ds.M % 16 == 0 else (P + 15) // 16
+    out = {"metric": f"mixed IRLS steps/s at N={ds.N},G={ds.J},M={ds.M}", "value": 1e3 / ms_step,
+           "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": ms_step, "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+           "dtype": "f16x2-mfma/f32-acc/f64",
+           "data": "synthetic (Poisson counts from the SE-ARD GP recipe, seed 0; PCA-initialised X)",
+           "config": {"workload": f"GLM-phase IRLS steps, N={ds.N} cells, G={ds.J} genes, Q={ds.Q}, "
+                                  f"M={ds.M}, P={P}, no covariates",
+                      "N": ds.N, "G": ds.J, "Q": ds.Q, "M": ds.M, "P": P},
+           "roofline": {"kernel": f"k_gram_irls<{T}> (fused IRLS weights + Gram)", "bound": "mfma",
+                        "achieved": fl / (kt * 1e-3) / 1e12, "peak": PEAK["mixed"] / 1e12,
+                        "unit": "TFLOP/s", "frac": fl / (kt * 1e-3) / PEAK["mixed"],
+                        "traffic": None, "kernel_ms": kt, "flops_per_launch": fl},
+           "forms": {k: {"ms_per_step": float(np.median(v["ms"])), "kernels_ms": v["kern"],
+                         "objective_rel_err_vs_fp64": v.get("obj_rel_err")} for k, v in res.items()},
+           "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -682,10 +741,12 @@ def main():
     ap.add_argument("--enc-epochs", type=int, default=30, help="warmstart workload: encoder epochs")
     ap.add_argument("--enc-target", default="residual,aligned",
                     help="warmstart workload: encoder targets, comma-separated (first = headline)")
-    ap.add_argument("--workload", choices=["elbo", "batched", "fit", "fitcheck", "warmstart", "score"],
+    ap.add_argument("--workload", choices=["elbo", "batched", "fit", "fitcheck", "warmstart", "score",
+                                           "irls"],
                     default="elbo",
                     help="elbo: the ELBO+gradient unit (default); batched: uneven per-gene GLM fits; "
-                         "fitcheck: mixed-vs-FP64 ELBO and gradient errors at every L-BFGS iterate")
+                         "fitcheck: mixed-vs-FP64 ELBO and gradient errors at every L-BFGS iterate; "
+                         "irls: GLM-phase IRLS steps, fused and separate weight pass")
     ap.add_argument("--nmin", type=int, default=500)
     ap.add_argument("--donors", type=int, default=200, help="score workload: donors")
     ap.add_argument("--snps-per-gene", type=int, default=10, help="score workload: cis SNPs per gene")
@@ -727,6 +788,9 @@ def main():
         return
     if args.workload == "warmstart":
         main_warmstart(args)
+        return
+    if args.workload == "irls":
+        main_irls(args)
         return
 
     from gaspacho2_amd import lib, synth

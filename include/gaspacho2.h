@@ -92,6 +92,10 @@ int gp_set_precision(gp_ctx *ctx, int precision);
  *   "irls_mixed"  0/1       mixed path: IRLS steps on the whitened FP16
  *                           hi/lo Gram (1) or in FP64 in the reference
  *                           basis (0)
+ *   "irls_fused"  0/1       mixed IRLS steps: the weight update (W/Omega,
+ *                           nu, R and the per-gene sums) inside the Gram
+ *                           kernel, W and Nu written only when read (1), or
+ *                           on separate kernels before the Gram (0)
  *   "autotune"    0/1       gp_fit calls gp_autotune before its first
  *                           quasi-likelihood phase (mixed precision only)
  * Unknown names and out-of-range values return a status > 0. */
@@ -124,7 +128,10 @@ int gp_init_fit(gp_ctx *ctx);
 
 /* ggplvm.cu:1156-1181: one IRLS/Laplace step of the GLM phase
  * (getWForBeta, getPsi, cholPsi, getBetaPhi(jtr), getLkhdPois, collectLkhd
- * with N=0).  *obj receives lkhd[jtr]. */
+ * with N=0).  *obj receives lkhd[jtr].  With "irls_fused" (the default of a
+ * mixed context) getWForBeta runs inside the Gram kernel and W / Nu are
+ * written on the next call that reads them; every other entry point sees
+ * them as the reference leaves them. */
 int gp_irls_step(gp_ctx *ctx, int jtr, double *obj);
 
 /* ggplvm.cu:1205-1211: getOinvt, cholDeltaInv, getLkhd, collectLkhd(jtr=0):
