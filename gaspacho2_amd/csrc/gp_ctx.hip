@@ -622,9 +622,15 @@ extern "C" int gp_irls_step(gp_ctx *c, int jtr, double *obj) {
 }
 
 // getLkhd: b_j, beta-tilde_j (scratch), the factor diagonals and the sums.
-static int lkhd_genes(gp_ctx *c) {
+// r_ready: R, gq and gs already hold the current W, nu and Omega (the ELBO's
+// side stream wrote them), so only b_j = Z^T R_j is left.
+static int lkhd_genes(gp_ctx *c, bool r_ready = false) {
     const bool w = c->psi_whitened;
-    GP_TRY(gpk_r_and_sums(c, w ? c->Zw : c->Z));
+    if (r_ready)
+        GP_TRY(gpk_gemm_tn_splitk(c, c->P, c->J, c->Npad, w ? c->Zw : c->Z, c->Npad, c->R, c->Npad,
+                                  c->bvec, c->P));
+    else
+        GP_TRY(gpk_r_and_sums(c, w ? c->Zw : c->Z));
     GP_TRY(gpk_cholsl_batched(c, c->Psi, c->P, (size_t)c->P * c->P, c->bvec, c->pwork, c->J));
     GP_TRY(gpk_lkhd_genes_with(c, c->pwork, w ? c->DeltaWChol : c->DeltaChol));
     c->lkhd_fresh = 1;
@@ -642,16 +648,18 @@ extern "C" int gp_ql_start(gp_ctx *c, double *obj) {
     return 0;
 }
 
-// The ELBO's side chain (see gp_elbo): Oinvt, and for the mixed Gram the
-// FP32 weights, on the side stream from the point this is called.
+// The ELBO's side chain (see gp_elbo): Oinvt, and the likelihood's R, gq
+// and gs together with (mixed) the Gram's FP32 weights, on the side stream
+// from the point this is called.
 static int fork_weights_impl(gp_ctx *c, bool mixed) {
+    if (mixed) GP_TRY(gpk_mixed_alloc(c));   // allocations first (no stream work)
     GP_HIP(hipEventRecord(c->ev_efork, c->stream));
     GP_HIP(hipStreamWaitEvent(c->stream2, c->ev_efork, 0));
     c->efork_pending = 1;
     {
         SideStream side(c);
         GP_TRY(gpk_oinvt(c));
-        if (mixed) GP_TRY(gpk_wprep(c, c->W));
+        GP_TRY(gpk_r_sums(c, mixed));
     }
     GP_HIP(hipEventRecord(c->ev_ejoin, c->stream2));
     return 0;
@@ -667,8 +675,9 @@ extern "C" int gp_elbo(gp_ctx *c, int slot, double *obj) {
     gp_timer_begin(c, c->t_elbo);
     GP_TRY(clear_info(c));
     {
-        // Oinvt and (mixed) the FP32 weights W/Omega depend on neither the
-        // kernel matrices nor the basis: forked to the side stream after the
+        // Oinvt, the likelihood's R = W nu / Omega with its sums and (mixed)
+        // the FP32 weights W/Omega depend on neither the kernel matrices nor
+        // the basis: forked to the side stream after the
         // HBM-bound Kuf, they run beside the single-matrix Kmm factor and its
         // small follow-ups, which leave the chip idle, and are joined before
         // the Gram; the join also runs on an early error return, so no side
@@ -688,7 +697,7 @@ extern "C" int gp_elbo(gp_ctx *c, int slot, double *obj) {
         }
         GP_TRY(gram_chol(c, c->precision, mixed));
     }
-    GP_TRY(lkhd_genes(c));
+    GP_TRY(lkhd_genes(c, true));
     GP_TRY(gpk_collect(c, c->N, slot, obj));
     gp_timer_end(c, c->t_elbo);
     int bad = 0;

@@ -1489,6 +1489,8 @@ int gpk_split_z(gp_ctx *c, const double *Zsrc) {
     return 0;
 }
 
+int gpk_mixed_alloc(gp_ctx *c) { return ensure_mixed(c, c->gram_splits); }
+
 int gpk_wprep(gp_ctx *c, const double *W) {
     GP_TRY(ensure_mixed(c, c->gram_splits));
     k_wprep<<<c->J, 256, 0, c->stream>>>(c->N, c->Npad, W, c->Omega, c->w32, c->wsc, c->wun);

@@ -208,6 +208,9 @@ int gpk_oinvt(gp_ctx *c);
 int gpk_titsias(gp_ctx *c);                   // A = Knm Kinv (rocBLAS) + titsias
 int gpk_w_nu(gp_ctx *c);                      // IRLS W / Nu from ZB = Z*Beta
 int gpk_r_and_sums(gp_ctx *c, const double *Zsrc);  // R = W nu / Omega, gq, gs, b = Zsrc^T R
+// R, gq and gs (the bits of gpk_r_and_sums), and with w32 the mixed Gram's
+// FP32 weights and scales (gpk_wprep's values), in one pass over W, nu, Omega
+int gpk_r_sums(gp_ctx *c, bool w32);
 int gpk_init_y0(gp_ctx *c);                   // R = log((y+0.05)/ls) dense
 int gpk_beta_phi(gp_ctx *c, int jtr);
 int gpk_beta_phi_in(gp_ctx *c, int jtr, double *Beta);     // damped update of Beta, phi
@@ -265,6 +268,8 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
                    const double *Delta, bool weights_ready = false);
 // w32 = W/Omega in FP32 and the per-gene power-of-two scales (all genes)
 int gpk_wprep(gp_ctx *c, const double *W);
+// allocates the mixed Gram's buffers (FP16/FP32 model matrix, FP32 weights)
+int gpk_mixed_alloc(gp_ctx *c);
 // fused IRLS step on the mixed path: eta = Z beta, then one kernel that builds
 // W / Omega, nu and R = W nu / Omega from eta and accumulates the whitened
 // Gram; writes Psi (whitened, unfactored), R, w32, gq, gs and lkhds

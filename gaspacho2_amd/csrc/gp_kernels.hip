@@ -254,7 +254,79 @@ __global__ void __launch_bounds__(NT) k_r_sums(int N, int Npad, const double *__
     if (threadIdx.x == 0) { gq[j] = q; gs[j] = s; }
 }
 
-// init.cu:193-221 right-hand side: log((y + 0.05) / ls) at non-zeros,
+// k_r_sums (same traversal and expressions, so the same bits) that also
+// writes, when w32 is given, the mixed Gram's FP32 weights W / Omega and
+// their per-gene power-of-two scale (the values k_wprep writes): one pass
+// over W, nu and Omega serves the likelihood and the Gram.
+__global__ void __launch_bounds__(NT) k_r_sums_w32(int N, int Npad, const double *__restrict__ W,
+        const double *__restrict__ Nu, const double *__restrict__ Omega, double *__restrict__ R,
+        double *__restrict__ gq, double *__restrict__ gs, float *__restrict__ w32,
+        float *__restrict__ wsc, double *__restrict__ wun) {
+    __shared__ double sh[4];
+    __shared__ float shm[4];
+    const int j = blockIdx.x;
+    const double *Wj = W + (size_t)j * Npad, *nj = Nu + (size_t)j * Npad;
+    double *Rj = R + (size_t)j * Npad;
+    float *wj = w32 ? w32 + (size_t)j * Npad : nullptr;
+    double q = 0.0, s = 0.0;
+    float mx = 0.0f;
+    int i = threadIdx.x;
+    for (; i + (RSU - 1) * NT < N; i += RSU * NT) {
+        double w[RSU], nu[RSU], om[RSU];
+#pragma unroll
+        for (int u = 0; u < RSU; u++) {
+            w[u] = Wj[i + u * NT];
+            nu[u] = nj[i + u * NT];
+            om[u] = Omega[i + u * NT];
+        }
+#pragma unroll
+        for (int u = 0; u < RSU; u++) {
+            const double wo = w[u] / om[u];
+            Rj[i + u * NT] = wo * nu[u];
+            q += nu[u] * nu[u] * w[u] / om[u];
+            s += log(om[u] / w[u]);
+            if (wj) {
+                const float v = (float)wo;
+                wj[i + u * NT] = v;
+                mx = fmaxf(mx, fabsf(v));
+            }
+        }
+    }
+    for (; i < Npad; i += NT) {
+        if (i < N) {
+            const double w = Wj[i], nu = nj[i], om = Omega[i];
+            const double wo = w / om;
+            Rj[i] = wo * nu;
+            q += nu * nu * w / om;
+            s += log(om / w);
+            if (wj) {
+                const float v = (float)wo;
+                wj[i] = v;
+                mx = fmaxf(mx, fabsf(v));
+            }
+        } else {
+            Rj[i] = 0.0;
+            if (wj) wj[i] = 0.0f;
+        }
+    }
+    q = block_sum256(q, sh);
+    s = block_sum256(s, sh);
+    if (threadIdx.x == 0) { gq[j] = q; gs[j] = s; }
+    if (wj) {
+        for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_down(mx, off, 64));
+        if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = mx;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const float m = fmaxf(fmaxf(shm[0], shm[1]), fmaxf(shm[2], shm[3]));
+            int e = 0;
+            if (m > 0.0f) (void)frexpf(m, &e);      // m <= 2^e
+            wsc[j] = ldexpf(1.0f, -e);
+            wun[j] = ldexp(1.0, e);
+        }
+    }
+}
+
+// init.cu:193-221 right-hand side: log((y + 0.05) / ls) at non-zeros,// init.cu:193-221 right-hand side: log((y + 0.05) / ls) at non-zeros,
 // log(0.05 / ls) elsewhere.
 __global__ void k_y0_dense(int N, int Npad, const double *ls, double *R) {
     const int i = blockIdx.x * NT + threadIdx.x;
@@ -875,6 +947,16 @@ int gpk_w_nu(gp_ctx *c) {
     k_nu_sparse<<<c->J, NT, 0, c->stream>>>(c->Npad, c->Yp, c->Yi, c->Yx, c->ZB, c->ls,
                                             c->titsias, dt, 0.1, c->Nu);
     LAUNCH_CHECK();
+    return 0;
+}
+
+int gpk_r_sums(gp_ctx *c, bool w32) {
+    if (w32 && (!c->w32 || !c->wsc || !c->wun)) { gp_set_error("FP32 weight buffers not allocated"); return 1; }
+    k_r_sums_w32<<<c->J, NT, 0, c->stream>>>(c->N, c->Npad, c->W, c->Nu, c->Omega, c->R, c->gq, c->gs,
+                                            w32 ? c->w32 : nullptr, w32 ? c->wsc : nullptr,
+                                            w32 ? c->wun : nullptr);
+    LAUNCH_CHECK();
+    if (w32) c->w32_valid = 1;
     return 0;
 }
 
