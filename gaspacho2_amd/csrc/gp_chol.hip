@@ -164,6 +164,141 @@ __global__ void __launch_bounds__(CT) k_chol(double *__restrict__ Aall, int n, s
     }
 }
 
+// One matrix (the Kmm factor): right-looking blocked Cholesky on a
+// single 1024-thread workgroup.  Per 32-column panel: the panel's rows are
+// staged in LDS, the 32 x 32 diagonal block is factored on wave 0 (rows in
+// lanes, pivots and columns by shuffles), the rows below are solved against
+// it, the panel is written back, and the trailing lower triangle takes the
+// panel's rank-32 update in 4 x 4 register blocks.  The batched kernel gives
+// a lone matrix one 256-thread workgroup and reloads every earlier column per
+// panel; here the trailing update runs over all the workgroup's threads.
+// Same layout and zero-fill rule as k_chol (reads the lower triangle, L to
+// lower + diag, the strict upper triangle untouched; a non-positive pivot
+// zero-fills the matrix and counts it).
+constexpr int C1T = 1024;   // threads
+constexpr int NB1 = 32;     // panel width
+constexpr int PS1 = NB1 + 1;  // LDS row stride of the panel (doubles)
+constexpr int MAXN1 = 512;  // largest n (the panel's rows take n * 33 * 8 bytes of LDS)
+
+__global__ void __launch_bounds__(C1T) k_chol1(double *__restrict__ a, int n, int *__restrict__ info) {
+    extern __shared__ double pan[];    // [rows][PS1]
+    __shared__ int bad;
+    const int tid = threadIdx.x;
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    for (int kb = 0; kb < n; kb += NB1) {
+        const int nb = min(NB1, n - kb);
+        const int rows = n - kb;
+        // 1. the panel's lower part (earlier panels' updates already applied)
+        for (int t = tid; t < rows * nb; t += C1T) {
+            const int r = t % rows, cc = t / rows;
+            pan[r * PS1 + cc] = (r >= cc) ? a[(kb + r) + (size_t)(kb + cc) * n] : 0.0;
+        }
+        __syncthreads();
+        // 2. the diagonal block on wave 0
+        if (tid < 64) {
+            const int r = tid;
+            double x[NB1];
+#pragma unroll
+            for (int cc = 0; cc < NB1; cc++) x[cc] = (r < nb && cc < nb) ? pan[r * PS1 + cc] : 0.0;
+            // the pivot is the same on every lane, so fail is wave-uniform;
+            // after a failure the remaining columns are skipped
+            int fail = 0;
+#pragma unroll
+            for (int cc = 0; cc < NB1; cc++) {
+                const double piv = __shfl(x[cc], cc);
+                if (cc < nb && piv <= 0.0) fail = 1;    // same test as chol.cu:26
+                if (cc < nb && !fail) {
+                    const double d = sqrt(piv);
+                    if (r == cc) x[cc] = d;
+                    else if (r > cc) x[cc] /= d;
+                    const double lc = x[cc];
+#pragma unroll
+                    for (int c2 = cc + 1; c2 < NB1; c2++) {
+                        const double l2 = __shfl(lc, c2);
+                        if (r >= c2 && c2 < nb) x[c2] -= lc * l2;
+                    }
+                }
+            }
+            if (r < nb) {
+#pragma unroll
+                for (int cc = 0; cc < NB1; cc++)
+                    if (cc <= r) pan[r * PS1 + cc] = x[cc];
+            }
+            if (tid == 0 && fail) bad = 1;
+        }
+        __syncthreads();
+        if (bad) break;
+        // 3. the rows below the diagonal block: forward substitution
+        for (int r = nb + tid; r < rows; r += C1T) {
+            double x[NB1];
+#pragma unroll
+            for (int cc = 0; cc < NB1; cc++) x[cc] = (cc < nb) ? pan[r * PS1 + cc] : 0.0;
+#pragma unroll
+            for (int cc = 0; cc < NB1; cc++) {
+                if (cc < nb) {
+                    double v = x[cc];
+#pragma unroll
+                    for (int c = 0; c < cc; c++) v -= x[c] * pan[cc * PS1 + c];
+                    x[cc] = v / pan[cc * PS1 + cc];
+                }
+            }
+#pragma unroll
+            for (int cc = 0; cc < NB1; cc++)
+                if (cc < nb) pan[r * PS1 + cc] = x[cc];
+        }
+        __syncthreads();
+        // 4. the factored panel back to the matrix
+        for (int t = tid; t < rows * nb; t += C1T) {
+            const int r = t % rows, cc = t / rows;
+            if (r >= cc) a[(kb + r) + (size_t)(kb + cc) * n] = pan[r * PS1 + cc];
+        }
+        // 5. trailing update A[i, j] -= sum_c L[i, c] L[j, c], kb + nb <= j <= i < n,
+        //    in 4 x 4 blocks (block row bi >= block column bj of the trailing part)
+        const int m = rows - nb;
+        const int nbl = (m + 3) / 4;
+        const int nblk = nbl * (nbl + 1) / 2;
+        for (int t = tid; t < nblk; t += C1T) {
+            int bi = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+            while ((bi + 1) * (bi + 2) / 2 <= t) bi++;
+            while (bi * (bi + 1) / 2 > t) bi--;
+            const int bj = t - bi * (bi + 1) / 2;
+            const int i0 = nb + 4 * bi, j0 = nb + 4 * bj;
+            double acc[4][4];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) acc[u][v] = 0.0;
+            for (int c = 0; c < nb; c++) {
+                double pi[4], pj[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    pi[u] = (i0 + u < rows) ? pan[(i0 + u) * PS1 + c] : 0.0;
+                    pj[u] = (j0 + u < rows) ? pan[(j0 + u) * PS1 + c] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++) acc[u][v] += pi[u] * pj[v];
+            }
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int j = j0 + v;
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int i = i0 + u;
+                    if (i < rows && j <= i) a[(kb + i) + (size_t)(kb + j) * n] -= acc[u][v];
+                }
+            }
+        }
+        __syncthreads();   // the updates are visible to the next panel's loads
+    }
+    if (bad) {
+        for (size_t t = tid; t < (size_t)n * n; t += C1T) a[t] = 0.0;
+        if (tid == 0) atomicAdd(info, 1);
+    }
+}
+
 // x_j = L_j^{-T} L_j^{-1} b_j, one workgroup per right-hand side.
 __global__ void __launch_bounds__(CT) k_cholsl(const double *__restrict__ Lall, int n,
                                                size_t lstride, const double *__restrict__ b,
@@ -385,6 +520,15 @@ __global__ void __launch_bounds__(CT) k_vtv(const double *__restrict__ Vall, int
 
 int gpk_chol_batched(gp_ctx *c, double *A, int n, int batch, size_t stride, int *info) {
     if (n > MAXN) { gp_set_error("Cholesky size exceeds 640"); return 1; }
+    if (batch == 1 && n <= MAXN1 && c->chol1 && A == c->Kchol) {
+        const size_t shm = (size_t)n * PS1 * sizeof(double);
+        if (shm > 65536 && !c->capturing)
+            GP_HIP(hipFuncSetAttribute((const void *)k_chol1, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)shm));
+        k_chol1<<<1, C1T, shm, c->stream>>>(A, n, info);
+        GP_HIP(hipGetLastError());
+        return 0;
+    }
     const size_t shm = ((size_t)n * NB + (size_t)NB * KC) * sizeof(double);
     if (shm > 65536 && !c->capturing)   // set before any recording: not a stream operation
         GP_HIP(hipFuncSetAttribute((const void *)k_chol, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));

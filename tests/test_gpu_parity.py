@@ -393,6 +393,35 @@ def test_mixed_gradient_contraction_at_m256():
     assert ge < 1e-4
 
 
+@pytest.mark.parametrize("M", [32, 40, 256])
+def test_single_matrix_cholesky_matches_batched_kernel(M):
+    """The one-matrix Cholesky of the Kmm factor (option chol1, the default;
+    cholKmm / choldc chol.cu:19-43) against the batched kernel on the same
+    matrix: the factor within 1e-11 of its largest entry (right-looking
+    against left-looking order; M = 40 ends on a partial 32-column panel) and
+    the ELBO after two IRLS steps within 1e-10.  The M = 256 FP64 oracle
+    comparison at N = 100K runs with it as well."""
+    from gaspacho2_amd import lib, synth
+    ds = synth.make_dataset(1500, 24, 5, M, seed=6)
+    T0 = lib.init_ta_with_per(ds.M, ds.Q)
+    out = {}
+    for one in (0, 1):
+        g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls,
+                          ds.X0, T0, precision=lib.PREC_FP64)
+        g.set_option("chol1", one)
+        g.init_fit()
+        out[one] = {"Kinv": g.get("Kinv")}
+        for jtr in range(2):
+            g.irls_step(jtr)
+        g.ql_start()
+        out[one]["elbo"] = g.elbo(1)
+        g.close()
+    rk = rel(out[1]["Kinv"], out[0]["Kinv"])
+    re = abs(out[1]["elbo"] - out[0]["elbo"]) / abs(out[0]["elbo"])
+    print(f"one-matrix Cholesky vs batched, M={M}: Kmm factor {rk:.2e}, ELBO {re:.2e}")
+    assert rk < 1e-11 and re < 1e-10
+
+
 def test_options_reject_bad_values(tiny_ds):
     from gaspacho2_amd import lib
     _, g = _pair(tiny_ds)
