@@ -95,6 +95,8 @@ static void free_all(gp_ctx *c) {
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->ev_efork) (void)hipEventDestroy(c->ev_efork);
     if (c->ev_ejoin) (void)hipEventDestroy(c->ev_ejoin);
+    if (c->ev_gram) (void)hipEventDestroy(c->ev_gram);
+    if (c->ev_bdone) (void)hipEventDestroy(c->ev_bdone);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -145,7 +147,9 @@ extern "C" int gp_create(gp_ctx **out, int N, int J, int M, int Q, int p, const 
         (e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&c->ev_efork, hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&c->ev_ejoin, hipEventDisableTiming)) != hipSuccess) {
+        (e = hipEventCreateWithFlags(&c->ev_ejoin, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&c->ev_gram, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&c->ev_bdone, hipEventDisableTiming)) != hipSuccess) {
         gp_set_error(std::string("side stream: ") + hipGetErrorString(e)); free_all(c); delete c; return (int)e;
     }
     if (rocblas_create_handle(&c->blas2) != rocblas_status_success) { gp_set_error("rocblas_create_handle failed"); free_all(c); delete c; return 950; }
@@ -626,14 +630,18 @@ extern "C" int gp_irls_step(gp_ctx *c, int jtr, double *obj) {
 
 // getLkhd: b_j, beta-tilde_j (scratch), the factor diagonals and the sums.
 // r_ready: R, gq and gs already hold the current W, nu and Omega (the ELBO's
-// side stream wrote them), so only b_j = Z^T R_j is left.
-static int lkhd_genes(gp_ctx *c, bool r_ready = false) {
+// side stream wrote them), so only b_j = Z^T R_j is left; b_ready: b is
+// computed too (the ELBO's side stream, joined before the solve).
+static int lkhd_genes(gp_ctx *c, bool r_ready = false, bool b_ready = false) {
     const bool w = c->psi_whitened;
-    if (r_ready)
+    if (b_ready) {
+        // nothing left before the solve
+    } else if (r_ready) {
         GP_TRY(gpk_gemm_tn_splitk(c, c->P, c->J, c->Npad, w ? c->Zw : c->Z, c->Npad, c->R, c->Npad,
                                   c->bvec, c->P));
-    else
+    } else {
         GP_TRY(gpk_r_and_sums(c, w ? c->Zw : c->Z));
+    }
     GP_TRY(gpk_cholsl_batched(c, c->Psi, c->P, (size_t)c->P * c->P, c->bvec, c->pwork, c->J));
     GP_TRY(gpk_lkhd_genes_with(c, c->pwork, w ? c->DeltaWChol : c->DeltaChol));
     c->lkhd_fresh = 1;
@@ -677,6 +685,8 @@ extern "C" int gp_elbo(gp_ctx *c, int slot, double *obj) {
     if (slot < 0 || slot >= GP_LKHD_SLOTS) { gp_set_error("slot out of range"); return 1; }
     gp_timer_begin(c, c->t_elbo);
     GP_TRY(clear_info(c));
+    c->gram_ev_pending = 0;
+    bool b_ready = false;
     {
         // Oinvt, the likelihood's R = W nu / Omega with its sums and (mixed)
         // the FP32 weights W/Omega depend on neither the kernel matrices nor
@@ -699,8 +709,24 @@ extern "C" int gp_elbo(gp_ctx *c, int slot, double *obj) {
             GP_TRY(gpk_expand_delta(c));
         }
         GP_TRY(gram_chol(c, c->precision, mixed));
+        if (c->gram_ev_pending) {
+            // b = Zw^T R needs only R (side stream) and the basis: from the
+            // Gram kernel's end it runs on the side stream beside the
+            // segment reduction, the intercept row and the batched Cholesky
+            // (HBM- and latency-bound), joined before the solve
+            c->gram_ev_pending = 0;
+            GP_HIP(hipStreamWaitEvent(c->stream2, c->ev_gram, 0));
+            {
+                SideStream side(c);
+                GP_TRY(gpk_gemm_tn_splitk(c, c->P, c->J, c->Npad, c->psi_whitened ? c->Zw : c->Z,
+                                          c->Npad, c->R, c->Npad, c->bvec, c->P));
+            }
+            GP_HIP(hipEventRecord(c->ev_bdone, c->stream2));
+            GP_HIP(hipStreamWaitEvent(c->stream, c->ev_bdone, 0));
+            b_ready = true;
+        }
     }
-    GP_TRY(lkhd_genes(c, true));
+    GP_TRY(lkhd_genes(c, true, b_ready));
     GP_TRY(gpk_collect(c, c->N, slot, obj));
     gp_timer_end(c, c->t_elbo);
     int bad = 0;

@@ -37,6 +37,10 @@ struct gp_ctx {
     // the ELBO's fork: Oinvt and the FP32 weights beside the kernel matrices
     hipEvent_t ev_efork = nullptr, ev_ejoin = nullptr;
     int efork_pending = 0;   // the ELBO's side chain is forked and not yet joined
+    // the mixed Gram kernel's end (recorded by gpk_gram_mixed) and the end of
+    // the likelihood's b = Zw^T R product run from it on the side stream
+    hipEvent_t ev_gram = nullptr, ev_bdone = nullptr;
+    int gram_ev_pending = 0;
 
     int N = 0, J = 0, M = 0, Q = 0, P = 0, p = 0, g0 = 0, Npad = 0;
     long long L = 0;
