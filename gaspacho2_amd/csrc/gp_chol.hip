@@ -327,6 +327,88 @@ __global__ void __launch_bounds__(CT) k_cholsl(const double *__restrict__ Lall, 
 
 
 
+// x_j = L_j^{-T} L_j^{-1} b_j blocked by 16 rows: each 16 x 16 diagonal block
+// is solved on wave 0 (row r in lane r, each solved value broadcast by a
+// shuffle), then every thread updates the rows beyond the block with the 16
+// new values -- two barriers per 16 rows instead of two per row.
+constexpr int SB = 16;
+__global__ void __launch_bounds__(CT) k_cholsl_blk(const double *__restrict__ Lall, int n,
+                                                   size_t lstride, const double *__restrict__ b,
+                                                   double *__restrict__ x) {
+    extern __shared__ double xs[];
+    const int j = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    const double *L = Lall + (size_t)j * lstride;
+    for (int k = tid; k < n; k += CT) xs[k] = b[(size_t)j * n + k];
+    __syncthreads();
+    // L y = b
+    for (int kb = 0; kb < n; kb += SB) {
+        const int nb = min(SB, n - kb);
+        if (tid < 64) {
+            const int r = lane;
+            double t = (r < nb) ? xs[kb + r] : 0.0;
+            double lrow[SB];
+#pragma unroll
+            for (int c = 0; c < SB; c++)
+                lrow[c] = (r < nb && c <= r && c < nb) ? L[(kb + r) + (size_t)(kb + c) * n] : 0.0;
+            double y = 0.0;
+#pragma unroll
+            for (int c = 0; c < SB; c++) {
+                if (r == c && c < nb) y = t / lrow[c];
+                const double yc = __shfl(y, c);
+                if (r > c && c < nb) t -= lrow[c] * yc;
+            }
+            if (r < nb) xs[kb + r] = y;
+        }
+        __syncthreads();
+        double yb[SB];
+#pragma unroll
+        for (int c = 0; c < SB; c++) yb[c] = (c < nb) ? xs[kb + c] : 0.0;
+        for (int r = kb + nb + tid; r < n; r += CT) {
+            double v = xs[r];
+#pragma unroll
+            for (int c = 0; c < SB; c++)
+                if (c < nb) v -= L[r + (size_t)(kb + c) * n] * yb[c];
+            xs[r] = v;
+        }
+        __syncthreads();
+    }
+    // L^T x = y
+    const int nblk = (n + SB - 1) / SB;
+    for (int bk = nblk - 1; bk >= 0; bk--) {
+        const int kb = bk * SB, nb = min(SB, n - kb);
+        if (tid < 64) {
+            const int r = lane;
+            double t = (r < nb) ? xs[kb + r] : 0.0;
+            double lcol[SB];   // L[kb + c, kb + r], c >= r
+#pragma unroll
+            for (int c = 0; c < SB; c++)
+                lcol[c] = (r < nb && c >= r && c < nb) ? L[(kb + c) + (size_t)(kb + r) * n] : 0.0;
+            double xv = 0.0;
+#pragma unroll
+            for (int c = SB - 1; c >= 0; c--) {
+                if (r == c && c < nb) xv = t / lcol[c];
+                const double xc = __shfl(xv, c);
+                if (r < c && c < nb) t -= lcol[c] * xc;
+            }
+            if (r < nb) xs[kb + r] = xv;
+        }
+        __syncthreads();
+        double xb[SB];
+#pragma unroll
+        for (int c = 0; c < SB; c++) xb[c] = (c < nb) ? xs[kb + c] : 0.0;
+        for (int r = tid; r < kb; r += CT) {
+            double v = xs[r];
+            const double *Lc = L + (size_t)r * n + kb;    // L[kb .. kb + nb - 1, r]
+#pragma unroll
+            for (int c = 0; c < SB; c++)
+                if (c < nb) v -= Lc[c] * xb[c];
+            xs[r] = v;
+        }
+        __syncthreads();
+    }
+    for (int k = tid; k < n; k += CT) x[(size_t)j * n + k] = xs[k];
+}
+
 // One matrix: column j of V = L^{-1} by column-oriented forward substitution
 // (one workgroup per column; the strict upper triangle of V is written as 0).
 __global__ void __launch_bounds__(CT) k_trinv_cols(const double *__restrict__ L, int n,
@@ -539,6 +621,11 @@ int gpk_chol_batched(gp_ctx *c, double *A, int n, int batch, size_t stride, int 
 
 int gpk_cholsl_batched(gp_ctx *c, const double *L, int n, size_t lstride, const double *b,
                        double *x, int batch) {
+    if (c->cholsl_blk) {
+        k_cholsl_blk<<<batch, CT, (size_t)n * sizeof(double), c->stream>>>(L, n, lstride, b, x);
+        GP_HIP(hipGetLastError());
+        return 0;
+    }
     k_cholsl<<<batch, CT, (size_t)n * sizeof(double), c->stream>>>(L, n, lstride, b, x);
     GP_HIP(hipGetLastError());
     return 0;

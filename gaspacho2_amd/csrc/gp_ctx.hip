@@ -294,6 +294,8 @@ extern "C" int gp_set_option(gp_ctx *c, const char *name, int value) {
         c->irls_fused = value ? 1 : 0;
     } else if (n == "chol1") {
         c->chol1 = value ? 1 : 0;
+    } else if (n == "cholsl_blk") {
+        c->cholsl_blk = value ? 1 : 0;
     } else if (n == "autotune") {
         c->autotune = value ? 1 : 0;
     } else {
@@ -319,6 +321,7 @@ extern "C" int gp_get_option(gp_ctx *c, const char *name, int *value) {
     else if (n == "irls_mixed") *value = c->irls_mixed;
     else if (n == "irls_fused") *value = c->irls_fused;
     else if (n == "chol1") *value = c->chol1;
+    else if (n == "cholsl_blk") *value = c->cholsl_blk;
     else if (n == "autotune") *value = c->autotune;
     else { gp_set_error("unknown option " + n); return 1; }
     return 0;

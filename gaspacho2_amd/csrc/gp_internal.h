@@ -157,6 +157,7 @@ struct gp_ctx {
     int wnu_pending = 0;
     int autotune = 0;          // gp_fit runs gp_autotune before its first quasi-likelihood phase
     int chol1 = 1;             // the Kmm factor takes the one-matrix Cholesky kernel (0: the batched kernel)
+    int cholsl_blk = 1;        // per-gene solves blocked by 16 rows (0: row by row)
     // 1 while R, bvec, Oinvt and pwork = Psi^-1 bvec hold the likelihood
     // evaluation (gp_elbo / gp_ql_start) at the current parameters and factors,
     // so gp_grad takes beta-tilde from pwork instead of solving again; every

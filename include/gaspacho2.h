@@ -98,6 +98,8 @@ int gp_set_precision(gp_ctx *ctx, int precision);
  *                           on separate kernels before the Gram (0)
  *   "chol1"       0/1       the Kmm factor on the one-matrix Cholesky
  *                           kernel (1) or the batched kernel (0)
+ *   "cholsl_blk"  0/1       per-gene solves blocked by 16 rows (1) or row
+ *                           by row (0)
  *   "autotune"    0/1       gp_fit calls gp_autotune before its first
  *                           quasi-likelihood phase (mixed precision only)
  * Unknown names and out-of-range values return a status > 0. */
