@@ -80,7 +80,7 @@ static void free_all(gp_ctx *c) {
                     c->Vtmp, c->Sh, c->Sl, c->gc_perm, c->gc_aexp, c->gc_sexp, c->gc_bexp,
                     c->gc_gun, c->gc_part, c->gc_zexp, c->skw, c->gc_kappa, c->gc_t0, c->gc_s0,
                     c->gc_y, c->gram_row0, c->irls_cc, c->irls_segptr, c->irls_sums,
-                    c->irls_segun};
+                    c->irls_segun, c->tn_b32, c->tn_c1a32};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     gp_timer *ts[] = {&c->t_gram, &c->t_chol, &c->t_gc, &c->t_kuf, &c->t_elbo, &c->t_grad,
@@ -296,6 +296,8 @@ extern "C" int gp_set_option(gp_ctx *c, const char *name, int value) {
         c->chol1 = value ? 1 : 0;
     } else if (n == "cholsl_blk") {
         c->cholsl_blk = value ? 1 : 0;
+    } else if (n == "tn_f32") {
+        c->tn_f32 = value ? 1 : 0;
     } else if (n == "autotune") {
         c->autotune = value ? 1 : 0;
     } else {
@@ -322,6 +324,7 @@ extern "C" int gp_get_option(gp_ctx *c, const char *name, int *value) {
     else if (n == "irls_fused") *value = c->irls_fused;
     else if (n == "chol1") *value = c->chol1;
     else if (n == "cholsl_blk") *value = c->cholsl_blk;
+    else if (n == "tn_f32") *value = c->tn_f32;
     else if (n == "autotune") *value = c->autotune;
     else { gp_set_error("unknown option " + n); return 1; }
     return 0;
@@ -818,7 +821,8 @@ extern "C" int gp_eval_graph(gp_ctx *c, int slot, double *obj) {
     if (slot < 0 || slot >= GP_LKHD_SLOTS) { gp_set_error("slot out of range"); return 1; }
     const int key[GP_GRAPH_KEY] = {c->precision, slot, c->gram_splits, c->gc_groups, c->gram_waves,
                                    c->gc_form, c->gc_mixed, c->gc_peel, c->gc_wave_rows,
-                                   c->gc_prefetch, c->gram_peel, c->gc_waves};
+                                   c->gc_prefetch, c->gram_peel, c->gc_waves, c->chol1,
+                                   c->cholsl_blk, c->tn_f32};
     if (!c->graph_exec || !std::equal(key, key + GP_GRAPH_KEY, c->graph_key)) {
         drop_graph(c);
         double tmp = 0.0;

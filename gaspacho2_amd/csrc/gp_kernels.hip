@@ -4,6 +4,7 @@
 // Every reduction is two-stage with a fixed partition, so results are
 // bit-reproducible from run to run.
 #include "gp_internal.h"
+#include "../../include/gaspacho2.h"
 
 #include <algorithm>
 
@@ -590,7 +591,72 @@ __global__ void k_tn_sum(int N, int Npad, const double *Epart, double *E) {
     E[i] = e;
 }
 
-// ---------------------------------------------------------------- per-cell gradient
+// FP32 form of the tn chain (mixed path): the same tn and E from an FP32
+// product z^T tBeta (the Gram's FP32 whitened model matrix z32 = zw 2^s_c
+// times the whitened beta-tilde scaled by 2^-s_c), TN written in FP32 for an
+// FP32 C1a product.  tn = o nu - o zb and E are formed and summed in FP64.
+__global__ void k_tn_prep32(int P, int M, int g0, int J, const double *__restrict__ tBw,
+                            const double *__restrict__ zun, const double *__restrict__ tBeta,
+                            float *__restrict__ B32, float *__restrict__ Bgp32) {
+    const int idx = blockIdx.x * NT + threadIdx.x;
+    if (idx < P * J) {
+        const int c = idx % P;
+        B32[idx] = (float)(tBw[idx] * zun[c]);
+    }
+    if (idx < M * J) {
+        const int m = idx % M, j = idx / M;
+        Bgp32[idx] = (float)tBeta[g0 + m + (size_t)j * P];
+    }
+}
+__global__ void k_tn32(int N, int Npad, int J, const double *__restrict__ W,
+                       const double *__restrict__ Nu, const float *__restrict__ ZB32,
+                       const double *__restrict__ Omega, const double *__restrict__ phi,
+                       float *__restrict__ TN32, double *__restrict__ Epart) {
+    const int i = blockIdx.x * NT + threadIdx.x;
+    const int j0 = (int)((long long)J * blockIdx.y / TN_SPLITS);
+    const int j1 = (int)((long long)J * (blockIdx.y + 1) / TN_SPLITS);
+    if (i >= Npad) return;
+    if (i >= N) {
+        for (int j = j0; j < j1; j++) TN32[i + (size_t)j * Npad] = 0.0f;
+        return;
+    }
+    const double om = Omega[i];
+    double e = 0.0;
+    int j = j0;
+    for (; j + TNU <= j1; j += TNU) {
+        double w[TNU], nu[TNU], ph[TNU];
+        float zb[TNU];
+#pragma unroll
+        for (int u = 0; u < TNU; u++) {
+            const size_t a = i + (size_t)(j + u) * Npad;
+            w[u] = W[a];
+            nu[u] = Nu[a];
+            zb[u] = ZB32[a];
+            ph[u] = phi[j + u];
+        }
+#pragma unroll
+        for (int u = 0; u < TNU; u++) {
+            const double o = w[u] / om;
+            const double tn = o * nu[u] - o * (double)zb[u];
+            e += tn * tn / w[u] / ph[u] / 2.0;
+            TN32[i + (size_t)(j + u) * Npad] = (float)(tn / ph[u]);
+        }
+    }
+    for (; j < j1; j++) {
+        const size_t a = i + (size_t)j * Npad;
+        const double w = W[a], o = w / om;
+        const double tn = o * Nu[a] - o * (double)ZB32[a];
+        e += tn * tn / w / phi[j] / 2.0;
+        TN32[a] = (float)(tn / phi[j]);
+    }
+    Epart[(size_t)blockIdx.y * Npad + i] = e;
+}
+__global__ void k_f32_to_f64(size_t n, const float *__restrict__ a, double *__restrict__ b) {
+    const size_t idx = (size_t)blockIdx.x * NT + threadIdx.x;
+    if (idx < n) b[idx] = (double)a[idx];
+}
+
+// ---------------------------------------------------------------- per-cell gradient// ---------------------------------------------------------------- per-cell gradient
 // ggplvm.cu:346-375: gOmega, C1 and gX from U = sum_j o_ij Psi_j^{-1} z_i.
 __global__ void k_cell_grad(int N, int Npad, int J, int M, int Q, int P, int g0, int ard,
         const double *Z, const double *U, const double *C1a, const double *A,
@@ -1104,7 +1170,39 @@ int gpk_grad_delta(gp_ctx *c, int whitened) {
     return 0;
 }
 
+// the tn chain with FP32 products (mixed path, whitened factor, the Gram's
+// z32 current): ZB32 and TN32 occupy the FP64 ZB buffer's two halves
+static int gpk_tn32(gp_ctx *c) {
+    const size_t nj = (size_t)c->Npad * c->J;
+    if (!c->tn_b32) {
+        GP_HIP(hipMalloc((void **)&c->tn_b32, (size_t)(c->P + c->M) * c->J * sizeof(float)));
+        GP_HIP(hipMalloc((void **)&c->tn_c1a32, (size_t)c->Npad * c->M * sizeof(float)));
+    }
+    float *B32 = c->tn_b32, *Bgp32 = c->tn_b32 + (size_t)c->P * c->J;
+    float *ZB32 = reinterpret_cast<float *>(c->ZB), *TN32 = ZB32 + nj;
+    k_tn_prep32<<<nblk((size_t)c->P * c->J), NT, 0, c->stream>>>(c->P, c->M, c->g0, c->J, c->tBw, c->zun,
+                                                                c->tBeta, B32, Bgp32);
+    LAUNCH_CHECK();
+    const float one = 1.0f, zero = 0.0f;
+    GP_BLAS(rocblas_sgemm(c->blas, rocblas_operation_none, rocblas_operation_none, c->Npad, c->J, c->P,
+                          &one, c->z32, c->Npad, B32, c->P, &zero, ZB32, c->Npad));
+    k_tn32<<<dim3(nblk(c->Npad), TN_SPLITS), NT, 0, c->stream>>>(c->N, c->Npad, c->J, c->W, c->Nu, ZB32,
+                                                                  c->Omega, c->phi, TN32, c->Epart);
+    LAUNCH_CHECK();
+    k_tn_sum<<<nblk(c->N), NT, 0, c->stream>>>(c->N, c->Npad, c->Epart, c->Ecell);
+    LAUNCH_CHECK();
+    // C1a = (TN / phi) tBeta_GP^T
+    GP_BLAS(rocblas_sgemm(c->blas, rocblas_operation_none, rocblas_operation_transpose, c->Npad, c->M, c->J,
+                          &one, TN32, c->Npad, Bgp32, c->M, &zero, c->tn_c1a32, c->Npad));
+    const size_t nc = (size_t)c->Npad * c->M;
+    k_f32_to_f64<<<nblk(nc), NT, 0, c->stream>>>(nc, c->tn_c1a32, c->C1a);
+    LAUNCH_CHECK();
+    return 0;
+}
+
 int gpk_tn(gp_ctx *c) {
+    if (c->tn_f32 && c->precision == GP_PREC_MIXED && c->psi_whitened && c->zsplit_valid && c->z32)
+        return gpk_tn32(c);
     // ZB = Z * tBeta
     const double one = 1.0, zero = 0.0;
     GP_BLAS(rocblas_dgemm(c->blas, rocblas_operation_none, rocblas_operation_none, c->Npad,

@@ -100,6 +100,8 @@ int gp_set_precision(gp_ctx *ctx, int precision);
  *                           kernel (1) or the batched kernel (0)
  *   "cholsl_blk"  0/1       per-gene solves blocked by 16 rows (1) or row
  *                           by row (0)
+ *   "tn_f32"      0/1       mixed path: the gradient's z^T beta-tilde and
+ *                           C1a products in FP32 (1) or FP64 (0)
  *   "autotune"    0/1       gp_fit calls gp_autotune before its first
  *                           quasi-likelihood phase (mixed precision only)
  * Unknown names and out-of-range values return a status > 0. */

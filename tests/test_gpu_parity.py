@@ -272,6 +272,42 @@ def test_mixed_precision_gradient_close_to_fp64():
     assert gr < 1e-4
 
 
+@pytest.mark.parametrize("shape", [(2000, 200, 5, 32, 0, 0), (3000, 48, 10, 256, 0, 0)])
+def test_fp32_tn_products_keep_the_mixed_gradient(shape):
+    """The mixed path's z^T beta-tilde and C1a products in FP32 (option
+    tn_f32, the default; ggplvm.cu:335-345 tn and C1 pieces) against the same
+    products in FP64: the mixed gradient's distance from the FP64 unit's
+    changes by no more than 2e-6 of its largest entry (at N = 100K, M = 256:
+    1.61e-6 in FP64 and 1.59e-6 in FP32), and the two gradients agree within
+    1e-5 of it; the ELBO is unchanged (the products are gradient-only)."""
+    from gaspacho2_amd import lib, synth
+    N, G, Q, M, nnum, nlev = shape
+    ds = synth.make_dataset(N, G, Q, M, seed=5, n_numeric=nnum, n_levels=nlev)
+    T0 = lib.init_ta_with_per(ds.M, ds.Q)
+    g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0,
+                      T0, precision=lib.PREC_FP64)
+    g.init_fit()
+    for jtr in range(2):
+        g.irls_step(jtr)
+    g.ql_start()
+    g.elbo(1)
+    g.grad()
+    g64 = g.get("grad")
+    g.set_precision(lib.PREC_MIXED)
+    out = {}
+    for v in (0, 1):
+        g.set_option("tn_f32", v)
+        e = g.elbo(1)
+        g.grad()
+        out[v] = (e, g.get("grad"))
+    d0, d1 = rel(out[0][1], g64), rel(out[1][1], g64)
+    d01 = rel(out[1][1], out[0][1])
+    print(f"tn products FP64 / FP32: gradient vs FP64 unit {d0:.2e} / {d1:.2e}, between them {d01:.2e}")
+    assert out[0][0] == out[1][0]
+    assert d1 <= d0 + 2e-6 and d01 < 1e-5
+    g.close()
+
+
 @pytest.mark.parametrize("shape", [(300, 20, 3, 8, 2, 3), (2000, 200, 5, 32, 0, 0)])
 def test_mixed_irls_steps_close_to_fp64(shape):
     """IRLS steps on the whitened FP16 hi/lo Gram (option irls_mixed, the
