@@ -97,6 +97,8 @@ static void free_all(gp_ctx *c) {
     if (c->ev_ejoin) (void)hipEventDestroy(c->ev_ejoin);
     if (c->ev_gram) (void)hipEventDestroy(c->ev_gram);
     if (c->ev_bdone) (void)hipEventDestroy(c->ev_bdone);
+    if (c->ev_rfork) (void)hipEventDestroy(c->ev_rfork);
+    if (c->ev_row0) (void)hipEventDestroy(c->ev_row0);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
 }
@@ -149,7 +151,9 @@ extern "C" int gp_create(gp_ctx **out, int N, int J, int M, int Q, int p, const 
         (e = hipEventCreateWithFlags(&c->ev_efork, hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&c->ev_ejoin, hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&c->ev_gram, hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&c->ev_bdone, hipEventDisableTiming)) != hipSuccess) {
+        (e = hipEventCreateWithFlags(&c->ev_bdone, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&c->ev_rfork, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&c->ev_row0, hipEventDisableTiming)) != hipSuccess) {
         gp_set_error(std::string("side stream: ") + hipGetErrorString(e)); free_all(c); delete c; return (int)e;
     }
     if (rocblas_create_handle(&c->blas2) != rocblas_status_success) { gp_set_error("rocblas_create_handle failed"); free_all(c); delete c; return 950; }
@@ -508,18 +512,6 @@ static int kernel_mats(gp_ctx *c, int (*after_kuf)(gp_ctx *) = nullptr) {
     gp_timer_end(c, c->t_kuf);
     return 0;
 }
-
-// For its scope, the context's stream and rocBLAS handle are the side ones,
-// so the gpk_* launchers enqueue there.
-struct SideStream {
-    gp_ctx *c;
-    explicit SideStream(gp_ctx *cc) : c(cc) { swap_in(); }
-    ~SideStream() { swap_in(); }
-    void swap_in() {
-        std::swap(c->stream, c->stream2);
-        std::swap(c->blas, c->blas2);
-    }
-};
 
 // Weighted Grams + batched Cholesky.  mixed: FP16 hi/lo MFMA Gram in the
 // whitened basis [covariates | Knm L^-T] with the diagonal prior DeltaW, whose

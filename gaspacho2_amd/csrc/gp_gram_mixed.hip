@@ -1565,6 +1565,26 @@ static int gram_kernel_done(gp_ctx *c) {
     return 0;
 }
 
+// The intercept row and column (FP32 products of w32 and z32, summed in
+// FP64) need only the weights and the model matrix and write only row and
+// column 0 of Psi: forked to the side stream after the Gram kernel, they run
+// beside the segment reduction (which writes the GP block); the caller's
+// factorization waits for them (gram_row_join).
+static int gram_row_fork(gp_ctx *c, int batch, const double *Delta, double *Psi) {
+    GP_HIP(hipEventRecord(c->ev_rfork, c->stream));
+    GP_HIP(hipStreamWaitEvent(c->stream2, c->ev_rfork, 0));
+    {
+        SideStream side(c);
+        GP_TRY(gram_intercept_row(c, batch, Delta, Psi));
+    }
+    GP_HIP(hipEventRecord(c->ev_row0, c->stream2));
+    return 0;
+}
+static int gram_row_join(gp_ctx *c) {
+    GP_HIP(hipStreamWaitEvent(c->stream, c->ev_row0, 0));
+    return 0;
+}
+
 int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const double *Zsrc,
                    const double *Delta, bool weights_ready) {
     const int T16 = (c->P + 15) / 16, P16 = 16 * T16;
@@ -1615,11 +1635,12 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
         if (r) return r;
         gp_timer_end(c, c->t_gramk);
         GP_TRY(gram_kernel_done(c));
+        GP_TRY(gram_row_fork(c, batch, Delta, Psi));
         dim3 gr((unsigned)(TG * (TG + 1) / 2), (unsigned)((batch + RSG - 1) / RSG));
         k_gram_reduce_sym<<<gr, 256, 0, c->stream>>>(c->P, TG, nseg, batch, 1, c->gram_part, c->wun, 0,
                                                       c->zun, Delta, Psi);
         GP_HIP(hipGetLastError());
-        return gram_intercept_row(c, batch, Delta, Psi);
+        return gram_row_join(c);
     }
     // the same peeling for designs wider than the symmetric kernel's tiles
     // (M = 512): the tiled general kernel over the GP block of the split
@@ -1782,11 +1803,12 @@ int gpk_gram_irls(gp_ctx *c, double *Psi, const double *Delta) {
     k_irls_sums<<<(unsigned)((c->J + 255) / 256), 256, 0, c->stream>>>(c->J, nseg, c->irls_sums, c->gq,
                                                                       c->gs, c->lkhds);
     GP_HIP(hipGetLastError());
+    if (peel) GP_TRY(gram_row_fork(c, c->J, Delta, Psi));
     dim3 gr((unsigned)(T * (T + 1) / 2), (unsigned)((c->J + RSG - 1) / RSG));
     k_gram_reduce_sym<<<gr, 256, 0, c->stream>>>(c->P, T, nseg, c->J, peel ? 1 : 0, c->gram_part,
                                                   c->irls_segun, (size_t)c->J, c->zun, Delta, Psi);
     GP_HIP(hipGetLastError());
-    if (peel) GP_TRY(gram_intercept_row(c, c->J, Delta, Psi));
+    if (peel) GP_TRY(gram_row_join(c));
     return 0;
 }
 

@@ -41,6 +41,9 @@ struct gp_ctx {
     // the likelihood's b = Zw^T R product run from it on the side stream
     hipEvent_t ev_gram = nullptr, ev_bdone = nullptr;
     int gram_ev_pending = 0;
+    // the peeled Gram's intercept row on the side stream: fork after the
+    // Gram kernel, join before the factorization
+    hipEvent_t ev_rfork = nullptr, ev_row0 = nullptr;
 
     int N = 0, J = 0, M = 0, Q = 0, P = 0, p = 0, g0 = 0, Npad = 0;
     long long L = 0;
@@ -184,6 +187,22 @@ struct gp_ctx {
     std::vector<int> fit_step;           // per entry: outer iteration, inner step, line-search trials
     std::vector<double> fit_sr;          // per entry: the line search's final step fraction
     std::string err;
+};
+
+// For its scope, the context's stream and rocBLAS handle are the side ones,
+// so the gpk_* launchers enqueue there.
+struct SideStream {
+    gp_ctx *c;
+    explicit SideStream(gp_ctx *cc) : c(cc) { swap_in(); }
+    ~SideStream() { swap_in(); }
+    void swap_in() {
+        hipStream_t s = c->stream;
+        c->stream = c->stream2;
+        c->stream2 = s;
+        rocblas_handle b = c->blas;
+        c->blas = c->blas2;
+        c->blas2 = b;
+    }
 };
 
 // ---------------- error handling ----------------
