@@ -95,7 +95,6 @@ static void free_all(gp_ctx *c) {
     if (c->ev_join) (void)hipEventDestroy(c->ev_join);
     if (c->ev_efork) (void)hipEventDestroy(c->ev_efork);
     if (c->ev_ejoin) (void)hipEventDestroy(c->ev_ejoin);
-    if (c->ev_gram) (void)hipEventDestroy(c->ev_gram);
     if (c->ev_bdone) (void)hipEventDestroy(c->ev_bdone);
     if (c->ev_rfork) (void)hipEventDestroy(c->ev_rfork);
     if (c->ev_row0) (void)hipEventDestroy(c->ev_row0);
@@ -150,7 +149,6 @@ extern "C" int gp_create(gp_ctx **out, int N, int J, int M, int Q, int p, const 
         (e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&c->ev_efork, hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&c->ev_ejoin, hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&c->ev_gram, hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&c->ev_bdone, hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&c->ev_rfork, hipEventDisableTiming)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&c->ev_row0, hipEventDisableTiming)) != hipSuccess) {
@@ -670,7 +668,16 @@ static int fork_weights_impl(gp_ctx *c, bool mixed) {
         GP_TRY(gpk_oinvt(c));
         GP_TRY(gpk_r_sums(c, mixed));
     }
-    GP_HIP(hipEventRecord(c->ev_ejoin, c->stream2));
+    GP_HIP(hipEventRecord(c->ev_ejoin, c->stream2));    // what the Gram reads
+    // then the likelihood's b = Z^T R in the reference basis (Knm is in Z
+    // from k_kuf on): beside the Kmm chain and the Gram's first workgroups
+    // instead of after the batched Cholesky; joined before the solve
+    {
+        SideStream side(c);
+        GP_TRY(gpk_gemm_tn_splitk(c, c->P, c->J, c->Npad, c->Z, c->Npad, c->R, c->Npad, c->bvec, c->P));
+    }
+    GP_HIP(hipEventRecord(c->ev_bdone, c->stream2));
+    c->bfork_pending = 1;
     return 0;
 }
 static int fork_weights(gp_ctx *c) { return fork_weights_impl(c, false); }
@@ -683,16 +690,23 @@ extern "C" int gp_elbo(gp_ctx *c, int slot, double *obj) {
     if (slot < 0 || slot >= GP_LKHD_SLOTS) { gp_set_error("slot out of range"); return 1; }
     gp_timer_begin(c, c->t_elbo);
     GP_TRY(clear_info(c));
-    c->gram_ev_pending = 0;
-    bool b_ready = false;
+    // the side stream's b product is joined before the solve, or on an early
+    // error return, so no side work outlives the call unjoined
+    struct BJoin {
+        gp_ctx *c;
+        ~BJoin() {
+            if (c->bfork_pending) (void)hipStreamWaitEvent(c->stream, c->ev_bdone, 0);
+            c->bfork_pending = 0;
+        }
+    } bjoin{c};
     {
         // Oinvt, the likelihood's R = W nu / Omega with its sums and (mixed)
         // the FP32 weights W/Omega depend on neither the kernel matrices nor
         // the basis: forked to the side stream after the
         // HBM-bound Kuf, they run beside the single-matrix Kmm factor and its
         // small follow-ups, which leave the chip idle, and are joined before
-        // the Gram; the join also runs on an early error return, so no side
-        // work outlives the call unjoined
+        // the Gram (b = Z^T R follows them on the side stream); the join also
+        // runs on an early error return
         const bool mixed = c->precision == GP_PREC_MIXED;
         struct Join {
             gp_ctx *c;
@@ -707,24 +721,13 @@ extern "C" int gp_elbo(gp_ctx *c, int slot, double *obj) {
             GP_TRY(gpk_expand_delta(c));
         }
         GP_TRY(gram_chol(c, c->precision, mixed));
-        if (c->gram_ev_pending) {
-            // b = Zw^T R needs only R (side stream) and the basis: from the
-            // Gram kernel's end it runs on the side stream beside the
-            // segment reduction, the intercept row and the batched Cholesky
-            // (HBM- and latency-bound), joined before the solve
-            c->gram_ev_pending = 0;
-            GP_HIP(hipStreamWaitEvent(c->stream2, c->ev_gram, 0));
-            {
-                SideStream side(c);
-                GP_TRY(gpk_gemm_tn_splitk(c, c->P, c->J, c->Npad, c->psi_whitened ? c->Zw : c->Z,
-                                          c->Npad, c->R, c->Npad, c->bvec, c->P));
-            }
-            GP_HIP(hipEventRecord(c->ev_bdone, c->stream2));
-            GP_HIP(hipStreamWaitEvent(c->stream, c->ev_bdone, 0));
-            b_ready = true;
-        }
     }
-    GP_TRY(lkhd_genes(c, true, b_ready));
+    // b from the side stream, in the reference basis; the whitened basis
+    // Zw = Z blockdiag(I, L^-T) takes b_w = blockdiag(I, L^-1) b
+    GP_HIP(hipStreamWaitEvent(c->stream, c->ev_bdone, 0));
+    c->bfork_pending = 0;
+    if (c->psi_whitened) GP_TRY(gpk_b_whiten(c));
+    GP_TRY(lkhd_genes(c, true, true));
     GP_TRY(gpk_collect(c, c->N, slot, obj));
     gp_timer_end(c, c->t_elbo);
     int bad = 0;
@@ -768,8 +771,11 @@ extern "C" int gp_grad(gp_ctx *c) {
         GP_HIP(hipMemcpyAsync(tB, c->pwork, (size_t)c->P * c->J * sizeof(double),
                               hipMemcpyDeviceToDevice, c->stream));
     } else {
+        // b as the ELBO forms it (reference basis, then L^-1 on the GP rows),
+        // so both paths give the same bits
         GP_TRY(gpk_oinvt(c));
-        GP_TRY(gpk_r_and_sums(c, w ? c->Zw : c->Z));
+        GP_TRY(gpk_r_and_sums(c, c->Z));
+        if (w) GP_TRY(gpk_b_whiten(c));
         GP_TRY(gpk_cholsl_batched(c, c->Psi, c->P, (size_t)c->P * c->P, c->bvec, tB, c->J));
     }
     GP_TRY(fork_tn_chain(c));

@@ -1557,14 +1557,6 @@ static int gram_intercept_row(gp_ctx *c, int batch, const double *Delta, double 
     return 0;
 }
 
-// the main Gram kernel of this call is enqueued: work that needs only its
-// inputs (and not Psi) may start from here on another stream
-static int gram_kernel_done(gp_ctx *c) {
-    GP_HIP(hipEventRecord(c->ev_gram, c->stream));
-    c->gram_ev_pending = 1;
-    return 0;
-}
-
 // The intercept row and column (FP32 products of w32 and z32, summed in
 // FP64) need only the weights and the model matrix and write only row and
 // column 0 of Psi: forked to the side stream after the Gram kernel, they run
@@ -1634,7 +1626,6 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
         }
         if (r) return r;
         gp_timer_end(c, c->t_gramk);
-        GP_TRY(gram_kernel_done(c));
         GP_TRY(gram_row_fork(c, batch, Delta, Psi));
         dim3 gr((unsigned)(TG * (TG + 1) / 2), (unsigned)((batch + RSG - 1) / RSG));
         k_gram_reduce_sym<<<gr, 256, 0, c->stream>>>(c->P, TG, nseg, batch, 1, c->gram_part, c->wun, 0,
@@ -1656,7 +1647,6 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
                                                 c->gram_part);
         GP_HIP(hipGetLastError());
         gp_timer_end(c, c->t_gramk);
-        GP_TRY(gram_kernel_done(c));
         dim3 gr((unsigned)(((size_t)c->M * c->M + 255) / 256), batch);
         k_gram_reduce<<<gr, 256, 0, c->stream>>>(c->P, c->M, 1, nbk_g, nseg, batch, c->gram_part, c->wun,
                                                   c->zun, Delta, Psi);
@@ -1687,7 +1677,6 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
         }
         if (r) return r;
         gp_timer_end(c, c->t_gramk);
-        GP_TRY(gram_kernel_done(c));
         dim3 gr((unsigned)(T16 * (T16 + 1) / 2), (unsigned)((batch + RSG - 1) / RSG));
         k_gram_reduce_sym<<<gr, 256, 0, c->stream>>>(c->P, T16, nseg, batch, 0, c->gram_part, c->wun, 0,
                                                       c->zun, Delta, Psi);
@@ -1698,7 +1687,6 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
     k_gram_mixed<<<g, 256, 0, c->stream>>>(c->Npad, P16, T16, nbk, seg, (const _Float16 *)c->Zh,
                                             (const _Float16 *)c->Zl, c->w32, c->wsc, c->gram_part);
     GP_HIP(hipGetLastError());
-    GP_TRY(gram_kernel_done(c));
     dim3 gr((unsigned)(((size_t)c->P * c->P + 255) / 256), batch);
     k_gram_reduce<<<gr, 256, 0, c->stream>>>(c->P, c->P, 0, nbk, nseg, batch, c->gram_part, c->wun,
                                               c->zun, Delta, Psi);

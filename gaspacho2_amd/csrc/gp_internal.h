@@ -37,10 +37,9 @@ struct gp_ctx {
     // the ELBO's fork: Oinvt and the FP32 weights beside the kernel matrices
     hipEvent_t ev_efork = nullptr, ev_ejoin = nullptr;
     int efork_pending = 0;   // the ELBO's side chain is forked and not yet joined
-    // the mixed Gram kernel's end (recorded by gpk_gram_mixed) and the end of
-    // the likelihood's b = Zw^T R product run from it on the side stream
-    hipEvent_t ev_gram = nullptr, ev_bdone = nullptr;
-    int gram_ev_pending = 0;
+    // the end of the likelihood's b = Z^T R product on the ELBO's side stream
+    hipEvent_t ev_bdone = nullptr;
+    int bfork_pending = 0;
     // the peeled Gram's intercept row on the side stream: fork after the
     // Gram kernel, join before the factorization
     hipEvent_t ev_rfork = nullptr, ev_row0 = nullptr;
@@ -244,6 +243,7 @@ int gpk_init_y0(gp_ctx *c);                   // R = log((y+0.05)/ls) dense
 int gpk_beta_phi(gp_ctx *c, int jtr);
 int gpk_beta_phi_in(gp_ctx *c, int jtr, double *Beta);     // damped update of Beta, phi
 int gpk_beta_whiten(gp_ctx *c, const double *B, double *Bw);
+int gpk_b_whiten(gp_ctx *c);                  // bvec's GP rows <- L^-1 bvec (reference -> whitened basis)
 int gpk_beta_unwhiten(gp_ctx *c, const double *Bw, double *B);
 int gpk_lkhd_pois(gp_ctx *c);
 int gpk_lkhd_genes_with(gp_ctx *c, const double *tb, const double *Dchol);  // tb = Psi_j^{-1} b_j

@@ -1110,6 +1110,16 @@ int gpk_beta_whiten(gp_ctx *c, const double *B, double *Bw) {
     return 0;
 }
 
+// b_w = T^T b with T = blockdiag(I, L^-T): GP rows L^-1 b_GP (a triangular
+// solve with the Kmm factor), covariate rows unchanged; in place.
+int gpk_b_whiten(gp_ctx *c) {
+    const double one = 1.0;
+    GP_BLAS(rocblas_dtrsm(c->blas, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none,
+                          rocblas_diagonal_non_unit, c->M, c->J, &one, c->Kchol, c->M, c->bvec + c->g0,
+                          c->P));
+    return 0;
+}
+
 // B = T Bw: covariate rows copied, GP rows = L^-T Bw_GP.
 int gpk_beta_unwhiten(gp_ctx *c, const double *Bw, double *B) {
     const double one = 1.0, zero = 0.0;

