@@ -300,6 +300,8 @@ extern "C" int gp_set_option(gp_ctx *c, const char *name, int value) {
         c->cholsl_blk = value ? 1 : 0;
     } else if (n == "tn_f32") {
         c->tn_f32 = value ? 1 : 0;
+    } else if (n == "gram_wide") {
+        c->gram_wide = value ? 1 : 0;
     } else if (n == "autotune") {
         c->autotune = value ? 1 : 0;
     } else {
@@ -327,6 +329,7 @@ extern "C" int gp_get_option(gp_ctx *c, const char *name, int *value) {
     else if (n == "chol1") *value = c->chol1;
     else if (n == "cholsl_blk") *value = c->cholsl_blk;
     else if (n == "tn_f32") *value = c->tn_f32;
+    else if (n == "gram_wide") *value = c->gram_wide;
     else if (n == "autotune") *value = c->autotune;
     else { gp_set_error("unknown option " + n); return 1; }
     return 0;

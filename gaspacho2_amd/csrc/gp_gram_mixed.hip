@@ -428,9 +428,17 @@ struct GramRounds {
 // one FP64 exp, one FP64 division, the count scatter and a dozen FP64 ops)
 constexpr int IRLS_CYC = 400;
 
-constexpr GramRounds make_gram_rounds(int T16, bool irls = false) {
+// Wide designs (M = 512, 32 GP tiles): the upper triangle is cut into two
+// 16-tile symmetric halves (the T16 = 16 plan on an offset column range) and
+// four 8 x 8-tile rectangles of cross terms.  A rectangle stages 128 row and
+// 128 column columns as one 256-column operand (4-tile groups 0, 1 = rows,
+// 2, 3 = columns) and computes the four off-diagonal 4 x 4 blocks (0|1, 2|3):
+// one block on each of waves 0-3, waves 4-7 only staging.
+constexpr GramPlan kRectPlan = {{0, -1, 0, -1, 1, -1, 1, -1, -1, -1, -1, -1, -1, -1, -1, -1},
+                                {2, -1, 3, -1, 2, -1, 3, -1, -1, -1, -1, -1, -1, -1, -1, -1}};
+
+constexpr GramRounds make_gram_rounds_p(const GramPlan pl, int T16, bool irls) {
     GramRounds g{};
-    const GramPlan pl = make_gram_plan(T16);
     int t[GW] = {};
     if (irls) t[0] += IRLS_CYC;
     for (int w = 0; w < GW; w++)
@@ -445,6 +453,10 @@ constexpr GramRounds make_gram_rounds(int T16, bool irls = false) {
     }
     for (int w = 1; w < GW; w++) g.r0[w] = g.r0[w - 1] + g.nr[w - 1];
     return g;
+}
+
+constexpr GramRounds make_gram_rounds(int T16, bool irls = false) {
+    return make_gram_rounds_p(make_gram_plan(T16), T16, irls);
 }
 
 __device__ inline void mfma3(f4 &acc, const h8 &ah, const h8 &al, const h8 &bh, const h8 &bl) {
@@ -498,11 +510,33 @@ __device__ __forceinline__ void gram_block_store(const f4 (&acc)[4][4], float *p
         }
 }
 
+// block (A, B)'s tiles stored at their place in a TGL-tile upper triangle,
+// group q of the staged operand starting at tile tmap[q]
+template <int T16, int A, int B>
+__device__ __forceinline__ void gram_block_store_map(const f4 (&acc)[4][4], float *pg, int lane,
+                                                     const int (&tmap)[4], int TGL) {
+    constexpr int NG = (T16 + 3) / 4, RL = T16 - 4 * (NG - 1);
+    constexpr int SA = A < NG - 1 ? 4 : RL, SB = B < NG - 1 ? 4 : RL;
+#pragma unroll
+    for (int x = 0; x < SA; x++)
+#pragma unroll
+        for (int y = (A == B ? x : 0); y < SB; y++) {
+            const int ta = tmap[A] + x, tb = tmap[B] + y;
+            const int t = ta * TGL - ta * (ta - 1) / 2 + (tb - ta);
+            *(f4 *)(pg + (size_t)t * 256 + 4 * lane) = acc[x][y];
+        }
+}
+
 struct GramArgs {
     int Npad, c0, nch;
     float sc;
     const float *z32, *wj;
     float *pg;
+    // wide designs: the staged 64-column groups' sources, their first tiles
+    // in the whole GP block and its tile count (the partials' layout)
+    const float *zg[4] = {nullptr, nullptr, nullptr, nullptr};
+    int tmap[4] = {0, 4, 8, 12};
+    int TGL = 0;
     // fused IRLS pass (k_gram_irls): wave 0 builds each chunk's weights from
     // eta = z^T beta instead of reading them, together with nu, R and the sums
     int N = 0;
@@ -585,14 +619,17 @@ __device__ __forceinline__ float irls_finish(const GramArgs &g, int ch, int &nzp
 
 // the chunk loop of wave W (its blocks are compile-time constants); IR: the
 // fused IRLS form, whose wave 0 builds the weights from eta
-template <int T16, int W, bool IR = false>
+// MODE 0: the design's own columns; 1: a 16-tile symmetric half of a wide
+// design; 2: a wide design's 8 x 8-tile rectangle (kRectPlan)
+template <int T16, int W, bool IR = false, int MODE = 0>
 __device__ __forceinline__ void gram_wave(const GramArgs &g, _Float16 *Xs, float (*wsl)[GCH]) {
     constexpr int P16 = 16 * T16;
-    constexpr GramPlan pl = make_gram_plan(T16);
+    static_assert(MODE == 0 || T16 == 16, "wide-design parts stage 256 columns");
+    constexpr GramPlan pl = MODE == 2 ? kRectPlan : make_gram_plan(T16);
     constexpr int A0 = pl.a[W * GSL], B0 = pl.b[W * GSL];
     constexpr int A1 = pl.a[W * GSL + 1], B1 = pl.b[W * GSL + 1];
     constexpr int QC = 4 * GKC;                   // 8-cell groups per chunk
-    constexpr GramRounds rp = make_gram_rounds(T16, IR);
+    constexpr GramRounds rp = make_gram_rounds_p(pl, T16, IR);
     constexpr int RB = rp.r0[W], IPT = rp.nr[W];  // this wave's staging rounds
     const int tid = threadIdx.x, lane = tid & 63;
     const int lrow = lane & 15, lk = 8 * (lane >> 4);
@@ -620,7 +657,10 @@ __device__ __forceinline__ void gram_wave(const GramArgs &g, _Float16 *Xs, float
 #pragma unroll
         for (int r = r0; r < r1; r++) {
             const int it = 64 * (RB + r) + lane;
-            const float *src = g.z32 + (size_t)(it / QC) * g.Npad + cc + 8 * (it % QC);
+            const int col = it / QC;              // (RB + r) / 8 = col / 64 is a compile-time group
+            const float *src = (MODE == 0 ? g.z32 + (size_t)col * g.Npad
+                                          : g.zg[col >> 6] + (size_t)(col & 63) * g.Npad) +
+                               cc + 8 * (it % QC);
             pz[r][0] = *(const f4 *)src;
             pz[r][1] = *(const f4 *)(src + 4);
         }
@@ -703,8 +743,13 @@ __device__ __forceinline__ void gram_wave(const GramArgs &g, _Float16 *Xs, float
         }
         __syncthreads();
     }
-    if constexpr (A0 >= 0) gram_block_store<T16, A0, B0>(acc0, g.pg, lane);
-    if constexpr (A1 >= 0) gram_block_store<T16, A1, B1>(acc1, g.pg, lane);
+    if constexpr (MODE == 0) {
+        if constexpr (A0 >= 0) gram_block_store<T16, A0, B0>(acc0, g.pg, lane);
+        if constexpr (A1 >= 0) gram_block_store<T16, A1, B1>(acc1, g.pg, lane);
+    } else {
+        if constexpr (A0 >= 0) gram_block_store_map<T16, A0, B0>(acc0, g.pg, lane, g.tmap, g.TGL);
+        if constexpr (A1 >= 0) gram_block_store_map<T16, A1, B1>(acc1, g.pg, lane, g.tmap, g.TGL);
+    }
     if constexpr (IR && W == 0) {
         // the wave's sums over the segment, lanes added in a fixed tree order
 #pragma unroll
@@ -942,6 +987,71 @@ __global__ void k_irls_nu_nz(int Npad, const int *__restrict__ Yp, const int *__
         const double y = Yx[l];
         const double q = y / W[a];
         Nu[a] = (eta[a] + (y > 0.0 ? q : 0.0)) - 1.0;
+    }
+}
+
+// Wide design (the GP block of M = 512 columns, 32 tiles) in six parts per
+// (gene, cell segment): MODE 1 = the symmetric halves h = 0, 1 (GP columns
+// 256 h .. 256 h + 255), MODE 2 = the rectangles (ra, cb) (GP row columns
+// 128 ra .., column columns 256 + 128 cb ..).  zg is the GP block's first
+// column (z32 + Npad); partials go to the 528-tile layout k_gram_reduce_sym
+// reads.  The segment-major XCD mapping of k_gram_sym, per (gene, part).
+template <int MODE>
+__global__ void __launch_bounds__(GW * 64) k_gram_wide(int Npad, int seg, int nseg, int J,
+        const float *__restrict__ zg, const float *__restrict__ w32,
+        const float *__restrict__ wsc, float *__restrict__ part) {
+    constexpr int NP = MODE == 1 ? 2 : 4;         // parts of this kind
+    constexpr int TGL = 32, NTG = TGL * (TGL + 1) / 2;
+    extern __shared__ __attribute__((aligned(16))) _Float16 Xs[];
+    __shared__ __attribute__((aligned(16))) float wsl[3][GCH];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    int sgi, jp;
+    {
+        const int id = blockIdx.x;
+        if (nseg % 8 == 0) {
+            const int xcd = id % 8, loc = id / 8, nsx = nseg / 8;
+            sgi = xcd + 8 * (loc % nsx);
+            jp = loc / nsx;
+        } else {
+            sgi = id % nseg;
+            jp = id / nseg;
+        }
+    }
+    const int j = jp / NP, pt = jp % NP;
+    GramArgs g;
+    g.Npad = Npad;
+    g.c0 = sgi * seg;
+    g.nch = (min(Npad, g.c0 + seg) - g.c0) / GCH;
+    g.sc = wsc[j];
+    g.z32 = zg;
+    g.wj = w32 + (size_t)j * Npad;
+    g.pg = part + ((size_t)sgi * J + j) * NTG * 256;
+    g.TGL = TGL;
+    if constexpr (MODE == 1) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            g.zg[q] = zg + (size_t)(256 * pt + 64 * q) * Npad;
+            g.tmap[q] = 16 * pt + 4 * q;
+        }
+    } else {
+        const int ra = pt >> 1, cb = pt & 1;
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            g.zg[q] = zg + (size_t)(128 * ra + 64 * q) * Npad;
+            g.tmap[q] = 8 * ra + 4 * q;
+            g.zg[2 + q] = zg + (size_t)(256 + 128 * cb + 64 * q) * Npad;
+            g.tmap[2 + q] = 16 + 8 * cb + 4 * q;
+        }
+    }
+    switch (w) {
+    case 0: gram_wave<16, 0, false, MODE>(g, Xs, wsl); break;
+    case 1: gram_wave<16, 1, false, MODE>(g, Xs, wsl); break;
+    case 2: gram_wave<16, 2, false, MODE>(g, Xs, wsl); break;
+    case 3: gram_wave<16, 3, false, MODE>(g, Xs, wsl); break;
+    case 4: gram_wave<16, 4, false, MODE>(g, Xs, wsl); break;
+    case 5: gram_wave<16, 5, false, MODE>(g, Xs, wsl); break;
+    case 6: gram_wave<16, 6, false, MODE>(g, Xs, wsl); break;
+    default: gram_wave<16, 7, false, MODE>(g, Xs, wsl); break;
     }
 }
 
@@ -1633,9 +1743,36 @@ int gpk_gram_mixed(gp_ctx *c, const double *W, double *Psi, int batch, const dou
         GP_HIP(hipGetLastError());
         return gram_row_join(c);
     }
-    // the same peeling for designs wider than the symmetric kernel's tiles
-    // (M = 512): the tiled general kernel over the GP block of the split
-    // model matrix (columns 1..M of Zh / Zl), its reduction written at offset 1
+    if (c->gram_peel && c->gram_wide && c->g0 == 1 && c->M == 512 && c->Npad % GRK_SEG == 0) {
+        // M = 512: two symmetric halves and four rectangles of the 32-tile GP
+        // block on the symmetric kernel's chunk loop, reduced as one 32-tile
+        // triangle at offset 1
+        constexpr size_t shm = (size_t)2 * 2 * 256 * GLDX * sizeof(_Float16);
+        if (!c->capturing) {
+            GP_HIP(hipFuncSetAttribute((const void *)k_gram_wide<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)shm));
+            GP_HIP(hipFuncSetAttribute((const void *)k_gram_wide<2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)shm));
+        }
+        gp_timer_begin(c, c->t_gramk);
+        const float *zg = c->z32 + c->Npad;
+        k_gram_wide<1><<<(unsigned)(2 * batch * nseg), GW * 64, shm, c->stream>>>(c->Npad, seg, nseg, batch, zg,
+                                                                                c->w32, c->wsc, c->gram_part);
+        GP_HIP(hipGetLastError());
+        k_gram_wide<2><<<(unsigned)(4 * batch * nseg), GW * 64, shm, c->stream>>>(c->Npad, seg, nseg, batch, zg,
+                                                                                c->w32, c->wsc, c->gram_part);
+        GP_HIP(hipGetLastError());
+        gp_timer_end(c, c->t_gramk);
+        GP_TRY(gram_row_fork(c, batch, Delta, Psi));
+        dim3 gr((unsigned)(32 * 33 / 2), (unsigned)((batch + RSG - 1) / RSG));
+        k_gram_reduce_sym<<<gr, 256, 0, c->stream>>>(c->P, 32, nseg, batch, 1, c->gram_part, c->wun, 0,
+                                                      c->zun, Delta, Psi);
+        GP_HIP(hipGetLastError());
+        return gram_row_join(c);
+    }
+    // the same peeling for other designs wider than the symmetric kernel's
+    // tiles: the tiled general kernel over the GP block of the split model
+    // matrix (columns 1..M of Zh / Zl), its reduction written at offset 1
     if (c->gram_peel && c->g0 == 1 && c->M % 16 == 0 && TG > MAX_SYM_T16 && c->Npad % GRK_SEG == 0) {
         const int nbk_g = (c->M + BLK - 1) / BLK;
         const int nbp_g = nbk_g * (nbk_g + 1) / 2;

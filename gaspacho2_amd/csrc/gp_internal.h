@@ -123,6 +123,7 @@ struct gp_ctx {
     int gram_splits = 8;       // cell segments of the mixed Gram
     int gram_waves = 8;        // waves per workgroup of the symmetric mixed Gram (8 or 4)
     int gram_peel = 1;         // symmetric Gram on the GP block only when the intercept is the sole covariate column
+    int gram_wide = 1;         // M = 512: the GP block as symmetric halves and rectangles (0: the general tiled kernel)
     float *gram_row0 = nullptr;  // per-segment FP32 intercept-row products (peeled Gram)
     size_t gram_row0_len = 0;
     int zsplit_valid = 0;

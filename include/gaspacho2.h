@@ -102,6 +102,10 @@ int gp_set_precision(gp_ctx *ctx, int precision);
  *                           by row (0)
  *   "tn_f32"      0/1       mixed path: the gradient's z^T beta-tilde and
  *                           C1a products in FP32 (1) or FP64 (0)
+ *   "gram_wide"   0/1       mixed Gram at M = 512 (intercept-peeled): the
+ *                           GP block as two symmetric halves and four
+ *                           rectangles on the symmetric kernel's chunk
+ *                           loop (1) or on the general tiled kernel (0)
  *   "autotune"    0/1       gp_fit calls gp_autotune before its first
  *                           quasi-likelihood phase (mixed precision only)
  * Unknown names and out-of-range values return a status > 0. */

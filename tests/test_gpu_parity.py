@@ -1045,12 +1045,15 @@ def test_wide_design_fp64_matches_oracle(M):
     _cmp(o, g, "C2")
 
 
-@pytest.mark.parametrize("M", [288, 512])
-def test_wide_design_mixed_close_to_fp64(M):
+@pytest.mark.parametrize("M,wide", [(288, 1), (512, 1), (512, 0)])
+def test_wide_design_mixed_close_to_fp64(M, wide):
     """The mixed path at P16 = 304 and 528 against the FP64 whitened path of
     the same state (pinned to the oracle by the whitened-basis test): Gram
     entries within 5e-7 of each gene's largest, ELBO within 1e-7, gradient
-    within 1e-4 of its largest entry."""
+    within 1e-4 of its largest entry.  M = 512 runs both Gram forms: the GP
+    block as two symmetric halves and four rectangles on the symmetric
+    kernel's chunk loop (option gram_wide, the default) and the general tiled
+    kernel."""
     from gaspacho2_amd import lib, synth
     ds = synth.make_dataset(3000, 24, 20, M, seed=9, pca=False)
     T0 = lib.init_ta_with_per(ds.M, ds.Q)
@@ -1068,14 +1071,15 @@ def test_wide_design_mixed_close_to_fp64(M):
     g.grad()
     g64 = g.get("grad")
     g.set_precision(lib.PREC_MIXED)
+    g.set_option("gram_wide", wide)
     em = g.elbo(1)
     Pm = g.get("Psi")[:J * P * P].reshape(J, P, P)[:, iu[1], iu[0]]
     g.grad()
     gm = g.get("grad")
     err = np.max(np.abs(Pm - Pw), axis=1) / np.max(np.abs(Pw), axis=1)
     ge = rel(gm, g64)
-    print(f"M={M} mixed: Gram max rel err {err.max():.2e}, ELBO {abs(em - e64) / abs(e64):.2e}, "
-          f"gradient {ge:.2e}")
+    print(f"M={M} gram_wide={wide} mixed: Gram max rel err {err.max():.2e}, "
+          f"ELBO {abs(em - e64) / abs(e64):.2e}, gradient {ge:.2e}")
     assert err.max() < 5e-7
     assert abs(em - e64) <= 1e-7 * abs(e64)
     assert ge < 1e-4
