@@ -4,8 +4,8 @@
 // inverses, then the block columns on FP64 MFMA), then the symmetric product
 // S = V^T V -- all hand-written.
 //
-// Cholesky: left-looking, blocked by 16-column panels held in LDS; one
-// workgroup per matrix.  Input is the lower triangle (the Gram writes both
+// Cholesky: left-looking, blocked by 8-column panels held in LDS (16-column
+// panels as an option); one workgroup per matrix.  Input is the lower triangle (the Gram writes both
 // triangles), L overwrites lower+diag and the strict upper triangle keeps the
 // original matrix -- the layout choldc (chol.cu:19-43) leaves behind.  A
 // matrix that meets a non-positive pivot is zero-filled (chol.cu:26-29) and
@@ -15,17 +15,19 @@
 namespace {
 
 constexpr int CT = 256;   // threads per workgroup
-constexpr int NB = 16;    // panel width
-// k-chunk of previous columns staged in LDS: 48 keeps a P = 257 workgroup at
-// 39 KB of LDS, four per CU (64 made it 41 KB, three per CU)
-constexpr int KC = 48;
 constexpr int MAXN = 640; // largest n handled (panel rows in LDS)
 constexpr int PLD = 8;    // panel-load batch per thread
-static_assert(KC % NB == 0, "k-chunks hold whole panels");
-static_assert(NB * KC % CT == 0, "the k-chunk stage splits evenly over the threads");
 
-__global__ void __launch_bounds__(CT) k_chol(double *__restrict__ Aall, int n, size_t stride,
+// Panel width NB and k-chunk KC of the batched Cholesky.  NB = 16, KC = 48
+// keeps a P = 257 workgroup at 39 KB of LDS, four per CU (KC = 64 made it
+// 41 KB, three per CU); NB = 8, KC = 32 takes 18.5 KB, eight per CU, so 2,048
+// matrices are in flight at once (twice the panels, half the work per panel).
+template <int NB, int KC>
+__global__ void __launch_bounds__(CT, NB == 8 ? 8 : 1) k_chol(double *__restrict__ Aall, int n, size_t stride,
                                              int *__restrict__ info) {
+    static_assert(KC % NB == 0, "k-chunks hold whole panels");
+    static_assert(NB * KC % CT == 0, "the k-chunk stage splits evenly over the threads");
+    constexpr int PL = NB == 8 ? 4 : PLD;   // panel loads in flight (the 8-column form fits 64 VGPRs)
     extern __shared__ double lds[];
     double *pan = lds;                 // [n][NB] panel, row-major by panel row
     double *lc = lds + (size_t)n * NB; // [NB][KC] rows kb..kb+NB of L, k-chunk
@@ -38,17 +40,17 @@ __global__ void __launch_bounds__(CT) k_chol(double *__restrict__ Aall, int n, s
     for (int kb = 0; kb < n; kb += NB) {
         const int nbk = min(NB, n - kb);
         const int rows = n - kb;
-        // 1. load the panel's lower part, PLD loads in flight per thread
-        for (int t0 = tid; t0 < rows * nbk; t0 += CT * PLD) {
-            double v[PLD];
+        // 1. load the panel's lower part, PL loads in flight per thread
+        for (int t0 = tid; t0 < rows * nbk; t0 += CT * PL) {
+            double v[PL];
 #pragma unroll
-            for (int u = 0; u < PLD; u++) {
+            for (int u = 0; u < PL; u++) {
                 const int t = t0 + u * CT;
                 const int r = t % rows, cc = t / rows;
                 v[u] = (t < rows * nbk && r >= cc) ? a[(kb + r) + (size_t)(kb + cc) * n] : 0.0;
             }
 #pragma unroll
-            for (int u = 0; u < PLD; u++) {
+            for (int u = 0; u < PL; u++) {
                 const int t = t0 + u * CT;
                 if (t < rows * nbk) pan[(t % rows) * NB + t / rows] = v[u];
             }
@@ -591,10 +593,20 @@ int gpk_chol_batched(gp_ctx *c, double *A, int n, int batch, size_t stride, int 
         GP_HIP(hipGetLastError());
         return 0;
     }
-    const size_t shm = ((size_t)n * NB + (size_t)NB * KC) * sizeof(double);
+    if (c->chol_nb == 8) {
+        const size_t shm = ((size_t)n * 8 + (size_t)8 * 32) * sizeof(double);
+        if (shm > 65536 && !c->capturing)
+            GP_HIP(hipFuncSetAttribute((const void *)k_chol<8, 32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)shm));
+        k_chol<8, 32><<<batch, CT, shm, c->stream>>>(A, n, stride, info);
+        GP_HIP(hipGetLastError());
+        return 0;
+    }
+    const size_t shm = ((size_t)n * 16 + (size_t)16 * 48) * sizeof(double);
     if (shm > 65536 && !c->capturing)   // set before any recording: not a stream operation
-        GP_HIP(hipFuncSetAttribute((const void *)k_chol, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    k_chol<<<batch, CT, shm, c->stream>>>(A, n, stride, info);
+        GP_HIP(hipFuncSetAttribute((const void *)k_chol<16, 48>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)shm));
+    k_chol<16, 48><<<batch, CT, shm, c->stream>>>(A, n, stride, info);
     GP_HIP(hipGetLastError());
     return 0;
 }

@@ -1488,57 +1488,79 @@ __global__ void __launch_bounds__((PC_CW + PC_PW) * 64) k_gram_pc(int Npad, int 
 constexpr int RSG = 4;       // genes per block of the symmetric reduction
 constexpr int GRK_SEG = 64;  // cell segments of the intercept row's FP32 products
 
-// Psi_j from the symmetric-form partials: one 256-thread block per (tile, RSG genes);
-// sums the segments in FP64, each rescaled by its weight unscale 2^e
-// (wun[s wstride + j]: wstride 0 for one scale per gene, J for the fused IRLS
-// pass's per-segment scales; powers of two, so a common scale gives the bits
-// of scaling the sum), rescales by 2^-s_r 2^-s_c, adds the prior and writes
-// the tile and its mirror (upper-triangle values only, so Psi is exactly
-// symmetric).
+// Psi_j from the symmetric-form partials: one 256-thread block per (tile, RSG
+// genes), 64 threads per gene, each thread one 16-byte load per segment (the
+// four values of one MFMA lane: tile rows 4 (l / 16) + r, column l % 16);
+// sums the segments in FP64 in segment order, each rescaled by its weight
+// unscale 2^e (wun[s wstride + j]: wstride 0 for one scale per gene, J for the
+// fused IRLS pass's per-segment scales; powers of two, so a common scale gives
+// the bits of scaling the sum), rescales by 2^-s_r 2^-s_c, adds the prior and
+// writes the tile and its mirror (upper-triangle values only, so Psi is
+// exactly symmetric).
 // off: the tiles cover columns off .. off + 16 T16 - 1 of Psi (1 when the
 // intercept row and column come from k_gram_row0)
 __global__ void __launch_bounds__(256) k_gram_reduce_sym(int P, int T16, int nseg, int J, int off,
         const float *__restrict__ part, const double *__restrict__ wun, size_t wstride,
         const double *__restrict__ zun, const double *__restrict__ Delta, double *__restrict__ Psi) {
-    __shared__ double tile[16][17];
+    static_assert(RSG * 64 == 256, "one 64-thread slot per gene of the block");
+    __shared__ double tile[RSG][16][17];
     const int t = blockIdx.x, tid = threadIdx.x;
     const int NT = T16 * (T16 + 1) / 2;
     int ta = 0, rem = t;
     while (rem >= T16 - ta) { rem -= T16 - ta; ta++; }
     const int tb = ta + rem;
-    // thread tid = 4*lane + r  <->  tile element (4(lane/16) + r, lane%16)
-    const int lane = tid >> 2, r = tid & 3;
-    const int rr = 4 * (lane >> 4) + r, cc = lane & 15;
-    const int row = off + 16 * ta + rr, col = off + 16 * tb + cc;
-    const int i1 = tid & 15, i2 = tid >> 4;
-    const int j0 = blockIdx.y * RSG, j1 = min(J, j0 + RSG);
-    for (int j = j0; j < j1; j++) {
-        if (j > j0) __syncthreads();              // the previous gene's tile reads are done
-        // the segments' partials in order, GRU loads in flight at a time
+    const int q = tid >> 6, l = tid & 63;
+    const int j = blockIdx.y * RSG + q;
+    const int cc = l & 15, r0 = 4 * (l >> 4);
+    const int col = off + 16 * tb + cc;
+    if (j < J) {
+        // the segments' partials in order, GRU 16-byte loads in flight at a time
         constexpr int GRU = 8;
-        double s = 0.0;
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
         int g = 0;
         for (; g + GRU <= nseg; g += GRU) {
-            float pv[GRU];
+            float4 pv[GRU];
 #pragma unroll
-            for (int u = 0; u < GRU; u++) pv[u] = part[(((size_t)(g + u) * J + j) * NT + t) * 256 + tid];
+            for (int u = 0; u < GRU; u++)
+                pv[u] = *(const float4 *)(part + (((size_t)(g + u) * J + j) * NT + t) * 256 + 4 * l);
 #pragma unroll
-            for (int u = 0; u < GRU; u++) s += (double)pv[u] * wun[(size_t)(g + u) * wstride + j];
+            for (int u = 0; u < GRU; u++) {
+                const double wu = wun[(size_t)(g + u) * wstride + j];
+                s[0] += (double)pv[u].x * wu;
+                s[1] += (double)pv[u].y * wu;
+                s[2] += (double)pv[u].z * wu;
+                s[3] += (double)pv[u].w * wu;
+            }
         }
-        for (; g < nseg; g++)
-            s += (double)part[(((size_t)g * J + j) * NT + t) * 256 + tid] * wun[(size_t)g * wstride + j];
-        double v = 0.0;
-        if (row < P && col < P) v = s * (zun[row] * zun[col]) + Delta[row + (size_t)col * P];
-        tile[rr][cc] = v;
-        __syncthreads();
-        double *Pj = Psi + (size_t)j * P * P;
+        for (; g < nseg; g++) {
+            const float4 pv = *(const float4 *)(part + (((size_t)g * J + j) * NT + t) * 256 + 4 * l);
+            const double wu = wun[(size_t)g * wstride + j];
+            s[0] += (double)pv.x * wu;
+            s[1] += (double)pv.y * wu;
+            s[2] += (double)pv.z * wu;
+            s[3] += (double)pv.w * wu;
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int row = off + 16 * ta + r0 + r;
+            double v = 0.0;
+            if (row < P && col < P) v = s[r] * (zun[row] * zun[col]) + Delta[row + (size_t)col * P];
+            tile[q][r0 + r][cc] = v;
+        }
+    }
+    __syncthreads();
+    if (j >= J) return;
+    double *Pj = Psi + (size_t)j * P * P;
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+        const int e = l + 64 * m, i1 = e & 15, i2 = e >> 4;
         {   // Psi[16ta + i1, 16tb + i2]   (coalesced over i1)
             const int R = off + 16 * ta + i1, C = off + 16 * tb + i2;
-            if (R < P && C < P && (ta != tb || i1 <= i2)) Pj[R + (size_t)C * P] = tile[i1][i2];
+            if (R < P && C < P && (ta != tb || i1 <= i2)) Pj[R + (size_t)C * P] = tile[q][i1][i2];
         }
         {   // mirror: Psi[16tb + i1, 16ta + i2] = tile[i2][i1]
             const int R = off + 16 * tb + i1, C = off + 16 * ta + i2;
-            if (R < P && C < P && (ta != tb || i2 <= i1)) Pj[R + (size_t)C * P] = tile[i2][i1];
+            if (R < P && C < P && (ta != tb || i2 <= i1)) Pj[R + (size_t)C * P] = tile[q][i2][i1];
         }
     }
 }

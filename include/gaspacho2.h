@@ -102,6 +102,8 @@ int gp_set_precision(gp_ctx *ctx, int precision);
  *                           by row (0)
  *   "tn_f32"      0/1       mixed path: the gradient's z^T beta-tilde and
  *                           C1a products in FP32 (1) or FP64 (0)
+ *   "chol_nb"     8/16      panel width of the batched Cholesky of the
+ *                           Psi_j (8: eight workgroups per CU, the default)
  *   "gram_wide"   0/1       mixed Gram at M = 512 (intercept-peeled): the
  *                           GP block as two symmetric halves and four
  *                           rectangles on the symmetric kernel's chunk
