@@ -209,6 +209,13 @@ def _trace_summary(o, sec, kind):
             "lbfgs_s": spent[2], "lbfgs_steps": counts[2]}
 
 
+def _ctx_options(g, args):
+    """Apply the --set NAME=VALUE context options to a new GPContext."""
+    for item in args.set:
+        name, _, val = item.partition("=")
+        g.set_option(name, int(val))
+
+
 def _run_fit(args, ds, prec):
     """One gp_fit of ds in precision prec ("f64", "mixed", "f64w") from its
     PCA latent: wall time, final objective, trace and fitted latent means."""
@@ -217,6 +224,7 @@ def _run_fit(args, ds, prec):
     T0 = lib.init_ta_with_per(ds.M, ds.Q)
     g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0, T0,
                       ard_only=args.kernel == "ard", precision=pmap[prec])
+    _ctx_options(g, args)
     if prec == "mixed" and not args.no_tune:
         g.set_option("autotune", 1)
     log(f"{prec} fit: {args.fit_itr} outer iteration(s), L-BFGS cap {args.fit_cap or 'none'}")
@@ -366,6 +374,7 @@ def main_fitcheck(args):
     T0 = lib.init_ta_with_per(ds.M, ds.Q)
     g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0, T0,
                       ard_only=args.kernel == "ard", precision=lib.PREC_MIXED)
+    _ctx_options(g, args)
     scratch = 49999                              # an objective slot the schedule never reaches
     lk = {}
     jtrmax = min(200, args.fit_cap) if args.fit_cap > 0 else 200   # first outer iteration
@@ -747,6 +756,9 @@ def main():
                     help="elbo: the ELBO+gradient unit (default); batched: uneven per-gene GLM fits; "
                          "fitcheck: mixed-vs-FP64 ELBO and gradient errors at every L-BFGS iterate; "
                          "irls: GLM-phase IRLS steps, fused and separate weight pass")
+    ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
+                    help="context option (gp_set_option) for the contexts of the elbo, fit and fitcheck "
+                         "workloads; repeatable")
     ap.add_argument("--nmin", type=int, default=500)
     ap.add_argument("--donors", type=int, default=200, help="score workload: donors")
     ap.add_argument("--snps-per-gene", type=int, default=10, help="score workload: cis SNPs per gene")
@@ -801,6 +813,7 @@ def main():
     T0 = lib.init_ta_with_per(ds.M, ds.Q)
     g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls,
                       ds.X0, T0, ard_only=args.kernel == "ard", precision=lib.PREC_FP64)
+    _ctx_options(g, args)
     g.init_fit()
     for jtr in range(2):
         g.irls_step(jtr)
@@ -940,6 +953,35 @@ def main():
         out["mixed_vs_fp64"] = {"speedup": f["t"] / m["t"],
                                 "elbo_rel_err": abs(m["elbo"] - f["elbo"]) / abs(f["elbo"]),
                                 "grad_rel_err": float(np.max(np.abs(gd)) / np.max(np.abs(f["grad"])))}
+        # the same unit with the contraction's S_j in FP16 alone (option gc_slo 0:
+        # two MFMAs per tile instead of three), for reference: its time and its
+        # gradient's distance from FP64, whole and in the inducing-point block
+        g.set_precision(lib.PREC_MIXED)
+        g.set_option("gc_slo", 0)
+        g.elbo(1)
+        g.grad()
+        t1 = time.perf_counter()
+        for s in range(args.steps):
+            e0 = g.elbo(1)
+            g.grad()
+        t0s = time.perf_counter() - t1
+        g0d = g.get("grad") - f["grad"]
+        npar = len(ds.nh) - 1
+        a = ds.N + npar + ds.N * ds.Q
+        b = a + ds.M * ds.Q
+        g.timing(True)
+        g.timing_reset()
+        g.elbo(1)
+        g.grad()
+        g.timing(False)
+        out["option_gc_slo0"] = {
+            "value": args.steps / t0s, "ms_per_step": t0s / args.steps * 1e3,
+            "gradcontract_kernel_ms": g.timing_get("gradcontract_kernel")[0],
+            "elbo_rel_err": abs(e0 - f["elbo"]) / abs(f["elbo"]),
+            "grad_rel_err": float(np.max(np.abs(g0d)) / np.max(np.abs(f["grad"]))),
+            "grad_T_block_rel_err": float(np.max(np.abs(g0d[a:b])) / np.max(np.abs(f["grad"][a:b]))),
+            "grad_T_block_rel_err_default": float(np.max(np.abs(gd[a:b])) / np.max(np.abs(f["grad"][a:b])))}
+        g.set_option("gc_slo", 1)
     g.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ds, ard_only=args.kernel == "ard")

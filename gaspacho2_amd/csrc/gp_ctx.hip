@@ -302,6 +302,8 @@ extern "C" int gp_set_option(gp_ctx *c, const char *name, int value) {
         c->tn_f32 = value ? 1 : 0;
     } else if (n == "gram_wide") {
         c->gram_wide = value ? 1 : 0;
+    } else if (n == "gc_slo") {
+        c->gc_slo = value ? 1 : 0;
     } else if (n == "chol_nb") {
         if (value != 8 && value != 16) { gp_set_error("chol_nb must be 8 or 16"); return 1; }
         c->chol_nb = value;
@@ -334,6 +336,7 @@ extern "C" int gp_get_option(gp_ctx *c, const char *name, int *value) {
     else if (n == "tn_f32") *value = c->tn_f32;
     else if (n == "gram_wide") *value = c->gram_wide;
     else if (n == "chol_nb") *value = c->chol_nb;
+    else if (n == "gc_slo") *value = c->gc_slo;
     else if (n == "autotune") *value = c->autotune;
     else { gp_set_error("unknown option " + n); return 1; }
     return 0;
@@ -827,7 +830,7 @@ extern "C" int gp_eval_graph(gp_ctx *c, int slot, double *obj) {
     const int key[GP_GRAPH_KEY] = {c->precision, slot, c->gram_splits, c->gc_groups, c->gram_waves,
                                    c->gc_form, c->gc_mixed, c->gc_peel, c->gc_wave_rows,
                                    c->gc_prefetch, c->gram_peel, c->gc_waves, c->chol1,
-                                   c->cholsl_blk, c->tn_f32, c->gram_wide, c->chol_nb};
+                                   c->cholsl_blk, c->tn_f32, c->gram_wide, c->chol_nb, c->gc_slo};
     if (!c->graph_exec || !std::equal(key, key + GP_GRAPH_KEY, c->graph_key)) {
         drop_graph(c);
         double tmp = 0.0;

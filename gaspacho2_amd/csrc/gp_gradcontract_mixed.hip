@@ -895,7 +895,7 @@ __global__ void k_ssplit_gp(int P, int M, const double *__restrict__ S, const in
 template <int TG, int RG, int CG>
 constexpr int gc_gp_waves() { return (8 / RG) * (TG / CG); }
 
-template <int TG, int RG, int CG, int PD>
+template <int TG, int RG, int CG, int PD, bool SL = true>
 __global__ void __launch_bounds__((gc_gp_waves<TG, RG, CG>() * 64))
 k_gc_gp(int N, int Npad, int gsize, int J,
         const float *__restrict__ zg, const float *__restrict__ w32,
@@ -1011,7 +1011,7 @@ k_gc_gp(int N, int Npad, int gsize, int J,
 #pragma unroll
             for (int c = 0; c < CG; c++) {
                 bh[d][c] = *(const h8 *)(hj + bcol[c] + 32 * d);
-                bl[d][c] = *(const h8 *)(lj + bcol[c] + 32 * d);
+                if constexpr (SL) bl[d][c] = *(const h8 *)(lj + bcol[c] + 32 * d);
             }
     }
     for (int jj = j0; jj < j1; jj++) {
@@ -1040,7 +1040,7 @@ k_gc_gp(int N, int Npad, int gsize, int J,
 #pragma unroll
             for (int c = 0; c < CG; c++) {
                 bh[sl][c] = *(const h8 *)(ph + bcol[c]);
-                bl[sl][c] = *(const h8 *)(pl + bcol[c]);
+                if constexpr (SL) bl[sl][c] = *(const h8 *)(pl + bcol[c]);
             }
 #pragma unroll
             for (int r = 0; r < RG; r++) {
@@ -1051,7 +1051,8 @@ k_gc_gp(int N, int Npad, int gsize, int J,
 #pragma unroll
                 for (int c = 0; c < CG; c++) {
                     acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bh[s0][c], acc[r][c], 0, 0, 0);
-                    acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[s0][c], acc[r][c], 0, 0, 0);
+                    if constexpr (SL)
+                        acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ah, bl[s0][c], acc[r][c], 0, 0, 0);
                     acc[r][c] = __builtin_amdgcn_mfma_f32_16x16x32_f16(al, bh[s0][c], acc[r][c], 0, 0, 0);
                 }
                 __builtin_amdgcn_sched_barrier(0);
@@ -1103,15 +1104,15 @@ __global__ void k_gc_u0(int Npad, int M, const double *__restrict__ Zw, const fl
     Uw[i] = s;
 }
 
-template <int TG, int RG, int CG, int PD>
+template <int TG, int RG, int CG, int PD, bool SL = true>
 static int launch_gc_gp(gp_ctx *c, int gsize, int ngroup) {
     constexpr int M = 16 * TG;
     const size_t shm = (size_t)2 * GRC * (M + 8) * sizeof(_Float16);
     if (shm > 65536 && !c->capturing)
-        GP_HIP(hipFuncSetAttribute((const void *)k_gc_gp<TG, RG, CG, PD>,
+        GP_HIP(hipFuncSetAttribute((const void *)k_gc_gp<TG, RG, CG, PD, SL>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     dim3 g(c->Npad / GRC, ngroup);
-    k_gc_gp<TG, RG, CG, PD><<<g, gc_gp_waves<TG, RG, CG>() * 64, shm, c->stream>>>(c->N, c->Npad, gsize, c->J, c->z32 + c->Npad,
+    k_gc_gp<TG, RG, CG, PD, SL><<<g, gc_gp_waves<TG, RG, CG>() * 64, shm, c->stream>>>(c->N, c->Npad, gsize, c->J, c->z32 + c->Npad,
                                                            c->w32, c->gc_perm, c->gc_aexp, c->gc_zexp + 1,
                                                            (const _Float16 *)c->Sh, (const _Float16 *)c->Sl,
                                                            c->gc_t0, c->gc_part);
@@ -1122,30 +1123,38 @@ static int launch_gc_gp(gp_ctx *c, int gsize, int ngroup) {
 // prefetch depth: the deepest of 1, 3 and 7 whose ring divides the k-steps
 // (M / 32) and does not exceed the option gc_prefetch (at most 1 for the
 // 4-wave shape)
-template <int TG, int RG, int CG>
+template <int TG, int RG, int CG, bool SL>
 static int launch_gc_gp_pd(gp_ctx *c, int gsize, int ngroup) {
     constexpr int KS = TG / 2;
     if constexpr (KS % 8 == 0 && RG * CG <= 16) {
-        if (c->gc_prefetch >= 7) return launch_gc_gp<TG, RG, CG, 7>(c, gsize, ngroup);
+        if (c->gc_prefetch >= 7) return launch_gc_gp<TG, RG, CG, 7, SL>(c, gsize, ngroup);
     }
-    if constexpr (KS % 4 == 0 && RG * CG <= 16) {     // 4 B sets of 32 tiles spill
-        if (c->gc_prefetch >= 3) return launch_gc_gp<TG, RG, CG, 3>(c, gsize, ngroup);
+    // 4 B sets of 32 tiles spill with hi/lo B (not with B in FP16 alone)
+    if constexpr (KS % 4 == 0 && (RG * CG <= 16 || !SL)) {
+        if (c->gc_prefetch >= 3) return launch_gc_gp<TG, RG, CG, 3, SL>(c, gsize, ngroup);
     }
     if constexpr (KS % 2 == 0) {
-        if (c->gc_prefetch >= 1) return launch_gc_gp<TG, RG, CG, 1>(c, gsize, ngroup);
+        if (c->gc_prefetch >= 1) return launch_gc_gp<TG, RG, CG, 1, SL>(c, gsize, ngroup);
     }
-    return launch_gc_gp<TG, RG, CG, 0>(c, gsize, ngroup);
+    return launch_gc_gp<TG, RG, CG, 0, SL>(c, gsize, ngroup);
 }
 
 // wave shape: gc_waves 4 (RG = 8, CG = 4; M a multiple of 64), else
 // gc_wave_rows 4 (RG = 4, CG = 4) or the default RG = 8, CG = 2
-template <int TG>
+template <int TG, bool SL>
 static int launch_gc_gp_rg(gp_ctx *c, int gsize, int ngroup) {
     if constexpr (TG % 4 == 0) {
-        if (c->gc_waves == 4) return launch_gc_gp_pd<TG, 8, 4>(c, gsize, ngroup);
-        if (c->gc_wave_rows == 4) return launch_gc_gp_pd<TG, 4, 4>(c, gsize, ngroup);
+        if (c->gc_waves == 4) return launch_gc_gp_pd<TG, 8, 4, SL>(c, gsize, ngroup);
+        if (c->gc_wave_rows == 4) return launch_gc_gp_pd<TG, 4, 4, SL>(c, gsize, ngroup);
     }
-    return launch_gc_gp_pd<TG, 8, 2>(c, gsize, ngroup);
+    return launch_gc_gp_pd<TG, 8, 2, SL>(c, gsize, ngroup);
+}
+
+// operand precision of S_j: hi/lo (option gc_slo 1) or FP16 alone
+template <int TG>
+static int launch_gc_gp_sl(gp_ctx *c, int gsize, int ngroup) {
+    if (c->gc_slo) return launch_gc_gp_rg<TG, true>(c, gsize, ngroup);
+    return launch_gc_gp_rg<TG, false>(c, gsize, ngroup);
 }
 
 // the peeled form applies: the intercept is the only column before the GP
@@ -1174,14 +1183,14 @@ static int gradcontract_gp(gp_ctx *c, int gsize, int ngroup) {
     int r = 0;
     gp_timer_begin(c, c->t_gck);
     switch (TG) {
-    case 2: r = launch_gc_gp_rg<2>(c, gsize, ngroup); break;
-    case 4: r = launch_gc_gp_rg<4>(c, gsize, ngroup); break;
-    case 6: r = launch_gc_gp_rg<6>(c, gsize, ngroup); break;
-    case 8: r = launch_gc_gp_rg<8>(c, gsize, ngroup); break;
-    case 10: r = launch_gc_gp_rg<10>(c, gsize, ngroup); break;
-    case 12: r = launch_gc_gp_rg<12>(c, gsize, ngroup); break;
-    case 14: r = launch_gc_gp_rg<14>(c, gsize, ngroup); break;
-    default: r = launch_gc_gp_rg<16>(c, gsize, ngroup); break;
+    case 2: r = launch_gc_gp_sl<2>(c, gsize, ngroup); break;
+    case 4: r = launch_gc_gp_sl<4>(c, gsize, ngroup); break;
+    case 6: r = launch_gc_gp_sl<6>(c, gsize, ngroup); break;
+    case 8: r = launch_gc_gp_sl<8>(c, gsize, ngroup); break;
+    case 10: r = launch_gc_gp_sl<10>(c, gsize, ngroup); break;
+    case 12: r = launch_gc_gp_sl<12>(c, gsize, ngroup); break;
+    case 14: r = launch_gc_gp_sl<14>(c, gsize, ngroup); break;
+    default: r = launch_gc_gp_sl<16>(c, gsize, ngroup); break;
     }
     if (r) return r;
     gp_timer_end(c, c->t_gck);

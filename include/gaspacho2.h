@@ -102,6 +102,11 @@ int gp_set_precision(gp_ctx *ctx, int precision);
  *                           by row (0)
  *   "tn_f32"      0/1       mixed path: the gradient's z^T beta-tilde and
  *                           C1a products in FP32 (1) or FP64 (0)
+ *   "gc_slo"      0/1       intercept-peeled mixed contraction: S_j as
+ *                           hi/lo against zw as hi/lo, three MFMAs per tile
+ *                           (1, the default), or S_j in FP16 alone, two (0:
+ *                           ~35 % faster contraction, gradient ~10x less
+ *                           accurate, the inducing-point block most)
  *   "chol_nb"     8/16      panel width of the batched Cholesky of the
  *                           Psi_j (8: eight workgroups per CU, the default)
  *   "gram_wide"   0/1       mixed Gram at M = 512 (intercept-peeled): the
