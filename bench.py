@@ -492,6 +492,7 @@ def main_score(args):
     g = lib.GPContext(N, G, M, Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0, T0,
                       ard_only=True, precision=lib.PREC_MIXED)
     g.set_option("gram_splits", args.gram_splits)     # FP32 Gram partials: segments x genes x blocks
+    _ctx_options(g, args)
     g.init_fit()
     irls = []
     for jtr in range(args.irls):
@@ -757,8 +758,8 @@ def main():
                          "fitcheck: mixed-vs-FP64 ELBO and gradient errors at every L-BFGS iterate; "
                          "irls: GLM-phase IRLS steps, fused and separate weight pass")
     ap.add_argument("--set", action="append", default=[], metavar="NAME=VALUE",
-                    help="context option (gp_set_option) for the contexts of the elbo, fit and fitcheck "
-                         "workloads; repeatable")
+                    help="context option (gp_set_option) for the contexts of the elbo, fit, fitcheck and "
+                         "score workloads (the score workload's mixed context); repeatable")
     ap.add_argument("--nmin", type=int, default=500)
     ap.add_argument("--donors", type=int, default=200, help="score workload: donors")
     ap.add_argument("--snps-per-gene", type=int, default=10, help="score workload: cis SNPs per gene")

@@ -302,6 +302,8 @@ extern "C" int gp_set_option(gp_ctx *c, const char *name, int value) {
         c->tn_f32 = value ? 1 : 0;
     } else if (n == "gram_wide") {
         c->gram_wide = value ? 1 : 0;
+    } else if (n == "score_mfma") {
+        c->score_mfma = value ? 1 : 0;
     } else if (n == "gc_slo") {
         c->gc_slo = value ? 1 : 0;
     } else if (n == "chol_nb") {
@@ -337,6 +339,7 @@ extern "C" int gp_get_option(gp_ctx *c, const char *name, int *value) {
     else if (n == "gram_wide") *value = c->gram_wide;
     else if (n == "chol_nb") *value = c->chol_nb;
     else if (n == "gc_slo") *value = c->gc_slo;
+    else if (n == "score_mfma") *value = c->score_mfma;
     else if (n == "autotune") *value = c->autotune;
     else { gp_set_error("unknown option " + n); return 1; }
     return 0;

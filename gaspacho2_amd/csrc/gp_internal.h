@@ -162,6 +162,7 @@ struct gp_ctx {
     int chol1 = 1;             // the Kmm factor takes the one-matrix Cholesky kernel (0: the batched kernel)
     int cholsl_blk = 1;        // per-gene solves blocked by 16 rows (0: row by row)
     int chol_nb = 8;           // panel width of the batched Cholesky (8 or 16)
+    int score_mfma = 1;        // score tests: partials on FP64 MFMA, all columns per pass (P <= 576)
     int gc_slo = 1;            // peeled mixed contraction: S_j as hi/lo (1, three MFMAs) or in FP16 (0, two)
     // mixed path: the gradient's z^T tBeta and C1a products in FP32 (0: FP64)
     int tn_f32 = 1;

@@ -167,6 +167,122 @@ __global__ void __launch_bounds__(ST) k_score_acc(int N, int Npad, int P, int J,
     }
 }
 
+// The same partials with every model-matrix column in one pass over the
+// cells: per 64-cell tile the staged h_i o_ij (as above) is the B operand of
+// v_mfma_f64_16x16x4f64 and the tile's Z rows the A operand, read straight
+// from Zsrc; wave w owns the 16-column tiles w, w + 4, ... (KTW of them, FP64
+// accumulators), so the cell weights are staged once per tile instead of
+// once per 256 columns.  ZT = float reads the mixed path's FP32 copy of Zw
+// (z32 = zw 2^(14 - zc), half the bytes; each column's sum is unscaled by
+// zun[k] = 2^(zc - 14) at the end), ZT = double the FP64 model matrix (zun
+// null).  Fragment layouts (lane l): A[m = l % 16][k = l / 16]
+// = Z[cell 4 s + l / 16][column 16 kt + l % 16], B[k = l / 16][n = l % 16] =
+// v[cell 4 s + l / 16][test l % 16], D[m = 4 r + l / 16][n = l % 16].
+typedef double sc_d4 __attribute__((ext_vector_type(4)));
+template <typename GT, int KTW, typename ZT>
+__global__ void __launch_bounds__(ST) k_score_mfma(int N, int Npad, int P, int J, int S, int T,
+        int t0, const GT *__restrict__ G, const int *__restrict__ donor, int ld,
+        const double *__restrict__ d,
+        const double *__restrict__ cmean, const int *__restrict__ gene,
+        const int *__restrict__ snp, const double *__restrict__ W,
+        const double *__restrict__ Omega, const ZT *__restrict__ Zsrc, const double *__restrict__ zun,
+        double *__restrict__ part, int *__restrict__ err) {
+    static_assert(TT == 16 && CT == 64 && ST == 256, "fragment layout: 16 tests, 64-cell tiles, 4 waves");
+    __shared__ double v[CT][TT];
+    __shared__ int sj[TT], sl[TT];
+    __shared__ double sc[TT];
+    const int tid = threadIdx.x;
+    const int tb = t0 + blockIdx.y * TT;
+    if (tb >= T) return;
+    const int nt = min(TT, T - tb);
+    if (tid < TT) {
+        int j = -1, l = -1;
+        double c = 0.0;
+        if (tid < nt) {
+            j = gene[tb + tid];
+            l = snp[tb + tid];
+            if (!test_ok(j, l, J, S)) {
+                if (blockIdx.x == 0) err[0] = 1;
+                j = -1;
+                l = -1;
+            } else if (d) {
+                c = cmean[l];
+            }
+        }
+        sj[tid] = j;
+        sl[tid] = l;
+        sc[tid] = c;
+    }
+    __syncthreads();
+    double *pb = part + ((size_t)blockIdx.y * NBK + blockIdx.x) * TT * (P + 2);
+    const int ci = tid & (CT - 1), w = tid >> 6, lane = tid & 63;
+    const int lm = lane & 15, lk = lane >> 4;
+    const int ntile = Npad / CT;
+    const int KT = (P + 15) / 16;
+    sc_d4 acc[KTW];
+#pragma unroll
+    for (int q = 0; q < KTW; q++) acc[q] = sc_d4{0.0, 0.0, 0.0, 0.0};
+    double sa[4] = {0.0, 0.0, 0.0, 0.0}, sw[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int tile = blockIdx.x; tile < ntile; tile += NBK) {
+        const int i = tile * CT + ci;
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int t = w + 4 * u;
+            double val = 0.0;
+            if (i < N && sj[t] >= 0) {
+                double h = gat(G, donor, ld, sl[t], i);
+                if (d) h = h * d[i] - sc[t];
+                const double o = W[i + (size_t)sj[t] * Npad] / Omega[i];
+                val = h * o;
+                sa[u] += h * val;
+                sw[u] += val;
+            }
+            v[ci][t] = val;
+        }
+        __syncthreads();
+        double bf[CT / 4];
+#pragma unroll
+        for (int st = 0; st < CT / 4; st++) bf[st] = v[4 * st + lk][lm];
+        const size_t cell0 = (size_t)tile * CT + lk;
+#pragma unroll
+        for (int q = 0; q < KTW; q++) {
+            const int kt = w + 4 * q;
+            if (kt < KT) {                             // uniform over the wave
+                const int col = min(16 * kt + lm, P - 1);
+                const bool colok = 16 * kt + lm < P;
+                const ZT *zc = Zsrc + (size_t)col * Npad + cell0;
+                double af[CT / 4];
+#pragma unroll
+                for (int st = 0; st < CT / 4; st++) af[st] = (double)zc[4 * st];
+#pragma unroll
+                for (int st = 0; st < CT / 4; st++)
+                    acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(colok ? af[st] : 0.0, bf[st], acc[q], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < KTW; q++) {
+        const int kt = w + 4 * q;
+        if (kt < KT) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int k = 16 * kt + 4 * r + lk;
+                if (k < P) pb[(size_t)lm * (P + 2) + k] = zun ? acc[q][r] * zun[k] : acc[q][r];
+            }
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const double a = wave_sum(sa[u]), s = wave_sum(sw[u]);
+        if ((tid & 63) == 0) {
+            const int t = w + 4 * u;
+            pb[(size_t)t * (P + 2) + P] = a;
+            pb[(size_t)t * (P + 2) + P + 1] = s;
+        }
+    }
+}
+
 // One workgroup per test of the launch: sum the NBK partials, add the count
 // term over the gene's non-zeros, forward substitution, statistic, p-value.
 template <typename GT>
@@ -200,23 +316,77 @@ __global__ void __launch_bounds__(ST) k_score_fin(int N, int P, int T, int t0,
     }
     const double cm = d ? cmean[l] : 0.0;
     double sy = 0.0;
-    for (int q = Yp[j] + tid; q < Yp[j + 1]; q += ST) {
-        const int i = Yi[q];
-        double h = gat(G, donor, ld, l, i);
-        if (d) h = h * d[i] - cm;
-        sy += h * Yx[q] / Omega[i];
+    {   // the count term over the gene's non-zeros, SYU entries per thread in
+        // flight (the cell index, then its genotype, factor and Omega)
+        constexpr int SYU = 8;
+        const int q1 = Yp[j + 1];
+        int q = Yp[j] + tid;
+        for (; q + (SYU - 1) * ST < q1; q += SYU * ST) {
+            int ii[SYU];
+            double yx[SYU], h[SYU], om[SYU], dv[SYU];
+#pragma unroll
+            for (int u = 0; u < SYU; u++) {
+                ii[u] = Yi[q + u * ST];
+                yx[u] = Yx[q + u * ST];
+            }
+#pragma unroll
+            for (int u = 0; u < SYU; u++) {
+                h[u] = gat(G, donor, ld, l, ii[u]);
+                om[u] = Omega[ii[u]];
+                dv[u] = d ? d[ii[u]] : 1.0;
+            }
+#pragma unroll
+            for (int u = 0; u < SYU; u++) {
+                const double hh = d ? h[u] * dv[u] - cm : h[u];
+                sy += hh * yx[u] / om[u];
+            }
+        }
+        for (; q < q1; q += ST) {
+            const int i = Yi[q];
+            double h = gat(G, donor, ld, l, i);
+            if (d) h = h * d[i] - cm;
+            sy += h * Yx[q] / Omega[i];
+        }
     }
     sy = block_sum(sy, sh);                // also orders the xs writes before the reads below
     const double a = xs[P], sw = xs[P + 1];
     const double *L = Psi + (size_t)j * P * P;
-    for (int i = 0; i < P; i++) {
+    // L y = b by blocks of SBF rows: the diagonal block on wave 0 (row r in
+    // lane r, each y_c broadcast by a shuffle), then every later row takes the
+    // block's SBF terms; two barriers per block instead of two per row
+    constexpr int SBF = 16;
+    const int lane = tid & 63;
+    for (int kb = 0; kb < P; kb += SBF) {
+        const int nb = min(SBF, P - kb);
+        if (tid < 64) {
+            const int r = lane;
+            double t = (r < nb) ? xs[kb + r] : 0.0;
+            double lrow[SBF];
+#pragma unroll
+            for (int cc = 0; cc < SBF; cc++)
+                lrow[cc] = (r < nb && cc <= r) ? L[(kb + r) + (size_t)(kb + cc) * P] : 0.0;
+            double y = 0.0;
+#pragma unroll
+            for (int cc = 0; cc < SBF; cc++) {
+                if (r == cc && cc < nb) y = t / lrow[cc];
+                const double yc = __shfl(y, cc);
+                if (r > cc && cc < nb) t -= lrow[cc] * yc;
+            }
+            if (r < nb) xs[kb + r] = y;
+        }
         __syncthreads();
-        const double xi = xs[i] / L[i + (size_t)i * P];
+        double yb[SBF];
+#pragma unroll
+        for (int cc = 0; cc < SBF; cc++) yb[cc] = (cc < nb) ? xs[kb + cc] : 0.0;
+        for (int r = kb + nb + tid; r < P; r += ST) {
+            double v = xs[r];
+#pragma unroll
+            for (int cc = 0; cc < SBF; cc++)
+                if (cc < nb) v -= L[r + (size_t)(kb + cc) * P] * yb[cc];
+            xs[r] = v;
+        }
         __syncthreads();
-        if (tid == 0) xs[i] = xi;
-        for (int r = i + 1 + tid; r < P; r += ST) xs[r] -= L[r + (size_t)i * P] * xi;
     }
-    __syncthreads();
     double q = 0.0;
     for (int k = tid; k < P; k += ST) q += xs[k] * xs[k];
     q = block_sum(q, sh);
@@ -240,9 +410,29 @@ int score_run(gp_ctx *c, int S, const GT *G, const int *donor, int ld, const dou
     for (int t0 = 0; t0 < T; t0 += per) {
         const int n = std::min(per, T - t0);
         const int nch = (n + TT - 1) / TT;
-        k_score_acc<GT><<<dim3(NBK, nch), ST, 0, c->stream>>>(c->N, c->Npad, P, c->J, S, T, t0, G, donor,
-                                                             ld, d, cmean, gene, snp, c->W, c->Omega,
-                                                             Zsrc, part, err);
+        const int ktw = ((P + 15) / 16 + 3) / 4;   // 16-column tiles per wave of the MFMA form
+        // the FP32 copy of Zw when the mixed path holds it for the current basis
+        const bool z32 = c->precision == GP_PREC_MIXED && c->psi_whitened && c->zsplit_valid && c->z32;
+#define SCORE_MFMA(K)                                                                                    \
+    case K:                                                                                              \
+        if (z32)                                                                                         \
+            k_score_mfma<GT, K, float><<<dim3(NBK, nch), ST, 0, c->stream>>>(                            \
+                c->N, c->Npad, P, c->J, S, T, t0, G, donor, ld, d, cmean, gene, snp, c->W, c->Omega,    \
+                c->z32, c->zun, part, err);                                                              \
+        else                                                                                             \
+            k_score_mfma<GT, K, double><<<dim3(NBK, nch), ST, 0, c->stream>>>(                           \
+                c->N, c->Npad, P, c->J, S, T, t0, G, donor, ld, d, cmean, gene, snp, c->W, c->Omega,    \
+                Zsrc, nullptr, part, err);                                                               \
+        break;
+        switch (c->score_mfma ? ktw : 0) {
+            SCORE_MFMA(1) SCORE_MFMA(2) SCORE_MFMA(3) SCORE_MFMA(4) SCORE_MFMA(5)
+            SCORE_MFMA(6) SCORE_MFMA(7) SCORE_MFMA(8) SCORE_MFMA(9)
+        default:
+            k_score_acc<GT><<<dim3(NBK, nch), ST, 0, c->stream>>>(c->N, c->Npad, P, c->J, S, T, t0, G, donor,
+                                                                 ld, d, cmean, gene, snp, c->W, c->Omega,
+                                                                 Zsrc, part, err);
+        }
+#undef SCORE_MFMA
         GP_HIP(hipGetLastError());
         k_score_fin<GT><<<n, ST, (size_t)(P + 2) * sizeof(double), c->stream>>>(
             c->N, P, T, t0, G, donor, ld, d, cmean, gene, snp, c->J, S, c->Yp, c->Yi, c->Yx, c->Omega, c->Psi,

@@ -107,6 +107,10 @@ int gp_set_precision(gp_ctx *ctx, int precision);
  *                           (1, the default), or S_j in FP16 alone, two (0:
  *                           ~35 % faster contraction, gradient ~10x less
  *                           accurate, the inducing-point block most)
+ *   "score_mfma"  0/1       score tests: partials on FP64 MFMA over every
+ *                           model-matrix column per pass (1, the default;
+ *                           P <= 576, the FP32 copy of Zw in a mixed
+ *                           context) or one column per thread (0)
  *   "chol_nb"     8/16      panel width of the batched Cholesky of the
  *                           Psi_j (8: eight workgroups per CU, the default)
  *   "gram_wide"   0/1       mixed Gram at M = 512 (intercept-peeled): the

@@ -941,6 +941,45 @@ def test_score_tests_mixed_basis_close_to_oracle():
     assert rel(sg, so) < 1e-6
 
 
+@pytest.mark.parametrize("prec", ["f64", "mixed"])
+def test_score_mfma_form_matches_column_loop(prec):
+    """The score-test partials on FP64 MFMA with every model-matrix column in
+    one pass over the cells (option score_mfma, the default) against the
+    column-per-thread loop (score_mfma 0), interaction tests with per-donor
+    genotypes at P = 97: in an FP64 context both read the FP64 model matrix
+    (statistics within 1e-10 of each other; measured equal); in a mixed
+    context after gp_elbo the MFMA form reads the FP32 copy of Zw (within
+    1e-8, FP32's rounding of the model matrix; measured 4.2e-10)."""
+    from gaspacho2_amd import lib, synth
+    ds = synth.make_dataset(3000, 24, 5, 96, seed=4)
+    T0 = lib.init_ta_with_per(ds.M, ds.Q)
+    g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0, T0,
+                      precision=lib.PREC_FP64)
+    g.init_fit()
+    for jtr in range(2):
+        g.irls_step(jtr)
+    g.ql_start()
+    if prec == "mixed":
+        g.set_precision(lib.PREC_MIXED)
+        g.elbo(1)
+    rng = np.random.default_rng(21)
+    D, S = 40, 6
+    donor = rng.integers(0, D, ds.N).astype(np.int32)
+    dos = rng.integers(0, 3, (D, ds.J * S)).astype(np.uint8)
+    factor = rng.uniform(0.0, 2.0, ds.N)
+    gene = np.repeat(np.arange(ds.J), S).astype(np.int32)
+    snp = np.arange(ds.J * S).astype(np.int32)
+    out = {}
+    for form in (1, 0):
+        g.set_option("score_mfma", form)
+        out[form] = g.score_tests_donor(donor, dos, factor, gene, snp)
+    g.close()
+    rs, rp = rel(out[1][0], out[0][0]), rel(out[1][1], out[0][1])
+    print(f"score partials, MFMA vs column loop ({prec}): statistics {rs:.2e}, p-values {rp:.2e}")
+    assert np.all(np.isfinite(out[1][0]))
+    assert rs < (1e-10 if prec == "f64" else 1e-8)
+
+
 def _batched_oracle(ds, bd, g, T0):
     """The oracle on gene g's own cells and design (one-gene state)."""
     from oracle.oracle import OracleState
