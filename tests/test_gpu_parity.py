@@ -443,6 +443,38 @@ def test_mixed_gradient_contraction_at_m256(slo):
     assert ge < 1e-4
 
 
+@pytest.mark.parametrize("M", [32, 96])
+def test_fp16_s_contraction_other_widths(M):
+    """The peeled contraction with S_j in FP16 alone (option gc_slo 0) at two
+    more column-tile counts (M / 16 = 2 and 6): Uw within FP16's unit
+    roundoff (5e-4 of its largest entry) of the FP64 contraction of the same
+    whitened inverses, and the ELBO untouched (the contraction is
+    gradient-only)."""
+    from gaspacho2_amd import lib, synth
+    ds = synth.make_dataset(2000, 40, 5, M, seed=13)
+    T0 = lib.init_ta_with_per(ds.M, ds.Q)
+    g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0,
+                      T0, precision=lib.PREC_FP64)
+    g.init_fit()
+    for jtr in range(2):
+        g.irls_step(jtr)
+    g.ql_start()
+    g.set_precision(lib.PREC_MIXED)
+    e1 = g.elbo(1)
+    g.set_option("gc_mixed", 0)
+    g.grad()
+    uw64 = g.get("Uw")
+    g.set_option("gc_mixed", 1)
+    g.set_option("gc_slo", 0)
+    e0 = g.elbo(1)
+    g.grad()
+    ue = rel(g.get("Uw"), uw64)
+    g.close()
+    print(f"M={M} FP16-S contraction: Uw rel err {ue:.3e}")
+    assert e0 == e1
+    assert ue < 5e-4
+
+
 @pytest.mark.parametrize("M", [32, 40, 256])
 def test_single_matrix_cholesky_matches_batched_kernel(M):
     """The one-matrix Cholesky of the Kmm factor (option chol1, the default;
