@@ -580,6 +580,74 @@ __global__ void __launch_bounds__(CT) k_vtv(const double *__restrict__ Vall, int
         }
 }
 
+// k_vtv on FP64 MFMA (v_mfma_f64_16x16x4f64): the same 64x64 tiles and
+// 32-row chunks of V staged in LDS; wave w owns the 16 rows a0 + 16 w .. of
+// the tile against its four 16-column blocks, so per 4-row step one A
+// fragment and four B fragments feed four MFMAs (the VALU form reads eight
+// values from LDS per 16 FMAs).  Fragment layouts (lane l): A[m = l % 16][k =
+// l / 16] = V[k0 + 4 s + l / 16][a0 + 16 w + l % 16], B[k = l / 16][n = l % 16]
+// = V[k0 + 4 s + l / 16][b0 + 16 cb + l % 16], D[m = 4 r + l / 16][n = l % 16].
+__global__ void __launch_bounds__(CT) k_vtv_mfma(const double *__restrict__ Vall, int n, size_t stride,
+                                                 int ntile, double *__restrict__ Sall) {
+    __shared__ double As[32][65];
+    __shared__ double Bs[32][65];
+    int ta = 0, t = blockIdx.x;
+    while (t >= ntile - ta) { t -= ntile - ta; ta++; }
+    const int tb = ta + t;
+    const int a0 = ta * 64, b0 = tb * 64;
+    const double *V = Vall + (size_t)blockIdx.y * stride;
+    double *S = Sall + (size_t)blockIdx.y * stride;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int lm = lane & 15, lk = lane >> 4;
+    tri_d4 acc[4];
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++) acc[cb] = tri_d4{0.0, 0.0, 0.0, 0.0};
+    // staging as k_vtv: thread t stages column t % 64 of both tiles, rows
+    // 8 (t / 64) .. +7 of the chunk, the next chunk loaded during this one
+    const int col = tid % 64, rg = 8 * (tid / 64);
+    const int ga = a0 + col, gb = b0 + col;
+    const double *va = V + (size_t)min(ga, n - 1) * n;
+    const double *vb = V + (size_t)min(gb, n - 1) * n;
+    double pa[8], pb[8];
+    auto fetch = [&](int k0) {
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int k = k0 + rg + u;
+            pa[u] = (k < n && ga < n && k >= ga) ? va[k] : 0.0;
+            pb[u] = (k < n && gb < n && k >= gb) ? vb[k] : 0.0;
+        }
+    };
+    fetch(b0);
+    for (int k0 = b0; k0 < n; k0 += 32) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            As[rg + u][col] = pa[u];
+            Bs[rg + u][col] = pb[u];
+        }
+        __syncthreads();
+        if (k0 + 32 < n) fetch(k0 + 32);
+#pragma unroll
+        for (int s4 = 0; s4 < 8; s4++) {
+            const double av = As[4 * s4 + lk][16 * w + lm];
+#pragma unroll
+            for (int cb = 0; cb < 4; cb++)
+                acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, Bs[4 * s4 + lk][16 * cb + lm], acc[cb], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int cb = 0; cb < 4; cb++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int ra = a0 + 16 * w + 4 * r + lk, cbn = b0 + 16 * cb + lm;
+            if (ra < n && cbn < n && ra <= cbn) {
+                S[ra + (size_t)cbn * n] = acc[cb][r];
+                S[cbn + (size_t)ra * n] = acc[cb][r];
+            }
+        }
+}
+
 }  // namespace
 
 int gpk_chol_batched(gp_ctx *c, double *A, int n, int batch, size_t stride, int *info) {
@@ -642,7 +710,10 @@ int gpk_chol_inverse_batched(gp_ctx *c, const double *L, int n, int batch, size_
     }
     const int nt = (n + 63) / 64;
     dim3 g(nt * (nt + 1) / 2, batch);
-    k_vtv<<<g, CT, 0, c->stream>>>(c->Vtmp, n, st, nt, S);
+    if (c->vtv_mfma)
+        k_vtv_mfma<<<g, CT, 0, c->stream>>>(c->Vtmp, n, st, nt, S);
+    else
+        k_vtv<<<g, CT, 0, c->stream>>>(c->Vtmp, n, st, nt, S);
     GP_HIP(hipGetLastError());
     return 0;
 }
@@ -651,7 +722,10 @@ int gpk_vtv(gp_ctx *c, const double *V, int n, int batch, size_t stride, double 
     const size_t st = stride ? stride : (size_t)n * n;
     const int nt = (n + 63) / 64;
     dim3 g(nt * (nt + 1) / 2, batch);
-    k_vtv<<<g, CT, 0, c->stream>>>(V, n, st, nt, S);
+    if (c->vtv_mfma)
+        k_vtv_mfma<<<g, CT, 0, c->stream>>>(V, n, st, nt, S);
+    else
+        k_vtv<<<g, CT, 0, c->stream>>>(V, n, st, nt, S);
     GP_HIP(hipGetLastError());
     return 0;
 }

@@ -304,6 +304,8 @@ extern "C" int gp_set_option(gp_ctx *c, const char *name, int value) {
         c->gram_wide = value ? 1 : 0;
     } else if (n == "score_mfma") {
         c->score_mfma = value ? 1 : 0;
+    } else if (n == "vtv_mfma") {
+        c->vtv_mfma = value ? 1 : 0;
     } else if (n == "gc_slo") {
         c->gc_slo = value ? 1 : 0;
     } else if (n == "chol_nb") {
@@ -339,6 +341,7 @@ extern "C" int gp_get_option(gp_ctx *c, const char *name, int *value) {
     else if (n == "gram_wide") *value = c->gram_wide;
     else if (n == "chol_nb") *value = c->chol_nb;
     else if (n == "gc_slo") *value = c->gc_slo;
+    else if (n == "vtv_mfma") *value = c->vtv_mfma;
     else if (n == "score_mfma") *value = c->score_mfma;
     else if (n == "autotune") *value = c->autotune;
     else { gp_set_error("unknown option " + n); return 1; }
@@ -833,7 +836,8 @@ extern "C" int gp_eval_graph(gp_ctx *c, int slot, double *obj) {
     const int key[GP_GRAPH_KEY] = {c->precision, slot, c->gram_splits, c->gc_groups, c->gram_waves,
                                    c->gc_form, c->gc_mixed, c->gc_peel, c->gc_wave_rows,
                                    c->gc_prefetch, c->gram_peel, c->gc_waves, c->chol1,
-                                   c->cholsl_blk, c->tn_f32, c->gram_wide, c->chol_nb, c->gc_slo};
+                                   c->cholsl_blk, c->tn_f32, c->gram_wide, c->chol_nb, c->gc_slo,
+                                   c->vtv_mfma};
     if (!c->graph_exec || !std::equal(key, key + GP_GRAPH_KEY, c->graph_key)) {
         drop_graph(c);
         double tmp = 0.0;

@@ -13,7 +13,7 @@
 
 #define GP_HIST 10            // L-BFGS history length (ggplvm.cu:872-880)
 #define GP_LKHD_SLOTS 50000   // objective history length (ggplvm.cu:1025)
-#define GP_GRAPH_KEY 18       // settings a recorded ELBO + gradient graph depends on
+#define GP_GRAPH_KEY 19       // settings a recorded ELBO + gradient graph depends on
 
 struct gp_timer {
     hipEvent_t a = nullptr, b = nullptr;
@@ -162,6 +162,7 @@ struct gp_ctx {
     int chol1 = 1;             // the Kmm factor takes the one-matrix Cholesky kernel (0: the batched kernel)
     int cholsl_blk = 1;        // per-gene solves blocked by 16 rows (0: row by row)
     int chol_nb = 8;           // panel width of the batched Cholesky (8 or 16)
+    int vtv_mfma = 1;          // S = V^T V of the inverses on FP64 MFMA (1) or VALU (0)
     int score_mfma = 1;        // score tests: partials on FP64 MFMA, all columns per pass (P <= 576)
     int gc_slo = 1;            // peeled mixed contraction: S_j as hi/lo (1, three MFMAs) or in FP16 (0, two)
     // mixed path: the gradient's z^T tBeta and C1a products in FP32 (0: FP64)
@@ -183,8 +184,8 @@ struct gp_ctx {
     hipGraphExec_t graph_exec = nullptr;
     // precision, slot, gram_splits, gc_groups, gram_waves, gc_form, gc_mixed, gc_peel,
     // gc_wave_rows, gc_prefetch, gram_peel, gc_waves, chol1, cholsl_blk, tn_f32, gram_wide,
-    // chol_nb, gc_slo
-    int graph_key[GP_GRAPH_KEY] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+    // chol_nb, gc_slo, vtv_mfma
+    int graph_key[GP_GRAPH_KEY] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
     // objective and elapsed seconds after every accepted step of the last gp_fit
     std::vector<double> fit_obj, fit_sec;
     std::vector<int> fit_kind;           // 0 IRLS step, 1 quasi-likelihood start, 2 L-BFGS step

@@ -111,6 +111,8 @@ int gp_set_precision(gp_ctx *ctx, int precision);
  *                           model-matrix column per pass (1, the default;
  *                           P <= 576, the FP32 copy of Zw in a mixed
  *                           context) or one column per thread (0)
+ *   "vtv_mfma"    0/1       S = V^T V of the per-gene inverses on FP64
+ *                           MFMA (1, the default) or FP64 VALU (0)
  *   "chol_nb"     8/16      panel width of the batched Cholesky of the
  *                           Psi_j (8: eight workgroups per CU, the default)
  *   "gram_wide"   0/1       mixed Gram at M = 512 (intercept-peeled): the
