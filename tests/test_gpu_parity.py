@@ -443,6 +443,35 @@ def test_mixed_gradient_contraction_at_m256(slo):
     assert ge < 1e-4
 
 
+@pytest.mark.parametrize("M", [32, 160])
+def test_inverse_vtv_forms_agree(M):
+    """The per-gene inverses S_j = V_j^T V_j (solvePsi, ggplvm.cu:46-53) with
+    the product on FP64 MFMA (option vtv_mfma, the default) against the FP64
+    VALU form: the inverses within 1e-12 of their largest entry and the
+    gradient within the FP64 path's array tolerance (1e-9; both measured
+    equal: the MFMA adds the rows' products in the loop's order); P = 33 (one
+    64-column tile) and P = 161 (three tiles, the last partial)."""
+    from gaspacho2_amd import lib, synth
+    ds = synth.make_dataset(1500, 24, 5, M, seed=17)
+    T0 = lib.init_ta_with_per(ds.M, ds.Q)
+    out = {}
+    for form in (1, 0):
+        g = lib.GPContext(ds.N, ds.J, ds.M, ds.Q, ds.nh, ds.Yp, ds.Yi, ds.Yx, ds.Zcov, ds.ls, ds.X0,
+                          T0, precision=lib.PREC_FP64)
+        g.set_option("vtv_mfma", form)
+        g.init_fit()
+        for jtr in range(2):
+            g.irls_step(jtr)
+        g.ql_start()
+        g.grad()
+        J, P = ds.J, ds.P
+        out[form] = (g.get("Psi")[J * P * P:].copy(), g.get("grad").copy())
+        g.close()
+    ri, rg = rel(out[1][0], out[0][0]), rel(out[1][1], out[0][1])
+    print(f"V^T V on MFMA vs VALU, P={ds.P}: inverses {ri:.2e}, gradient {rg:.2e}")
+    assert ri < 1e-12 and rg < TOL
+
+
 @pytest.mark.parametrize("M", [32, 96])
 def test_fp16_s_contraction_other_widths(M):
     """The peeled contraction with S_j in FP16 alone (option gc_slo 0) at two
