@@ -76,7 +76,7 @@ static void free_all(gp_ctx *c) {
                     c->SS, c->XX, c->lbq, c->gradDeltaTmp, c->gradLogRhoTmp, c->C1, c->C2, c->U,
                     c->ZB, c->R, c->C1a, c->AO, c->G2, c->B2, c->S2, c->Ecell, c->Epart, c->ci1, c->red, c->scal,
                     c->lkhd, c->info, c->Zh, c->Zl, c->z32, c->gram_part, c->pwork, c->w32, c->wsc,
-                    c->wun, c->zun, c->Linv, c->Zw, c->DeltaW, c->DeltaWChol, c->Uw, c->tBw,
+                    c->wun, c->zun, c->Linv, c->LinvT, c->Zw, c->DeltaW, c->DeltaWChol, c->Uw, c->tBw,
                     c->Vtmp, c->Sh, c->Sl, c->gc_perm, c->gc_aexp, c->gc_sexp, c->gc_bexp,
                     c->gc_gun, c->gc_part, c->gc_zexp, c->skw, c->gc_kappa, c->gc_t0, c->gc_s0,
                     c->gc_y, c->gram_row0, c->irls_cc, c->irls_segptr, c->irls_sums,
@@ -178,6 +178,7 @@ extern "C" int gp_create(gp_ctx **out, int N, int J, int M, int Q, int p, const 
     ALLOC(c->DeltaInv, P * P);
     ALLOC(c->DeltaChol, P * P);
     ALLOC(c->Linv, (size_t)M * M);
+    ALLOC(c->LinvT, (size_t)M * M);
     ALLOC(c->Zw, Np * P);
     ALLOC(c->DeltaW, P * P);
     ALLOC(c->DeltaWChol, P * P);

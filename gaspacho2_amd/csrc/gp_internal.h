@@ -71,6 +71,7 @@ struct gp_ctx {
     double *DeltaChol = nullptr; // P x P factor of DeltaInv (lower+diag)
     // whitened basis (mixed path): GP columns Knm L^-T with Kmm = L L^T
     double *Linv = nullptr;      // M x M  L^{-1}
+    double *LinvT = nullptr;     // M x M  L^{-T}: the whitening product's right operand, untransposed
     double *Zw = nullptr;        // Npad x P  [covariates | Knm L^-T]
     double *DeltaW = nullptr;    // P x P  diag(1/delta_l): the whitened prior
     double *DeltaWChol = nullptr;// P x P  diag(sqrt(1/delta_l))
