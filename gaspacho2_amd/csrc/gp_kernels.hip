@@ -1056,13 +1056,7 @@ int gpk_gemm_tn_splitk(gp_ctx *c, int M, int N, int K, const double *A, int lda,
                               &one, A, lda, B, ldb, &zero, C, ldc));
         return 0;
     }
-    // a split count in [want, 2 want] that cuts K into equal 16-aligned pieces
-    // gives one batched call; otherwise 64-aligned pieces and a remainder GEMM,
-    // which has only the output's few tiles to spread over the chip (0.42 ms
-    // for the last 3,328 of 100,096 cells in Z^T R at P = 257, J = 2,000)
-    int Kc = ((K + want - 1) / want + 63) / 64 * 64;
-    for (int s = want; s <= 2 * want && 16 * s <= K; s++)
-        if (K % s == 0 && (K / s) % 16 == 0) { Kc = K / s; break; }
+    const int Kc = ((K + want - 1) / want + 63) / 64 * 64;
     const int nfull = K / Kc, rem = K - nfull * Kc;
     const int nsplit = nfull + (rem > 0);
     const size_t mn = (size_t)M * N;
