@@ -255,7 +255,7 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
                 if (part == 0) nu = log((y + 0.05) / lsv);
             } else {
                 const double l1 = l1v, c = cvn, e2 = e2v;
-                // round 1: exp(eta) (part 0), log(y) (part 1)
+                // first transcendental step: exp(eta) (part 0), log(y) (part 1)
                 double r1 = 0.0;
                 if (part == 0) r1 = exp(eta);
                 else if (part == 1 && y > 0.0) r1 = log(y);
@@ -263,7 +263,7 @@ __global__ void __launch_bounds__(BTH) k_bf_pass(int mode, int J, int M, int PB,
                 const double ly = __shfl(r1, cb0 + 1, 64);
                 const double logW = eta + l1 + c;
                 const double Wi = eeta * lsv * e2;
-                // round 2: exp(log y - logW) (part 0), sqrt(W / Omega) (part 1), log(W) (part 2)
+                // second step: exp(log y - logW) (part 0), sqrt(W / Omega) (part 1), log(W) (part 2)
                 double r2 = 0.0;
                 if (part == 0 && y > 0.0) r2 = exp(ly - logW);
                 else if (part == 1) r2 = sqrt(Wi / omv);
