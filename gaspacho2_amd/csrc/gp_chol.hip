@@ -418,6 +418,22 @@ __global__ void __launch_bounds__(CT) k_cholsl_blk(const double *__restrict__ La
 // independent: one workgroup per (block column, gene) walks I downwards.
 constexpr int TB = 32;
 
+// Workgroup -> (x, matrix y) for the batched inverse kernels: the hardware
+// deals consecutive workgroup ids round robin over the 8 XCDs, so the id
+// xcd + 8 (x + nx q) puts every x of matrix y = 8 q + xcd on XCD xcd, where
+// they share its L2 (with (x, y) as a 2-D grid a matrix's workgroups span
+// all eight L2s and each fetches the matrix from HBM).  False past the last
+// matrix; the grid is xcd_grid(nx, batch) workgroups.
+__device__ __forceinline__ bool xcd_pick(int nx, int batch, int &x, int &y) {
+    const int id = blockIdx.x, xcd = id & 7, q = id >> 3;
+    x = q % nx;
+    y = 8 * (q / nx) + xcd;
+    return y < batch;
+}
+static inline unsigned xcd_grid(int nx, int batch) {
+    return (unsigned)nx * (unsigned)((batch + 7) / 8) * 8u;
+}
+
 // V_II = L_II^{-1} for every diagonal block: lane c solves column c by
 // forward substitution in LDS.
 __global__ void __launch_bounds__(64) k_trinv_diag(const double *__restrict__ Lall, int n, size_t stride,
@@ -447,7 +463,7 @@ __global__ void __launch_bounds__(64) k_trinv_diag(const double *__restrict__ La
     }
 }
 
-// Off-diagonal blocks of block column J = blockIdx.x (all but the last block
+// Off-diagonal blocks of block column J (from the workgroup id; all but the last block
 // column) on v_mfma_f64_16x16x4f64: 4 waves, wave w owns the 16 x 16 tile
 // (w / 2, w % 2) of each 32 x 32 block.  Per block row I: T = sum_K L_IK V_KJ
 // with both fragments read from global memory (L from the factorization; V_KJ
@@ -459,11 +475,14 @@ __global__ void __launch_bounds__(64) k_trinv_diag(const double *__restrict__ La
 typedef double tri_d4 __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(CT) k_trinv_mfma(const double *__restrict__ Lall, int n, size_t stride,
-                                                   double *__restrict__ Vall) {
+                                                   double *__restrict__ Vall, int batch) {
     __shared__ double Tt[TB][TB + 1];
-    const double *L = Lall + (size_t)blockIdx.y * stride;
-    double *V = Vall + (size_t)blockIdx.y * stride;
-    const int NB = (n + TB - 1) / TB, J = blockIdx.x, c0 = TB * J;
+    const int NB = (n + TB - 1) / TB;
+    int J, y;
+    if (!xcd_pick(NB - 1, batch, J, y)) return;
+    const double *L = Lall + (size_t)y * stride;
+    double *V = Vall + (size_t)y * stride;
+    const int c0 = TB * J;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int tr = w >> 1, tc = w & 1;
@@ -513,15 +532,17 @@ __global__ void __launch_bounds__(CT) k_trinv_mfma(const double *__restrict__ La
 // prefetched into registers), 4x4 per thread.
 // Entry (a, b), a <= b, sums rows k >= b only (V[k, b] = 0 above the diagonal).
 __global__ void __launch_bounds__(CT) k_vtv(const double *__restrict__ Vall, int n, size_t stride,
-                                            int ntile, double *__restrict__ Sall) {
+                                            int ntile, double *__restrict__ Sall, int batch) {
     __shared__ double As[32][65];
     __shared__ double Bs[32][65];
-    int ta = 0, t = blockIdx.x;
+    int t, y;
+    if (!xcd_pick(ntile * (ntile + 1) / 2, batch, t, y)) return;
+    int ta = 0;
     while (t >= ntile - ta) { t -= ntile - ta; ta++; }
     const int tb = ta + t;
     const int a0 = ta * 64, b0 = tb * 64;
-    const double *V = Vall + (size_t)blockIdx.y * stride;
-    double *S = Sall + (size_t)blockIdx.y * stride;
+    const double *V = Vall + (size_t)y * stride;
+    double *S = Sall + (size_t)y * stride;
     const int tid = threadIdx.x, tx = tid % 16, ty = tid / 16;
     double acc[4][4];
 #pragma unroll
@@ -588,15 +609,17 @@ __global__ void __launch_bounds__(CT) k_vtv(const double *__restrict__ Vall, int
 // l / 16] = V[k0 + 4 s + l / 16][a0 + 16 w + l % 16], B[k = l / 16][n = l % 16]
 // = V[k0 + 4 s + l / 16][b0 + 16 cb + l % 16], D[m = 4 r + l / 16][n = l % 16].
 __global__ void __launch_bounds__(CT) k_vtv_mfma(const double *__restrict__ Vall, int n, size_t stride,
-                                                 int ntile, double *__restrict__ Sall) {
+                                                 int ntile, double *__restrict__ Sall, int batch) {
     __shared__ double As[32][65];
     __shared__ double Bs[32][65];
-    int ta = 0, t = blockIdx.x;
+    int t, y;
+    if (!xcd_pick(ntile * (ntile + 1) / 2, batch, t, y)) return;
+    int ta = 0;
     while (t >= ntile - ta) { t -= ntile - ta; ta++; }
     const int tb = ta + t;
     const int a0 = ta * 64, b0 = tb * 64;
-    const double *V = Vall + (size_t)blockIdx.y * stride;
-    double *S = Sall + (size_t)blockIdx.y * stride;
+    const double *V = Vall + (size_t)y * stride;
+    double *S = Sall + (size_t)y * stride;
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int lm = lane & 15, lk = lane >> 4;
@@ -705,15 +728,15 @@ int gpk_chol_inverse_batched(gp_ctx *c, const double *L, int n, int batch, size_
     k_trinv_diag<<<dim3(nb, batch), 64, 0, c->stream>>>(L, n, st, c->Vtmp);
     GP_HIP(hipGetLastError());
     if (nb > 1) {
-        k_trinv_mfma<<<dim3(nb - 1, batch), CT, 0, c->stream>>>(L, n, st, c->Vtmp);
+        k_trinv_mfma<<<xcd_grid(nb - 1, batch), CT, 0, c->stream>>>(L, n, st, c->Vtmp, batch);
         GP_HIP(hipGetLastError());
     }
     const int nt = (n + 63) / 64;
-    dim3 g(nt * (nt + 1) / 2, batch);
+    const unsigned g = xcd_grid(nt * (nt + 1) / 2, batch);
     if (c->vtv_mfma)
-        k_vtv_mfma<<<g, CT, 0, c->stream>>>(c->Vtmp, n, st, nt, S);
+        k_vtv_mfma<<<g, CT, 0, c->stream>>>(c->Vtmp, n, st, nt, S, batch);
     else
-        k_vtv<<<g, CT, 0, c->stream>>>(c->Vtmp, n, st, nt, S);
+        k_vtv<<<g, CT, 0, c->stream>>>(c->Vtmp, n, st, nt, S, batch);
     GP_HIP(hipGetLastError());
     return 0;
 }
@@ -721,11 +744,11 @@ int gpk_chol_inverse_batched(gp_ctx *c, const double *L, int n, int batch, size_
 int gpk_vtv(gp_ctx *c, const double *V, int n, int batch, size_t stride, double *S) {
     const size_t st = stride ? stride : (size_t)n * n;
     const int nt = (n + 63) / 64;
-    dim3 g(nt * (nt + 1) / 2, batch);
+    const unsigned g = xcd_grid(nt * (nt + 1) / 2, batch);
     if (c->vtv_mfma)
-        k_vtv_mfma<<<g, CT, 0, c->stream>>>(V, n, st, nt, S);
+        k_vtv_mfma<<<g, CT, 0, c->stream>>>(V, n, st, nt, S, batch);
     else
-        k_vtv<<<g, CT, 0, c->stream>>>(V, n, st, nt, S);
+        k_vtv<<<g, CT, 0, c->stream>>>(V, n, st, nt, S, batch);
     GP_HIP(hipGetLastError());
     return 0;
 }
@@ -743,7 +766,7 @@ int gpk_tri_inverse(gp_ctx *c, const double *L, int n, double *V) {
     k_trinv_diag<<<dim3(nb, 1), 64, 0, c->stream>>>(L, n, st, V);
     GP_HIP(hipGetLastError());
     if (nb > 1) {
-        k_trinv_mfma<<<dim3(nb - 1, 1), CT, 0, c->stream>>>(L, n, st, V);
+        k_trinv_mfma<<<xcd_grid(nb - 1, 1), CT, 0, c->stream>>>(L, n, st, V, 1);
         GP_HIP(hipGetLastError());
     }
     return 0;
